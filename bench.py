@@ -1,0 +1,133 @@
+#!/usr/bin/env python3
+"""Headline benchmark: ptnee camera samples per second at 720p on one MI355X.
+
+Workload (one step): a 1280x720 frame at 64 samples per pixel, max depth 10, Russian roulette
+(prr 0.8) from vertex 3, on the procedural scene of a 512x512-cell heightfield, 2000 80-triangle
+icospheres with GGX-reflect materials, four emissive quads and a 1024x512 sky (684,296
+triangles).  The scene is built and resident in HBM before the timed region; a step is one
+nr_render_ptnee call (all wavefront batches of the frame, FP64 accumulation included).
+
+Printed JSON line: value = Msamples/s over the timed steps, the traversal roofline (HIP events
+around every closest-hit launch; algorithmic bytes from the kernel's own node / triangle
+counters) and the CPU baseline (the test oracle's restatement of the reference estimator timed
+on a bounded slice of the same frame with all host threads).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes of one closest-hit ray: 64 B per node-pair fetch, 48 B per triangle test,
+# 32 B ray in, 16 B hit out
+B_PAIR, B_TRI, B_RAY = 64, 48, 48
+
+
+def cpu_baseline(sc, spp_frame, target_s=12.0):
+    """Times the oracle (CPU restatement of RendererPTLS, OpenMP over 8x8 tiles, all host
+    threads) on whole rows of the same frame at 1 spp, sized to ~target_s of CPU work."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    from nuray_amd.scenes import camera_array
+
+    threads = len(os.sched_getaffinity(0))
+    W, H = sc["width"], sc["height"]
+    o = po.Scene(sc)
+    cam = camera_array(sc["camera"])
+    kw = dict(maxdepth=10, rrdepth=3, nthreads=threads, seed=7)
+    y0 = H // 2 - 4
+    t0 = time.perf_counter()
+    o.render(cam, W, H, 1, rect=(0, y0, W, y0 + 8), **kw)
+    dt = time.perf_counter() - t0
+    rows = int(np.clip(8 * target_s / max(dt, 1e-3), 8, H))
+    y0 = max(0, H // 2 - rows // 2)
+    t0 = time.perf_counter()
+    o.render(cam, W, H, 1, rect=(0, y0, W, y0 + rows), **kw)
+    dt = time.perf_counter() - t0
+    samples = W * rows
+    return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"rows {y0}..{y0 + rows} of the 1280x720 frame at 1 spp ({samples} camera samples, {dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; a frame shards by pixels x samples "
+                                   "(see DESIGN.md) but no multi-GPU run is part of it", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import nuray_amd as nr
+    from nuray_amd import scenes
+
+    nr.init(0)
+    sc = scenes.terrain_spheres()
+    W, H, spp = sc["width"], sc["height"], args.spp
+    t0 = time.perf_counter()
+    dev = nr.Scene(sc)
+    prep_s = time.perf_counter() - t0
+    info = dev.info()
+    rp = dict(spp=spp, seed=7, maxdepth=10, rrdepth=3, prr=0.8, fetch=False)
+    print(f"# scene {sc['name']}: {len(sc['p'])} triangles, BVH {info['n_nodes']} nodes depth {info['depth']}, "
+          f"build {info['build_ms']:.0f} ms, prepare {prep_s:.2f} s", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        dev.render(sc["camera"], W, H, **rp)
+    # counters pass (untimed): node-pair fetches / triangle tests / rays of the closest-hit kernel
+    dev.set_profiling(2)
+    dev.render(sc["camera"], W, H, **rp)
+    cnt = dev.stats()
+    dev.set_profiling(1)
+    nr.synchronize()
+    closest_ms = 0.0
+    launches = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dev.render(sc["camera"], W, H, **rp)
+        st = dev.stats()
+        closest_ms += st["closest_ms"]
+        launches += st["closest_launches"]
+    nr.synchronize()
+    elapsed = time.perf_counter() - t0
+    samples = W * H * spp * args.steps
+    value = samples / elapsed / 1e6
+    print(f"# timed {args.steps} frames in {elapsed:.3f} s; closest-hit kernel {closest_ms:.1f} ms over "
+          f"{launches} launches; last frame: {st['ms']:.1f} ms, {st['iterations']:.0f} iterations",
+          file=sys.stderr, flush=True)
+
+    bytes_frame = B_PAIR * cnt["pair_fetch"] + B_TRI * cnt["tri_tests"] + B_RAY * cnt["rays"]
+    achieved = bytes_frame * args.steps / (closest_ms / 1e3) / 1e9 if closest_ms > 0 else None
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                "kernel": "k_trace<closest>", "bytes_per_launch": bytes_frame / max(cnt["closest_launches"], 1),
+                "avg_launch_ms": closest_ms / max(launches, 1),
+                "rays_per_frame": cnt["rays"], "pair_fetch_per_ray": cnt["pair_fetch"] / max(cnt["rays"], 1),
+                "tri_tests_per_ray": cnt["tri_tests"] / max(cnt["rays"], 1)}
+    cpu = None if args.no_cpu else cpu_baseline(sc, spp)
+    out = {
+        "metric": "Msamples/s at 720p", "value": value, "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"ptnee {W}x{H}x{spp}spp maxdepth10 rrdepth3 prr0.8, {sc['name']} "
+                               f"({len(sc['p'])} triangles) + 1024x512 sky", "global_batch": W * H * spp,
+                   "seq_len": None, "parallelism": "single-gpu", "bvh_build_ms": info["build_ms"]},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

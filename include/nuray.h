@@ -1,0 +1,124 @@
+/* nuray.h -- C ABI of the MI355X ptnee path (libnuray.so).
+ *
+ * The reference (mollnn/nuRay) has no C ABI: its boundary is the C++ virtual class
+ * Renderer { prepare(triangles); render(camera, triangles, img, config, ...); setEnvmap(env); }
+ * (src/renderer/renderer.h:15-35) plus the Triangle / Material / Texture / Camera / Config value
+ * types.  Each entry point below names the reference interface it replaces.  Host C++ (the
+ * nuray::Renderer facade, include/nuray.hpp) and Python tests bind to these functions.
+ *
+ * Conventions: every function returns NR_OK (0) or a negative error code; nr_last_error()
+ * returns the message of the last failure on the calling thread.  No exceptions cross the ABI.
+ * Buffers are caller-owned host memory unless a name ends in _device.  One scene is used from
+ * one host thread at a time; `cancel` may be written from any thread.
+ */
+#ifndef NURAY_H
+#define NURAY_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NR_OK 0
+#define NR_CANCELLED 1
+#define NR_ERROR (-1)
+
+typedef struct nr_scene nr_scene;
+
+/* Material rows passed to nr_scene_create: 8 floats {type, r, g, b, alpha, ior, texture, 0}.
+ * type: NR_MAT_LAMBERT (MatLambert, Kd), NR_MAT_LIGHT (MatLight, Ke), NR_MAT_GGXREFL (MatGGXRefl,
+ * F0 + roughness alpha), NR_MAT_GGX (MatGGX translucent dielectric, F0 + alpha + ior).
+ * texture: index of an albedo texture (nr_scene_set_texture_rgba8) or -1. */
+#define NR_MAT_LAMBERT 0
+#define NR_MAT_LIGHT 1
+#define NR_MAT_GGXREFL 2
+#define NR_MAT_GGX 3
+
+/* Camera(fov, asp, pos, yaw, pitch, roll), src/scene/camera.h:18 and camera.cpp:8-22,88-105. */
+typedef struct nr_camera {
+    float pos[3];
+    float euler[3]; /* yaw, pitch, roll in degrees */
+    float fov;      /* vertical field of view in degrees (camera.cpp:28) */
+    float aspect;   /* film width / height */
+} nr_camera;
+
+/* The Config keys RendererPTLS::render / ::trace read (rendererptls.cpp:65,81,98-100,130,191)
+ * plus this build's seed / maxdepth / rrdepth.  maxdepth = 0 and rrdepth = 0 are the reference's
+ * semantics (no depth limit, Russian roulette at every vertex). */
+typedef struct nr_params {
+    int32_t width, height; /* imgw, imgh */
+    int32_t spp;           /* spp */
+    int32_t spp_offset;    /* first sample index (progressive / split renders) */
+    uint32_t seed;         /* Philox key */
+    float prr;             /* prr, survival probability of Russian roulette */
+    int32_t mis;           /* mis: 1 = balance heuristic, 0 = fixed 0.5 weights */
+    int32_t maxdepth;      /* path vertices with index >= maxdepth are not created (0 = unbounded) */
+    int32_t rrdepth;       /* Russian roulette applies at vertex index >= rrdepth */
+    int32_t batch_spp;     /* samples per pixel per wavefront batch (0 = automatic) */
+    int32_t accumulate;    /* 1: add to the previous render's sums instead of clearing them */
+} nr_params;
+
+typedef void (*nr_progress_fn)(float fraction, void* user);
+
+const char* nr_last_error(void);
+/* Selects the HIP device for this host thread. */
+int nr_init(int32_t device);
+int nr_synchronize(void);
+
+/* Replaces Loader::getTriangles() + Triangle::Triangle (src/scene/loader.h:15-19,
+ * triangle.cpp:5-25): triangles in loader order, p = N x 9 floats (p0, p1, p2), n = N x 9 vertex
+ * normals or NULL (flat normals when |n0|^2 < 1e-4), uv = N x 6 or NULL, one material row per
+ * triangle.  Triangle ids reported by nr_intersect are indices into this order. */
+nr_scene* nr_scene_create(const float* p, const float* n, const float* uv, const int32_t* mat_id,
+                          uint32_t n_tris, const float* mats, uint32_t n_mats);
+int nr_scene_destroy(nr_scene* s);
+/* Texture::load result (texture.cpp:5-31): RGBA8 texels, row-major, first row at v = 1. */
+int nr_scene_set_texture_rgba8(nr_scene* s, int32_t id, int32_t w, int32_t h, const uint8_t* rgba);
+/* Renderer::setEnvmap / Envmap(texture, gain) (renderer.h:30, envmap.cpp:17-24): linear RGB float
+ * texels, row-major, looked up with the reference's bilinear rule. */
+int nr_scene_set_envmap(nr_scene* s, int32_t w, int32_t h, const float* rgb, float gain);
+
+/* Renderer::prepare (renderer.cpp:30-39): LightSampler::setPrimitives + BVH::build with the
+ * reference's SAH rule (bvh.cpp:5-97), flattened and uploaded to HBM.  nthreads <= 0: all cores. */
+int nr_bvh_build_sah(nr_scene* s, int32_t nthreads);
+int nr_bvh_info(nr_scene* s, int32_t* n_nodes, int32_t* depth, double* build_ms);
+/* Topology dump in preorder over all nodes: children (-1 for leaves), boxes (min xyz, max xyz),
+ * leaf primitive ranges into prim_order (loader ids in DFS leaf order). */
+int nr_bvh_export(nr_scene* s, int32_t* left, int32_t* right, float* box, int32_t* leaf_first,
+                  int32_t* leaf_count, int32_t* prim_order);
+
+/* BVH::build alone, on the host, without a device (same rule and export format as above).
+ * Arrays sized for 2 * n_tris - 1 nodes and n_tris primitives; returns the node count or < 0. */
+int nr_bvh_build_host(const float* p, uint32_t n_tris, int32_t nthreads, int32_t* depth, int32_t* left,
+                      int32_t* right, float* box, int32_t* leaf_first, int32_t* leaf_count, int32_t* prim_order);
+
+/* BVH::intersection (bvh.cpp:129-137) on a ray batch: o, d = n x 3 floats.
+ * any_hit = 0: closest hit with the reference's tie order; tri = loader id or -1, t = 1e18 on a miss.
+ * any_hit = 1: the shadow-ray test of rendererptls.cpp:70-71 with segment length tmax[i]: the ray
+ * is blocked iff the reference's closest hit satisfies (double)t + 2e-3 <= tmax[i]; tri = a blocking
+ * triangle or -1 when the segment is clear. */
+int nr_intersect(nr_scene* s, const float* o, const float* d, uint32_t n, int32_t any_hit, const float* tmax,
+                 int32_t* tri, float* t, float* b1, float* b2);
+
+/* RendererPTLS::render (rendererptls.cpp:96-236) into the scene's device accumulators.
+ * Outputs may be NULL (then fetch later with nr_render_fetch): rgb_linear = mean radiance
+ * (W*H*3 floats), sum / sum2 = FP64 per-pixel sums of the samples and their squares, rgb8 = the
+ * reference's display transform.  progress is called after each wavefront batch; a non-zero
+ * *cancel stops the render (returns NR_CANCELLED; the partial sums stay readable). */
+int nr_render_ptnee(nr_scene* s, const nr_camera* cam, const nr_params* params, float* rgb_linear, double* sum,
+                    double* sum2, uint8_t* rgb8, nr_progress_fn progress, void* user, volatile int32_t* cancel);
+int nr_render_fetch(nr_scene* s, float* rgb_linear, double* sum, double* sum2, uint8_t* rgb8);
+/* Statistics of the last render: [0] wall ms, [1] samples, [2] wavefront iterations,
+ * [3] closest-hit kernel ms, [4] closest-hit launches, [5] rays traced (closest),
+ * [6] node-pair fetches (closest), [7] triangle tests (closest), [8] shadow kernel ms,
+ * [9] shadow rays, [10] shadow node-pair fetches, [11] shadow triangle tests.
+ * Kernel times are filled when the render ran with timing on (nr_set_profiling). */
+int nr_render_stats(nr_scene* s, double* out12);
+/* 1: time the traversal launches with HIP events; 2: also count node-pair fetches / triangle tests. */
+int nr_set_profiling(nr_scene* s, int32_t mode);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,224 @@
+// extern "C" entry points of libnuray.so (include/nuray.h).
+#include "../../include/nuray.h"
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include "render.h"
+#include "scene.h"
+
+struct nr_scene {
+    nr::Scene* sc;
+    int profiling = 0;
+    double stats[12] = {0};
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+    try {
+        return f();
+    } catch (const std::exception& e) {
+        g_err = e.what();
+    } catch (...) {
+        g_err = "unknown error";
+    }
+    return NR_ERROR;
+}
+}  // namespace
+
+extern "C" {
+
+const char* nr_last_error(void) { return g_err.c_str(); }
+
+int nr_init(int32_t device) {
+    return guard([&] {
+        nr::check_hip(hipSetDevice(device), "hipSetDevice");
+        nr::check_hip(hipFree(nullptr), "context init");
+        return NR_OK;
+    });
+}
+
+int nr_synchronize(void) {
+    return guard([&] {
+        nr::check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        return NR_OK;
+    });
+}
+
+nr_scene* nr_scene_create(const float* p, const float* n, const float* uv, const int32_t* mat_id, uint32_t n_tris,
+                          const float* mats, uint32_t n_mats) {
+    nr_scene* out = nullptr;
+    guard([&] {
+        if (!p || !mat_id || !mats) throw std::runtime_error("null scene arrays");
+        out = new nr_scene{new nr::Scene(p, n, uv, mat_id, n_tris, mats, n_mats)};
+        return NR_OK;
+    });
+    return out;
+}
+
+int nr_scene_destroy(nr_scene* s) {
+    return guard([&] {
+        if (s) {
+            delete s->sc;
+            delete s;
+        }
+        return NR_OK;
+    });
+}
+
+int nr_scene_set_texture_rgba8(nr_scene* s, int32_t id, int32_t w, int32_t h, const uint8_t* rgba) {
+    return guard([&] {
+        s->sc->add_texture(id, w, h, 0, rgba);
+        return NR_OK;
+    });
+}
+
+int nr_scene_set_envmap(nr_scene* s, int32_t w, int32_t h, const float* rgb, float gain) {
+    return guard([&] {
+        // the environment map lives in the texture pool after the albedo textures
+        int id = int(s->sc->textures.size());
+        if (s->sc->env_tex >= 0) id = s->sc->env_tex;
+        s->sc->add_texture(id, w, h, 1, rgb);
+        s->sc->env_tex = id;
+        s->sc->env_gain = gain;
+        return NR_OK;
+    });
+}
+
+int nr_bvh_build_sah(nr_scene* s, int32_t nthreads) {
+    return guard([&] {
+        s->sc->build(nthreads);
+        return NR_OK;
+    });
+}
+
+int nr_bvh_info(nr_scene* s, int32_t* n_nodes, int32_t* depth, double* build_ms) {
+    return guard([&] {
+        if (!s->sc->built) throw std::runtime_error("BVH not built");
+        if (n_nodes) *n_nodes = int32_t(s->sc->bvh.left.size());
+        if (depth) *depth = s->sc->bvh.depth;
+        if (build_ms) *build_ms = s->sc->bvh.build_ms;
+        return NR_OK;
+    });
+}
+
+int nr_bvh_export(nr_scene* s, int32_t* left, int32_t* right, float* box, int32_t* leaf_first, int32_t* leaf_count,
+                  int32_t* prim_order) {
+    return guard([&] {
+        const nr::BvhBuild& b = s->sc->bvh;
+        if (!s->sc->built) throw std::runtime_error("BVH not built");
+        const size_t nn = b.left.size();
+        std::memcpy(left, b.left.data(), nn * 4);
+        std::memcpy(right, b.right.data(), nn * 4);
+        std::memcpy(box, b.box.data(), nn * 24);
+        std::memcpy(leaf_first, b.leaf_first.data(), nn * 4);
+        std::memcpy(leaf_count, b.leaf_count.data(), nn * 4);
+        std::memcpy(prim_order, b.order.data(), b.order.size() * 4);
+        return NR_OK;
+    });
+}
+
+int nr_bvh_build_host(const float* p, uint32_t n_tris, int32_t nthreads, int32_t* depth, int32_t* left,
+                      int32_t* right, float* box, int32_t* leaf_first, int32_t* leaf_count, int32_t* prim_order) {
+    int nn = NR_ERROR;
+    int rc = guard([&] {
+        nr::BvhBuild b = nr::build_sah(p, int(n_tris), nthreads);
+        nn = int(b.left.size());
+        if (depth) *depth = b.depth;
+        std::memcpy(left, b.left.data(), nn * 4);
+        std::memcpy(right, b.right.data(), nn * 4);
+        std::memcpy(box, b.box.data(), size_t(nn) * 24);
+        std::memcpy(leaf_first, b.leaf_first.data(), nn * 4);
+        std::memcpy(leaf_count, b.leaf_count.data(), nn * 4);
+        std::memcpy(prim_order, b.order.data(), b.order.size() * 4);
+        return NR_OK;
+    });
+    return rc == NR_OK ? nn : rc;
+}
+
+int nr_intersect(nr_scene* s, const float* o, const float* d, uint32_t n, int32_t any_hit, const float* tmax,
+                 int32_t* tri, float* t, float* b1, float* b2) {
+    return guard([&] {
+        nr::Scene& sc = *s->sc;
+        if (!sc.built) throw std::runtime_error("BVH not built");
+        if (any_hit && !tmax) throw std::runtime_error("any_hit needs tmax");
+        if (n == 0) return NR_OK;
+        std::vector<nr::Ray> rays(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            rays[i].o = make_float4(o[3 * i], o[3 * i + 1], o[3 * i + 2], any_hit ? tmax[i] : 0.0f);
+            rays[i].d = make_float4(d[3 * i], d[3 * i + 1], d[3 * i + 2], 0.0f);
+        }
+        nr::DevBuf dr, dh, dc;
+        dr.upload(rays.data(), n * sizeof(nr::Ray), sc.stream);
+        dh.ensure(n * sizeof(nr::HitRec));
+        dc.ensure(4 * sizeof(int));
+        int hc[4] = {int(n), 0, 0, 0};
+        nr::check_hip(hipMemcpyAsync(dc.ptr, hc, sizeof(hc), hipMemcpyHostToDevice, sc.stream), "H2D");
+        int* c = dc.as<int>();
+        if (any_hit)
+            nr::launch_shadow(sc.view, dr.as<nr::Ray>(), c, nullptr, dh.as<nr::HitRec>(), nullptr, nullptr, c + 1,
+                              c + 2, nullptr, sc.stream);
+        else
+            nr::launch_closest(sc.view, dr.as<nr::Ray>(), c, dh.as<nr::HitRec>(), c + 1, c + 2, nullptr, sc.stream);
+        std::vector<nr::HitRec> h(n);
+        nr::check_hip(hipMemcpyAsync(h.data(), dh.ptr, n * sizeof(nr::HitRec), hipMemcpyDeviceToHost, sc.stream), "D2H");
+        nr::check_hip(hipMemcpyAsync(hc, dc.ptr, sizeof(hc), hipMemcpyDeviceToHost, sc.stream), "D2H");
+        nr::check_hip(hipStreamSynchronize(sc.stream), "intersect sync");
+        if (hc[2]) throw std::runtime_error("traversal stack overflow");
+        for (uint32_t i = 0; i < n; ++i) {
+            tri[i] = h[i].prim >= 0 ? sc.bvh.order[h[i].prim] : -1;
+            t[i] = h[i].t;
+            b1[i] = h[i].b1;
+            b2[i] = h[i].b2;
+        }
+        return NR_OK;
+    });
+}
+
+int nr_render_ptnee(nr_scene* s, const nr_camera* cam, const nr_params* pp, float* rgb_linear, double* sum,
+                    double* sum2, uint8_t* rgb8, nr_progress_fn progress, void* user, volatile int32_t* cancel) {
+    return guard([&] {
+        nr::CamView cv = nr::make_camera(cam->pos, cam->euler, cam->fov, cam->aspect);
+        nr::RenderParams p;
+        p.W = pp->width; p.H = pp->height; p.spp = pp->spp; p.spp_offset = pp->spp_offset;
+        p.seed = pp->seed; p.prr = pp->prr; p.mis = pp->mis; p.maxdepth = pp->maxdepth; p.rrdepth = pp->rrdepth;
+        p.batch_spp = pp->batch_spp; p.accumulate = pp->accumulate;
+        nr::RenderStats rs;
+        nr::TraceStats ts[2] = {};
+        nr::KernelTimes kt;
+        int st = nr::render_ptnee(*s->sc, cv, p, progress, user, (volatile int*)cancel, &rs, s->profiling >= 2, ts,
+                                  s->profiling >= 1 ? &kt : nullptr);
+        double* o = s->stats;
+        o[0] = rs.ms; o[1] = double(rs.samples); o[2] = rs.iterations;
+        o[3] = kt.closest_ms; o[4] = kt.closest_launches;
+        o[5] = double(ts[0].rays); o[6] = double(ts[0].pair_fetch); o[7] = double(ts[0].tri_tests);
+        o[8] = kt.shadow_ms; o[9] = double(ts[1].rays); o[10] = double(ts[1].pair_fetch); o[11] = double(ts[1].tri_tests);
+        if (rgb_linear || sum || sum2 || rgb8) nr::render_fetch(*s->sc, rgb_linear, sum, sum2, rgb8);
+        return st ? NR_CANCELLED : NR_OK;
+    });
+}
+
+int nr_render_fetch(nr_scene* s, float* rgb_linear, double* sum, double* sum2, uint8_t* rgb8) {
+    return guard([&] {
+        nr::render_fetch(*s->sc, rgb_linear, sum, sum2, rgb8);
+        return NR_OK;
+    });
+}
+
+int nr_render_stats(nr_scene* s, double* out12) {
+    return guard([&] {
+        std::memcpy(out12, s->stats, sizeof(s->stats));
+        return NR_OK;
+    });
+}
+
+int nr_set_profiling(nr_scene* s, int32_t mode) {
+    return guard([&] {
+        s->profiling = mode;
+        return NR_OK;
+    });
+}
+
+}  // extern "C"

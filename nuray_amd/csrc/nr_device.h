@@ -1,0 +1,175 @@
+// Device-side records and inline helpers of the MI355X ptnee path.
+//
+// Layout in HBM (all arrays 64-byte aligned by hipMalloc):
+//   NodePair  64 B  one per internal BVH node, preorder; holds both children's boxes and refs
+//   TriGeom   48 B  one per triangle, in DFS leaf order: p0, e1 = p1-p0, e2 = p2-p0, loader id
+//   TriShade  64 B  same order: n0, n1, n2, uv0, uv1, uv2, material row
+//   MatRec    32 B  one per material: {type, r, g, b, alpha, ior, texture, 0}
+//   LightRec 112 B  one per emissive triangle, loader order, with the area CDF beside it
+//
+// Every float step that decides a hit follows src/hierarchy/bvhnode.cpp and
+// src/scene/triangle.cpp in order and precision; this header is compiled with
+// -ffp-contract=off so no multiply-add is fused.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nr {
+
+enum MatType { MAT_LAMBERT = 0, MAT_LIGHT = 1, MAT_GGXREFL = 2, MAT_GGX = 3 };
+
+// Leaf references inside NodePair::ref: bit 31 set, bits 27..30 = primitive count (1..5),
+// bits 0..26 = first primitive in TriGeom order.  Internal references are NodePair indices.
+constexpr uint32_t LEAF_BIT = 0x80000000u;
+__host__ __device__ inline bool ref_is_leaf(int32_t r) { return (uint32_t(r) & LEAF_BIT) != 0; }
+__host__ __device__ inline int ref_first(int32_t r) { return int(uint32_t(r) & 0x07FFFFFFu); }
+__host__ __device__ inline int ref_count(int32_t r) { return int((uint32_t(r) >> 27) & 0xFu); }
+__host__ __device__ inline int32_t make_leaf(int first, int count) {
+    return int32_t(LEAF_BIT | (uint32_t(count) << 27) | uint32_t(first));
+}
+
+struct alignas(64) NodePair {
+    float4 a;  // lmin.x lmin.y lmin.z lmax.x
+    float4 b;  // lmax.y lmax.z rmin.x rmin.y
+    float4 c;  // rmin.z rmax.x rmax.y rmax.z
+    int4 d;    // lref rref 0 0
+};
+struct alignas(16) TriGeom {
+    float4 g0;  // p0.xyz e1.x
+    float4 g1;  // e1.yz e2.xy
+    float4 g2;  // e2.z, loader index (bits), 0, 0
+};
+struct alignas(16) TriShade {
+    float4 s0;  // n0.xyz n1.x
+    float4 s1;  // n1.yz n2.xy
+    float4 s2;  // n2.z uv0.xy uv1.x
+    float4 s3;  // uv1.y uv2.xy material (bits)
+};
+struct alignas(16) MatRec {
+    float4 a;  // type, r, g, b
+    float4 b;  // alpha, ior, texture, 0
+};
+struct alignas(16) LightRec {
+    float4 p0, p1, p2;  // xyz used
+    float4 n0, n1, n2;  // xyz used
+    float4 ke;          // emission
+};
+struct TexDesc {
+    int32_t w, h, fmt, pad;  // fmt 0: RGBA8 (sRGB-like pow 2.2 decode through a LUT), 1: RGB32F
+    uint64_t offset;         // byte offset into the texture pool
+};
+
+// Everything a kernel needs to read the scene.
+struct SceneView {
+    const NodePair* nodes;
+    const TriGeom* tris;
+    const TriShade* shade;
+    const MatRec* mats;
+    const LightRec* lights;
+    const float* light_cdf;
+    const TexDesc* tex;
+    const uint8_t* texpool;
+    const float* lut;  // 256-entry RGBA8 decode table
+    float root_min[3], root_max[3];
+    int32_t root_ref;  // root as a NodePair index, or a leaf reference
+    int32_t n_lights;
+    float light_p;
+    int32_t env_tex;   // texture index of the environment map or -1
+    float env_gain;
+    int32_t stack_cap; // traversal stack entries per lane
+};
+
+// Queued ray: origin + tmax / shadow length, direction + payload.
+struct alignas(16) Ray {
+    float4 o;  // xyz, w = segment length for shadow rays (unused for closest hit)
+    float4 d;  // xyz, w = path slot (bits)
+};
+struct alignas(16) HitRec {
+    float t, b1, b2;
+    int32_t prim;  // TriGeom index, -1 for a miss
+};
+
+// ------------------------------------------------------------------ exact hit arithmetic
+struct F3 {
+    float x, y, z;
+};
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ F3 sub(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 add(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ F3 mul(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ F3 mulv(F3 a, F3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ F3 divs(F3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ F3 neg(F3 a) { return {-a.x, -a.y, -a.z}; }
+// vec3::dot and vec3::cross, src/math/vec3.cpp:54-61 (left-to-right, unfused).
+__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ F3 cross(F3 a, F3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float norm(F3 a) { return sqrtf(dot(a, a)); }
+// vec3::normalized, vec3.cpp:70-73.
+__device__ __forceinline__ F3 normalized(F3 a) { return divs(a, norm(a) + 1e-8f); }
+
+// BVHNode::intersect, bvhnode.cpp:35-48.  dd = d + 1e-9f per axis (computed once per ray).
+// std::max(a, b) is (a < b) ? b : a and std::min(a, b) is (b < a) ? b : a: NaN in the second
+// operand leaves the first one.
+__device__ __forceinline__ bool slab(float mnx, float mny, float mnz, float mxx, float mxy, float mxz,
+                                     F3 o, F3 dd) {
+    float t_in = -1e18f, t_out = 1e18f;
+    {
+        float t1 = (mnx - o.x) / dd.x, t2 = (mxx - o.x) / dd.x;
+        if (t1 > t2) { float s = t1; t1 = t2; t2 = s; }
+        t_in = (t_in < t1) ? t1 : t_in;
+        t_out = (t2 < t_out) ? t2 : t_out;
+    }
+    {
+        float t1 = (mny - o.y) / dd.y, t2 = (mxy - o.y) / dd.y;
+        if (t1 > t2) { float s = t1; t1 = t2; t2 = s; }
+        t_in = (t_in < t1) ? t1 : t_in;
+        t_out = (t2 < t_out) ? t2 : t_out;
+    }
+    {
+        float t1 = (mnz - o.z) / dd.z, t2 = (mxz - o.z) / dd.z;
+        if (t1 > t2) { float s = t1; t1 = t2; t2 = s; }
+        t_in = (t_in < t1) ? t1 : t_in;
+        t_out = (t2 < t_out) ? t2 : t_out;
+    }
+    return t_in <= t_out && t_out >= 0.0f;
+}
+
+// Triangle::intersection, triangle.cpp:27-33; q rounds (double)(s1.e1) + 1e-6 to float.
+__device__ __forceinline__ void tri_test(const TriGeom& g, F3 o, F3 d, float& t, float& b1, float& b2) {
+    F3 p0 = f3(g.g0.x, g.g0.y, g.g0.z);
+    F3 e1 = f3(g.g0.w, g.g1.x, g.g1.y);
+    F3 e2 = f3(g.g1.z, g.g1.w, g.g2.x);
+    F3 s = sub(o, p0), s1 = cross(d, e2), s2 = cross(s, e1);
+    float tn = dot(s2, e2), b1n = dot(s1, s), b2n = dot(s2, d);
+    float q = float(double(dot(s1, e1)) + 1e-6);
+    t = tn / q;
+    b1 = b1n / q;
+    b2 = b2n / q;
+}
+// Leaf acceptance, bvhnode.cpp:57 (the running best is handled by the caller).
+__device__ __forceinline__ bool tri_valid(float t, float b1, float b2) {
+    return t > 0.0f && t < 1e18f && b1 >= 0.0f && b2 >= 0.0f && b1 + b2 <= 1.0f;
+}
+
+// ------------------------------------------------------------------ Philox4x32-10
+// Salmon et al., SC'11.  Counter = (pixel, sample, c2, c3), key = (seed, 0x6E755261).
+__device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+__device__ __forceinline__ float u01(uint32_t v) { return float(v >> 8) * (1.0f / 16777216.0f); }
+__device__ __forceinline__ float4 rng4(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t c2, uint32_t c3) {
+    uint4 r = philox(make_uint4(pixel, sample, c2, c3), seed, 0x6E755261u);
+    return make_float4(u01(r.x), u01(r.y), u01(r.z), u01(r.w));
+}
+
+}  // namespace nr

@@ -1,0 +1,415 @@
+// Wavefront ptnee integrator for gfx950 (RendererPTLS::trace / ::render,
+// src/renderer/rendererptls.cpp:13-236), made iterative:
+//
+//   raygen  -> camera rays for kb samples of every pixel (slot = s * npix + pixel)
+//   repeat: closest-hit trace (persistent) -> shade -> shadow any-hit trace (persistent, adds
+//           the NEE term of visible rays straight into the path's radiance)
+//   finalize-> per-pixel FP64 sums of max(0, L) and of its square (the only FP64 on the path)
+//
+// shade() handles one path vertex: miss -> environment, emitter -> Le * fac (MIS weight of the
+// previous vertex's NEE direction, rendererptls.cpp:33-36), back face -> stop, NEE with the
+// reference's MIS weight and shadow-ray test, Russian roulette, BSDF sampling.  New shadow
+// and extension rays are appended with one atomic per wave: ballot the lanes that emit,
+// prefix-count them, and let lane 0 reserve the wave's range.
+//
+// Random numbers come from Philox4x32-10 keyed by (seed, pixel, sample, vertex, slot), so any
+// split of pixels or samples over batches or devices gives the same per-sample results.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include "nr_material.h"
+#include "render.h"
+
+namespace nr {
+
+CamView make_camera(const float pos[3], const float euler[3], float fov, float aspect) {
+    CamView c;
+    float cs[3], sn[3];
+    for (int i = 0; i < 3; ++i) {
+        cs[i] = float(std::cos(double(euler[i] / 180) * 3.14159));
+        sn[i] = float(std::sin(double(euler[i] / 180) * 3.14159));
+    }
+    const float c1 = cs[0], s1 = sn[0], c2 = cs[1], s2 = sn[1], c3 = cs[2], s3 = sn[2];
+    const float m12 = c3 * s1 * s2 - c1 * s3, m13 = c2 * s1, m22 = c2 * c3, m23 = -s2;
+    const float m32 = s1 * s3 + c1 * c3 * s2, m33 = c1 * c2;
+    const float up[3] = {m12, m22, m32}, gaze[3] = {-m13, -m23, -m33};
+    for (int i = 0; i < 3; ++i) { c.pos[i] = pos[i]; c.up[i] = up[i]; c.gaze[i] = gaze[i]; }
+    float h[3] = {gaze[1] * up[2] - gaze[2] * up[1], gaze[2] * up[0] - gaze[0] * up[2],
+                  gaze[0] * up[1] - gaze[1] * up[0]};
+    const float nrm = std::sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]) + 1e-8f;
+    for (int i = 0; i < 3; ++i) c.hand[i] = h[i] / nrm;
+    c.fh = float(2 * std::tan(double(fov) * 3.14159 / 180 / 2));
+    c.fw = c.fh * aspect;
+    return c;
+}
+
+RenderTarget::~RenderTarget() {
+    if (pinned) (void)hipHostFree(pinned);
+}
+
+void RenderTarget::ensure(int w, int h, size_t c) {
+    if (w != W || h != H) {
+        sum.ensure(sizeof(double) * 3 * size_t(w) * h);
+        sum2.ensure(sizeof(double) * 3 * size_t(w) * h);
+        W = w; H = h;
+        spp_total = 0;
+    }
+    if (c > cap) {
+        rays[0].ensure(sizeof(Ray) * c);
+        rays[1].ensure(sizeof(Ray) * c);
+        hits.ensure(sizeof(HitRec) * c);
+        shadow.ensure(sizeof(Ray) * c);
+        contrib.ensure(sizeof(float4) * c);
+        state.ensure(sizeof(PathState) * c);
+        cap = c;
+    }
+    counters.ensure(sizeof(int) * 16);
+    stats.ensure(sizeof(TraceStats) * 2);
+    if (!pinned) check_hip(hipHostMalloc((void**)&pinned, sizeof(int) * 16, hipHostMallocDefault), "hipHostMalloc");
+}
+
+namespace {
+
+constexpr int BLK = 256;
+
+// Counter slots
+enum { C_RAY0 = 0, C_RAY1 = 1, C_SHADOW = 2, C_WORK0 = 3, C_WORK1 = 4, C_ERR = 5 };
+
+__global__ void k_zero_doubles(double* a, double* b, size_t n) {
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+        a[i] = 0.0;
+        b[i] = 0.0;
+    }
+}
+
+__global__ void k_clear_counters(int* c) {
+    if (threadIdx.x < 16) c[threadIdx.x] = 0;
+}
+
+__global__ void k_set_counters(int* c, int ray_slot, int ray_value, int clear_slot) {
+    if (threadIdx.x == 0) {
+        if (ray_slot >= 0) c[ray_slot] = ray_value;
+        if (clear_slot >= 0) c[clear_slot] = 0;
+        c[C_SHADOW] = 0;
+        c[C_WORK0] = 0;
+        c[C_WORK1] = 0;
+    }
+}
+
+// Camera rays: rendererptls.cpp:172 jitter (x + U, y + U), Camera::generateRay (camera.cpp:24-32).
+__global__ void k_raygen(CamView cam, int W, int npix, int P, uint32_t seed, uint32_t sample0, Ray* rays,
+                         PathState* state) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= P) return;
+    const int pixel = slot % npix;
+    const uint32_t sample = sample0 + uint32_t(slot / npix);
+    const int x = pixel % W, y = pixel / W;
+    const float4 u = rng4(seed, uint32_t(pixel), sample, 0u, 0u);
+    const float ix = float(x) + u.x, iy = float(y) + u.y;
+    const float cx = ix / float(W) * 2.0f - 1.0f;
+    const float cy = iy / float(npix / W) * 2.0f - 1.0f;
+    const F3 gaze = f3(cam.gaze[0], cam.gaze[1], cam.gaze[2]);
+    const F3 hand = f3(cam.hand[0], cam.hand[1], cam.hand[2]);
+    const F3 up = f3(cam.up[0], cam.up[1], cam.up[2]);
+    const F3 d = normalized(sub(add(gaze, mul(mul(mul(hand, cam.fw), cx), 0.5f)), mul(mul(mul(up, cam.fh), cy), 0.5f)));
+    rays[slot].o = make_float4(cam.pos[0], cam.pos[1], cam.pos[2], 0.0f);
+    rays[slot].d = make_float4(d.x, d.y, d.z, __int_as_float(slot));
+    state[slot].beta_fac = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    state[slot].L_depth = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0));
+}
+
+struct ShadeArgs {
+    SceneView sv;
+    int npix;
+    uint32_t seed, sample0;
+    float prr;
+    int mis, maxdepth, rrdepth;
+    const Ray* rin;
+    const HitRec* hits;
+    const int* n_in;
+    PathState* state;
+    Ray* rout;
+    int* n_out;
+    Ray* shadow;
+    float4* contrib;
+    int* n_shadow;
+};
+
+__device__ __forceinline__ int wave_append(bool emit, int* counter, int lane) {
+    const unsigned long long m = __ballot(emit);
+    const int total = __popcll(m);
+    int base = 0;
+    if (lane == 0 && total) base = atomicAdd(counter, total);
+    base = __shfl(base, 0, 64);
+    return base + __popcll(m & ((1ull << lane) - 1ull));
+}
+
+__global__ __launch_bounds__(BLK) void k_shade(ShadeArgs a) {
+    const SceneView& sv = a.sv;
+    const int lane = threadIdx.x & 63;
+    const int n = *a.n_in;
+    const int stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
+        const int j = base + lane;
+        const bool active = j < n;
+        bool emit_sh = false, emit_next = false;
+        Ray sh_ray, next_ray;
+        float4 sh_c = make_float4(0, 0, 0, 0);
+        if (active) {
+            const Ray r = a.rin[j];
+            const HitRec h = a.hits[j];
+            const int slot = __float_as_int(r.d.w);
+            PathState st = a.state[slot];
+            F3 beta = f3(st.beta_fac.x, st.beta_fac.y, st.beta_fac.z);
+            const float fac = st.beta_fac.w;
+            F3 L = f3(st.L_depth.x, st.L_depth.y, st.L_depth.z);
+            const int depth = __float_as_int(st.L_depth.w);
+            const int pixel = slot % a.npix;
+            const uint32_t sample = a.sample0 + uint32_t(slot / a.npix);
+            const F3 o = f3(r.o.x, r.o.y, r.o.z), d = f3(r.d.x, r.d.y, r.d.z);
+            if (h.prim < 0) {
+                L = add(L, mulv(beta, env_radiance(sv, d)));
+            } else {
+                const TriShade ts = sv.shade[h.prim];
+                const float b1 = h.b1, b2 = h.b2, w0 = 1.0f - b1 - b2;
+                const F3 n0 = f3(ts.s0.x, ts.s0.y, ts.s0.z), n1 = f3(ts.s0.w, ts.s1.x, ts.s1.y);
+                const F3 n2 = f3(ts.s1.z, ts.s1.w, ts.s2.x);
+                const F3 nrm = normalized(add(add(mul(n0, w0), mul(n1, b1)), mul(n2, b2)));  // triangle.cpp:35-38
+                const Mat m = load_mat(sv, __float_as_int(ts.s3.w));
+                if (m.type == MAT_LIGHT) {
+                    L = add(L, mulv(beta, mul(m.kd, fac)));  // rendererptls.cpp:33-36
+                } else if (!(dot(nrm, d) > 0.0f && !mat_transmissive(m))) {
+                    const float tu = w0 * ts.s2.y + b1 * ts.s2.w + b2 * ts.s3.y;  // triangle.cpp:40-43
+                    const float tv = w0 * ts.s2.z + b1 * ts.s3.x + b2 * ts.s3.z;
+                    const F3 hp = add(o, mul(d, h.t));
+                    const F3 wo = neg(d);
+                    const float4 ua = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 0u);
+                    float fac_next = 1.0f;
+                    if (mat_needs_nee(m) && sv.n_lights > 0) {  // rendererptls.cpp:50-77
+                        const LightRec lr = sv.lights[pick_light(sv, ua.x)];
+                        float r1 = ua.y, r2 = ua.z;
+                        if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
+                        const float lw = 1.0f - r1 - r2;
+                        const F3 lp = add(add(mul(f3(lr.p0.x, lr.p0.y, lr.p0.z), lw), mul(f3(lr.p1.x, lr.p1.y, lr.p1.z), r1)),
+                                          mul(f3(lr.p2.x, lr.p2.y, lr.p2.z), r2));
+                        const F3 ln = normalized(add(add(mul(f3(lr.n0.x, lr.n0.y, lr.n0.z), lw),
+                                                         mul(f3(lr.n1.x, lr.n1.y, lr.n1.z), r1)),
+                                                     mul(f3(lr.n2.x, lr.n2.y, lr.n2.z), r2)));
+                        const F3 lv = sub(lp, hp);
+                        const F3 wl = normalized(lv);
+                        const float lpdf = sv.light_p;
+                        const float lcos = dot(ln, neg(wl));
+                        const float lop = lpdf * dot(lv, lv) / ((1e-6f < lcos) ? lcos : 1e-6f);
+                        const float bop = bsdf_pdf(m, wo, nrm, wl);
+                        float mis = lop / (lop + bop);
+                        if (a.mis == 0) mis = 0.5f;
+                        fac_next = 1.0f - mis;
+                        const F3 f = bsdf_eval(sv, m, wo, nrm, wl, tu, tv);
+                        const float ncos = dot(nrm, normalized(sub(lp, hp)));
+                        const F3 Ll = mul(divs(f3(lr.ke.x, lr.ke.y, lr.ke.z), dot(lv, lv)), (0.0f < lcos ? lcos : 0.0f) / lpdf);
+                        const F3 c = mul(mulv(mulv(beta, Ll), f), (0.0f < ncos ? ncos : 0.0f) * mis);
+                        if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
+                            emit_sh = true;
+                            const F3 so = add(hp, mul(wl, 1e-3f));
+                            sh_ray.o = make_float4(so.x, so.y, so.z, norm(lv));
+                            sh_ray.d = make_float4(wl.x, wl.y, wl.z, __int_as_float(slot));
+                            sh_c = make_float4(c.x, c.y, c.z, 0.0f);
+                        }
+                    }
+                    bool survive = true;
+                    float prr = 1.0f;
+                    if (depth >= a.rrdepth) {  // rendererptls.cpp:80-83
+                        survive = !(ua.w > a.prr);
+                        prr = a.prr;
+                    }
+                    if (survive && (a.maxdepth <= 0 || depth + 1 < a.maxdepth)) {  // rendererptls.cpp:85-91
+                        const float4 ub = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 1u);
+                        const F3 wi = bsdf_sample(m, ub.x, ub.y, ub.z, wo, nrm);
+                        const float pdf = bsdf_pdf(m, wo, nrm, wi);
+                        const F3 f = bsdf_eval(sv, m, wo, nrm, wi, tu, tv);
+                        const float c = fabsf(dot(wi, nrm));
+                        const F3 nb = f3(beta.x * c * f.x / pdf / prr, beta.y * c * f.y / pdf / prr,
+                                         beta.z * c * f.z / pdf / prr);
+                        if (nb.x != 0.0f || nb.y != 0.0f || nb.z != 0.0f) {
+                            emit_next = true;
+                            const F3 no = add(hp, mul(wi, 1e-3f));
+                            next_ray.o = make_float4(no.x, no.y, no.z, 0.0f);
+                            next_ray.d = make_float4(wi.x, wi.y, wi.z, __int_as_float(slot));
+                            st.beta_fac = make_float4(nb.x, nb.y, nb.z, fac_next);
+                        }
+                    }
+                }
+            }
+            st.L_depth = make_float4(L.x, L.y, L.z, __int_as_float(depth + 1));
+            a.state[slot] = st;
+        }
+        const int ps = wave_append(emit_sh, a.n_shadow, lane);
+        if (emit_sh) {
+            a.shadow[ps] = sh_ray;
+            a.contrib[ps] = sh_c;
+        }
+        const int pn = wave_append(emit_next, a.n_out, lane);
+        if (emit_next) a.rout[pn] = next_ray;
+    }
+}
+
+// Per-pixel reduction of the batch's kb samples: sums of max(0, L) and its square in FP64
+// (rendererptls.cpp:173-175: max(0, trace(...)) per component, which also maps NaN to 0).
+__global__ void k_finalize(const PathState* __restrict__ state, int npix, int kb, double* __restrict__ sum,
+                           double* __restrict__ sum2) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    double s[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
+    for (int k = 0; k < kb; ++k) {
+        const float4 L = state[size_t(k) * npix + p].L_depth;
+        const float v[3] = {0.0f < L.x ? L.x : 0.0f, 0.0f < L.y ? L.y : 0.0f, 0.0f < L.z ? L.z : 0.0f};
+        for (int c = 0; c < 3; ++c) {
+            s[c] += double(v[c]);
+            s2[c] += double(v[c]) * double(v[c]);
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        sum[3 * size_t(p) + c] += s[c];
+        sum2[3 * size_t(p) + c] += s2[c];
+    }
+}
+
+// Mean, then the reference's display transform (rendererptls.cpp:175-181): pow(1/2.2), *255,
+// clamp to [0, 255], truncate.
+__global__ void k_tonemap(const double* __restrict__ sum, size_t n3, double inv_spp, float* __restrict__ rgb,
+                          uint8_t* __restrict__ rgb8) {
+    const size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+    if (i >= n3) return;
+    const float m = float(sum[i] * inv_spp);
+    rgb[i] = m;
+    float g = powf(m, float(1.0 / 2.2)) * 255.0f;
+    g = (0.0f < g) ? g : 0.0f;
+    g = (g < 255.0f) ? g : 255.0f;
+    rgb8[i] = uint8_t(int(g));
+}
+
+}  // namespace
+
+int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressFn cb, void* user,
+                 volatile int* cancel, RenderStats* st, bool trace_stats, TraceStats* tstats, KernelTimes* kt) {
+    if (!sc.built) throw std::runtime_error("scene not built (call nr_bvh_build_sah)");
+    if (p.W <= 0 || p.H <= 0 || p.spp <= 0) throw std::runtime_error("bad image size or spp");
+    const int npix = p.W * p.H;
+    int kb = p.batch_spp > 0 ? p.batch_spp : std::max(1, int((size_t(4) << 20) / size_t(npix)));
+    kb = std::min(kb, p.spp);
+    if (!sc.target) sc.target = new RenderTarget();
+    RenderTarget& t = *sc.target;
+    t.ensure(p.W, p.H, size_t(kb) * npix);
+    hipStream_t s = sc.stream;
+    int* cnt = t.counters.as<int>();
+    TraceStats* ts = trace_stats ? t.stats.as<TraceStats>() : nullptr;
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(64), 0, s, cnt);
+    if (ts) check_hip(hipMemsetAsync(ts, 0, 2 * sizeof(TraceStats), s), "memset stats");
+    if (!p.accumulate) {
+        hipLaunchKernelGGL(k_zero_doubles, dim3(1024), dim3(BLK), 0, s, t.sum.as<double>(), t.sum2.as<double>(),
+                           size_t(npix) * 3);
+        t.spp_total = 0;
+    }
+    const int grid_shade = device_cu_count() * 8;
+    auto t0 = std::chrono::steady_clock::now();
+    RenderStats rs;
+    int status = 0;
+    std::vector<hipEvent_t> ev_c, ev_s;  // start/stop pairs per traversal launch
+    auto mark = [&](std::vector<hipEvent_t>& v) {
+        hipEvent_t e;
+        check_hip(hipEventCreate(&e), "hipEventCreate");
+        check_hip(hipEventRecord(e, s), "hipEventRecord");
+        v.push_back(e);
+    };
+    for (int b = 0; b < p.spp; b += kb) {
+        if (cancel && *cancel) { status = 1; break; }
+        const int k = std::min(kb, p.spp - b);
+        const int P = k * npix;
+        hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, int(C_RAY0), P, -1);
+        hipLaunchKernelGGL(k_raygen, dim3((P + BLK - 1) / BLK), dim3(BLK), 0, s, cam, p.W, npix, P, p.seed,
+                           uint32_t(p.spp_offset + b), t.rays[0].as<Ray>(), t.state.as<PathState>());
+        for (int it = 0;; ++it) {
+            const int cur = it & 1, nxt = cur ^ 1;
+            hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, -1, 0, int(C_RAY0 + nxt));
+            if (kt) mark(ev_c);
+            launch_closest(sc.view, t.rays[cur].as<Ray>(), cnt + C_RAY0 + cur, t.hits.as<HitRec>(), cnt + C_WORK0,
+                           cnt + C_ERR, ts, s);
+            if (kt) mark(ev_c);
+            ShadeArgs a{sc.view, npix, p.seed, uint32_t(p.spp_offset + b), p.prr, p.mis, p.maxdepth, p.rrdepth,
+                        t.rays[cur].as<Ray>(), t.hits.as<HitRec>(), cnt + C_RAY0 + cur, t.state.as<PathState>(),
+                        t.rays[nxt].as<Ray>(), cnt + C_RAY0 + nxt, t.shadow.as<Ray>(), t.contrib.as<float4>(),
+                        cnt + C_SHADOW};
+            hipLaunchKernelGGL(k_shade, dim3(grid_shade), dim3(BLK), 0, s, a);
+            if (kt) mark(ev_s);
+            launch_shadow(sc.view, t.shadow.as<Ray>(), cnt + C_SHADOW, nullptr, nullptr, t.contrib.as<float4>(),
+                          t.state.as<PathState>(), cnt + C_WORK1, cnt + C_ERR, ts ? ts + 1 : nullptr, s);
+            if (kt) mark(ev_s);
+            rs.iterations++;
+            if (p.maxdepth > 0) {
+                if (it + 1 >= p.maxdepth) break;
+            } else {
+                check_hip(hipMemcpyAsync(t.pinned, cnt + C_RAY0 + nxt, sizeof(int), hipMemcpyDeviceToHost, s), "cnt");
+                check_hip(hipStreamSynchronize(s), "sync");
+                if (*t.pinned == 0) break;
+                if (cancel && *cancel) { status = 1; break; }
+            }
+        }
+        if (status) break;
+        hipLaunchKernelGGL(k_finalize, dim3((npix + BLK - 1) / BLK), dim3(BLK), 0, s, t.state.as<PathState>(), npix, k,
+                           t.sum.as<double>(), t.sum2.as<double>());
+        check_hip(hipGetLastError(), "ptnee launch");
+        t.spp_total += k;
+        rs.samples += (long long)P;
+        if (cb) {
+            check_hip(hipStreamSynchronize(s), "sync");
+            cb(float(b + k) / float(p.spp), user);
+        }
+    }
+    check_hip(hipMemcpyAsync(t.pinned, cnt, sizeof(int) * 8, hipMemcpyDeviceToHost, s), "counters");
+    check_hip(hipStreamSynchronize(s), "render sync");
+    if (t.pinned[C_ERR]) throw std::runtime_error("traversal stack overflow");
+    rs.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (kt) {
+        *kt = KernelTimes{};
+        for (size_t i = 0; i + 1 < ev_c.size(); i += 2) {
+            float ms = 0;
+            check_hip(hipEventElapsedTime(&ms, ev_c[i], ev_c[i + 1]), "elapsed");
+            kt->closest_ms += ms;
+            kt->closest_launches++;
+        }
+        for (size_t i = 0; i + 1 < ev_s.size(); i += 2) {
+            float ms = 0;
+            check_hip(hipEventElapsedTime(&ms, ev_s[i], ev_s[i + 1]), "elapsed");
+            kt->shadow_ms += ms;
+            kt->shadow_launches++;
+        }
+    }
+    for (hipEvent_t e : ev_c) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev_s) (void)hipEventDestroy(e);
+    if (ts && tstats) check_hip(hipMemcpy(tstats, ts, 2 * sizeof(TraceStats), hipMemcpyDeviceToHost), "stats");
+    if (st) *st = rs;
+    return status;
+}
+
+void render_fetch(Scene& sc, float* rgb, double* sum, double* sum2, uint8_t* rgb8) {
+    if (!sc.target) throw std::runtime_error("nothing rendered");
+    RenderTarget& t = *sc.target;
+    const size_t n3 = size_t(t.W) * t.H * 3;
+    hipStream_t s = sc.stream;
+    if (sum) check_hip(hipMemcpyAsync(sum, t.sum.ptr, n3 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+    if (sum2) check_hip(hipMemcpyAsync(sum2, t.sum2.ptr, n3 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+    if (rgb || rgb8) {
+        DevBuf drgb, d8;
+        drgb.ensure(n3 * sizeof(float));
+        d8.ensure(n3);
+        hipLaunchKernelGGL(k_tonemap, dim3((n3 + BLK - 1) / BLK), dim3(BLK), 0, s, t.sum.as<double>(), n3,
+                           t.spp_total ? 1.0 / double(t.spp_total) : 0.0, drgb.as<float>(), d8.as<uint8_t>());
+        if (rgb) check_hip(hipMemcpyAsync(rgb, drgb.ptr, n3 * sizeof(float), hipMemcpyDeviceToHost, s), "D2H");
+        if (rgb8) check_hip(hipMemcpyAsync(rgb8, d8.ptr, n3, hipMemcpyDeviceToHost, s), "D2H");
+        check_hip(hipStreamSynchronize(s), "fetch sync");
+    }
+    check_hip(hipStreamSynchronize(s), "fetch sync");
+}
+
+}  // namespace nr

@@ -1,0 +1,61 @@
+// Wavefront ptnee renderer: buffers and host orchestration.
+#pragma once
+#include "scene.h"
+#include "trace.h"
+
+namespace nr {
+
+// Camera::generateRay inputs (camera.cpp:24-32), precomputed once per render on the host.
+struct CamView {
+    float pos[3], gaze[3], up[3], hand[3];
+    float fw, fh;  // film width / height
+};
+// Camera(fov, asp, pos, yaw, pitch, roll) with fromEuler (camera.cpp:88-105).
+CamView make_camera(const float pos[3], const float euler[3], float fov, float aspect);
+
+struct RenderParams {
+    int W = 1, H = 1, spp = 1, spp_offset = 0;
+    uint32_t seed = 0;
+    float prr = 0.8f;
+    int mis = 1;
+    int maxdepth = 0;   // 0: unbounded (reference)
+    int rrdepth = 0;    // Russian roulette from this vertex index (reference: 0)
+    int batch_spp = 0;  // samples per pixel per wavefront batch (0: auto)
+    int accumulate = 0; // keep previous sums instead of clearing them
+};
+
+struct RenderStats {
+    double ms = 0;
+    long long samples = 0, extension_rays = 0, shadow_rays = 0;
+    int iterations = 0;
+};
+
+// HIP-event timing of the traversal launches of one render (on the scene's stream).
+struct KernelTimes {
+    double closest_ms = 0, shadow_ms = 0;
+    int closest_launches = 0, shadow_launches = 0;
+};
+
+struct RenderTarget {
+    int W = 0, H = 0;
+    size_t cap = 0;
+    DevBuf sum, sum2;                  // double, W*H*3
+    DevBuf rays[2], hits, shadow, contrib, state, occ;
+    DevBuf counters;                   // int[16]
+    DevBuf stats;                      // TraceStats[2]
+    int* pinned = nullptr;             // host-visible counter readback
+    long long spp_total = 0;
+    ~RenderTarget();
+    void ensure(int W, int H, size_t cap);
+};
+
+typedef void (*ProgressFn)(float, void*);
+
+// Renders p.spp samples per pixel into the scene's render target.  Returns 0, or 1 if cancelled.
+int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressFn cb, void* user,
+                 volatile int* cancel, RenderStats* st, bool trace_stats = false, TraceStats* tstats = nullptr,
+                 KernelTimes* kt = nullptr);
+// Copies the accumulated result: linear mean (float W*H*3), FP64 sums, gamma-2.2 RGB8.
+void render_fetch(Scene& sc, float* rgb, double* sum, double* sum2, uint8_t* rgb8);
+
+}  // namespace nr

@@ -1,0 +1,199 @@
+// Host scene assembly and upload.
+#include "scene.h"
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include "render.h"
+
+namespace nr {
+
+void check_hip(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+DevBuf::~DevBuf() {
+    if (ptr) (void)hipFree(ptr);
+}
+void DevBuf::ensure(size_t n) {
+    if (n <= bytes && ptr) return;
+    if (ptr) check_hip(hipFree(ptr), "hipFree");
+    ptr = nullptr;
+    bytes = 0;
+    check_hip(hipMalloc(&ptr, n ? n : 64), "hipMalloc");
+    bytes = n;
+}
+void DevBuf::upload(const void* src, size_t n, hipStream_t s) {
+    ensure(n);
+    if (n) check_hip(hipMemcpyAsync(ptr, src, n, hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+}
+
+namespace {
+struct V {
+    float x, y, z;
+};
+inline V vsub(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V vcross(V a, V b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline float vnorm2(V a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+inline V vnormalized(V a) {  // vec3::normalized, vec3.cpp:70-73
+    float d = std::sqrt(vnorm2(a)) + 1e-8f;
+    return {a.x / d, a.y / d, a.z / d};
+}
+inline V at(const float* q, int k) { return {q[3 * k], q[3 * k + 1], q[3 * k + 2]}; }
+}  // namespace
+
+Scene::Scene(const float* P, const float* N, const float* UV, const int32_t* M, uint32_t ntri, const float* MATS,
+             uint32_t nmat)
+    : p(P, P + 9 * size_t(ntri)), n(9 * size_t(ntri), 0.0f), uv(6 * size_t(ntri), 0.0f), mat(M, M + ntri),
+      mats(MATS, MATS + 8 * size_t(nmat)) {
+    if (N) std::memcpy(n.data(), N, sizeof(float) * 9 * size_t(ntri));
+    if (UV) std::memcpy(uv.data(), UV, sizeof(float) * 6 * size_t(ntri));
+    for (uint32_t i = 0; i < ntri; ++i) {
+        if (M[i] < 0 || uint32_t(M[i]) >= nmat) throw std::runtime_error("material id out of range");
+        // Triangle::Triangle: flat normal when |n0|^2 < 1e-4 (triangle.cpp:14-25)
+        float* nn = &n[9 * size_t(i)];
+        if (double(vnorm2(at(nn, 0))) < 1e-4) {
+            const float* q = &p[9 * size_t(i)];
+            V f = vnormalized(vcross(vsub(at(q, 1), at(q, 0)), vsub(at(q, 2), at(q, 0))));
+            for (int k = 0; k < 3; ++k) { nn[3 * k] = f.x; nn[3 * k + 1] = f.y; nn[3 * k + 2] = f.z; }
+        }
+    }
+    check_hip(hipGetDevice(&device), "hipGetDevice");
+    check_hip(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+}
+
+Scene::~Scene() {
+    delete target;
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int Scene::add_texture(int id, int w, int h, int fmt, const void* data) {
+    if (id < 0 || w <= 0 || h <= 0) throw std::runtime_error("bad texture");
+    if (int(textures.size()) <= id) textures.resize(id + 1);
+    Texture& t = textures[id];
+    t.w = w; t.h = h; t.fmt = fmt;
+    size_t bytes = size_t(w) * h * (fmt == 0 ? 4 : 12);
+    t.bytes.assign(static_cast<const uint8_t*>(data), static_cast<const uint8_t*>(data) + bytes);
+    built = false;
+    return id;
+}
+
+void Scene::build(int nthreads) {
+    const int N = n_tris();
+    // every texture a material or the environment names must exist (kernels index without checks)
+    for (size_t j = 0; j < mats.size() / 8; ++j) {
+        const int tex = int(mats[8 * j + 6]);
+        if (tex >= 0 && (tex >= int(textures.size()) || textures[tex].w <= 0))
+            throw std::runtime_error("material " + std::to_string(j) + " names missing texture " + std::to_string(tex));
+        const int type = int(mats[8 * j]);
+        if (type < MAT_LAMBERT || type > MAT_GGX) throw std::runtime_error("unknown material type");
+    }
+    if (env_tex >= 0 && (env_tex >= int(textures.size()) || textures[env_tex].w <= 0))
+        throw std::runtime_error("environment map missing");
+    bvh = build_sah(p.data(), N, nthreads);
+
+    // LightSampler::setPrimitives (lightsampler.cpp:4-39): emissive triangles in loader order,
+    // weight = area() = |e1 x e2| (twice the area), CDF from 1e-6.
+    lights.clear();
+    light_cdf.clear();
+    std::vector<float> w;
+    float sum = 0.0f;
+    for (int i = 0; i < N; ++i) {
+        const float* m = &mats[8 * size_t(mat[i])];
+        if (int(m[0]) != MAT_LIGHT) continue;
+        const float* q = &p[9 * size_t(i)];
+        const float* nn = &n[9 * size_t(i)];
+        float a = std::sqrt(vnorm2(vcross(vsub(at(q, 1), at(q, 0)), vsub(at(q, 2), at(q, 0)))));
+        w.push_back(a);
+        sum += a;
+        LightRec L;
+        L.p0 = make_float4(q[0], q[1], q[2], 0); L.p1 = make_float4(q[3], q[4], q[5], 0);
+        L.p2 = make_float4(q[6], q[7], q[8], 0);
+        L.n0 = make_float4(nn[0], nn[1], nn[2], 0); L.n1 = make_float4(nn[3], nn[4], nn[5], 0);
+        L.n2 = make_float4(nn[6], nn[7], nn[8], 0);
+        L.ke = make_float4(m[1], m[2], m[3], 0);
+        lights.push_back(L);
+    }
+    sum += 1e-8f;
+    light_sum = sum;
+    float acc = 1e-6f;
+    for (float x : w) {
+        acc += x / sum;
+        light_cdf.push_back(acc);
+    }
+
+    // Geometry and shading records in DFS leaf order.
+    std::vector<TriGeom> g(N);
+    std::vector<TriShade> s(N);
+    for (int k = 0; k < N; ++k) {
+        int i = bvh.order[k];
+        const float* q = &p[9 * size_t(i)];
+        float e1[3] = {q[3] - q[0], q[4] - q[1], q[5] - q[2]};
+        float e2[3] = {q[6] - q[0], q[7] - q[1], q[8] - q[2]};
+        int32_t id = i;
+        float idf;
+        std::memcpy(&idf, &id, 4);
+        g[k].g0 = make_float4(q[0], q[1], q[2], e1[0]);
+        g[k].g1 = make_float4(e1[1], e1[2], e2[0], e2[1]);
+        g[k].g2 = make_float4(e2[2], idf, 0, 0);
+        const float* nn = &n[9 * size_t(i)];
+        const float* t = &uv[6 * size_t(i)];
+        int32_t m = mat[i];
+        float mf;
+        std::memcpy(&mf, &m, 4);
+        s[k].s0 = make_float4(nn[0], nn[1], nn[2], nn[3]);
+        s[k].s1 = make_float4(nn[4], nn[5], nn[6], nn[7]);
+        s[k].s2 = make_float4(nn[8], t[0], t[1], t[2]);
+        s[k].s3 = make_float4(t[3], t[4], t[5], mf);
+    }
+    std::vector<MatRec> mr(mats.size() / 8);
+    for (size_t j = 0; j < mr.size(); ++j) {
+        const float* m = &mats[8 * j];
+        mr[j].a = make_float4(m[0], m[1], m[2], m[3]);
+        mr[j].b = make_float4(m[4], m[5], m[6], 0);
+    }
+    // texture pool
+    std::vector<TexDesc> td(textures.size());
+    size_t pool = 0;
+    for (size_t j = 0; j < textures.size(); ++j) {
+        td[j] = TexDesc{textures[j].w, textures[j].h, textures[j].fmt, 0, pool};
+        pool += (textures[j].bytes.size() + 255) & ~size_t(255);
+    }
+    std::vector<uint8_t> poolh(pool);
+    for (size_t j = 0; j < textures.size(); ++j)
+        std::memcpy(poolh.data() + td[j].offset, textures[j].bytes.data(), textures[j].bytes.size());
+    // texture.cpp:18-20 decode: pow(c / 255, 2.2f) in double, stored as float
+    float lut[256];
+    for (int c = 0; c < 256; ++c) lut[c] = float(std::pow(double(c) / 255.0, double(2.2f)));
+
+    d_nodes.upload(bvh.pairs.data(), bvh.pairs.size() * sizeof(NodePair), stream);
+    d_tris.upload(g.data(), g.size() * sizeof(TriGeom), stream);
+    d_shade.upload(s.data(), s.size() * sizeof(TriShade), stream);
+    d_mats.upload(mr.data(), mr.size() * sizeof(MatRec), stream);
+    d_lights.upload(lights.data(), lights.size() * sizeof(LightRec), stream);
+    d_cdf.upload(light_cdf.data(), light_cdf.size() * sizeof(float), stream);
+    d_tex.upload(td.data(), td.size() * sizeof(TexDesc), stream);
+    d_texpool.upload(poolh.data(), poolh.size(), stream);
+    d_lut.upload(lut, sizeof(lut), stream);
+    check_hip(hipStreamSynchronize(stream), "upload sync");
+
+    view = SceneView{};
+    view.nodes = d_nodes.as<NodePair>();
+    view.tris = d_tris.as<TriGeom>();
+    view.shade = d_shade.as<TriShade>();
+    view.mats = d_mats.as<MatRec>();
+    view.lights = d_lights.as<LightRec>();
+    view.light_cdf = d_cdf.as<float>();
+    view.tex = d_tex.as<TexDesc>();
+    view.texpool = d_texpool.as<uint8_t>();
+    view.lut = d_lut.as<float>();
+    for (int j = 0; j < 3; ++j) { view.root_min[j] = bvh.root_min[j]; view.root_max[j] = bvh.root_max[j]; }
+    view.root_ref = N ? bvh.root_ref : make_leaf(0, 0);
+    view.n_lights = int(lights.size());
+    view.light_p = float(1.0 / (double(light_sum) + 1e-8));  // LightSampler::p, lightsampler.cpp:100-103
+    view.env_tex = env_tex;
+    view.env_gain = env_gain;
+    view.stack_cap = bvh.depth + 2;
+    built = true;
+}
+
+}  // namespace nr

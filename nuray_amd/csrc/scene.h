@@ -1,0 +1,67 @@
+// Host scene: owns the loader-order arrays, the SAH BVH and the device copies.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "bvh_build.h"
+#include "nr_device.h"
+
+namespace nr {
+
+void check_hip(hipError_t e, const char* what);  // throws std::runtime_error on failure
+
+// A device allocation that frees itself.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf();
+    void ensure(size_t n);          // grow (contents not kept)
+    void upload(const void* src, size_t n, hipStream_t s);
+    template <class T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+struct Texture {
+    int w = 0, h = 0, fmt = 0;  // 0 RGBA8, 1 RGB32F
+    std::vector<uint8_t> bytes;
+};
+
+struct RenderTarget;  // defined in render.h
+
+class Scene {
+public:
+    // loader-order inputs (triangle.h:8-36 fields)
+    std::vector<float> p, n, uv;  // N*9, N*9, N*6
+    std::vector<int32_t> mat;
+    std::vector<float> mats;      // M*8
+    std::vector<Texture> textures;
+    int env_tex = -1;
+    float env_gain = 1.0f;
+
+    // derived host data
+    BvhBuild bvh;
+    bool built = false;
+    std::vector<LightRec> lights;
+    std::vector<float> light_cdf;
+    float light_sum = 0.0f;
+
+    // device mirror
+    DevBuf d_nodes, d_tris, d_shade, d_mats, d_lights, d_cdf, d_tex, d_texpool, d_lut;
+    SceneView view{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    RenderTarget* target = nullptr;  // lazily created render buffers
+
+    Scene(const float* p, const float* n, const float* uv, const int32_t* mat, uint32_t ntri,
+          const float* mats, uint32_t nmat);
+    ~Scene();
+    int n_tris() const { return int(mat.size()); }
+    int add_texture(int id, int w, int h, int fmt, const void* data);
+    // Host SAH build (reference rule), light sampler, upload of every device array.
+    void build(int nthreads);
+};
+
+}  // namespace nr

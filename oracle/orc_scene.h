@@ -1,0 +1,294 @@
+// ORACLE -- test infrastructure only (see orc_common.h).
+//
+// Scene records, textures, materials and the area-light sampler of the reference, restated.
+#pragma once
+#include "orc_common.h"
+
+namespace orc {
+
+enum MatType { MAT_LAMBERT = 0, MAT_LIGHT = 1, MAT_GGXREFL = 2, MAT_GGX = 3 };
+
+// One material row as passed through the C API: type, rgb (Kd / F0 / Ke), alpha, ior, texture.
+struct Material {
+    int type = MAT_LAMBERT;
+    float kd[3] = {0, 0, 0};
+    float alpha = 0.5f;
+    float ior = 1.0f;
+    int tex = -1;
+};
+
+// src/scene/triangle.h:8-36, triangle.cpp:5-25.  A flat normal normalize((p1-p0)x(p2-p0))
+// replaces all three vertex normals when |n0|^2 < 1e-4 (:14-17).
+struct Tri {
+    V3f p[3], t[3], n[3];
+    int mat = 0;
+    void fix_normal() {
+        if (double(n[0].norm2()) < 1e-4) {
+            n[0] = (p[1] - p[0]).cross(p[2] - p[0]).normalized();
+            n[1] = n[0];
+            n[2] = n[0];
+        }
+    }
+    V3f pmin() const { return vmin(p[0], vmin(p[1], p[2])); }  // triangle.cpp:68-71
+    V3f pmax() const { return vmax(p[0], vmax(p[1], p[2])); }  // triangle.cpp:73-76
+    // triangle.cpp:57-60: |e1 x e2|, twice the geometric area.
+    float area() const { return (p[1] - p[0]).cross(p[2] - p[0]).norm(); }
+    template <class T>
+    V3<T> normal(T b1, T b2) const {  // triangle.cpp:35-38
+        return ((T(1) - b1 - b2) * V3<T>(n[0]) + b1 * V3<T>(n[1]) + b2 * V3<T>(n[2])).normalized();
+    }
+    template <class T>
+    V3<T> texc(T b1, T b2) const {  // triangle.cpp:40-43
+        return (T(1) - b1 - b2) * V3<T>(t[0]) + b1 * V3<T>(t[1]) + b2 * V3<T>(t[2]);
+    }
+};
+
+// src/scene/texture.cpp:54-72: texels outside [0,W)x[0,H) read as black; bilinear weights between
+// the floor texel and floor+1 (no wrap of the +1 neighbour); pixelUV wraps u and 1-v into [0,1).
+struct Texture {
+    int w = 0, h = 0;
+    std::vector<float> rgb;  // linear RGB, row-major, y down
+    V3f px(int x, int y) const {
+        if (x < 0 || y < 0 || x >= w || y >= h) return V3f(0.0f);
+        const float* q = &rgb[3 * (size_t(y) * w + x)];
+        return {q[0], q[1], q[2]};
+    }
+    V3f bilinear(float x, float y) const {
+        int x0 = int(std::floor(x)), y0 = int(std::floor(y));
+        int x1 = x0 + 1, y1 = y0 + 1;
+        return px(x1, y1) * (x - x0) * (y - y0) + px(x1, y0) * (x - x0) * (y1 - y) +
+               px(x0, y1) * (x1 - x) * (y - y0) + px(x0, y0) * (x1 - x) * (y1 - y);
+    }
+    V3f uv(float u, float v) const {
+        float fu = std::fmod(std::fmod(u, 1.0f) + 1.0f, 1.0f);
+        float fv = std::fmod(std::fmod(1.0f - v, 1.0f) + 1.0f, 1.0f);
+        return bilinear(fu * w, fv * h);
+    }
+};
+// texture.cpp:18-20: QColor::redF() is c/255 in double, raised to the float literal 2.2f.
+inline float srgb8_decode(int c) { return float(std::pow(double(c) / 255.0, double(2.2f))); }
+
+struct Light {  // one emissive triangle (lightsampler.cpp:13-22)
+    int tri;
+};
+
+struct Scene {
+    std::vector<Tri> tris;
+    std::vector<Material> mats;
+    std::vector<Texture> textures;
+    Texture env;
+    float env_gain = 1.0f;
+    bool has_env = false;
+    // light sampler state (lightsampler.cpp:4-39)
+    std::vector<int> light_tris;
+    std::vector<float> light_cdf;
+    float light_sum = 0.0f;
+
+    bool is_emitter(int tri) const { return mats[tris[tri].mat].type == MAT_LIGHT; }
+
+    // LightSampler::setPrimitives, lightsampler.cpp:4-39.
+    void build_light_sampler() {
+        light_tris.clear();
+        light_cdf.clear();
+        std::vector<float> pdf;
+        float sum = 0;
+        for (size_t i = 0; i < tris.size(); ++i) {
+            if (is_emitter(int(i))) {
+                float w = tris[i].area();
+                pdf.push_back(w);
+                sum += w;
+                light_tris.push_back(int(i));
+            }
+        }
+        sum += 1e-8f;
+        for (auto& x : pdf) x /= sum;
+        light_sum = sum;
+        float acc = 1e-6f;
+        for (float x : pdf) {
+            acc += x;
+            light_cdf.push_back(acc);
+        }
+    }
+    // LightSampler::sampleLight, lightsampler.cpp:49-58: lower_bound over the CDF.
+    int pick_light(float u) const {
+        if (light_tris.empty()) return -1;
+        size_t id = std::lower_bound(light_cdf.begin(), light_cdf.end(), u) - light_cdf.begin();
+        if (id >= light_tris.size()) id = light_tris.size() - 1;
+        return light_tris[id];
+    }
+    // LightSampler::p, lightsampler.cpp:100-103 (double division, stored as float).
+    float light_p() const { return float(1.0 / (double(light_sum) + 1e-8)); }
+};
+
+// ---------------------------------------------------------------- materials
+// Shading frame shared by MatLambert (matlambert.cpp:11-13) and MatGGXRefl (matggxrefl.cpp:20-22).
+// The reference's `abs` is read as the floating-point absolute value (a g++ build
+// without Qt headers in the material TUs binds int abs; the published renders show float abs).
+template <class T>
+inline void frame(const V3<T>& n, V3<T>& ax1, V3<T>& ax2) {
+    V3<T> ax0 = double(std::fabs(n.x)) < 0.8 ? V3<T>(1, 0, 0) : V3<T>(0, 1, 0);
+    ax1 = ax0.cross(n).normalized();
+    ax2 = n.cross(ax1).normalized();
+}
+
+template <class T>
+struct MatEval {
+    const Scene* sc;
+    const Material* m;
+
+    V3<T> albedo(const V3<T>& uv) const {
+        if (m->tex >= 0) return V3<T>(sc->textures[m->tex].uv(float(uv.x), float(uv.y)));
+        return V3<T>(T(m->kd[0]), T(m->kd[1]), T(m->kd[2]));
+    }
+    bool needs_nee() const { return m->type == MAT_LAMBERT || m->type == MAT_GGXREFL; }
+    bool transmissive() const { return m->type == MAT_LIGHT || m->type == MAT_GGX; }
+
+    // sampleBxdf: matlambert.cpp:5-18, matggxrefl.cpp:5-34, matggx.cpp:5-47.
+    V3<T> sample(const float u[3], const V3<T>& wo, const V3<T>& n) const {
+        if (m->type == MAT_LAMBERT) {
+            T r2 = T(u[0]) * T(0.99f);
+            T phi = T(double(u[1]) * 3.14159 * 2);
+            T r = std::sqrt(r2), h = std::sqrt(T(1) - r2);
+            V3<T> ax1, ax2;
+            frame(n, ax1, ax2);
+            return h * n + (r * T(std::cos(phi))) * ax1 + (r * T(std::sin(phi))) * ax2;
+        }
+        if (m->type == MAT_GGXREFL) {
+            V3<T> ax1, ax2;
+            frame(n, ax1, ax2);
+            T a = T(m->alpha);
+            T r1 = T(u[0]), r2 = T(u[1]);
+            T theta = T(double(std::atan(a * std::sqrt(r1 / (T(1) - r1)))) * 0.9999);
+            T phi = T(2.0f * 3.14159f) * r2;
+            V3<T> hv = T(std::cos(theta)) * n + (T(std::sin(theta)) * T(std::cos(phi))) * ax1 +
+                       (T(std::sin(theta)) * T(std::sin(phi))) * ax2;
+            return (T(2) * hv.dot(wo)) * hv - wo;
+        }
+        if (m->type == MAT_GGX) {
+            V3<T> ax1, ax2;
+            frame(n, ax1, ax2);
+            T a = T(m->alpha);
+            T r1 = T(u[0]), r2 = T(u[1]);
+            T ct = std::sqrt((T(1) - r1) / (T(1) + (a * a - T(1)) * r1));
+            T theta = T(double(std::acos(ct)) * 0.9999);
+            T phi = T(2.0f * 3.14159f) * r2;
+            V3<T> hv = T(std::cos(theta)) * n + (T(std::sin(theta)) * T(std::cos(phi))) * ax1 +
+                       (T(std::sin(theta)) * T(std::sin(phi))) * ax2;
+            V3<T> wi;
+            if (u[2] < 0.5f) {
+                wi = (T(2) * hv.dot(wo)) * hv - wo;
+            } else {
+                // matggx.cpp:43 reads the still-zero `wi`, so eta is always 1/ior.
+                T eta = T(1) / T(m->ior);
+                T nh = std::fabs((eta + T(1)) * wo.dot(hv));
+                wi = -eta * wo - hv * nh;
+            }
+            return wi.normalized();
+        }
+        return V3<T>(0);
+    }
+
+    static T ggx_d(T a, T c) {  // D of matggxrefl.cpp:46 / matggx.cpp:77-82 (for n.m >= 0)
+        T a2 = a * a;
+        T k = c * c * (a2 - T(1)) + T(1);
+        return a2 / pi_f() / (k * k);
+    }
+    static T pi_f() { return T(3.14159f); }
+
+    // bxdf: matlambert.cpp:20-32, matggxrefl.cpp:36-52, matggx.cpp:54-121.
+    V3<T> f(const V3<T>& wo, const V3<T>& n, const V3<T>& wi, const V3<T>& uv) const {
+        if (m->type == MAT_LAMBERT) {
+            if (wi.dot(n) * wo.dot(n) < 0) return V3<T>(0);
+            return albedo(uv) / T(3.14159);
+        }
+        if (m->type == MAT_GGXREFL) {
+            if (wi.dot(n) * wo.dot(n) < 0) return V3<T>(0);
+            V3<T> kd = albedo(uv);
+            T a = T(m->alpha);
+            V3<T> wh = (wo + wi).normalized();
+            T D = ggx_d(a, n.dot(wh));
+            T k = a * T(0.5) + T(0.5);
+            T ci = n.dot(wi), co = n.dot(wo);
+            T Gl = T(2) * ci / (ci + std::sqrt(k * k + (T(1) - k * k * ci * ci)));
+            T Gv = T(2) * co / (co + std::sqrt(k * k + (T(1) - k * k * co * co)));
+            T fr = std::pow(T(1) - wh.dot(wi), T(5));
+            V3<T> F = kd + (V3<T>(1) - kd) * fr;
+            T s = D * Gl * Gv / T(4) / (co + T(1e-6f)) / (ci + T(1e-6f));
+            return F * s;
+        }
+        if (m->type == MAT_GGX) {
+            V3<T> kd = albedo(uv);
+            T bi = wi.dot(n), bo = wo.dot(n);
+            if (bi * bo >= 0) return ggx_fr(wi, wo, n, kd);
+            return ggx_ft(wi, wo, n, kd);
+        }
+        return V3<T>(0);
+    }
+
+    // pdf: matlambert.cpp:34-37, matggxrefl.cpp:54-64, matggx.cpp:123-148, matlight.cpp:15-18.
+    T pdf(const V3<T>& wo, const V3<T>& n, const V3<T>& wi) const {
+        if (m->type == MAT_LAMBERT) return T(1.0 / 3.14159) * (std::fabs(wi.dot(n)) + T(1e-4f));
+        if (m->type == MAT_GGXREFL) {
+            if (double(wi.dot(n)) <= 1e-4 || wo.dot(n) <= T(1e-4f)) return T(1e18f);
+            V3<T> wh = (wo + wi).normalized();
+            T D = ggx_d(T(m->alpha), n.dot(wh));
+            return D * n.dot(wh) / T(4) + T(1e-8f);
+        }
+        if (m->type == MAT_GGX) {
+            if (wi.dot(n) * wo.dot(n) > 0) {
+                V3<T> wh = (wo + wi).normalized();
+                T d = ggx_D_signed(wh, n);
+                if (d < T(1e-9f)) return T(1e18f);
+                return d * n.dot(wh) / T(4);
+            }
+            T eta = wi.dot(n) > 0 ? T(m->ior) : T(1) / T(m->ior);
+            V3<T> wh = -(wi + eta * wo).normalized();
+            if (wh.dot(n) < 0) wh = -wh;
+            T d = ggx_D_signed(wh, n);
+            if (d < T(1e-6f)) return T(1e18f);
+            return std::max(d * n.dot(wh) / T(4), T(1));
+        }
+        return T(1.0 / 3.14159);
+    }
+
+    // MatGGX helpers (matggx.cpp:49-121).
+    T ggx_D_signed(const V3<T>& mm, const V3<T>& n) const {
+        if (n.dot(mm) < 0) return 0;
+        return ggx_d(T(m->alpha), n.dot(mm));
+    }
+    V3<T> ggx_F(const V3<T>& v, const V3<T>& mm, const V3<T>& kd) const {
+        if (v.dot(mm) < 0) return V3<T>(0);
+        return kd + (V3<T>(1) - kd) * std::pow(T(1) - v.dot(mm), T(5));
+    }
+    T ggx_G1(const V3<T>& v, const V3<T>& mm, const V3<T>& n) const {
+        T vm = v.dot(mm), vn = v.dot(n);
+        if (vm * vn < 0) return 0;
+        T tn = T(m->alpha) * std::tan(std::acos(vn));
+        T r2 = T(1) + tn * tn;
+        return T(2) / (T(1) + std::sqrt(r2));
+    }
+    V3<T> ggx_fr(const V3<T>& i, const V3<T>& o, const V3<T>& n, const V3<T>& kd) const {
+        if (n.dot(i) * n.dot(o) < 0) return V3<T>(0);
+        V3<T> h = (i + o).normalized();
+        V3<T> nn = i.dot(n) > 0 ? n : -n;
+        V3<T> F = ggx_F(i, h, kd);
+        T g = ggx_G1(i, h, nn) * ggx_G1(o, h, nn);
+        T d = ggx_D_signed(h, nn);
+        T denom = T(4) * i.dot(nn) * o.dot(nn);
+        return F * (g * d) / (denom + T(1e-6f));
+    }
+    V3<T> ggx_ft(const V3<T>& i, const V3<T>& o, const V3<T>& n, const V3<T>& kd) const {
+        if (n.dot(i) * n.dot(o) > 0) return V3<T>(0);
+        T eta = i.dot(n) > 0 ? T(m->ior) : T(1) / T(m->ior);
+        V3<T> h = -(i + eta * o).normalized();
+        T A = std::fabs(i.dot(h) * o.dot(h) / i.dot(n) / o.dot(n));
+        V3<T> F = ggx_F(i, h, kd);
+        T g = ggx_G1(i, h, n) * ggx_G1(o, h, n);
+        T d = ggx_D_signed(h, n);
+        V3<T> numer = (eta * eta) * ((V3<T>(1) - F) * (g * d));
+        T c = i.dot(h) + eta * o.dot(h);
+        return numer * (A / (c * c + T(1e-8f)));
+    }
+};
+
+}  // namespace orc

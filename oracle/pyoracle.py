@@ -1,0 +1,166 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes front-end of oracle/_build/liboracle.so, the CPU restatement of the reference's ptnee
+path.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module,
+and only as the checker; the product (nuray_amd / libnuray.so) never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        vp = C.c_void_p
+        L.orc_scene_create.restype = vp
+        L.orc_scene_create.argtypes = [f32p, vp, vp, i32p, C.c_int, f32p, C.c_int]
+        L.orc_scene_destroy.argtypes = [vp]
+        L.orc_scene_set_envmap.argtypes = [vp, C.c_int, C.c_int, f32p, C.c_float]
+        L.orc_scene_set_texture_rgba8.argtypes = [vp, C.c_int, C.c_int, C.c_int, u8p]
+        L.orc_bvh_build.argtypes = [vp]
+        L.orc_bvh_depth.argtypes = [vp]
+        L.orc_bvh_export.argtypes = [vp, i32p, i32p, f32p, i32p, i32p, i32p]
+        L.orc_intersect.argtypes = [vp, C.c_int, f32p, f32p, i32p, f32p, f32p, f32p, C.c_int]
+        L.orc_trav_stats.argtypes = [vp, C.c_int, f32p, f32p, u64p]
+        L.orc_render.argtypes = [vp, f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 C.c_int, C.c_uint32, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 f64p, f64p]
+        L.orc_philox.argtypes = [u32p, u32p, u32p]
+        L.orc_srgb8_decode.argtypes = [C.c_int]
+        L.orc_srgb8_decode.restype = C.c_float
+        L.orc_camera_rays.argtypes = [f32p, C.c_int, f32p, f32p, C.c_float, C.c_float, f32p]
+        L.orc_texture_lookup.argtypes = [vp, C.c_int, C.c_int, f32p, f32p, f32p]
+        L.orc_material_eval.argtypes = [vp, C.c_int, C.c_int, f32p, f32p, f32p, f32p, C.c_int, f32p, f32p, f32p]
+        L.orc_light_pick.argtypes = [vp, C.c_float]
+        L.orc_light_p.argtypes = [vp]
+        L.orc_light_p.restype = C.c_float
+        _lib = L
+    return _lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+class Scene:
+    """Oracle scene built from a scene dict of nuray_amd.scenes."""
+
+    def __init__(self, sc, build_bvh=True):
+        L = lib()
+        p = _f32(sc["p"])
+        n = None if sc.get("n") is None else _f32(sc["n"])
+        uv = None if sc.get("uv") is None else _f32(sc["uv"])
+        self._keep = (p, n, uv)
+        self.h = L.orc_scene_create(p, None if n is None else n.ctypes.data, None if uv is None else uv.ctypes.data,
+                                    np.ascontiguousarray(sc["mat"], np.int32), len(p), _f32(sc["mats"]),
+                                    len(sc["mats"]))
+        self.n_tris = len(p)
+        env = sc.get("envmap")
+        if env is not None:
+            rgb = _f32(env["rgb"])
+            L.orc_scene_set_envmap(self.h, rgb.shape[1], rgb.shape[0], rgb, float(env.get("gain", 1.0)))
+        for tid, tex in enumerate(sc.get("textures") or []):
+            t = np.ascontiguousarray(tex, np.uint8)
+            L.orc_scene_set_texture_rgba8(self.h, tid, t.shape[1], t.shape[0], t)
+        self.n_nodes = L.orc_bvh_build(self.h) if build_bvh else 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def depth(self):
+        return lib().orc_bvh_depth(self.h)
+
+    def export(self):
+        n = self.n_nodes
+        left = np.zeros(n, np.int32); right = np.zeros(n, np.int32)
+        box = np.zeros((n, 6), np.float32)
+        lf = np.zeros(n, np.int32); lc = np.zeros(n, np.int32)
+        order = np.zeros(self.n_tris, np.int32)
+        lib().orc_bvh_export(self.h, left, right, box, lf, lc, order)
+        return dict(left=left, right=right, box=box, leaf_first=lf, leaf_count=lc, prim_order=order)
+
+    def intersect(self, o, d, brute=False):
+        o = _f32(o); d = _f32(d)
+        n = len(o)
+        tri = np.zeros(n, np.int32); t = np.zeros(n, np.float32)
+        b1 = np.zeros(n, np.float32); b2 = np.zeros(n, np.float32)
+        lib().orc_intersect(self.h, n, o, d, tri, t, b1, b2, int(brute))
+        return tri, t, b1, b2
+
+    def trav_stats(self, o, d):
+        out = np.zeros(2, np.uint64)
+        lib().orc_trav_stats(self.h, len(o), _f32(o), _f32(d), out)
+        return int(out[0]), int(out[1])
+
+    def render(self, cam, W, H, spp, spp0=0, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0, fp64=False,
+               nthreads=0, rect=None, sums=None):
+        x0, y0, x1, y1 = rect if rect is not None else (0, 0, W, H)
+        if sums is None:
+            sums = (np.zeros(W * H * 3), np.zeros(W * H * 3))
+        r = lib().orc_render(self.h, _f32(cam), W, H, x0, y0, x1, y1, spp0, spp, seed, prr, mis, maxdepth,
+                             rrdepth, int(fp64), nthreads, sums[0], sums[1])
+        if r != 0:
+            raise RuntimeError("oracle render failed")
+        return sums
+
+    def texture_lookup(self, tid, u, v):
+        u = _f32(u); v = _f32(v)
+        out = np.zeros((len(u), 3), np.float32)
+        lib().orc_texture_lookup(self.h, tid, len(u), u, v, out)
+        return out
+
+    def material_eval(self, m, wo, n, wi=None, u3=None):
+        wo = _f32(wo); n = _f32(n); k = len(wo)
+        wi_in = _f32(wi if wi is not None else np.zeros((k, 3)))
+        u_in = _f32(u3 if u3 is not None else np.zeros((k, 3)))
+        wi_out = np.zeros((k, 3), np.float32); f = np.zeros((k, 3), np.float32); pdf = np.zeros(k, np.float32)
+        lib().orc_material_eval(self.h, m, k, wo, n, wi_in, u_in, 1 if wi is None else 0, wi_out, f, pdf)
+        return wi_out, f, pdf
+
+    def light_pick(self, u):
+        return lib().orc_light_pick(self.h, float(u))
+
+    def light_p(self):
+        return lib().orc_light_p(self.h)
+
+
+def philox(ctr, key):
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox(np.asarray(ctr, np.uint32), np.asarray(key, np.uint32), out)
+    return out
+
+
+def srgb8_decode(c):
+    return lib().orc_srgb8_decode(int(c))
+
+
+def camera_rays(cam, px, py, W, H):
+    px = _f32(px); py = _f32(py)
+    out = np.zeros((len(px), 3), np.float32)
+    lib().orc_camera_rays(_f32(cam), len(px), px, py, float(W), float(H), out)
+    return out
