@@ -144,6 +144,9 @@ int nr_intersect(nr_scene* s, const float* o, const float* d, uint32_t n, int32_
         nr::Scene& sc = *s->sc;
         if (!sc.built) throw std::runtime_error("BVH not built");
         if (any_hit && !tmax) throw std::runtime_error("any_hit needs tmax");
+        if (any_hit)
+            for (uint32_t i = 0; i < n; ++i)
+                if (!(tmax[i] == tmax[i])) throw std::runtime_error("tmax must not be NaN");
         if (n == 0) return NR_OK;
         std::vector<nr::Ray> rays(n);
         for (uint32_t i = 0; i < n; ++i) {

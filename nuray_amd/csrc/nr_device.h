@@ -153,6 +153,70 @@ __device__ __forceinline__ bool tri_valid(float t, float b1, float b2) {
     return t > 0.0f && t < 1e18f && b1 >= 0.0f && b2 >= 0.0f && b1 + b2 <= 1.0f;
 }
 
+// ------------------------------------------------------------------ filtered exact tests
+// The same decisions as slab() / tri_test()+tri_valid(), with most divisions replaced by a
+// multiply and an error band.  Outside the band the cheap result provably equals the exact
+// one; inside it the exact division path runs.  Hit / miss outcomes and hit values are
+// therefore bit-identical to the unfiltered functions.
+//
+// Slab: per ray r = 1 / (d + 1e-9f) (correctly rounded).  For every bound,
+// t' = fl(fl(b - o) * r) differs from t = fl(fl(b - o) / dd) by at most ~2.5 ulp relative, so
+// t' has t's sign and |t' - t| <= 2^-22 |t|.  With finite quotients (dd != 0: `rays_ok`) the
+// reference's swap / std::max / std::min chain reduces to t_in = max(-1e18, min pairs) and
+// t_out = min(1e18, max pairs), each also within 2^-22 relative of its exact value.  Then
+//   t_out >= 0  <=>  t_out' >= 0                                   (sign preserved), and
+//   t_in <= t_out is decided by t_out' - t_in' against a band of 2^-20 (|t_in'| + |t_out'|).
+// Values below 1e-30 in magnitude (denormal range) always take the exact path.
+struct SlabRay {
+    F3 o, dd, r;
+    bool finite;  // every dd component non-zero
+};
+__device__ __forceinline__ SlabRay slab_ray(F3 o, F3 d) {
+    SlabRay s;
+    s.o = o;
+    s.dd = f3(d.x + 1e-9f, d.y + 1e-9f, d.z + 1e-9f);
+    s.r = f3(1.0f / s.dd.x, 1.0f / s.dd.y, 1.0f / s.dd.z);
+    s.finite = s.dd.x != 0.0f && s.dd.y != 0.0f && s.dd.z != 0.0f;
+    return s;
+}
+__device__ __forceinline__ bool slab_fast(float mnx, float mny, float mnz, float mxx, float mxy, float mxz,
+                                          const SlabRay& s) {
+    const float ax = (mnx - s.o.x) * s.r.x, bx = (mxx - s.o.x) * s.r.x;
+    const float ay = (mny - s.o.y) * s.r.y, by = (mxy - s.o.y) * s.r.y;
+    const float az = (mnz - s.o.z) * s.r.z, bz = (mxz - s.o.z) * s.r.z;
+    const float t_in = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), -1e18f));
+    const float t_out = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), 1e18f));
+    const float band = 9.5367431640625e-07f * (fabsf(t_in) + fabsf(t_out));  // 2^-20
+    const float diff = t_out - t_in;
+    const bool clear = s.finite && fabsf(t_out) >= 1e-30f && fabsf(t_in) >= 1e-30f && fabsf(diff) > band;
+    if (__builtin_expect(clear, 1)) return t_out >= 0.0f && diff > 0.0f;
+    return slab(mnx, mny, mnz, mxx, mxy, mxz, s.o, s.dd);
+}
+
+// Triangle: q is formed exactly as in tri_test (so |q| >= ~2.5e-15: a float plus 1e-6 in double
+// is never closer to zero), rq = v_rcp_f32(q) (1 ulp), and t', b1', b2' = numerator * rq lie within
+// ~2.5 ulp of the exact quotients.  A triangle is rejected without division when any of
+//   t' <= -1e-30, b1' <= -1e-30, b2' <= -1e-30  (sign of a quotient that cannot underflow),
+//   t' >= best * (1 + 2^-18) + 1e-30,  b1' + b2' >= 1 + 2^-18
+// holds -- each implies the exact acceptance test fails.  Everything else is decided on the
+// exact quotients.  Returns true iff tri_valid(t, b1, b2) and t <= best, with exact t, b1, b2.
+__device__ __forceinline__ bool tri_fast(const TriGeom& g, F3 o, F3 d, float best, float& t, float& b1, float& b2) {
+    const F3 p0 = f3(g.g0.x, g.g0.y, g.g0.z);
+    const F3 e1 = f3(g.g0.w, g.g1.x, g.g1.y);
+    const F3 e2 = f3(g.g1.z, g.g1.w, g.g2.x);
+    const F3 s = sub(o, p0), s1 = cross(d, e2), s2 = cross(s, e1);
+    const float tn = dot(s2, e2), b1n = dot(s1, s), b2n = dot(s2, d);
+    const float q = float(double(dot(s1, e1)) + 1e-6);
+    const float rq = __builtin_amdgcn_rcpf(q);
+    const float tq = tn * rq, b1q = b1n * rq, b2q = b2n * rq;
+    if (tq <= -1e-30f || b1q <= -1e-30f || b2q <= -1e-30f) return false;
+    if (tq >= best * 1.000003814697265625f + 1e-30f || b1q + b2q >= 1.000003814697265625f) return false;
+    t = tn / q;
+    b1 = b1n / q;
+    b2 = b2n / q;
+    return tri_valid(t, b1, b2) && t <= best;
+}
+
 // ------------------------------------------------------------------ Philox4x32-10
 // Salmon et al., SC'11.  Counter = (pixel, sample, c2, c3), key = (seed, 0x6E755261).
 __device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {

@@ -296,7 +296,10 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
     if (!sc.built) throw std::runtime_error("scene not built (call nr_bvh_build_sah)");
     if (p.W <= 0 || p.H <= 0 || p.spp <= 0) throw std::runtime_error("bad image size or spp");
     const int npix = p.W * p.H;
-    int kb = p.batch_spp > 0 ? p.batch_spp : std::max(1, int((size_t(4) << 20) / size_t(npix)));
+    // Default batch: up to 64M paths in flight (~160 B of queues and state each, ~10 GB of the
+    // 288 GB HBM), so a 720p frame at 64 spp runs as one wavefront and the short tail
+    // iterations of Russian roulette are shared by every sample of the frame.
+    int kb = p.batch_spp > 0 ? p.batch_spp : std::max(1, int((size_t(64) << 20) / size_t(npix)));
     kb = std::min(kb, p.spp);
     if (!sc.target) sc.target = new RenderTarget();
     RenderTarget& t = *sc.target;

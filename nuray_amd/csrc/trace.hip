@@ -46,72 +46,81 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const Ray* __rest
         const Ray r = rays[i];
         const F3 o = f3(r.o.x, r.o.y, r.o.z);
         const F3 d = f3(r.d.x, r.d.y, r.d.z);
-        const F3 dd = f3(d.x + 1e-9f, d.y + 1e-9f, d.z + 1e-9f);
+        const SlabRay sr = slab_ray(o, d);
         const double len = double(r.o.w);
-        float bt = 1e18f, bb1 = 0.0f, bb2 = 0.0f;
+        // closest hit: running winner; any hit: `best` caps the triangle filter at the segment
+        float bt = ANY ? fminf(r.o.w, 1e18f) : 1e18f, bb1 = 0.0f, bb2 = 0.0f;
         int bprim = -1, blf = -1;
         bool done = false, overflow = false;
         uint32_t n_pf = 0, n_tt = 0;
 
-        // Leaf loop, bvhnode.cpp:50-66, merged into the running winner with the DFS tie rule.
-        auto leaf = [&](int32_t ref) {
-            const int first = ref_first(ref), cnt = ref_count(ref);
+        // The stack holds internal-node and leaf references.  Internal nodes are expanded
+        // until this lane holds a leaf (or runs dry); the wave then tests its leaves together.
+        int sp = 0;
+        int32_t leaf = 0;
+        bool has_leaf = false;
+        if (slab_fast(sv.root_min[0], sv.root_min[1], sv.root_min[2], sv.root_max[0], sv.root_max[1],
+                      sv.root_max[2], sr)) {
+            if (ref_is_leaf(sv.root_ref)) leaf = sv.root_ref, has_leaf = true;
+            else stk[tid] = sv.root_ref, sp = 1;
+        }
+        for (;;) {
+            while (sp > 0 && !has_leaf) {
+                --sp;
+                const int32_t e = stk[sp * bdim + tid];
+                if (ref_is_leaf(e)) {
+                    leaf = e;
+                    has_leaf = true;
+                    break;
+                }
+                const NodePair* np = sv.nodes + e;
+                const float4 a = np->a, b = np->b, c = np->c;
+                const int4 ch = np->d;
+                if (STATS) ++n_pf;
+                const bool hl = slab_fast(a.x, a.y, a.z, a.w, b.x, b.y, sr);
+                const bool hr = slab_fast(b.z, b.w, c.x, c.y, c.z, c.w, sr);
+                // a hit leaf child is taken at once; everything else is pushed
+                int32_t push0 = 0, push1 = 0;
+                int np_ = 0;
+                if (hl) {
+                    if (ref_is_leaf(ch.x)) leaf = ch.x, has_leaf = true;
+                    else push0 = ch.x, np_ = 1;
+                }
+                if (hr) {
+                    if (ref_is_leaf(ch.y) && !has_leaf) leaf = ch.y, has_leaf = true;
+                    else if (np_ == 0) push0 = ch.y, np_ = 1;
+                    else push1 = ch.y, np_ = 2;
+                }
+                if (sp + np_ > cap) {
+                    overflow = true;
+                    sp = 0;
+                    has_leaf = false;
+                    break;
+                }
+                if (np_ >= 1) stk[sp * bdim + tid] = push0;
+                if (np_ == 2) stk[(sp + 1) * bdim + tid] = push1;
+                sp += np_;
+            }
+            if (!has_leaf) break;
+            has_leaf = false;
+            // Leaf loop, bvhnode.cpp:50-66, merged into the running winner with the DFS tie rule.
+            const int first = ref_first(leaf), cnt = ref_count(leaf);
             if (STATS) n_tt += cnt;
             for (int k = 0; k < cnt; ++k) {
                 const TriGeom g = sv.tris[first + k];
                 float t, b1, b2;
-                tri_test(g, o, d, t, b1, b2);
-                if (!tri_valid(t, b1, b2)) continue;
+                if (!tri_fast(g, o, d, bt, t, b1, b2)) continue;
                 if (ANY) {
                     if (double(t) + 2e-3 <= len) {
                         bt = t; bb1 = b1; bb2 = b2; bprim = first + k;
                         done = true;
-                        return;
+                        break;
                     }
-                } else if (t < bt || (t == bt && first > blf)) {
+                } else if (t < bt || first > blf) {  // t <= bt here: equal t -> later leaf wins
                     bt = t; bb1 = b1; bb2 = b2; bprim = first + k; blf = first;
                 }
             }
-        };
-
-        int sp = 0;
-        if (slab(sv.root_min[0], sv.root_min[1], sv.root_min[2], sv.root_max[0], sv.root_max[1],
-                 sv.root_max[2], o, dd)) {
-            if (ref_is_leaf(sv.root_ref)) leaf(sv.root_ref);
-            else stk[tid] = sv.root_ref, sp = 1;
-        }
-        while (sp > 0 && !done) {
-            --sp;
-            const int ni = stk[sp * bdim + tid];
-            const NodePair* np = sv.nodes + ni;
-            const float4 a = np->a, b = np->b, c = np->c;
-            const int4 e = np->d;
-            if (STATS) ++n_pf;
-            const bool hl = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, dd);
-            const bool hr = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, dd);
-            if (hl) {
-                if (ref_is_leaf(e.x)) {
-                    leaf(e.x);
-                } else if (sp < cap) {
-                    stk[sp * bdim + tid] = e.x;
-                    ++sp;
-                } else {
-                    overflow = true;
-                    break;
-                }
-            }
-            if (done) break;
-            if (hr) {
-                if (ref_is_leaf(e.y)) {
-                    leaf(e.y);
-                } else if (sp < cap) {
-                    stk[sp * bdim + tid] = e.y;
-                    ++sp;
-                } else {
-                    overflow = true;
-                    break;
-                }
-            }
+            if (ANY && done) break;
         }
         if (overflow) atomicOr(err, 1);
         if (STATS) {
