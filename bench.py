@@ -90,21 +90,23 @@ def main():
     cnt = dev.stats()
     dev.set_profiling(1)
     nr.synchronize()
-    closest_ms = 0.0
+    closest_ms = shade_ms = shadow_ms = 0.0
     launches = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
         dev.render(sc["camera"], W, H, **rp)
         st = dev.stats()
         closest_ms += st["closest_ms"]
+        shade_ms += st["shade_ms"]
+        shadow_ms += st["shadow_ms"]
         launches += st["closest_launches"]
     nr.synchronize()
     elapsed = time.perf_counter() - t0
     samples = W * H * spp * args.steps
     value = samples / elapsed / 1e6
-    print(f"# timed {args.steps} frames in {elapsed:.3f} s; closest-hit kernel {closest_ms:.1f} ms over "
-          f"{launches} launches; last frame: {st['ms']:.1f} ms, {st['iterations']:.0f} iterations",
-          file=sys.stderr, flush=True)
+    print(f"# timed {args.steps} frames in {elapsed:.3f} s; per frame: closest-hit {closest_ms / args.steps:.1f} ms, "
+          f"shade {shade_ms / args.steps:.1f} ms, shadow {shadow_ms / args.steps:.1f} ms; "
+          f"{st['iterations']:.0f} iterations", file=sys.stderr, flush=True)
 
     bytes_frame = B_PAIR * cnt["pair_fetch"] + B_TRI * cnt["tri_tests"] + B_RAY * cnt["rays"]
     achieved = bytes_frame * args.steps / (closest_ms / 1e3) / 1e9 if closest_ms > 0 else None

@@ -112,9 +112,10 @@ int nr_render_fetch(nr_scene* s, float* rgb_linear, double* sum, double* sum2, u
 /* Statistics of the last render: [0] wall ms, [1] samples, [2] wavefront iterations,
  * [3] closest-hit kernel ms, [4] closest-hit launches, [5] rays traced (closest),
  * [6] node-pair fetches (closest), [7] triangle tests (closest), [8] shadow kernel ms,
- * [9] shadow rays, [10] shadow node-pair fetches, [11] shadow triangle tests.
+ * [9] shadow rays, [10] shadow node-pair fetches, [11] shadow triangle tests, [12] shade kernel ms,
+ * [13] shade launches, [14] shadow launches, [15] reserved.
  * Kernel times are filled when the render ran with timing on (nr_set_profiling). */
-int nr_render_stats(nr_scene* s, double* out12);
+int nr_render_stats(nr_scene* s, double* out16);
 /* 1: time the traversal launches with HIP events; 2: also count node-pair fetches / triangle tests. */
 int nr_set_profiling(nr_scene* s, int32_t mode);
 

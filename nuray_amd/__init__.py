@@ -200,8 +200,9 @@ class Scene:
         check(lib().nr_set_profiling(self.h, mode))
 
     def stats(self):
-        out = np.zeros(12, np.float64)
+        out = np.zeros(16, np.float64)
         check(lib().nr_render_stats(self.h, out))
         keys = ["ms", "samples", "iterations", "closest_ms", "closest_launches", "rays", "pair_fetch", "tri_tests",
-                "shadow_ms", "shadow_rays", "shadow_pair_fetch", "shadow_tri_tests"]
+                "shadow_ms", "shadow_rays", "shadow_pair_fetch", "shadow_tri_tests", "shade_ms", "shade_launches",
+                "shadow_launches", "reserved"]
         return dict(zip(keys, out.tolist()))

@@ -9,7 +9,7 @@
 struct nr_scene {
     nr::Scene* sc;
     int profiling = 0;
-    double stats[12] = {0};
+    double stats[16] = {0};
 };
 
 namespace {
@@ -198,6 +198,7 @@ int nr_render_ptnee(nr_scene* s, const nr_camera* cam, const nr_params* pp, floa
         o[3] = kt.closest_ms; o[4] = kt.closest_launches;
         o[5] = double(ts[0].rays); o[6] = double(ts[0].pair_fetch); o[7] = double(ts[0].tri_tests);
         o[8] = kt.shadow_ms; o[9] = double(ts[1].rays); o[10] = double(ts[1].pair_fetch); o[11] = double(ts[1].tri_tests);
+        o[12] = kt.shade_ms; o[13] = kt.shade_launches; o[14] = kt.shadow_launches; o[15] = 0;
         if (rgb_linear || sum || sum2 || rgb8) nr::render_fetch(*s->sc, rgb_linear, sum, sum2, rgb8);
         return st ? NR_CANCELLED : NR_OK;
     });
@@ -210,9 +211,9 @@ int nr_render_fetch(nr_scene* s, float* rgb_linear, double* sum, double* sum2, u
     });
 }
 
-int nr_render_stats(nr_scene* s, double* out12) {
+int nr_render_stats(nr_scene* s, double* out16) {
     return guard([&] {
-        std::memcpy(out12, s->stats, sizeof(s->stats));
+        std::memcpy(out16, s->stats, sizeof(s->stats));
         return NR_OK;
     });
 }

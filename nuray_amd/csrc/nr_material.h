@@ -11,6 +11,10 @@
 
 namespace nr {
 
+// Optional material features a kernel instance is compiled with (the host launches the instance
+// matching the scene): F_GGX = translucent GGX dielectric present, F_TEX = albedo textures present.
+constexpr unsigned F_GGX = 1u, F_TEX = 2u, F_ALL = 3u;
+
 struct Mat {
     int type;
     F3 kd;
@@ -49,15 +53,18 @@ __device__ __forceinline__ F3 tex_uv(const SceneView& sv, int id, float u, float
     r = add(r, mul(mul(texel(sv, t, x0, y0), float(x1) - x), float(y1) - y));
     return r;
 }
-// Miss shading, rendererptls.cpp:18-28 (the angle math runs in double as in the reference).
+// Miss shading, rendererptls.cpp:18-28.  The reference's double promotion of acos / atan2 is a
+// C++ literal artefact; FP32 here moves u, v by at most a couple of ulps.
 __device__ __forceinline__ F3 env_radiance(const SceneView& sv, F3 d) {
     if (sv.env_tex < 0) return f3(0.0f, 0.0f, 0.0f);
-    const float v = float(1.0 - acos(double(d.y)) / 3.14159);
-    const float u = float(atan2(double(d.z), double(d.x)) / 2 / 3.14159 + 0.5);
+    const float v = 1.0f - acosf(d.y) / 3.14159f;
+    const float u = atan2f(d.z, d.x) / 2.0f / 3.14159f + 0.5f;
     return mul(tex_uv(sv, sv.env_tex, u, v), sv.env_gain);
 }
+template <unsigned F = F_ALL>
 __device__ __forceinline__ F3 mat_albedo(const SceneView& sv, const Mat& m, float u, float v) {
-    return m.tex >= 0 ? tex_uv(sv, m.tex, u, v) : m.kd;
+    if (F & F_TEX) return m.tex >= 0 ? tex_uv(sv, m.tex, u, v) : m.kd;
+    return m.kd;
 }
 
 // ---------------------------------------------------------------- BSDFs
@@ -73,6 +80,7 @@ __device__ __forceinline__ float ggx_d(float a, float c) {
 }
 
 // sampleBxdf; u = (r1, r2, lobe)
+template <unsigned F = F_ALL>
 __device__ __forceinline__ F3 bsdf_sample(const Mat& m, float u0, float u1, float u2, F3 wo, F3 n) {
     F3 ax1, ax2;
     frame(n, ax1, ax2);
@@ -93,7 +101,7 @@ __device__ __forceinline__ F3 bsdf_sample(const Mat& m, float u0, float u1, floa
         const F3 h = add(add(mul(n, ct), mul(ax1, st * cp)), mul(ax2, st * sp));
         return sub(mul(h, 2.0f * dot(h, wo)), wo);
     }
-    if (m.type == MAT_GGX) {
+    if ((F & F_GGX) && m.type == MAT_GGX) {
         const float a = m.alpha;
         const float c = sqrtf((1.0f - u0) / (1.0f + (a * a - 1.0f) * u0));
         const float theta = float(double(acosf(c)) * 0.9999);
@@ -132,14 +140,15 @@ __device__ __forceinline__ float ggx_G1(float a, F3 v, F3 mm, F3 n) {
     return 2.0f / (1.0f + sqrtf(1.0f + tn * tn));
 }
 
+template <unsigned F = F_ALL>
 __device__ __forceinline__ F3 bsdf_eval(const SceneView& sv, const Mat& m, F3 wo, F3 n, F3 wi, float u, float v) {
     if (m.type == MAT_LAMBERT) {
         if (dot(wi, n) * dot(wo, n) < 0.0f) return f3(0.0f, 0.0f, 0.0f);
-        return divs(mat_albedo(sv, m, u, v), 3.14159f);
+        return divs(mat_albedo<F>(sv, m, u, v), 3.14159f);
     }
     if (m.type == MAT_GGXREFL) {
         if (dot(wi, n) * dot(wo, n) < 0.0f) return f3(0.0f, 0.0f, 0.0f);
-        const F3 kd = mat_albedo(sv, m, u, v);
+        const F3 kd = mat_albedo<F>(sv, m, u, v);
         const F3 wh = normalized(add(wo, wi));
         const float D = ggx_d(m.alpha, dot(n, wh));
         const float k = m.alpha * 0.5f + 0.5f;
@@ -150,33 +159,34 @@ __device__ __forceinline__ F3 bsdf_eval(const SceneView& sv, const Mat& m, F3 wo
         const float s = D * Gl * Gv / 4.0f / (co + 1e-6f) / (ci + 1e-6f);
         return f3((kd.x + (1.0f - kd.x) * fr) * s, (kd.y + (1.0f - kd.y) * fr) * s, (kd.z + (1.0f - kd.z) * fr) * s);
     }
-    if (m.type == MAT_GGX) {
-        const F3 kd = mat_albedo(sv, m, u, v);
+    if ((F & F_GGX) && m.type == MAT_GGX) {
+        const F3 kd = mat_albedo<F>(sv, m, u, v);
         const float bi = dot(wi, n), bo = dot(wo, n);
         if (bi * bo >= 0.0f) {  // fr
             if (dot(n, wi) * dot(n, wo) < 0.0f) return f3(0.0f, 0.0f, 0.0f);
             const F3 h = normalized(add(wi, wo));
             const F3 nn = dot(wi, n) > 0.0f ? n : neg(n);
-            const F3 F = ggx_F(wi, h, kd);
+            const F3 Fr = ggx_F(wi, h, kd);
             const float g = ggx_G1(m.alpha, wi, h, nn) * ggx_G1(m.alpha, wo, h, nn);
             const float dd = ggx_D_signed(m.alpha, h, nn);
             const float denom = 4.0f * dot(wi, nn) * dot(wo, nn);
-            return mul(F, g * dd / (denom + 1e-6f));
+            return mul(Fr, g * dd / (denom + 1e-6f));
         }
         // ft
         const float eta = dot(wi, n) > 0.0f ? m.ior : 1.0f / m.ior;
         const F3 h = neg(normalized(add(wi, mul(wo, eta))));
         const float A = fabsf(dot(wi, h) * dot(wo, h) / dot(wi, n) / dot(wo, n));
-        const F3 F = ggx_F(wi, h, kd);
+        const F3 Fr = ggx_F(wi, h, kd);
         const float g = ggx_G1(m.alpha, wi, h, n) * ggx_G1(m.alpha, wo, h, n);
         const float dd = ggx_D_signed(m.alpha, h, n);
         const float c = dot(wi, h) + eta * dot(wo, h);
         const float s = A * (eta * eta) * g * dd / (c * c + 1e-8f);
-        return f3((1.0f - F.x) * s, (1.0f - F.y) * s, (1.0f - F.z) * s);
+        return f3((1.0f - Fr.x) * s, (1.0f - Fr.y) * s, (1.0f - Fr.z) * s);
     }
     return f3(0.0f, 0.0f, 0.0f);
 }
 
+template <unsigned F = F_ALL>
 __device__ __forceinline__ float bsdf_pdf(const Mat& m, F3 wo, F3 n, F3 wi) {
     if (m.type == MAT_LAMBERT) return float(1.0 / 3.14159) * (fabsf(dot(wi, n)) + 1e-4f);
     if (m.type == MAT_GGXREFL) {
@@ -185,7 +195,7 @@ __device__ __forceinline__ float bsdf_pdf(const Mat& m, F3 wo, F3 n, F3 wi) {
         const float c = dot(n, wh);
         return ggx_d(m.alpha, c) * c / 4.0f + 1e-8f;
     }
-    if (m.type == MAT_GGX) {
+    if ((F & F_GGX) && m.type == MAT_GGX) {
         if (dot(wi, n) * dot(wo, n) > 0.0f) {
             const F3 wh = normalized(add(wo, wi));
             const float d = ggx_D_signed(m.alpha, wh, n);

@@ -56,12 +56,14 @@ void RenderTarget::ensure(int w, int h, size_t c) {
         spp_total = 0;
     }
     if (c > cap) {
-        rays[0].ensure(sizeof(Ray) * c);
-        rays[1].ensure(sizeof(Ray) * c);
+        for (int k = 0; k < 2; ++k) {
+            rays[k].ensure(sizeof(Ray) * c);
+            beta[k].ensure(sizeof(float4) * c);
+        }
         hits.ensure(sizeof(HitRec) * c);
         shadow.ensure(sizeof(Ray) * c);
         contrib.ensure(sizeof(float4) * c);
-        state.ensure(sizeof(PathState) * c);
+        radiance.ensure(sizeof(float4) * c);
         cap = c;
     }
     counters.ensure(sizeof(int) * 16);
@@ -72,6 +74,7 @@ void RenderTarget::ensure(int w, int h, size_t c) {
 namespace {
 
 constexpr int BLK = 256;
+constexpr int LDS_LIGHTS = 2048;  // light CDFs up to this size are searched in LDS
 
 // Counter slots
 enum { C_RAY0 = 0, C_RAY1 = 1, C_SHADOW = 2, C_WORK0 = 3, C_WORK1 = 4, C_ERR = 5 };
@@ -98,8 +101,8 @@ __global__ void k_set_counters(int* c, int ray_slot, int ray_value, int clear_sl
 }
 
 // Camera rays: rendererptls.cpp:172 jitter (x + U, y + U), Camera::generateRay (camera.cpp:24-32).
-__global__ void k_raygen(CamView cam, int W, int npix, int P, uint32_t seed, uint32_t sample0, Ray* rays,
-                         PathState* state) {
+__global__ void k_raygen(CamView cam, int W, int H, int npix, int P, uint32_t seed, uint32_t sample0, Ray* rays,
+                         float4* beta, float4* radiance) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= P) return;
     const int pixel = slot % npix;
@@ -108,15 +111,15 @@ __global__ void k_raygen(CamView cam, int W, int npix, int P, uint32_t seed, uin
     const float4 u = rng4(seed, uint32_t(pixel), sample, 0u, 0u);
     const float ix = float(x) + u.x, iy = float(y) + u.y;
     const float cx = ix / float(W) * 2.0f - 1.0f;
-    const float cy = iy / float(npix / W) * 2.0f - 1.0f;
+    const float cy = iy / float(H) * 2.0f - 1.0f;
     const F3 gaze = f3(cam.gaze[0], cam.gaze[1], cam.gaze[2]);
     const F3 hand = f3(cam.hand[0], cam.hand[1], cam.hand[2]);
     const F3 up = f3(cam.up[0], cam.up[1], cam.up[2]);
     const F3 d = normalized(sub(add(gaze, mul(mul(mul(hand, cam.fw), cx), 0.5f)), mul(mul(mul(up, cam.fh), cy), 0.5f)));
-    rays[slot].o = make_float4(cam.pos[0], cam.pos[1], cam.pos[2], 0.0f);
+    rays[slot].o = make_float4(cam.pos[0], cam.pos[1], cam.pos[2], __int_as_float(0));
     rays[slot].d = make_float4(d.x, d.y, d.z, __int_as_float(slot));
-    state[slot].beta_fac = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-    state[slot].L_depth = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0));
+    beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    radiance[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 struct ShadeArgs {
@@ -126,50 +129,93 @@ struct ShadeArgs {
     float prr;
     int mis, maxdepth, rrdepth;
     const Ray* rin;
+    const float4* bin;
     const HitRec* hits;
     const int* n_in;
-    PathState* state;
+    float4* radiance;
     Ray* rout;
+    float4* bout;
     int* n_out;
     Ray* shadow;
     float4* contrib;
     int* n_shadow;
 };
 
-__device__ __forceinline__ int wave_append(bool emit, int* counter, int lane) {
-    const unsigned long long m = __ballot(emit);
-    const int total = __popcll(m);
-    int base = 0;
-    if (lane == 0 && total) base = atomicAdd(counter, total);
-    base = __shfl(base, 0, 64);
-    return base + __popcll(m & ((1ull << lane) - 1ull));
+constexpr int NWAVE = BLK / 64;
+
+// Stream compaction for the two output queues of one block-wide step, with a single atomic per
+// queue per block: every wave ballots its emitting lanes, the per-wave counts meet in LDS, one
+// lane reserves the block's range, and each lane's slot is the block base + the counts of the
+// waves before it + the emitting lanes below it in its own wave.  (Device-scope atomics on one
+// address serialise outside the per-XCD L2s, so they are kept to one per 256 lanes.)
+struct BlockAppend {
+    int cnt[2][2][NWAVE];  // [parity][queue][wave]
+    int base[2][2];        // [parity][queue]
+};
+__device__ __forceinline__ void block_append2(BlockAppend& sa, int parity, bool e0, bool e1, int* c0, int* c1,
+                                              int& p0, int& p1) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const unsigned long long m0 = __ballot(e0), m1 = __ballot(e1);
+    if (lane == 0) {
+        sa.cnt[parity][0][wave] = __popcll(m0);
+        sa.cnt[parity][1][wave] = __popcll(m1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t0 = 0, t1 = 0;
+        for (int w = 0; w < NWAVE; ++w) { t0 += sa.cnt[parity][0][w]; t1 += sa.cnt[parity][1][w]; }
+        sa.base[parity][0] = t0 ? atomicAdd(c0, t0) : 0;
+        sa.base[parity][1] = t1 ? atomicAdd(c1, t1) : 0;
+    }
+    __syncthreads();
+    int o0 = sa.base[parity][0], o1 = sa.base[parity][1];
+    for (int w = 0; w < wave; ++w) { o0 += sa.cnt[parity][0][w]; o1 += sa.cnt[parity][1][w]; }
+    p0 = o0 + __popcll(m0 & below);
+    p1 = o1 + __popcll(m1 & below);
 }
 
-__global__ __launch_bounds__(BLK) void k_shade(ShadeArgs a) {
+// LightSampler::sampleLight over an LDS copy of the CDF (lower_bound, lightsampler.cpp:55-56).
+__device__ __forceinline__ int pick_light_lds(const float* cdf, int n, float u) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cdf[mid] < u) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n ? lo : n - 1;
+}
+
+template <unsigned F>
+__device__ __forceinline__ void shade_body(const ShadeArgs& a) {
+    __shared__ float s_cdf[LDS_LIGHTS];
     const SceneView& sv = a.sv;
-    const int lane = threadIdx.x & 63;
+    const bool cdf_lds = sv.n_lights <= LDS_LIGHTS;
+    if (cdf_lds)
+        for (int k = threadIdx.x; k < sv.n_lights; k += blockDim.x) s_cdf[k] = sv.light_cdf[k];
+    __syncthreads();
+    __shared__ BlockAppend s_app;
     const int n = *a.n_in;
     const int stride = gridDim.x * blockDim.x;
-    for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
-        const int j = base + lane;
+    int parity = 0;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += stride, parity ^= 1) {
+        const int j = base + int(threadIdx.x);
         const bool active = j < n;
         bool emit_sh = false, emit_next = false;
         Ray sh_ray, next_ray;
-        float4 sh_c = make_float4(0, 0, 0, 0);
+        float4 sh_c = make_float4(0, 0, 0, 0), next_b = make_float4(0, 0, 0, 0);
         if (active) {
             const Ray r = a.rin[j];
             const HitRec h = a.hits[j];
+            const float4 bf = a.bin[j];
             const int slot = __float_as_int(r.d.w);
-            PathState st = a.state[slot];
-            F3 beta = f3(st.beta_fac.x, st.beta_fac.y, st.beta_fac.z);
-            const float fac = st.beta_fac.w;
-            F3 L = f3(st.L_depth.x, st.L_depth.y, st.L_depth.z);
-            const int depth = __float_as_int(st.L_depth.w);
-            const int pixel = slot % a.npix;
-            const uint32_t sample = a.sample0 + uint32_t(slot / a.npix);
+            const int depth = __float_as_int(r.o.w);
+            const F3 beta = f3(bf.x, bf.y, bf.z);
+            const float fac = bf.w;
             const F3 o = f3(r.o.x, r.o.y, r.o.z), d = f3(r.d.x, r.d.y, r.d.z);
+            F3 Le = f3(0.0f, 0.0f, 0.0f);  // radiance this vertex adds directly
             if (h.prim < 0) {
-                L = add(L, mulv(beta, env_radiance(sv, d)));
+                Le = mulv(beta, env_radiance(sv, d));
             } else {
                 const TriShade ts = sv.shade[h.prim];
                 const float b1 = h.b1, b2 = h.b2, w0 = 1.0f - b1 - b2;
@@ -178,16 +224,22 @@ __global__ __launch_bounds__(BLK) void k_shade(ShadeArgs a) {
                 const F3 nrm = normalized(add(add(mul(n0, w0), mul(n1, b1)), mul(n2, b2)));  // triangle.cpp:35-38
                 const Mat m = load_mat(sv, __float_as_int(ts.s3.w));
                 if (m.type == MAT_LIGHT) {
-                    L = add(L, mulv(beta, mul(m.kd, fac)));  // rendererptls.cpp:33-36
+                    Le = mulv(beta, mul(m.kd, fac));  // rendererptls.cpp:33-36
                 } else if (!(dot(nrm, d) > 0.0f && !mat_transmissive(m))) {
                     const float tu = w0 * ts.s2.y + b1 * ts.s2.w + b2 * ts.s3.y;  // triangle.cpp:40-43
                     const float tv = w0 * ts.s2.z + b1 * ts.s3.x + b2 * ts.s3.z;
                     const F3 hp = add(o, mul(d, h.t));
                     const F3 wo = neg(d);
+                    const int pixel = slot % a.npix;
+                    const uint32_t sample = a.sample0 + uint32_t(slot / a.npix);
                     const float4 ua = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 0u);
-                    float fac_next = 1.0f;
-                    if (mat_needs_nee(m) && sv.n_lights > 0) {  // rendererptls.cpp:50-77
-                        const LightRec lr = sv.lights[pick_light(sv, ua.x)];
+                    // NEE direction (rendererptls.cpp:50-69): light pick, point on it, wl
+                    const bool do_nee = mat_needs_nee(m) && sv.n_lights > 0;
+                    F3 wl = f3(0, 0, 0), lv = f3(0, 0, 0), ke = f3(0, 0, 0);
+                    float lcos = 0.0f, ncos = 0.0f;
+                    if (do_nee) {
+                        const int li = cdf_lds ? pick_light_lds(s_cdf, sv.n_lights, ua.x) : pick_light(sv, ua.x);
+                        const LightRec lr = sv.lights[li];
                         float r1 = ua.y, r2 = ua.z;
                         if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
                         const float lw = 1.0f - r1 - r2;
@@ -196,19 +248,47 @@ __global__ __launch_bounds__(BLK) void k_shade(ShadeArgs a) {
                         const F3 ln = normalized(add(add(mul(f3(lr.n0.x, lr.n0.y, lr.n0.z), lw),
                                                          mul(f3(lr.n1.x, lr.n1.y, lr.n1.z), r1)),
                                                      mul(f3(lr.n2.x, lr.n2.y, lr.n2.z), r2)));
-                        const F3 lv = sub(lp, hp);
-                        const F3 wl = normalized(lv);
+                        lv = sub(lp, hp);
+                        wl = normalized(lv);
+                        lcos = dot(ln, neg(wl));
+                        ncos = dot(nrm, normalized(sub(lp, hp)));
+                        ke = f3(lr.ke.x, lr.ke.y, lr.ke.z);
+                    }
+                    // Russian roulette and the continuation direction (rendererptls.cpp:80-88)
+                    bool do_next = true;
+                    float prr = 1.0f;
+                    if (depth >= a.rrdepth) {
+                        do_next = !(ua.w > a.prr);
+                        prr = a.prr;
+                    }
+                    if (a.maxdepth > 0 && depth + 1 >= a.maxdepth) do_next = false;
+                    F3 wi = f3(0, 0, 0);
+                    if (do_next) {
+                        const float4 ub = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 1u);
+                        wi = bsdf_sample<F>(m, ub.x, ub.y, ub.z, wo, nrm);
+                    }
+                    // f and pdf for both directions through one copy of the material code
+                    F3 f_l = f3(0, 0, 0), f_i = f3(0, 0, 0);
+                    float p_l = 1.0f, p_i = 1.0f;
+#pragma unroll 1
+                    for (int k = 0; k < 2; ++k) {
+                        const bool act = k == 0 ? do_nee : do_next;
+                        if (!act) continue;
+                        const F3 dir = k == 0 ? wl : wi;
+                        const F3 fr = bsdf_eval<F>(sv, m, wo, nrm, dir, tu, tv);
+                        const float pr = bsdf_pdf<F>(m, wo, nrm, dir);
+                        if (k == 0) f_l = fr, p_l = pr;
+                        else f_i = fr, p_i = pr;
+                    }
+                    float fac_next = 1.0f;
+                    if (do_nee) {  // rendererptls.cpp:61-77
                         const float lpdf = sv.light_p;
-                        const float lcos = dot(ln, neg(wl));
                         const float lop = lpdf * dot(lv, lv) / ((1e-6f < lcos) ? lcos : 1e-6f);
-                        const float bop = bsdf_pdf(m, wo, nrm, wl);
-                        float mis = lop / (lop + bop);
+                        float mis = lop / (lop + p_l);
                         if (a.mis == 0) mis = 0.5f;
                         fac_next = 1.0f - mis;
-                        const F3 f = bsdf_eval(sv, m, wo, nrm, wl, tu, tv);
-                        const float ncos = dot(nrm, normalized(sub(lp, hp)));
-                        const F3 Ll = mul(divs(f3(lr.ke.x, lr.ke.y, lr.ke.z), dot(lv, lv)), (0.0f < lcos ? lcos : 0.0f) / lpdf);
-                        const F3 c = mul(mulv(mulv(beta, Ll), f), (0.0f < ncos ? ncos : 0.0f) * mis);
+                        const F3 Ll = mul(divs(ke, dot(lv, lv)), (0.0f < lcos ? lcos : 0.0f) / lpdf);
+                        const F3 c = mul(mulv(mulv(beta, Ll), f_l), (0.0f < ncos ? ncos : 0.0f) * mis);
                         if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
                             emit_sh = true;
                             const F3 so = add(hp, mul(wl, 1e-3f));
@@ -217,52 +297,56 @@ __global__ __launch_bounds__(BLK) void k_shade(ShadeArgs a) {
                             sh_c = make_float4(c.x, c.y, c.z, 0.0f);
                         }
                     }
-                    bool survive = true;
-                    float prr = 1.0f;
-                    if (depth >= a.rrdepth) {  // rendererptls.cpp:80-83
-                        survive = !(ua.w > a.prr);
-                        prr = a.prr;
-                    }
-                    if (survive && (a.maxdepth <= 0 || depth + 1 < a.maxdepth)) {  // rendererptls.cpp:85-91
-                        const float4 ub = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 1u);
-                        const F3 wi = bsdf_sample(m, ub.x, ub.y, ub.z, wo, nrm);
-                        const float pdf = bsdf_pdf(m, wo, nrm, wi);
-                        const F3 f = bsdf_eval(sv, m, wo, nrm, wi, tu, tv);
+                    if (do_next) {  // rendererptls.cpp:86-91
                         const float c = fabsf(dot(wi, nrm));
-                        const F3 nb = f3(beta.x * c * f.x / pdf / prr, beta.y * c * f.y / pdf / prr,
-                                         beta.z * c * f.z / pdf / prr);
+                        const F3 nb = f3(beta.x * c * f_i.x / p_i / prr, beta.y * c * f_i.y / p_i / prr,
+                                         beta.z * c * f_i.z / p_i / prr);
                         if (nb.x != 0.0f || nb.y != 0.0f || nb.z != 0.0f) {
                             emit_next = true;
                             const F3 no = add(hp, mul(wi, 1e-3f));
-                            next_ray.o = make_float4(no.x, no.y, no.z, 0.0f);
+                            next_ray.o = make_float4(no.x, no.y, no.z, __int_as_float(depth + 1));
                             next_ray.d = make_float4(wi.x, wi.y, wi.z, __int_as_float(slot));
-                            st.beta_fac = make_float4(nb.x, nb.y, nb.z, fac_next);
+                            next_b = make_float4(nb.x, nb.y, nb.z, fac_next);
                         }
                     }
                 }
             }
-            st.L_depth = make_float4(L.x, L.y, L.z, __int_as_float(depth + 1));
-            a.state[slot] = st;
+            if (Le.x != 0.0f || Le.y != 0.0f || Le.z != 0.0f) {
+                float4 L = a.radiance[slot];
+                L.x += Le.x;
+                L.y += Le.y;
+                L.z += Le.z;
+                a.radiance[slot] = L;
+            }
         }
-        const int ps = wave_append(emit_sh, a.n_shadow, lane);
+        int ps, pn;
+        block_append2(s_app, parity, emit_sh, emit_next, a.n_shadow, a.n_out, ps, pn);
         if (emit_sh) {
             a.shadow[ps] = sh_ray;
             a.contrib[ps] = sh_c;
         }
-        const int pn = wave_append(emit_next, a.n_out, lane);
-        if (emit_next) a.rout[pn] = next_ray;
+        if (emit_next) {
+            a.rout[pn] = next_ray;
+            a.bout[pn] = next_b;
+        }
     }
 }
 
+// Lambert / GGX-reflect / light scenes without textures: 96 VGPRs, 5 waves per SIMD, no spills.
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_shade_basic(ShadeArgs a) {
+    shade_body<0u>(a);
+}
+__global__ __launch_bounds__(BLK) void k_shade_full(ShadeArgs a) { shade_body<F_ALL>(a); }
+
 // Per-pixel reduction of the batch's kb samples: sums of max(0, L) and its square in FP64
 // (rendererptls.cpp:173-175: max(0, trace(...)) per component, which also maps NaN to 0).
-__global__ void k_finalize(const PathState* __restrict__ state, int npix, int kb, double* __restrict__ sum,
+__global__ void k_finalize(const float4* __restrict__ radiance, int npix, int kb, double* __restrict__ sum,
                            double* __restrict__ sum2) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npix) return;
     double s[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
     for (int k = 0; k < kb; ++k) {
-        const float4 L = state[size_t(k) * npix + p].L_depth;
+        const float4 L = radiance[size_t(k) * npix + p];
         const float v[3] = {0.0f < L.x ? L.x : 0.0f, 0.0f < L.y ? L.y : 0.0f, 0.0f < L.z ? L.z : 0.0f};
         for (int c = 0; c < 3; ++c) {
             s[c] += double(v[c]);
@@ -318,7 +402,7 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
     auto t0 = std::chrono::steady_clock::now();
     RenderStats rs;
     int status = 0;
-    std::vector<hipEvent_t> ev_c, ev_s;  // start/stop pairs per traversal launch
+    std::vector<hipEvent_t> ev_c, ev_s, ev_h;  // start/stop pairs per launch
     auto mark = [&](std::vector<hipEvent_t>& v) {
         hipEvent_t e;
         check_hip(hipEventCreate(&e), "hipEventCreate");
@@ -330,8 +414,9 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
         const int k = std::min(kb, p.spp - b);
         const int P = k * npix;
         hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, int(C_RAY0), P, -1);
-        hipLaunchKernelGGL(k_raygen, dim3((P + BLK - 1) / BLK), dim3(BLK), 0, s, cam, p.W, npix, P, p.seed,
-                           uint32_t(p.spp_offset + b), t.rays[0].as<Ray>(), t.state.as<PathState>());
+        hipLaunchKernelGGL(k_raygen, dim3((P + BLK - 1) / BLK), dim3(BLK), 0, s, cam, p.W, p.H, npix, P, p.seed,
+                           uint32_t(p.spp_offset + b), t.rays[0].as<Ray>(), t.beta[0].as<float4>(),
+                           t.radiance.as<float4>());
         for (int it = 0;; ++it) {
             const int cur = it & 1, nxt = cur ^ 1;
             hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, -1, 0, int(C_RAY0 + nxt));
@@ -340,13 +425,16 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
                            cnt + C_ERR, ts, s);
             if (kt) mark(ev_c);
             ShadeArgs a{sc.view, npix, p.seed, uint32_t(p.spp_offset + b), p.prr, p.mis, p.maxdepth, p.rrdepth,
-                        t.rays[cur].as<Ray>(), t.hits.as<HitRec>(), cnt + C_RAY0 + cur, t.state.as<PathState>(),
-                        t.rays[nxt].as<Ray>(), cnt + C_RAY0 + nxt, t.shadow.as<Ray>(), t.contrib.as<float4>(),
-                        cnt + C_SHADOW};
-            hipLaunchKernelGGL(k_shade, dim3(grid_shade), dim3(BLK), 0, s, a);
+                        t.rays[cur].as<Ray>(), t.beta[cur].as<float4>(), t.hits.as<HitRec>(), cnt + C_RAY0 + cur,
+                        t.radiance.as<float4>(), t.rays[nxt].as<Ray>(), t.beta[nxt].as<float4>(), cnt + C_RAY0 + nxt,
+                        t.shadow.as<Ray>(), t.contrib.as<float4>(), cnt + C_SHADOW};
+            if (kt) mark(ev_h);
+            if (sc.features == 0) hipLaunchKernelGGL(k_shade_basic, dim3(grid_shade), dim3(BLK), 0, s, a);
+            else hipLaunchKernelGGL(k_shade_full, dim3(grid_shade), dim3(BLK), 0, s, a);
+            if (kt) mark(ev_h);
             if (kt) mark(ev_s);
             launch_shadow(sc.view, t.shadow.as<Ray>(), cnt + C_SHADOW, nullptr, nullptr, t.contrib.as<float4>(),
-                          t.state.as<PathState>(), cnt + C_WORK1, cnt + C_ERR, ts ? ts + 1 : nullptr, s);
+                          t.radiance.as<float4>(), cnt + C_WORK1, cnt + C_ERR, ts ? ts + 1 : nullptr, s);
             if (kt) mark(ev_s);
             rs.iterations++;
             if (p.maxdepth > 0) {
@@ -359,8 +447,8 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
             }
         }
         if (status) break;
-        hipLaunchKernelGGL(k_finalize, dim3((npix + BLK - 1) / BLK), dim3(BLK), 0, s, t.state.as<PathState>(), npix, k,
-                           t.sum.as<double>(), t.sum2.as<double>());
+        hipLaunchKernelGGL(k_finalize, dim3((npix + BLK - 1) / BLK), dim3(BLK), 0, s, t.radiance.as<float4>(), npix,
+                           k, t.sum.as<double>(), t.sum2.as<double>());
         check_hip(hipGetLastError(), "ptnee launch");
         t.spp_total += k;
         rs.samples += (long long)P;
@@ -375,21 +463,20 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
     rs.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (kt) {
         *kt = KernelTimes{};
-        for (size_t i = 0; i + 1 < ev_c.size(); i += 2) {
-            float ms = 0;
-            check_hip(hipEventElapsedTime(&ms, ev_c[i], ev_c[i + 1]), "elapsed");
-            kt->closest_ms += ms;
-            kt->closest_launches++;
-        }
-        for (size_t i = 0; i + 1 < ev_s.size(); i += 2) {
-            float ms = 0;
-            check_hip(hipEventElapsedTime(&ms, ev_s[i], ev_s[i + 1]), "elapsed");
-            kt->shadow_ms += ms;
-            kt->shadow_launches++;
-        }
+        auto total = [](std::vector<hipEvent_t>& v, double& ms, int& launches) {
+            for (size_t i = 0; i + 1 < v.size(); i += 2) {
+                float x = 0;
+                check_hip(hipEventElapsedTime(&x, v[i], v[i + 1]), "elapsed");
+                ms += x;
+                launches++;
+            }
+        };
+        total(ev_c, kt->closest_ms, kt->closest_launches);
+        total(ev_s, kt->shadow_ms, kt->shadow_launches);
+        total(ev_h, kt->shade_ms, kt->shade_launches);
     }
-    for (hipEvent_t e : ev_c) (void)hipEventDestroy(e);
-    for (hipEvent_t e : ev_s) (void)hipEventDestroy(e);
+    for (auto* v : {&ev_c, &ev_s, &ev_h})
+        for (hipEvent_t e : *v) (void)hipEventDestroy(e);
     if (ts && tstats) check_hip(hipMemcpy(tstats, ts, 2 * sizeof(TraceStats), hipMemcpyDeviceToHost), "stats");
     if (st) *st = rs;
     return status;

@@ -32,15 +32,17 @@ struct RenderStats {
 
 // HIP-event timing of the traversal launches of one render (on the scene's stream).
 struct KernelTimes {
-    double closest_ms = 0, shadow_ms = 0;
-    int closest_launches = 0, shadow_launches = 0;
+    double closest_ms = 0, shadow_ms = 0, shade_ms = 0;
+    int closest_launches = 0, shadow_launches = 0, shade_launches = 0;
 };
 
 struct RenderTarget {
     int W = 0, H = 0;
     size_t cap = 0;
     DevBuf sum, sum2;                  // double, W*H*3
-    DevBuf rays[2], hits, shadow, contrib, state, occ;
+    DevBuf rays[2], beta[2];           // extension-ray queues and their (throughput, MIS factor)
+    DevBuf hits, shadow, contrib;      // closest hits; shadow rays and their NEE contributions
+    DevBuf radiance;                   // float4 per path slot
     DevBuf counters;                   // int[16]
     DevBuf stats;                      // TraceStats[2]
     int* pinned = nullptr;             // host-visible counter readback

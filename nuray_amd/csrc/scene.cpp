@@ -87,6 +87,12 @@ void Scene::build(int nthreads) {
         const int type = int(mats[8 * j]);
         if (type < MAT_LAMBERT || type > MAT_GGX) throw std::runtime_error("unknown material type");
     }
+    features = 0;
+    for (int i = 0; i < N; ++i) {
+        const float* m = &mats[8 * size_t(mat[i])];
+        if (int(m[0]) == MAT_GGX) features |= 1u;
+        if (int(m[6]) >= 0) features |= 2u;
+    }
     if (env_tex >= 0 && (env_tex >= int(textures.size()) || textures[env_tex].w <= 0))
         throw std::runtime_error("environment map missing");
     bvh = build_sah(p.data(), N, nthreads);

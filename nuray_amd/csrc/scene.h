@@ -40,6 +40,7 @@ public:
     std::vector<Texture> textures;
     int env_tex = -1;
     float env_gain = 1.0f;
+    unsigned features = 0;        // material features present: 1 = GGX dielectric, 2 = albedo textures
 
     // derived host data
     BvhBuild bvh;

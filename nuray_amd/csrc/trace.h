@@ -4,11 +4,9 @@
 
 namespace nr {
 
-// Per camera-sample path state, indexed by path slot.
-struct alignas(16) PathState {
-    float4 beta_fac;  // throughput xyz, MIS factor carried to the next emitter hit (w)
-    float4 L_depth;   // radiance xyz, vertex depth (w, int bits)
-};
+// Path state of a queued extension ray lives beside it: Ray.o.w = vertex depth (int bits),
+// Ray.d.w = path slot (int bits), and a float4 (throughput xyz, MIS factor w) at the same queue
+// position.  Only the radiance accumulator is indexed by path slot (float4, xyz used).
 
 struct TraceStats {
     unsigned long long rays, pair_fetch, tri_tests, pad;
@@ -20,9 +18,9 @@ void launch_closest(const SceneView& sv, const Ray* rays, const int* n, HitRec* 
 // Shadow test for rays[0 .. *n) with segment length rays[i].o.w: occluded iff some accepted
 // triangle has (double)t + 2e-3 <= (double)len (rendererptls.cpp:70-71).  Writes occ[i] (0/1)
 // and the first occluder found (prim / t) when `hits` is non-null; when `contrib` is non-null
-// a visible ray adds contrib[i].xyz to state[slot].L, slot = bits of rays[i].d.w.
+// a visible ray adds contrib[i].xyz to radiance[slot], slot = bits of rays[i].d.w.
 void launch_shadow(const SceneView& sv, const Ray* rays, const int* n, uint8_t* occ, HitRec* hits,
-                   const float4* contrib, PathState* state, int* work, int* err, TraceStats* stats, hipStream_t s);
+                   const float4* contrib, float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s);
 
 int device_cu_count();
 

@@ -10,9 +10,11 @@
 //     "later leaf" is the one with the larger first-primitive index, which makes the winner
 //     independent of the order this kernel visits nodes in.
 //
-// Execution: persistent waves; lane 0 of each wave claims 64 rays with one atomic, every lane
-// traverses one ray with a private stack kept in LDS at stack[slot * blockDim + tid] (one
-// 4-byte bank per lane, conflict-free).  The stack depth is the BVH depth + 2, checked on
+// Execution: persistent waves.  Every lane traverses one ray with a private stack kept in LDS at
+// stack[slot * blockDim + tid] (one 4-byte bank per lane, conflict-free).  Internal nodes are
+// expanded until a lane holds a leaf, then leaves are tested together; a lane whose ray is done
+// goes idle, and once REFILL_MIN lanes are idle one atomic claims new rays for all of them
+// (ballot + prefix count), so long rays do not hold a whole wave hostage.  The stack depth is the BVH depth + 2, checked on
 // every push; an overflow is reported, never written past.
 #include "trace.h"
 #include "scene.h"
@@ -23,48 +25,84 @@ namespace {
 
 constexpr int BLOCK = 256;
 
+// Lanes whose ray has finished are refilled when at least this many are idle.  A wave claims
+// rays from the global counter CLAIM at a time and hands them out from that private range.
+constexpr int REFILL_MIN = 16;
+constexpr int CLAIM = 128;
+
 template <bool ANY, bool STATS>
 __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const Ray* __restrict__ rays,
                                                   const int* __restrict__ n_ptr, HitRec* __restrict__ hits,
                                                   uint8_t* __restrict__ occ, const float4* __restrict__ contrib,
-                                                  PathState* __restrict__ state, int* __restrict__ work,
+                                                  float4* __restrict__ radiance, int* __restrict__ work,
                                                   int* __restrict__ err, TraceStats* __restrict__ stats) {
     extern __shared__ int stk[];
     const int tid = threadIdx.x;
     const int bdim = blockDim.x;
     const int lane = tid & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
     const int cap = sv.stack_cap;
     const int n = *n_ptr;
+
+    // per-lane ray state; i < 0 means the lane is idle
+    int i = -1;
+    F3 o = f3(0, 0, 0), d = f3(0, 0, 0);
+    SlabRay sr;
+    float wr = 0.0f;  // ray payload word (segment length / path slot bits)
+    float pw = 0.0f;
+    float bt = 1e18f, bb1 = 0.0f, bb2 = 0.0f;
+    int bprim = -1, blf = -1, sp = 0;
+    int32_t leaf = 0;
+    bool has_leaf = false, done = false, overflow = false;
+    uint32_t n_pf = 0, n_tt = 0;
+    bool drained = false;  // wave-uniform: the global counter is exhausted
+    int rb = 0, re = 0;    // wave-uniform: claimed, not yet assigned ray indices [rb, re)
+
     for (;;) {
-        int base = 0;
-        if (lane == 0) base = atomicAdd(work, 64);
-        base = __shfl(base, 0, 64);
-        if (base >= n) break;
-        const int i = base + lane;
-        if (i >= n) continue;
-
-        const Ray r = rays[i];
-        const F3 o = f3(r.o.x, r.o.y, r.o.z);
-        const F3 d = f3(r.d.x, r.d.y, r.d.z);
-        const SlabRay sr = slab_ray(o, d);
-        const double len = double(r.o.w);
-        // closest hit: running winner; any hit: `best` caps the triangle filter at the segment
-        float bt = ANY ? fminf(r.o.w, 1e18f) : 1e18f, bb1 = 0.0f, bb2 = 0.0f;
-        int bprim = -1, blf = -1;
-        bool done = false, overflow = false;
-        uint32_t n_pf = 0, n_tt = 0;
-
-        // The stack holds internal-node and leaf references.  Internal nodes are expanded
-        // until this lane holds a leaf (or runs dry); the wave then tests its leaves together.
-        int sp = 0;
-        int32_t leaf = 0;
-        bool has_leaf = false;
-        if (slab_fast(sv.root_min[0], sv.root_min[1], sv.root_min[2], sv.root_max[0], sv.root_max[1],
-                      sv.root_max[2], sr)) {
-            if (ref_is_leaf(sv.root_ref)) leaf = sv.root_ref, has_leaf = true;
-            else stk[tid] = sv.root_ref, sp = 1;
+        // ---- refill idle lanes from the wave's claimed range; claim CLAIM more when it is empty
+        const unsigned long long idle_mask = __ballot(i < 0);
+        const int n_idle = __popcll(idle_mask);
+        if ((n_idle >= REFILL_MIN || n_idle == 64) && !(drained && rb == re)) {
+            if (rb == re) {
+                int base = 0;
+                if (lane == 0) base = atomicAdd(work, CLAIM);
+                base = __shfl(base, 0, 64);
+                if (base >= n) drained = true;
+                else rb = base, re = min(base + CLAIM, n);
+            }
+            const int take = min(n_idle, re - rb);
+            if (i < 0) {
+                const int rank = __popcll(idle_mask & below);
+                const int k = rb + rank;
+                if (rank < take) {
+                    i = k;
+                    const Ray r = rays[k];
+                    o = f3(r.o.x, r.o.y, r.o.z);
+                    d = f3(r.d.x, r.d.y, r.d.z);
+                    sr = slab_ray(o, d);
+                    wr = r.o.w;
+                    pw = r.d.w;
+                    bt = ANY ? fminf(wr, 1e18f) : 1e18f;
+                    bb1 = bb2 = 0.0f;
+                    bprim = blf = -1;
+                    sp = 0;
+                    has_leaf = done = overflow = false;
+                    n_pf = n_tt = 0;
+                    if (slab_fast(sv.root_min[0], sv.root_min[1], sv.root_min[2], sv.root_max[0], sv.root_max[1],
+                                  sv.root_max[2], sr)) {
+                        if (ref_is_leaf(sv.root_ref)) leaf = sv.root_ref, has_leaf = true;
+                        else stk[tid] = sv.root_ref, sp = 1;
+                    }
+                }
+            }
+            rb += take;
         }
-        for (;;) {
+        if (__ballot(i >= 0) == 0ull) {
+            if (drained && rb == re) break;
+            continue;
+        }
+        if (i >= 0) {
+            // ---- expand internal nodes until this lane holds a leaf (or runs dry)
             while (sp > 0 && !has_leaf) {
                 --sp;
                 const int32_t e = stk[sp * bdim + tid];
@@ -79,68 +117,71 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const Ray* __rest
                 if (STATS) ++n_pf;
                 const bool hl = slab_fast(a.x, a.y, a.z, a.w, b.x, b.y, sr);
                 const bool hr = slab_fast(b.z, b.w, c.x, c.y, c.z, c.w, sr);
-                // a hit leaf child is taken at once; everything else is pushed
                 int32_t push0 = 0, push1 = 0;
-                int np_ = 0;
+                int npush = 0;
                 if (hl) {
                     if (ref_is_leaf(ch.x)) leaf = ch.x, has_leaf = true;
-                    else push0 = ch.x, np_ = 1;
+                    else push0 = ch.x, npush = 1;
                 }
                 if (hr) {
                     if (ref_is_leaf(ch.y) && !has_leaf) leaf = ch.y, has_leaf = true;
-                    else if (np_ == 0) push0 = ch.y, np_ = 1;
-                    else push1 = ch.y, np_ = 2;
+                    else if (npush == 0) push0 = ch.y, npush = 1;
+                    else push1 = ch.y, npush = 2;
                 }
-                if (sp + np_ > cap) {
+                if (sp + npush > cap) {
                     overflow = true;
                     sp = 0;
                     has_leaf = false;
                     break;
                 }
-                if (np_ >= 1) stk[sp * bdim + tid] = push0;
-                if (np_ == 2) stk[(sp + 1) * bdim + tid] = push1;
-                sp += np_;
+                if (npush >= 1) stk[sp * bdim + tid] = push0;
+                if (npush == 2) stk[(sp + 1) * bdim + tid] = push1;
+                sp += npush;
             }
-            if (!has_leaf) break;
-            has_leaf = false;
-            // Leaf loop, bvhnode.cpp:50-66, merged into the running winner with the DFS tie rule.
-            const int first = ref_first(leaf), cnt = ref_count(leaf);
-            if (STATS) n_tt += cnt;
-            for (int k = 0; k < cnt; ++k) {
-                const TriGeom g = sv.tris[first + k];
-                float t, b1, b2;
-                if (!tri_fast(g, o, d, bt, t, b1, b2)) continue;
-                if (ANY) {
-                    if (double(t) + 2e-3 <= len) {
-                        bt = t; bb1 = b1; bb2 = b2; bprim = first + k;
-                        done = true;
-                        break;
+            // ---- leaf loop, bvhnode.cpp:50-66, merged with the DFS tie rule
+            if (has_leaf) {
+                has_leaf = false;
+                const int first = ref_first(leaf), cnt = ref_count(leaf);
+                if (STATS) n_tt += cnt;
+                for (int k = 0; k < cnt; ++k) {
+                    const TriGeom g = sv.tris[first + k];
+                    float t, b1, b2;
+                    if (!tri_fast(g, o, d, bt, t, b1, b2)) continue;
+                    if (ANY) {
+                        if (double(t) + 2e-3 <= double(wr)) {
+                            bt = t; bb1 = b1; bb2 = b2; bprim = first + k;
+                            done = true;
+                            break;
+                        }
+                    } else if (t < bt || first > blf) {  // t <= bt here: equal t -> later leaf wins
+                        bt = t; bb1 = b1; bb2 = b2; bprim = first + k; blf = first;
                     }
-                } else if (t < bt || first > blf) {  // t <= bt here: equal t -> later leaf wins
-                    bt = t; bb1 = b1; bb2 = b2; bprim = first + k; blf = first;
                 }
             }
-            if (ANY && done) break;
-        }
-        if (overflow) atomicOr(err, 1);
-        if (STATS) {
-            atomicAdd(&stats->rays, 1ull);
-            atomicAdd(&stats->pair_fetch, (unsigned long long)n_pf);
-            atomicAdd(&stats->tri_tests, (unsigned long long)n_tt);
-        }
-        if (!ANY) {
-            hits[i] = HitRec{bt, bb1, bb2, bprim};
-        } else {
-            if (occ) occ[i] = done ? 1 : 0;
-            if (hits) hits[i] = HitRec{done ? bt : 1e18f, bb1, bb2, done ? bprim : -1};
-            if (contrib && !done) {
-                const float4 cv = contrib[i];
-                const int slot = __float_as_int(r.d.w);
-                float4 L = state[slot].L_depth;
-                L.x += cv.x;
-                L.y += cv.y;
-                L.z += cv.z;
-                state[slot].L_depth = L;
+            // ---- finished: write the result, go idle
+            if ((sp == 0 && !has_leaf) || done || overflow) {
+                if (overflow) atomicOr(err, 1);
+                if (STATS) {
+                    atomicAdd(&stats->rays, 1ull);
+                    atomicAdd(&stats->pair_fetch, (unsigned long long)n_pf);
+                    atomicAdd(&stats->tri_tests, (unsigned long long)n_tt);
+                }
+                if (!ANY) {
+                    hits[i] = HitRec{bt, bb1, bb2, bprim};
+                } else {
+                    if (occ) occ[i] = done ? 1 : 0;
+                    if (hits) hits[i] = HitRec{done ? bt : 1e18f, bb1, bb2, done ? bprim : -1};
+                    if (contrib && !done) {
+                        const float4 cv = contrib[i];
+                        const int slot = __float_as_int(pw);
+                        float4 L = radiance[slot];
+                        L.x += cv.x;
+                        L.y += cv.y;
+                        L.z += cv.z;
+                        radiance[slot] = L;
+                    }
+                }
+                i = -1;
             }
         }
     }
@@ -162,15 +203,15 @@ int grid_for(size_t shmem) {
 
 template <bool ANY>
 void launch(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, uint8_t* occ, const float4* contrib,
-            PathState* state, int* work, int* err, TraceStats* stats, hipStream_t s) {
+            float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s) {
     const size_t shmem = size_t(sv.stack_cap) * BLOCK * sizeof(int);
     if (shmem > 160 * 1024) throw std::runtime_error("BVH too deep for the LDS traversal stack");
     if (stats) {
         hipLaunchKernelGGL((k_trace<ANY, true>), dim3(grid_for<ANY, true>(shmem)), dim3(BLOCK), shmem, s, sv, rays, n,
-                           hits, occ, contrib, state, work, err, stats);
+                           hits, occ, contrib, radiance, work, err, stats);
     } else {
         hipLaunchKernelGGL((k_trace<ANY, false>), dim3(grid_for<ANY, false>(shmem)), dim3(BLOCK), shmem, s, sv, rays,
-                           n, hits, occ, contrib, state, work, err, stats);
+                           n, hits, occ, contrib, radiance, work, err, stats);
     }
     check_hip(hipGetLastError(), "k_trace launch");
 }
@@ -193,8 +234,8 @@ void launch_closest(const SceneView& sv, const Ray* rays, const int* n, HitRec* 
 }
 
 void launch_shadow(const SceneView& sv, const Ray* rays, const int* n, uint8_t* occ, HitRec* hits,
-                   const float4* contrib, PathState* state, int* work, int* err, TraceStats* stats, hipStream_t s) {
-    launch<true>(sv, rays, n, hits, occ, contrib, state, work, err, stats, s);
+                   const float4* contrib, float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s) {
+    launch<true>(sv, rays, n, hits, occ, contrib, radiance, work, err, stats, s);
 }
 
 }  // namespace nr
