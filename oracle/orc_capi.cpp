@@ -186,6 +186,13 @@ int orc_render(void* h, const float* cam, int W, int H, int x0, int y0, int x1, 
 }
 
 // ---- small probes used by the oracle's own pinning tests
+int orc_slab(const float* mn, const float* mx, const float* o, const float* d) {
+    return slab_hit<float>(mn, mx, V3f(o[0], o[1], o[2]), V3f(d[0], d[1], d[2])) ? 1 : 0;
+}
+void orc_tri_test(const float* p, const float* o, const float* d, float* out3) {
+    V3f p0(p[0], p[1], p[2]), p1(p[3], p[4], p[5]), p2(p[6], p[7], p[8]);
+    tri_test_f(p0, p1 - p0, p2 - p0, V3f(o[0], o[1], o[2]), V3f(d[0], d[1], d[2]), out3[0], out3[1], out3[2]);
+}
 void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) { philox4x32_10(ctr, key, out); }
 float orc_srgb8_decode(int c) { return srgb8_decode(c); }
 

@@ -54,6 +54,8 @@ def lib():
         L.orc_camera_rays.argtypes = [f32p, C.c_int, f32p, f32p, C.c_float, C.c_float, f32p]
         L.orc_texture_lookup.argtypes = [vp, C.c_int, C.c_int, f32p, f32p, f32p]
         L.orc_material_eval.argtypes = [vp, C.c_int, C.c_int, f32p, f32p, f32p, f32p, C.c_int, f32p, f32p, f32p]
+        L.orc_slab.argtypes = [f32p, f32p, f32p, f32p]
+        L.orc_tri_test.argtypes = [f32p, f32p, f32p, f32p]
         L.orc_light_pick.argtypes = [vp, C.c_float]
         L.orc_light_p.argtypes = [vp]
         L.orc_light_p.restype = C.c_float
@@ -152,6 +154,16 @@ class Scene:
 def philox(ctr, key):
     out = np.zeros(4, np.uint32)
     lib().orc_philox(np.asarray(ctr, np.uint32), np.asarray(key, np.uint32), out)
+    return out
+
+
+def slab(mn, mx, o, d):
+    return bool(lib().orc_slab(_f32(mn), _f32(mx), _f32(o), _f32(d)))
+
+
+def tri_test(p9, o, d):
+    out = np.zeros(3, np.float32)
+    lib().orc_tri_test(_f32(p9), _f32(o), _f32(d), out)
     return out
 
 

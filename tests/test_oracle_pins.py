@@ -1,0 +1,399 @@
+"""Pins of the oracle (the CPU restatement of the reference's ptnee path) that do not rely on the
+oracle itself: published known-answer vectors, closed forms, invariants, brute force, and an
+independent NumPy restatement of the SAH split rule.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+
+# ------------------------------------------------------------------ Philox4x32-10
+# Random123 known-answer vectors (kat_vectors, philox4x32 10 rounds).
+PHILOX_KAT = [
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+]
+
+
+@pytest.mark.parametrize("ctr,key,out", PHILOX_KAT)
+def test_philox_known_answers(po, ctr, key, out):
+    assert po.philox(ctr, key).tolist() == out
+
+
+# ------------------------------------------------------------------ texture decode (texture.cpp:18-20)
+def test_srgb8_decode_closed_form(po):
+    vals = np.array([po.srgb8_decode(c) for c in range(256)], np.float64)
+    assert vals[0] == 0.0 and vals[255] == 1.0
+    assert np.all(np.diff(vals) > 0)
+    exact = (np.arange(256) / 255.0) ** 2.2
+    assert np.allclose(vals[1:], exact[1:], rtol=2e-7)  # exponent is the float literal 2.2f
+
+
+# ------------------------------------------------------------------ triangle test (triangle.cpp:27-33)
+def test_triangle_closed_form_and_bias(po):
+    p = [0, 0, 0, 1, 0, 0, 0, 1, 0]
+    t, b1, b2 = po.tri_test(p, [0.25, 0.25, 1.0], [0, 0, -1])
+    # s1.e1 = 1, so q = 1 + 1e-6: every output is scaled by 1 / (1 + 1e-6) (the reference's bias)
+    k = 1.0 / (1.0 + 1e-6)
+    assert abs(t - k) < 2e-7 and t < 1.0
+    assert abs(b1 - 0.25 * k) < 1e-7 and abs(b2 - 0.25 * k) < 1e-7
+    # double-sided: the reversed winding hits too, with q = -1 + 1e-6
+    t2, c1, c2 = po.tri_test([0, 0, 0, 0, 1, 0, 1, 0, 0], [0.25, 0.25, 1.0], [0, 0, -1])
+    k2 = -1.0 / (-1.0 + 1e-6)
+    assert abs(t2 - k2) < 2e-7 and t2 > 1.0
+    # edge: b1 = 0 exactly on the edge p0-p2
+    _, e1, _ = po.tri_test(p, [0.0, 0.5, 1.0], [0, 0, -1])
+    assert e1 == 0.0
+
+
+def test_triangle_bias_grows_for_small_triangles(po):
+    """With |e1 x e2| ~ 1e-6 the +1e-6 term halves t: the quirk is kept, not 'fixed'."""
+    s = 1e-3
+    t, _, _ = po.tri_test([0, 0, 0, s, 0, 0, 0, s, 0], [s / 4, s / 4, 1.0], [0, 0, -1])
+    assert abs(t - 0.5) < 1e-4
+
+
+# ------------------------------------------------------------------ slab test (bvhnode.cpp:35-48)
+def test_slab_cases(po):
+    mn, mx = [0, 0, 0], [1, 1, 1]
+    assert po.slab(mn, mx, [-1, 0.5, 0.5], [1, 0, 0])          # zero components get +1e-9
+    assert not po.slab(mn, mx, [-1, 2.0, 0.5], [1, 0, 0])      # parallel, outside the y slab
+    assert po.slab(mn, mx, [0.5, 0.5, 0.5], [0, 0, 1])         # origin inside
+    assert not po.slab(mn, mx, [0.5, 0.5, 2.0], [0, 0, 1])     # box behind: t_out < 0
+    assert po.slab(mn, mx, [0.5, 0.5, 2.0], [0, 0, -1])
+    # d = -1e-9 makes d + 1e-9 == 0: +-inf quotients, and 0/0 = NaN when the origin lies on the
+    # bound; std::max / std::min keep the first operand on NaN, so the slab is then ignored
+    assert po.slab(mn, mx, [0.5, 0.5, 0.5], [-1e-9, 0, 1])
+    assert po.slab(mn, mx, [0.0, 0.5, 0.5], [-1e-9, 0, 1])
+    assert not po.slab(mn, mx, [2.0, 0.5, 0.5], [-1e-9, 0, 1])
+
+
+# ------------------------------------------------------------------ BVH build (bvh.cpp:5-97)
+def _random_tris(n, seed, scale=100.0):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    c = rng.uniform(0, scale, (n, 1, 3))
+    return (c + rng.normal(0, scale / 30, (n, 3, 3))).reshape(n, 9).astype(np.float32)
+
+
+def _scene(p):
+    n = len(p)
+    return dict(p=p, n=None, uv=None, mat=np.zeros(n, np.int32),
+                mats=np.array([[0, 0.5, 0.5, 0.5, 0.5, 1, -1, 0]], np.float32))
+
+
+def _subtree_prims(ex, i):
+    out = []
+    stack = [i]
+    while stack:
+        v = stack.pop()
+        if ex["left"][v] < 0:
+            f, c = ex["leaf_first"][v], ex["leaf_count"][v]
+            out.extend(ex["prim_order"][f:f + c].tolist())
+        else:
+            stack += [ex["right"][v], ex["left"][v]]
+    return out
+
+
+def _check_invariants(ex, p):
+    n = len(p)
+    order = ex["prim_order"]
+    assert sorted(order.tolist()) == list(range(n))  # every triangle in exactly one leaf
+    P = p.reshape(n, 3, 3)
+    for v in range(len(ex["left"])):
+        if ex["left"][v] < 0:
+            f, c = ex["leaf_first"][v], ex["leaf_count"][v]
+            assert 1 <= c <= 5 or n == 0
+            pts = P[order[f:f + c]].reshape(-1, 3)
+            assert np.array_equal(ex["box"][v, :3], pts.min(0)) and np.array_equal(ex["box"][v, 3:], pts.max(0))
+        else:
+            l, r = ex["left"][v], ex["right"][v]
+            assert np.array_equal(ex["box"][v, :3], np.minimum(ex["box"][l, :3], ex["box"][r, :3]))
+            assert np.array_equal(ex["box"][v, 3:], np.maximum(ex["box"][l, 3:], ex["box"][r, 3:]))
+
+
+def _sah_split_numpy(prims, P, right_offset=0):
+    """Independent restatement of BVH::_divide for inputs without pMax ties: returns the set of
+    primitives that go left.  float32 arithmetic, same operation order as bvh.cpp:18-51; the
+    right count is (n - i) (bvh.cpp:44), or (n - i - right_offset) for the self-check below."""
+    prims = np.asarray(prims)
+    pmin = P[prims].min(1)
+    pmax = P[prims].max(1)
+    n = len(prims)
+    best = (np.float32(1e38), 0, 0)
+    f32 = np.float32
+
+    def sa(lo, hi):
+        d = (hi - lo).astype(np.float32)
+        return (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]) * f32(2)
+
+    for axis in range(3):
+        o = np.argsort(pmax[:, axis], kind="stable")
+        lo_s = np.minimum.accumulate(pmin[o][::-1], 0)[::-1]
+        hi_s = np.maximum.accumulate(pmax[o][::-1], 0)[::-1]
+        lo_p = np.minimum.accumulate(pmin[o], 0)
+        hi_p = np.maximum.accumulate(pmax[o], 0)
+        for i in range(n - 1):
+            cost = sa(lo_p[i], hi_p[i]) * f32(i + 1) + sa(lo_s[i + 1], hi_s[i + 1]) * f32(n - i - right_offset)
+            if cost < best[0]:
+                best = (cost, i + 1, axis)
+    _, mid, axis = best
+    o = np.argsort(pmax[:, axis], kind="stable")
+    return set(prims[o[:mid]].tolist())
+
+
+@pytest.mark.parametrize("n,seed", [(7, 1), (40, 2), (300, 3)])
+def test_sah_split_rule_independent(po, n, seed):
+    """Random triangles have distinct pMax values, so std::sort's order is unique and every split
+    of the oracle's tree must equal the NumPy restatement (incl. the (n - i) right count)."""
+    p = _random_tris(n, seed)
+    assert all(len(np.unique(p.reshape(n, 3, 3).max(1)[:, a])) == n for a in range(3))
+    ex = po.Scene(_scene(p)).export()
+    _check_invariants(ex, p)
+    P = p.reshape(n, 3, 3)
+    for v in range(len(ex["left"])):
+        if ex["left"][v] < 0:
+            continue
+        prims = _subtree_prims(ex, v)
+        assert len(prims) > 5
+        assert set(_subtree_prims(ex, ex["left"][v])) == _sah_split_numpy(prims, P)
+
+
+def test_sah_pin_detects_off_by_one(po):
+    """Self-check of the pin's power: the plausible slip (n - i - 1) for the right count
+    (bvh.cpp:44) changes some splits of this tree, so the test above would catch it."""
+    p = _random_tris(300, 3)
+    n = len(p)
+    P = p.reshape(n, 3, 3)
+    ex = po.Scene(_scene(p)).export()
+    differ = 0
+    for v in range(len(ex["left"])):
+        if ex["left"][v] < 0:
+            continue
+        prims = _subtree_prims(ex, v)
+        if set(_subtree_prims(ex, ex["left"][v])) != _sah_split_numpy(prims, P, right_offset=1):
+            differ += 1
+    assert differ > 0
+
+
+def test_box_room_tree_invariants(po, scenes):
+    sc = scenes.box_room()
+    ex = po.Scene(sc).export()
+    _check_invariants(ex, sc["p"])
+
+
+# ------------------------------------------------------------------ traversal (bvh.cpp:99-137)
+def test_traversal_equals_brute_force(po):
+    """The reference's brute-force loop (renderer.cpp:12-25) finds the same t on generic rays."""
+    p = _random_tris(2000, 5)
+    o = po.Scene(_scene(p))
+    rng = np.random.Generator(np.random.PCG64(6))
+    org = rng.uniform(-20, 120, (20000, 3)).astype(np.float32)
+    d = rng.normal(size=(20000, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    a = o.intersect(org, d)
+    b = o.intersect(org, d, brute=True)
+    assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+    assert np.array_equal(a[0], b[0])
+
+
+def test_traversal_tie_order(po):
+    """Duplicated triangles tie exactly; the winner is the one in the later DFS leaf, and within a
+    leaf the first (bvh.cpp:119-126, bvhnode.cpp:57)."""
+    base = _random_tris(30, 8)
+    p = np.concatenate([base, base, base])
+    o = po.Scene(_scene(p))
+    ex = o.export()
+    pos = np.empty(len(p), np.int64)
+    leaf_of = np.empty(len(p), np.int64)
+    for v in range(len(ex["left"])):
+        if ex["left"][v] < 0:
+            f, c = ex["leaf_first"][v], ex["leaf_count"][v]
+            for k in range(c):
+                pos[ex["prim_order"][f + k]] = f + k
+                leaf_of[ex["prim_order"][f + k]] = f
+    P = p.reshape(-1, 3, 3).astype(np.float64)
+    cen = P.mean(1)
+    org = (cen + np.array([0.0, 0.0, 500.0])).astype(np.float32)
+    d = np.tile(np.array([0, 0, -1], np.float32), (len(p), 1))
+    tri, t, _, _ = o.intersect(org, d)
+    bt, bb, _, _ = o.intersect(org, d, brute=True)
+    assert np.array_equal(t.view(np.uint32), bb.view(np.uint32))
+    for i in range(len(p)):
+        if tri[i] < 0:
+            continue
+        # candidates with the same t: the three copies of the winning triangle at least
+        same = [j for j in range(len(p)) if j % 30 == tri[i] % 30]
+        best = max(same, key=lambda j: (leaf_of[j], -pos[j]))
+        assert tri[i] == best
+
+
+# ------------------------------------------------------------------ camera (camera.cpp:24-32,88-105)
+def test_camera_closed_form(po):
+    W, H = 320.0, 240.0
+    cam = np.array([0, 100, 350, 0, 0, 0, 40, 4 / 3], np.float32)
+    d = po.camera_rays(cam, [160.0, 0.0, 320.0], [120.0, 0.0, 240.0], W, H)
+    assert np.allclose(d[0], [0, 0, -1], atol=1e-7)
+    fh = 2 * math.tan(math.radians(40) / 2)
+    fw = fh * 4 / 3
+    c = np.array([-fw / 2, fh / 2, -1.0]); c /= np.linalg.norm(c)
+    assert np.allclose(d[1], c, atol=2e-7)                 # top-left: y goes down the image
+    c = np.array([fw / 2, -fh / 2, -1.0]); c /= np.linalg.norm(c)
+    assert np.allclose(d[2], c, atol=2e-7)
+    # yaw 90: gaze = (-1, 0, 0); pitch 30: gaze.y = sin 30
+    g = po.camera_rays(np.array([0, 0, 0, 90, 0, 0, 40, 1], np.float32), [0.5], [0.5], 1, 1)[0]
+    assert np.allclose(g, [-1, 0, 0], atol=1e-5)
+    g = po.camera_rays(np.array([0, 0, 0, 0, 30, 0, 40, 1], np.float32), [0.5], [0.5], 1, 1)[0]
+    assert abs(g[1] - 0.5) < 1e-5 and g[2] < 0
+
+
+# ------------------------------------------------------------------ materials
+def _mat_scene(po, rows):
+    p = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32)
+    return po.Scene(dict(p=p, n=None, uv=None, mat=np.zeros(1, np.int32), mats=np.array(rows, np.float32)),
+                    build_bvh=False)
+
+
+def test_lambert_f_pdf_and_sampling(po):
+    """MatLambert (matlambert.cpp): f = Kd/pi on the same side, 0 across; pdf = (|cos| + 1e-4)/pi;
+    sampling has cos^2 uniform on [0.01, 1] (r^2 = 0.99 U)."""
+    s = _mat_scene(po, [[0, 0.5, 0.25, 0.125, 0.5, 1, -1, 0]])
+    n = np.tile([0, 0, 1.0], (4, 1))
+    wo = np.tile([0, 0.6, 0.8], (4, 1))
+    wi = np.array([[0, 0, 1.0], [0.6, 0, 0.8], [0, 0, -1.0], [0.8, 0, -0.6]])
+    _, f, pdf = s.material_eval(0, wo, n, wi=wi)
+    pi = 3.14159
+    assert np.allclose(f[:2], np.array([0.5, 0.25, 0.125]) / pi, rtol=1e-6) and not f[2:].any()
+    assert np.allclose(pdf, (np.abs(wi[:, 2]) + 1e-4) / pi, rtol=1e-6)
+    rng = np.random.Generator(np.random.PCG64(1))
+    k = 200000
+    u = rng.random((k, 3)).astype(np.float32)
+    w, _, _ = s.material_eval(0, np.tile([0, 0.6, 0.8], (k, 1)), np.tile([0, 0, 1.0], (k, 1)), u3=u)
+    c2 = np.sort(w[:, 2].astype(np.float64) ** 2)
+    assert abs(c2.min() - 0.01) < 1e-3 and c2.max() <= 1.0 + 1e-6
+    ks = np.abs(c2 - (0.01 + 0.99 * (np.arange(k) + 0.5) / k)).max()
+    assert ks < 0.01
+
+
+def test_ggx_ndf_normalised_and_sampling(po):
+    """MatGGXRefl: D integrates to 1 against cos(theta_h); pdf = D (n.h)/4 + 1e-8 away from the
+    horizon, 1e18 below it; sampled half vectors follow tan^2(theta_h)/alpha^2 = U/(1-U)."""
+    a = 0.3
+    th = np.linspace(0, np.pi / 2, 200001)
+    c = np.cos(th)
+    D = a * a / 3.14159 / (c * c * (a * a - 1) + 1) ** 2
+    integral = np.trapezoid(D * c * np.sin(th), th) * 2 * np.pi
+    assert abs(integral - 3.14159 / np.pi) < 1e-4   # the reference's pi is 3.14159
+    s = _mat_scene(po, [[2, 0.9, 0.9, 0.9, a, 1, -1, 0]])
+    wo = np.tile([0.0, 0.6, 0.8], (3, 1))
+    n = np.tile([0.0, 0.0, 1.0], (3, 1))
+    wi = np.array([[0.0, -0.6, 0.8], [0.6, 0.0, 0.8], [0, 0.6, -0.8]])
+    _, f, pdf = s.material_eval(0, wo, n, wi=wi)
+    h = (wo[0] + wi[0]) / np.linalg.norm(wo[0] + wi[0])
+    ch = h[2]
+    d0 = a * a / 3.14159 / (ch * ch * (a * a - 1) + 1) ** 2
+    assert abs(pdf[0] - (d0 * ch / 4 + 1e-8)) / pdf[0] < 1e-5
+    assert pdf[2] == np.float32(1e18) and not f[2].any()
+    rng = np.random.Generator(np.random.PCG64(2))
+    k = 100000
+    u = rng.random((k, 3)).astype(np.float32)
+    wos = np.tile([0.0, 0.0, 1.0], (k, 1))
+    w, _, _ = s.material_eval(0, wos, np.tile([0, 0, 1.0], (k, 1)), u3=u)
+    hv = w.astype(np.float64) + wos
+    hv /= np.linalg.norm(hv, axis=1, keepdims=True)
+    t2 = np.sort((1 - hv[:, 2] ** 2) / hv[:, 2] ** 2 / a ** 2)
+    x = t2 / (1 + t2)  # should be ~U (theta scaled by 0.9999)
+    assert np.abs(x - (np.arange(k) + 0.5) / k).max() < 0.01
+
+
+def test_light_sampler_area_weights_and_2x_pdf(po, scenes):
+    """LightSampler: CDF over emitters by area() = |e1 x e2| (twice the area), p() = 1/sum."""
+    sc = scenes.box_room()
+    o = po.Scene(sc, build_bvh=False)
+    assert abs(o.light_p() - 1.0 / 5000.0) < 1e-9   # 50 x 50 light: two triangles of |e1 x e2| = 2500
+    lights = np.nonzero(sc["mats"][sc["mat"]][:, 0] == 1)[0]
+    picks = [o.light_pick(u) for u in np.linspace(0.001, 0.999, 999)]
+    assert set(picks) == set(lights.tolist())
+    assert abs(np.mean(np.array(picks) == lights[0]) - 0.5) < 0.01
+
+
+def test_texture_bilinear_black_border(po):
+    """Texture::pixelUV / pixelBi (texture.cpp:54-72): bilinear between floor and floor+1, texels
+    outside the image read black (no wrap of the +1 neighbour), u and 1-v wrapped into [0, 1)."""
+    img = np.zeros((2, 2, 4), np.uint8)
+    img[0, 0, :3] = 255   # (x0, y0)
+    img[0, 1, 0] = 255    # (x1, y0) red
+    sc = dict(p=np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32), n=None, uv=None, mat=np.zeros(1, np.int32),
+              mats=np.array([[0, 1, 1, 1, 0.5, 1, 0, 0]], np.float32), textures=[img])
+    o = po.Scene(sc, build_bvh=False)
+    # v is flipped: v = 1 -> row 0.  (u, v) = (0, 1) -> x = 0, y = 0: exactly texel (0, 0)
+    out = o.texture_lookup(0, [0.0, 0.25, 0.5, 0.75, 1.25], [1.0, 1.0, 1.0, 1.0, 1.0])
+    assert np.allclose(out[0], [1, 1, 1])
+    assert np.allclose(out[1], [1, 0.5, 0.5])          # x = 0.5: half white + half red
+    assert np.allclose(out[2], [1, 0, 0])              # x = 1: texel (1, 0)
+    assert np.allclose(out[3], [0.5, 0, 0])            # x = 1.5: red fades into the black border
+    assert np.allclose(out[4], out[1])                 # u wraps
+
+
+# ------------------------------------------------------------------ estimator closed forms
+def _plane_scene(albedo, env_value, env_w=1024):
+    R = 1e4
+    p = np.array([[-R, 0, R, R, 0, R, R, 0, -R], [R, 0, -R, -R, 0, -R, -R, 0, R]], np.float32)
+    env = np.full((8, env_w, 3), env_value, np.float32)
+    return dict(p=p, n=None, uv=None, mat=np.zeros(2, np.int32),
+                mats=np.array([[0, albedo, albedo, albedo, 0.5, 1, -1, 0]], np.float32),
+                envmap=dict(rgb=env, gain=1.0))
+
+
+def test_lambert_plane_under_constant_sky(po):
+    """A Lambert plane of albedo rho under a constant environment E, seen from above: the camera
+    ray hits the plane, the BSDF-sampled ray escapes to the sky.  With no emitters there is no
+    NEE; RR divides by prr.  The expectation is E rho E[c/(c+1e-4)] with c^2 ~ U[0.01, 1]
+    (matlambert.cpp:7,36), times (1 - 0.5/W) for the black border at the u = 1 seam of a W-wide
+    environment texture (texture.cpp:54-67)."""
+    rho, E, Wenv = 0.6, 2.0, 1024
+    o = po.Scene(_plane_scene(rho, E, Wenv))
+    cam = np.array([0, 10, 0, 0, -90, 0, 10, 1], np.float32)
+    spp = 20000
+    s, s2 = o.render(cam, 4, 4, spp, seed=3)
+    cg = np.linspace(0.1, 1, 2000001)
+    ratio = np.trapezoid(2 * cg / 0.99 * cg / (cg + 1e-4), cg)
+    expect = E * rho * ratio * (1 - 0.5 / Wenv)
+    N = 16 * spp
+    mean = s.sum() / (3 * N)
+    var = s2.sum() / (3 * N) - mean ** 2
+    z = (mean - expect) / math.sqrt(var / N)
+    assert abs(z) < 5, (mean, expect)
+
+
+def test_nee_twice_weight_with_fixed_mis(po):
+    """mis = 0 fixes the MIS weight to 0.5 (rendererptls.cpp:65-68).  With LightSampler::p() being
+    half the true area pdf (triangle.cpp:57-60), the NEE term estimates twice the direct light,
+    the BSDF-sampled emitter hit the direct light itself: the total is 1.5 x E_direct."""
+    rho, Le, a, h = 0.5, 10.0, 2.0, 10.0
+    R = 1e4
+    p = np.array([[-R, 0, R, R, 0, R, R, 0, -R], [R, 0, -R, -R, 0, -R, -R, 0, R],
+                  [-a / 2, h, -a / 2, a / 2, h, -a / 2, a / 2, h, a / 2],
+                  [a / 2, h, a / 2, -a / 2, h, a / 2, -a / 2, h, -a / 2]], np.float32)
+    # light faces down: (p1 - p0) x (p2 - p0) must point to -y
+    e = np.cross(p[2, 3:6] - p[2, :3], p[2, 6:] - p[2, :3])
+    if e[1] > 0:
+        p[2] = np.concatenate([p[2, :3], p[2, 6:], p[2, 3:6]])
+        p[3] = np.concatenate([p[3, :3], p[3, 6:], p[3, 3:6]])
+    sc = dict(p=p, n=None, uv=None, mat=np.array([0, 0, 1, 1], np.int32),
+              mats=np.array([[0, rho, rho, rho, 0.5, 1, -1, 0], [1, Le, Le, Le, 0.5, 1, -1, 0]], np.float32))
+    o = po.Scene(sc)
+    cam = np.array([0, 5, 0, 0, -90, 0, 0.01, 1], np.float32)   # looks straight down at the centre
+    spp = 100000
+    s, s2 = o.render(cam, 1, 1, spp, seed=4, mis=0, maxdepth=2)
+    m = s[0] / spp
+    # irradiance at the centre from a square a x a at height h (radiance Le): numerical quadrature
+    x = np.linspace(-a / 2, a / 2, 1201)
+    X, Y = np.meshgrid(x, x)
+    Ed = Le * np.trapezoid(np.trapezoid(h * h / (X ** 2 + Y ** 2 + h * h) ** 2, x), x)
+    # BSDF term: Lambert samples cos >= 0.1 with density cos/(0.99 pi) but reports (cos + 1e-4)/pi,
+    # so its expectation carries a factor 1/0.99 (the light subtends cos ~ 1: 1e-4 is negligible)
+    expect = (0.5 * 2.0 + 0.5 / 0.99) * rho / 3.14159 * Ed
+    se = math.sqrt(s2[0] / spp - m * m) / math.sqrt(spp)
+    assert abs(m - expect) < 5 * se + 1e-3 * expect, (m, expect, se)
