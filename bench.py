@@ -31,7 +31,7 @@ B_PAIR, B_TRI, B_RAY = 64, 48, 48
 
 def cpu_baseline(sc, spp_frame, target_s=12.0):
     """Times the oracle (CPU restatement of RendererPTLS, OpenMP over 8x8 tiles, all host
-    threads) on whole rows of the same frame at 1 spp, sized to ~target_s of CPU work."""
+    threads) on the same frame: one calibration pass at 1 spp, then k spp sized to ~target_s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     from nuray_amd.scenes import camera_array
@@ -41,18 +41,27 @@ def cpu_baseline(sc, spp_frame, target_s=12.0):
     o = po.Scene(sc)
     cam = camera_array(sc["camera"])
     kw = dict(maxdepth=10, rrdepth=3, nthreads=threads, seed=7)
-    y0 = H // 2 - 4
     t0 = time.perf_counter()
-    o.render(cam, W, H, 1, rect=(0, y0, W, y0 + 8), **kw)
+    o.render(cam, W, H, 1, **kw)
     dt = time.perf_counter() - t0
-    rows = int(np.clip(8 * target_s / max(dt, 1e-3), 8, H))
-    y0 = max(0, H // 2 - rows // 2)
+    k = int(np.clip(round(target_s / max(dt, 1e-3)), 1, spp_frame))
     t0 = time.perf_counter()
-    o.render(cam, W, H, 1, rect=(0, y0, W, y0 + rows), **kw)
+    o.render(cam, W, H, k, spp0=1, **kw)
     dt = time.perf_counter() - t0
-    samples = W * rows
+    samples = W * H * k
     return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
-            "sample": f"rows {y0}..{y0 + rows} of the 1280x720 frame at 1 spp ({samples} camera samples, {dt:.1f} s)"}
+            "sample": f"the same 1280x720 frame at {k} spp (samples 1..{k}; {samples} camera samples in {dt:.1f} s)"}
+
+
+def measured_traffic():
+    """Calibrated HBM bytes per closest-hit launch from the committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", "trace_closest_720p_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split()[1])
+    return None
 
 
 def main():
@@ -62,6 +71,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only run the plain 1 GiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; a frame shards by pixels x samples "
@@ -72,6 +83,10 @@ def main():
     from nuray_amd import scenes
 
     nr.init(0)
+    copy_gbs = nr.bench_copy(1 << 30, 5)
+    if args.copy_only:
+        print(json.dumps({"copy_bytes": 1 << 30, "reps": 5, "copy_gbs": copy_gbs}), flush=True)
+        return
     sc = scenes.terrain_spheres()
     W, H, spp = sc["width"], sc["height"], args.spp
     t0 = time.perf_counter()
@@ -111,7 +126,7 @@ def main():
     bytes_frame = B_PAIR * cnt["pair_fetch"] + B_TRI * cnt["tri_tests"] + B_RAY * cnt["rays"]
     achieved = bytes_frame * args.steps / (closest_ms / 1e3) / 1e9 if closest_ms > 0 else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": measured_traffic(),
                 "kernel": "k_trace<closest>", "bytes_per_launch": bytes_frame / max(cnt["closest_launches"], 1),
                 "avg_launch_ms": closest_ms / max(launches, 1),
                 "rays_per_frame": cnt["rays"], "pair_fetch_per_ray": cnt["pair_fetch"] / max(cnt["rays"], 1),
@@ -124,7 +139,7 @@ def main():
         "config": {"workload": f"ptnee {W}x{H}x{spp}spp maxdepth10 rrdepth3 prr0.8, {sc['name']} "
                                f"({len(sc['p'])} triangles) + 1024x512 sky", "global_batch": W * H * spp,
                    "seq_len": None, "parallelism": "single-gpu", "bvh_build_ms": info["build_ms"]},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "hbm_copy_gbs": copy_gbs,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]

@@ -119,6 +119,11 @@ int nr_render_stats(nr_scene* s, double* out16);
 /* 1: time the traversal launches with HIP events; 2: also count node-pair fetches / triangle tests. */
 int nr_set_profiling(nr_scene* s, int32_t mode);
 
+/* No reference counterpart: a plain device-to-device copy of `bytes` bytes (16-byte vector loads
+ * and stores), run `reps` times; *gbs = bytes read + written per second.  Calibrates the
+ * hardware byte counters (a known number of HBM bytes per dispatch) and the achievable HBM rate. */
+int nr_bench_copy(uint64_t bytes, int32_t reps, double* gbs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,7 +24,7 @@ EXPORTS = [
     "nr_last_error", "nr_init", "nr_synchronize", "nr_scene_create", "nr_scene_destroy",
     "nr_scene_set_texture_rgba8", "nr_scene_set_envmap", "nr_bvh_build_sah", "nr_bvh_info", "nr_bvh_export",
     "nr_bvh_build_host", "nr_intersect", "nr_render_ptnee", "nr_render_fetch", "nr_render_stats",
-    "nr_set_profiling",
+    "nr_set_profiling", "nr_bench_copy",
 ]
 
 
@@ -72,6 +72,7 @@ def lib():
         L.nr_render_fetch.argtypes = [vp, vp, vp, vp, vp]
         L.nr_render_stats.argtypes = [vp, f64p]
         L.nr_set_profiling.argtypes = [vp, C.c_int32]
+        L.nr_bench_copy.argtypes = [C.c_uint64, C.c_int32, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -88,6 +89,13 @@ def init(device=0):
 
 def synchronize():
     check(lib().nr_synchronize())
+
+
+def bench_copy(nbytes, reps=5):
+    """GB/s (read + write) of a plain device copy of nbytes."""
+    g = C.c_double()
+    check(lib().nr_bench_copy(nbytes, reps, C.byref(g)))
+    return g.value
 
 
 def _f32(a):

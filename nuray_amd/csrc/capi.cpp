@@ -6,6 +6,10 @@
 #include "render.h"
 #include "scene.h"
 
+namespace nr {
+double bench_copy(size_t bytes, int reps);
+}
+
 struct nr_scene {
     nr::Scene* sc;
     int profiling = 0;
@@ -221,6 +225,13 @@ int nr_render_stats(nr_scene* s, double* out16) {
 int nr_set_profiling(nr_scene* s, int32_t mode) {
     return guard([&] {
         s->profiling = mode;
+        return NR_OK;
+    });
+}
+
+int nr_bench_copy(uint64_t bytes, int32_t reps, double* gbs) {
+    return guard([&] {
+        *gbs = nr::bench_copy(size_t(bytes), reps);
         return NR_OK;
     });
 }
