@@ -119,6 +119,20 @@ int nr_render_stats(nr_scene* s, double* out16);
 /* 1: time the traversal launches with HIP events; 2: also count node-pair fetches / triangle tests. */
 int nr_set_profiling(nr_scene* s, int32_t mode);
 
+/* ---- Differentiable ptnee (no counterpart in the reference, which has no adjoint).
+ * theta: one row {r, g, b, alpha} per material -- albedo / F0 and GGX roughness; rows of
+ * emitters are ignored.  Replaces the materials' Kd and alpha_ (matlambert.h, matggxrefl.h). */
+int nr_scene_set_params(nr_scene* s, const float* theta, uint32_t n_mats);
+/* Mean squared error of the last render's mean image against target (W*H*3 floats); dLoss/dImage
+ * stays on the device for nr_adjoint_ptnee and is also copied to dL_dimg when non-NULL. */
+int nr_loss_grad_l2(nr_scene* s, const float* target, double* loss, float* dL_dimg);
+/* Path-replay backpropagation: grad (n_mats x 4 doubles, same layout as theta) of the loss whose
+ * image gradient is dL_dimg (W*H*3 floats, NULL = the one nr_loss_grad_l2 left on the device),
+ * for the last nr_render_ptnee call, which must have used the same camera and params and run as a
+ * single wavefront batch.  Lambert and GGX-reflect materials. */
+int nr_adjoint_ptnee(nr_scene* s, const nr_camera* cam, const nr_params* params, const float* dL_dimg,
+                     double* grad);
+
 /* No reference counterpart: a plain device-to-device copy of `bytes` bytes (16-byte vector loads
  * and stores), run `reps` times; *gbs = bytes read + written per second.  Calibrates the
  * hardware byte counters (a known number of HBM bytes per dispatch) and the achievable HBM rate. */

@@ -24,7 +24,7 @@ EXPORTS = [
     "nr_last_error", "nr_init", "nr_synchronize", "nr_scene_create", "nr_scene_destroy",
     "nr_scene_set_texture_rgba8", "nr_scene_set_envmap", "nr_bvh_build_sah", "nr_bvh_info", "nr_bvh_export",
     "nr_bvh_build_host", "nr_intersect", "nr_render_ptnee", "nr_render_fetch", "nr_render_stats",
-    "nr_set_profiling", "nr_bench_copy",
+    "nr_set_profiling", "nr_bench_copy", "nr_scene_set_params", "nr_loss_grad_l2", "nr_adjoint_ptnee",
 ]
 
 
@@ -73,6 +73,9 @@ def lib():
         L.nr_render_stats.argtypes = [vp, f64p]
         L.nr_set_profiling.argtypes = [vp, C.c_int32]
         L.nr_bench_copy.argtypes = [C.c_uint64, C.c_int32, C.POINTER(C.c_double)]
+        L.nr_scene_set_params.argtypes = [vp, f32p, C.c_uint32]
+        L.nr_loss_grad_l2.argtypes = [vp, f32p, C.POINTER(C.c_double), vp]
+        L.nr_adjoint_ptnee.argtypes = [vp, C.POINTER(nr_camera), C.POINTER(nr_params), vp, f64p]
         _lib = L
     return _lib
 
@@ -203,6 +206,28 @@ class Scene:
         rgb8 = np.zeros((H, W, 3), np.uint8)
         check(lib().nr_render_fetch(self.h, rgb.ctypes.data, s.ctypes.data, s2.ctypes.data, rgb8.ctypes.data))
         return dict(rgb=rgb, sum=s, sum2=s2, rgb8=rgb8)
+
+    # ---- differentiable rendering
+    def set_params(self, theta):
+        """theta: (materials, 4) float32 rows {r, g, b, alpha}."""
+        theta = _f32(theta)
+        check(lib().nr_scene_set_params(self.h, theta.reshape(-1), len(theta)))
+
+    def loss_l2(self, target, want_grad=False):
+        t = _f32(target).reshape(-1)
+        loss = C.c_double()
+        g = np.zeros(len(t), np.float32) if want_grad else None
+        check(lib().nr_loss_grad_l2(self.h, t, C.byref(loss), None if g is None else g.ctypes.data))
+        return (loss.value, g) if want_grad else loss.value
+
+    def adjoint(self, cam, W, H, spp, n_mats, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0, spp_offset=0,
+                dl_dimg=None):
+        p = nr_params(W, H, spp, spp_offset, seed, prr, mis, maxdepth, rrdepth, 0, 0)
+        c = make_camera(cam)
+        grad = np.zeros(n_mats * 4, np.float64)
+        dl = None if dl_dimg is None else _f32(dl_dimg).reshape(-1)
+        check(lib().nr_adjoint_ptnee(self.h, C.byref(c), C.byref(p), None if dl is None else dl.ctypes.data, grad))
+        return grad.reshape(n_mats, 4)
 
     def set_profiling(self, mode):
         check(lib().nr_set_profiling(self.h, mode))

@@ -1,0 +1,16 @@
+// Differentiable ptnee (path-replay backpropagation) host entry points.
+#pragma once
+#include "render.h"
+
+namespace nr {
+
+// Material parameter rows {r, g, b, alpha} for every material (emitters keep their Ke).
+void set_material_params(Scene& sc, const float* theta, int n_mats);
+// Mean squared error of the last render's mean image against `target` (W*H*3); keeps
+// dLoss/dImage on the device (and copies it to dldi_host if non-null).
+double loss_l2(Scene& sc, const float* target, float* dldi_host);
+// Gradient of the loss with respect to every material's {r, g, b, alpha} (materials x 4 doubles),
+// replaying the paths of the last forward render (which must have run as one batch).
+void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const float* dldi_host, double* grad);
+
+}  // namespace nr

@@ -1,0 +1,378 @@
+// Differentiable ptnee: gradients of an image loss with respect to per-material albedo (RGB)
+// and GGX roughness, by path-replay backpropagation (Vicini, Speierer, Jakob, SIGGRAPH 2021).
+// There is no reference counterpart; the estimator being differentiated is the reference's
+// RendererPTLS::trace (rendererptls.cpp:13-93) with its quirks.
+//
+// What is differentiated: the expectation of the per-sample estimator.  Albedo enters each
+// sample multilinearly, so its per-sample derivative is exact.  For GGX roughness the sampled
+// direction is held fixed (detached) and the log-derivative of the sampler's density is added:
+// the reference sampler draws theta_h = 0.9999 * atan(alpha * sqrt(u / (1 - u)))
+// (matggxrefl.cpp:24-29), whose density in theta0 = theta_h / 0.9999 is
+// 2 alpha^2 tan(theta0) sec^2(theta0) / (alpha^2 + tan^2 theta0)^2, so
+//   d log p / d alpha = 2 / alpha - 4 alpha / (alpha^2 + tan^2 theta0)
+// (the reflection and 0.9999 Jacobians do not depend on alpha).  The MIS weight of NEE and the
+// factor `fac` carried to the next emitter hit depend on alpha through the reference's pdf
+// (matggxrefl.cpp:54-64) and are differentiated as well.
+//
+// Replay: the forward render keeps every sample's radiance L (path slot layout).  The backward
+// kernel re-derives each path from the same Philox counters and the same exact traversal, and
+// peels each vertex's own contribution off L, so the radiance still to come through the next
+// bounce is known at every vertex without storing any path.  Per-block gradients accumulate in
+// FP32 in LDS; blocks add into an FP64 buffer (the only FP64 arithmetic on this path).
+#include <cstring>
+#include "adjoint.h"
+#include "nr_material.h"
+#include "nr_traverse.h"
+#include "render.h"
+
+namespace nr {
+
+namespace {
+
+constexpr int ABLK = 256;
+constexpr int LDS_PARAMS = 4096;  // per-block FP32 gradient accumulators (materials x 4)
+
+// ---- derivatives of the reference's GGX-reflect model (matggxrefl.cpp:36-64)
+struct GgxParts {
+    F3 f;          // bxdf value
+    F3 df_dkd;     // d f_c / d Kd_c
+    F3 df_da;      // d f_c / d alpha
+    float pdf;     // reference pdf
+    float dpdf_da; // d pdf / d alpha
+};
+__device__ __forceinline__ void ggx_d_and_deriv(float a, float c, float& D, float& dD) {
+    const float k = c * c * (a * a - 1.0f) + 1.0f;
+    D = a * a / 3.14159f / (k * k);
+    dD = 2.0f * a / (3.14159f * k * k * k) * (k - 2.0f * a * a * c * c);
+}
+__device__ __forceinline__ void ggx_g_and_deriv(float a, float c, float& G, float& dG) {
+    const float kap = a * 0.5f + 0.5f;
+    const float S = sqrtf(kap * kap + (1.0f - kap * kap * c * c));
+    G = 2.0f * c / (c + S);
+    const float dS = kap * (1.0f - c * c) / (2.0f * S);
+    dG = -2.0f * c / ((c + S) * (c + S)) * dS;
+}
+__device__ __forceinline__ GgxParts ggx_parts(const Mat& m, F3 wo, F3 n, F3 wi) {
+    GgxParts r;
+    r.f = r.df_dkd = r.df_da = f3(0, 0, 0);
+    r.pdf = bsdf_pdf<0u>(m, wo, n, wi);
+    r.dpdf_da = 0.0f;
+    const float ci = dot(n, wi), co = dot(n, wo);
+    if (!(double(ci) <= 1e-4 || co <= 1e-4f)) {
+        const F3 wh = normalized(add(wo, wi));
+        const float ch = dot(n, wh);
+        float D, dD;
+        ggx_d_and_deriv(m.alpha, ch, D, dD);
+        r.dpdf_da = dD * ch / 4.0f;
+    }
+    if (dot(wi, n) * dot(wo, n) < 0.0f) return r;
+    const F3 wh = normalized(add(wo, wi));
+    float D, dD, Gl, dGl, Gv, dGv;
+    ggx_d_and_deriv(m.alpha, dot(n, wh), D, dD);
+    ggx_g_and_deriv(m.alpha, ci, Gl, dGl);
+    ggx_g_and_deriv(m.alpha, co, Gv, dGv);
+    const float fr = powf(1.0f - dot(wh, wi), 5.0f);
+    const float den = 4.0f * (co + 1e-6f) * (ci + 1e-6f);
+    const float s = D * Gl * Gv / 4.0f / (co + 1e-6f) / (ci + 1e-6f);
+    const float ds = (dD * Gl * Gv + D * dGl * Gv + D * Gl * dGv) / den;
+    const F3 F = f3(m.kd.x + (1.0f - m.kd.x) * fr, m.kd.y + (1.0f - m.kd.y) * fr, m.kd.z + (1.0f - m.kd.z) * fr);
+    r.f = mul(F, s);
+    r.df_dkd = f3(s * (1.0f - fr), s * (1.0f - fr), s * (1.0f - fr));
+    r.df_da = mul(F, ds);
+    return r;
+}
+
+// Per-material parts for the two models the adjoint supports.
+__device__ __forceinline__ GgxParts mat_parts(const SceneView& sv, const Mat& m, F3 wo, F3 n, F3 wi, float u, float v) {
+    if (m.type == MAT_GGXREFL) return ggx_parts(m, wo, n, wi);
+    GgxParts r;
+    r.f = bsdf_eval<0u>(sv, m, wo, n, wi, u, v);
+    r.pdf = bsdf_pdf<0u>(m, wo, n, wi);
+    const bool same = !(dot(wi, n) * dot(wo, n) < 0.0f);
+    const float inv = same ? 1.0f / 3.14159f : 0.0f;  // Lambert: f = Kd / pi
+    r.df_dkd = f3(inv, inv, inv);
+    r.df_da = f3(0, 0, 0);
+    r.dpdf_da = 0.0f;
+    return r;
+}
+
+// d log p(wi) / d alpha of the reference GGX sampler at a fixed direction.
+__device__ __forceinline__ float ggx_score(float a, F3 wo, F3 n, F3 wi) {
+    const F3 h = normalized(add(wo, wi));
+    const float ch = fminf(fmaxf(dot(n, h), -1.0f), 1.0f);
+    const float th0 = acosf(ch) / 0.9999f;
+    if (!(th0 < 1.5707963f)) return 0.0f;
+    const float t = tanf(th0);
+    return 2.0f / a - 4.0f * a / (a * a + t * t);
+}
+
+struct AdjArgs {
+    SceneView sv;
+    CamView cam;
+    int W, H, npix, P;
+    uint32_t seed, sample0;
+    float prr;
+    int mis, maxdepth, rrdepth;
+    const float4* radiance;  // per-sample radiance of the forward pass (slot layout)
+    const float* dldi;       // dLoss / dImage, W*H*3
+    float inv_spp;
+    int n_params;            // materials * 4
+    double* grad;            // n_params
+    int* err;
+};
+
+__device__ __forceinline__ void gadd(float* lds, double* glob, bool use_lds, int idx, float v) {
+    if (v == 0.0f || !(v == v)) return;
+    if (use_lds) atomicAdd(lds + idx, v);
+    else atomicAdd(glob + idx, double(v));
+}
+
+__global__ __launch_bounds__(ABLK) void k_adjoint(AdjArgs a) {
+    extern __shared__ char smem[];
+    const SceneView& sv = a.sv;
+    int* stk = reinterpret_cast<int*>(smem) + threadIdx.x;
+    float* gacc = reinterpret_cast<float*>(smem + size_t(sv.stack_cap) * ABLK * sizeof(int));
+    const bool use_lds = a.n_params <= LDS_PARAMS;
+    if (use_lds)
+        for (int k = threadIdx.x; k < a.n_params; k += blockDim.x) gacc[k] = 0.0f;
+    __syncthreads();
+    bool overflow = false;
+    for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < a.P; slot += gridDim.x * blockDim.x) {
+        const int pixel = slot % a.npix;
+        const uint32_t sample = a.sample0 + uint32_t(slot / a.npix);
+        const float4 Lt = a.radiance[slot];
+        // max(0, L) per channel at the camera (rendererptls.cpp:173): zero slope where L <= 0 or NaN
+        const F3 g = f3(Lt.x > 0.0f ? a.dldi[3 * pixel] * a.inv_spp : 0.0f,
+                        Lt.y > 0.0f ? a.dldi[3 * pixel + 1] * a.inv_spp : 0.0f,
+                        Lt.z > 0.0f ? a.dldi[3 * pixel + 2] * a.inv_spp : 0.0f);
+        if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) continue;
+        // camera ray, identical to k_raygen
+        const int x = pixel % a.W, y = pixel / a.W;
+        const float4 u0 = rng4(a.seed, uint32_t(pixel), sample, 0u, 0u);
+        const float cx = (float(x) + u0.x) / float(a.W) * 2.0f - 1.0f;
+        const float cy = (float(y) + u0.y) / float(a.H) * 2.0f - 1.0f;
+        const CamView& cam = a.cam;
+        F3 d = normalized(sub(add(f3(cam.gaze[0], cam.gaze[1], cam.gaze[2]),
+                                  mul(mul(mul(f3(cam.hand[0], cam.hand[1], cam.hand[2]), cam.fw), cx), 0.5f)),
+                              mul(mul(mul(f3(cam.up[0], cam.up[1], cam.up[2]), cam.fh), cy), 0.5f)));
+        F3 o = f3(cam.pos[0], cam.pos[1], cam.pos[2]);
+        F3 beta = f3(1.0f, 1.0f, 1.0f);
+        F3 Lrem = f3(Lt.x, Lt.y, Lt.z);  // radiance of this sample not yet accounted for
+        float fac = 1.0f, dfac = 0.0f;   // MIS factor for the next emitter hit and d/d alpha of it
+        int prev_mat = -1;
+        for (int depth = 0;; ++depth) {
+            const HitRec h = trace_ray<false>(sv, o, d, 0.0f, stk, ABLK, overflow);
+            if (h.prim < 0) break;  // environment: no parameter dependence
+            const TriShade ts = sv.shade[h.prim];
+            const float b1 = h.b1, b2 = h.b2, w0 = 1.0f - b1 - b2;
+            const F3 nrm = normalized(add(add(mul(f3(ts.s0.x, ts.s0.y, ts.s0.z), w0), mul(f3(ts.s0.w, ts.s1.x, ts.s1.y), b1)),
+                                          mul(f3(ts.s1.z, ts.s1.w, ts.s2.x), b2)));
+            const int mid = __float_as_int(ts.s3.w);
+            const Mat m = load_mat(sv, mid);
+            if (m.type == MAT_LIGHT) {
+                // Le * fac: fac depends on the previous vertex's roughness through its NEE MIS weight
+                if (prev_mat >= 0 && dfac != 0.0f) {
+                    const float v = g.x * beta.x * m.kd.x * dfac + g.y * beta.y * m.kd.y * dfac + g.z * beta.z * m.kd.z * dfac;
+                    gadd(gacc, a.grad, use_lds, 4 * prev_mat + 3, v);
+                }
+                break;
+            }
+            if (dot(nrm, d) > 0.0f && !mat_transmissive(m)) break;
+            const float tu = w0 * ts.s2.y + b1 * ts.s2.w + b2 * ts.s3.y;
+            const float tv = w0 * ts.s2.z + b1 * ts.s3.x + b2 * ts.s3.z;
+            const F3 hp = add(o, mul(d, h.t));
+            const F3 wo = neg(d);
+            const float4 ua = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 0u);
+            float fac_next = 1.0f, dfac_next = 0.0f;
+            if (mat_needs_nee(m) && sv.n_lights > 0) {
+                const LightRec lr = sv.lights[pick_light(sv, ua.x)];
+                float r1 = ua.y, r2 = ua.z;
+                if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
+                const float lw = 1.0f - r1 - r2;
+                const F3 lp = add(add(mul(f3(lr.p0.x, lr.p0.y, lr.p0.z), lw), mul(f3(lr.p1.x, lr.p1.y, lr.p1.z), r1)),
+                                  mul(f3(lr.p2.x, lr.p2.y, lr.p2.z), r2));
+                const F3 ln = normalized(add(add(mul(f3(lr.n0.x, lr.n0.y, lr.n0.z), lw), mul(f3(lr.n1.x, lr.n1.y, lr.n1.z), r1)),
+                                             mul(f3(lr.n2.x, lr.n2.y, lr.n2.z), r2)));
+                const F3 lv = sub(lp, hp);
+                const F3 wl = normalized(lv);
+                const float lcos = dot(ln, neg(wl));
+                const float ncos = dot(nrm, normalized(sub(lp, hp)));
+                const GgxParts pl = mat_parts(sv, m, wo, nrm, wl, tu, tv);
+                const float lpdf = sv.light_p;
+                const float lop = lpdf * dot(lv, lv) / ((1e-6f < lcos) ? lcos : 1e-6f);
+                float mis = lop / (lop + pl.pdf);
+                float dmis = -lop / ((lop + pl.pdf) * (lop + pl.pdf)) * pl.dpdf_da;
+                if (a.mis == 0) mis = 0.5f, dmis = 0.0f;
+                fac_next = 1.0f - mis;
+                dfac_next = -dmis;
+                const F3 Ll = mul(divs(f3(lr.ke.x, lr.ke.y, lr.ke.z), dot(lv, lv)), (0.0f < lcos ? lcos : 0.0f) / lpdf);
+                const float cn = (0.0f < ncos ? ncos : 0.0f);
+                const F3 c = mul(mulv(mulv(beta, Ll), pl.f), cn * mis);
+                if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
+                    const F3 so = add(hp, mul(wl, 1e-3f));
+                    const HitRec sh = trace_ray<true>(sv, so, wl, norm(lv), stk, ABLK, overflow);
+                    if (sh.prim < 0) {  // visible: d(beta Ll f cos mis)
+                        const F3 k = mul(mulv(beta, Ll), cn);
+                        if (m.tex < 0) {
+                            gadd(gacc, a.grad, use_lds, 4 * mid + 0, g.x * k.x * pl.df_dkd.x * mis);
+                            gadd(gacc, a.grad, use_lds, 4 * mid + 1, g.y * k.y * pl.df_dkd.y * mis);
+                            gadd(gacc, a.grad, use_lds, 4 * mid + 2, g.z * k.z * pl.df_dkd.z * mis);
+                        }
+                        const float va = g.x * k.x * (pl.df_da.x * mis + pl.f.x * dmis) +
+                                         g.y * k.y * (pl.df_da.y * mis + pl.f.y * dmis) +
+                                         g.z * k.z * (pl.df_da.z * mis + pl.f.z * dmis);
+                        gadd(gacc, a.grad, use_lds, 4 * mid + 3, va);
+                        Lrem = sub(Lrem, c);
+                    }
+                }
+            }
+            bool cont = true;
+            float prr = 1.0f;
+            if (depth >= a.rrdepth) {
+                cont = !(ua.w > a.prr);
+                prr = a.prr;
+            }
+            if (a.maxdepth > 0 && depth + 1 >= a.maxdepth) cont = false;
+            if (!cont) break;
+            const float4 ub = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 1u);
+            const F3 wi = bsdf_sample<0u>(m, ub.x, ub.y, ub.z, wo, nrm);
+            const GgxParts pi = mat_parts(sv, m, wo, nrm, wi, tu, tv);
+            const float cw = fabsf(dot(wi, nrm));
+            const F3 nb = f3(beta.x * cw * pi.f.x / pi.pdf / prr, beta.y * cw * pi.f.y / pi.pdf / prr,
+                             beta.z * cw * pi.f.z / pi.pdf / prr);
+            // everything still in Lrem arrives through this bounce: d log T = d log f - d log pdf + score
+            const float score = m.type == MAT_GGXREFL ? ggx_score(m.alpha, wo, nrm, wi) : 0.0f;
+            const float dlp = pi.pdf > 0.0f ? pi.dpdf_da / pi.pdf : 0.0f;
+            float va = 0.0f;
+            if (pi.f.x != 0.0f) {
+                if (m.tex < 0) gadd(gacc, a.grad, use_lds, 4 * mid + 0, g.x * Lrem.x * pi.df_dkd.x / pi.f.x);
+                va += g.x * Lrem.x * (pi.df_da.x / pi.f.x - dlp + score);
+            }
+            if (pi.f.y != 0.0f) {
+                if (m.tex < 0) gadd(gacc, a.grad, use_lds, 4 * mid + 1, g.y * Lrem.y * pi.df_dkd.y / pi.f.y);
+                va += g.y * Lrem.y * (pi.df_da.y / pi.f.y - dlp + score);
+            }
+            if (pi.f.z != 0.0f) {
+                if (m.tex < 0) gadd(gacc, a.grad, use_lds, 4 * mid + 2, g.z * Lrem.z * pi.df_dkd.z / pi.f.z);
+                va += g.z * Lrem.z * (pi.df_da.z / pi.f.z - dlp + score);
+            }
+            if (m.type == MAT_GGXREFL) gadd(gacc, a.grad, use_lds, 4 * mid + 3, va);
+            if (nb.x == 0.0f && nb.y == 0.0f && nb.z == 0.0f) break;
+            beta = nb;
+            fac = fac_next;
+            dfac = dfac_next;
+            prev_mat = mid;
+            o = add(hp, mul(wi, 1e-3f));
+            d = wi;
+        }
+        (void)fac;
+    }
+    if (overflow) atomicOr(a.err, 1);
+    __syncthreads();
+    if (use_lds)
+        for (int k = threadIdx.x; k < a.n_params; k += blockDim.x)
+            if (gacc[k] != 0.0f) atomicAdd(a.grad + k, double(gacc[k]));
+}
+
+// L2 loss (mean over pixels and channels) of the rendered mean against the target, and its
+// gradient with respect to the mean image.
+__global__ void k_loss_l2(const double* __restrict__ sum, double inv_spp, const float* __restrict__ target, size_t n3,
+                          float* __restrict__ dldi, double* __restrict__ loss) {
+    double part = 0.0;
+    const double inv_n = 1.0 / double(n3);
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n3; i += size_t(gridDim.x) * blockDim.x) {
+        const double r = sum[i] * inv_spp - double(target[i]);
+        part += r * r;
+        dldi[i] = float(2.0 * r * inv_n);
+    }
+    // block reduction in LDS, one FP64 atomic per block
+    __shared__ double red[256];
+    red[threadIdx.x] = part;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0] * inv_n);
+}
+
+__global__ void k_zero(double* a, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = 0.0;
+}
+
+}  // namespace
+
+void set_material_params(Scene& sc, const float* theta, int n_mats) {
+    if (n_mats != int(sc.mats.size() / 8)) throw std::runtime_error("parameter rows must match the material count");
+    for (int j = 0; j < n_mats; ++j) {
+        float* m = &sc.mats[8 * size_t(j)];
+        if (int(m[0]) == MAT_LIGHT) continue;  // emitters keep their Ke
+        m[1] = theta[4 * j + 0];
+        m[2] = theta[4 * j + 1];
+        m[3] = theta[4 * j + 2];
+        m[4] = theta[4 * j + 3];
+    }
+    if (!sc.built) return;
+    std::vector<MatRec> mr(sc.mats.size() / 8);
+    for (size_t j = 0; j < mr.size(); ++j) {
+        const float* m = &sc.mats[8 * j];
+        mr[j].a = make_float4(m[0], m[1], m[2], m[3]);
+        mr[j].b = make_float4(m[4], m[5], m[6], 0);
+    }
+    sc.d_mats.upload(mr.data(), mr.size() * sizeof(MatRec), sc.stream);
+    check_hip(hipStreamSynchronize(sc.stream), "params upload");
+}
+
+double loss_l2(Scene& sc, const float* target, float* dldi_host) {
+    if (!sc.target || sc.target->spp_total == 0) throw std::runtime_error("render before computing the loss");
+    RenderTarget& t = *sc.target;
+    const size_t n3 = size_t(t.W) * t.H * 3;
+    t.adj_target.upload(target, n3 * sizeof(float), sc.stream);
+    t.adj_dldi.ensure(n3 * sizeof(float));
+    t.adj_loss.ensure(sizeof(double));
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, sc.stream, t.adj_loss.as<double>(), 1);
+    hipLaunchKernelGGL(k_loss_l2, dim3(256), dim3(256), 0, sc.stream, t.sum.as<double>(), 1.0 / double(t.spp_total),
+                       t.adj_target.as<float>(), n3, t.adj_dldi.as<float>(), t.adj_loss.as<double>());
+    double loss = 0;
+    check_hip(hipMemcpyAsync(&loss, t.adj_loss.ptr, sizeof(double), hipMemcpyDeviceToHost, sc.stream), "D2H");
+    if (dldi_host)
+        check_hip(hipMemcpyAsync(dldi_host, t.adj_dldi.ptr, n3 * sizeof(float), hipMemcpyDeviceToHost, sc.stream), "D2H");
+    check_hip(hipStreamSynchronize(sc.stream), "loss sync");
+    return loss;
+}
+
+void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const float* dldi_host, double* grad_host) {
+    if (!sc.target || !sc.target->adj_valid) throw std::runtime_error("adjoint needs a single-batch forward render first");
+    RenderTarget& t = *sc.target;
+    if (t.W != p.W || t.H != p.H || t.adj_spp != p.spp || t.adj_seed != p.seed || t.adj_spp0 != p.spp_offset)
+        throw std::runtime_error("adjoint parameters differ from the last forward render");
+    if (sc.features & 1u) throw std::runtime_error("the adjoint supports Lambert and GGX-reflect materials");
+    const size_t n3 = size_t(p.W) * p.H * 3;
+    if (dldi_host) t.adj_dldi.upload(dldi_host, n3 * sizeof(float), sc.stream);
+    else if (!t.adj_dldi.ptr) throw std::runtime_error("no dL/dimage: pass one or call nr_loss_grad_l2");
+    const int n_params = int(sc.mats.size() / 8) * 4;
+    t.adj_grad.ensure(sizeof(double) * n_params);
+    hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, sc.stream, t.adj_grad.as<double>(), n_params);
+    int* cnt = t.counters.as<int>();
+    const int zero = 0;
+    check_hip(hipMemcpyAsync(cnt + 15, &zero, sizeof(int), hipMemcpyHostToDevice, sc.stream), "H2D");
+    AdjArgs a{sc.view, cam, p.W, p.H, p.W * p.H, p.W * p.H * p.spp, p.seed, uint32_t(p.spp_offset), p.prr, p.mis,
+              p.maxdepth, p.rrdepth, t.radiance.as<float4>(), t.adj_dldi.as<float>(), 1.0f / float(p.spp), n_params,
+              t.adj_grad.as<double>(), cnt + 15};
+    const size_t shmem = size_t(sc.view.stack_cap) * ABLK * sizeof(int) +
+                         (n_params <= LDS_PARAMS ? size_t(n_params) * sizeof(float) : 0);
+    if (shmem > 160 * 1024) throw std::runtime_error("adjoint LDS budget exceeded");
+    int per_cu = 0;
+    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adjoint, ABLK, shmem), "occupancy");
+    const int grid = std::max(1, per_cu) * device_cu_count();
+    hipLaunchKernelGGL(k_adjoint, dim3(grid), dim3(ABLK), shmem, sc.stream, a);
+    check_hip(hipGetLastError(), "k_adjoint launch");
+    int err = 0;
+    check_hip(hipMemcpyAsync(&err, cnt + 15, sizeof(int), hipMemcpyDeviceToHost, sc.stream), "D2H");
+    check_hip(hipMemcpyAsync(grad_host, t.adj_grad.ptr, sizeof(double) * n_params, hipMemcpyDeviceToHost, sc.stream),
+              "D2H");
+    check_hip(hipStreamSynchronize(sc.stream), "adjoint sync");
+    if (err) throw std::runtime_error("traversal stack overflow in the adjoint");
+}
+
+}  // namespace nr

@@ -3,6 +3,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include "adjoint.h"
 #include "render.h"
 #include "scene.h"
 
@@ -225,6 +226,36 @@ int nr_render_stats(nr_scene* s, double* out16) {
 int nr_set_profiling(nr_scene* s, int32_t mode) {
     return guard([&] {
         s->profiling = mode;
+        return NR_OK;
+    });
+}
+
+int nr_scene_set_params(nr_scene* s, const float* theta, uint32_t n_mats) {
+    return guard([&] {
+        nr::set_material_params(*s->sc, theta, int(n_mats));
+        return NR_OK;
+    });
+}
+
+int nr_loss_grad_l2(nr_scene* s, const float* target, double* loss, float* dL_dimg) {
+    return guard([&] {
+        *loss = nr::loss_l2(*s->sc, target, dL_dimg);
+        return NR_OK;
+    });
+}
+
+static nr::RenderParams to_params(const nr_params* pp) {
+    nr::RenderParams p;
+    p.W = pp->width; p.H = pp->height; p.spp = pp->spp; p.spp_offset = pp->spp_offset;
+    p.seed = pp->seed; p.prr = pp->prr; p.mis = pp->mis; p.maxdepth = pp->maxdepth; p.rrdepth = pp->rrdepth;
+    p.batch_spp = pp->batch_spp; p.accumulate = pp->accumulate;
+    return p;
+}
+
+int nr_adjoint_ptnee(nr_scene* s, const nr_camera* cam, const nr_params* params, const float* dL_dimg, double* grad) {
+    return guard([&] {
+        nr::CamView cv = nr::make_camera(cam->pos, cam->euler, cam->fov, cam->aspect);
+        nr::adjoint_ptnee(*s->sc, cv, to_params(params), dL_dimg, grad);
         return NR_OK;
     });
 }

@@ -1,0 +1,117 @@
+// Device-side BVH traversal building blocks (shared by the persistent trace kernels and by the
+// per-path replay kernels).  Semantics: see trace.hip -- no distance culling, the reference's
+// exact slab and triangle tests (filtered), DFS-order tie rule.
+#pragma once
+#include "nr_device.h"
+
+namespace nr {
+
+struct TravState {
+    int sp = 0;
+    int32_t leaf = 0;
+    bool has_leaf = false;
+    bool overflow = false;
+};
+
+// Start a ray: test the root box and seed the stack (stk points at this lane's slot 0; entries are
+// `stride` ints apart).
+__device__ __forceinline__ void trav_begin(const SceneView& sv, const SlabRay& sr, int* stk, TravState& s) {
+    s = TravState{};
+    if (slab_fast(sv.root_min[0], sv.root_min[1], sv.root_min[2], sv.root_max[0], sv.root_max[1], sv.root_max[2],
+                  sr)) {
+        if (ref_is_leaf(sv.root_ref)) s.leaf = sv.root_ref, s.has_leaf = true;
+        else stk[0] = sv.root_ref, s.sp = 1;
+    }
+}
+
+// Expand internal nodes until the lane holds a leaf or the stack is empty.
+template <bool STATS>
+__device__ __forceinline__ void trav_expand(const SceneView& sv, const SlabRay& sr, int* stk, int stride, int cap,
+                                            TravState& s, uint32_t& n_pf) {
+    while (s.sp > 0 && !s.has_leaf) {
+        --s.sp;
+        const int32_t e = stk[s.sp * stride];
+        if (ref_is_leaf(e)) {
+            s.leaf = e;
+            s.has_leaf = true;
+            break;
+        }
+        const NodePair* np = sv.nodes + e;
+        const float4 a = np->a, b = np->b, c = np->c;
+        const int4 ch = np->d;
+        if (STATS) ++n_pf;
+        const bool hl = slab_fast(a.x, a.y, a.z, a.w, b.x, b.y, sr);
+        const bool hr = slab_fast(b.z, b.w, c.x, c.y, c.z, c.w, sr);
+        int32_t push0 = 0, push1 = 0;
+        int npush = 0;
+        if (hl) {
+            if (ref_is_leaf(ch.x)) s.leaf = ch.x, s.has_leaf = true;
+            else push0 = ch.x, npush = 1;
+        }
+        if (hr) {
+            if (ref_is_leaf(ch.y) && !s.has_leaf) s.leaf = ch.y, s.has_leaf = true;
+            else if (npush == 0) push0 = ch.y, npush = 1;
+            else push1 = ch.y, npush = 2;
+        }
+        if (s.sp + npush > cap) {
+            s.overflow = true;
+            s.sp = 0;
+            s.has_leaf = false;
+            break;
+        }
+        if (npush >= 1) stk[s.sp * stride] = push0;
+        if (npush == 2) stk[(s.sp + 1) * stride] = push1;
+        s.sp += npush;
+    }
+}
+
+// Leaf loop, bvhnode.cpp:50-66, merged into the running winner with the DFS tie rule (closest
+// hit), or the shadow rule (double)t + 2e-3 <= len (any hit: returns true when blocked).
+template <bool ANY, bool STATS>
+__device__ __forceinline__ bool trav_leaf(const SceneView& sv, F3 o, F3 d, double len, TravState& s, float& bt,
+                                          float& bb1, float& bb2, int& bprim, int& blf, uint32_t& n_tt) {
+    s.has_leaf = false;
+    const int first = ref_first(s.leaf), cnt = ref_count(s.leaf);
+    if (STATS) n_tt += cnt;
+    for (int k = 0; k < cnt; ++k) {
+        const TriGeom g = sv.tris[first + k];
+        float t, b1, b2;
+        if (!tri_fast(g, o, d, bt, t, b1, b2)) continue;
+        if (ANY) {
+            if (double(t) + 2e-3 <= len) {
+                bt = t; bb1 = b1; bb2 = b2; bprim = first + k;
+                return true;
+            }
+        } else if (t < bt || first > blf) {  // t <= bt here: equal t -> later leaf wins
+            bt = t; bb1 = b1; bb2 = b2; bprim = first + k; blf = first;
+        }
+    }
+    return false;
+}
+
+// Whole traversal of one ray by one lane (no refill): closest hit, or shadow test with segment
+// length len (returns hit.prim >= 0 iff blocked).  *overflow is set if the stack ran out.
+template <bool ANY>
+__device__ __forceinline__ HitRec trace_ray(const SceneView& sv, F3 o, F3 d, float len, int* stk, int stride,
+                                            bool& overflow) {
+    const SlabRay sr = slab_ray(o, d);
+    TravState s;
+    trav_begin(sv, sr, stk, s);
+    float bt = ANY ? fminf(len, 1e18f) : 1e18f, bb1 = 0.0f, bb2 = 0.0f;
+    int bprim = -1, blf = -1;
+    uint32_t npf = 0, ntt = 0;
+    bool done = false;
+    for (;;) {
+        trav_expand<false>(sv, sr, stk, stride, sv.stack_cap, s, npf);
+        if (!s.has_leaf) break;
+        if (trav_leaf<ANY, false>(sv, o, d, double(len), s, bt, bb1, bb2, bprim, blf, ntt)) {
+            done = true;
+            break;
+        }
+    }
+    overflow = overflow || s.overflow;
+    if (ANY) return HitRec{done ? bt : 1e18f, bb1, bb2, done ? bprim : -1};
+    return HitRec{bt, bb1, bb2, bprim};
+}
+
+}  // namespace nr

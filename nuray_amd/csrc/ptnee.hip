@@ -388,6 +388,10 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
     if (!sc.target) sc.target = new RenderTarget();
     RenderTarget& t = *sc.target;
     t.ensure(p.W, p.H, size_t(kb) * npix);
+    t.adj_valid = kb >= p.spp && !p.accumulate;
+    t.adj_spp = p.spp;
+    t.adj_spp0 = p.spp_offset;
+    t.adj_seed = p.seed;
     hipStream_t s = sc.stream;
     int* cnt = t.counters.as<int>();
     TraceStats* ts = trace_stats ? t.stats.as<TraceStats>() : nullptr;
@@ -459,6 +463,7 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
     }
     check_hip(hipMemcpyAsync(t.pinned, cnt, sizeof(int) * 8, hipMemcpyDeviceToHost, s), "counters");
     check_hip(hipStreamSynchronize(s), "render sync");
+    if (status) t.adj_valid = false;
     if (t.pinned[C_ERR]) throw std::runtime_error("traversal stack overflow");
     rs.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (kt) {

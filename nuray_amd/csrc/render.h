@@ -47,6 +47,11 @@ struct RenderTarget {
     DevBuf stats;                      // TraceStats[2]
     int* pinned = nullptr;             // host-visible counter readback
     long long spp_total = 0;
+    // adjoint: the per-sample radiance of the last render is replayable when it ran as one batch
+    bool adj_valid = false;
+    int adj_spp = 0, adj_spp0 = 0;
+    uint32_t adj_seed = 0;
+    DevBuf adj_target, adj_dldi, adj_loss, adj_grad;
     ~RenderTarget();
     void ensure(int W, int H, size_t cap);
 };

@@ -17,6 +17,7 @@
 // (ballot + prefix count), so long rays do not hold a whole wave hostage.  The stack depth is the BVH depth + 2, checked on
 // every push; an overflow is reported, never written past.
 #include "trace.h"
+#include "nr_traverse.h"
 #include "scene.h"
 
 namespace nr {
@@ -36,10 +37,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const Ray* __rest
                                                   uint8_t* __restrict__ occ, const float4* __restrict__ contrib,
                                                   float4* __restrict__ radiance, int* __restrict__ work,
                                                   int* __restrict__ err, TraceStats* __restrict__ stats) {
-    extern __shared__ int stk[];
+    extern __shared__ int stk_all[];
     const int tid = threadIdx.x;
     const int bdim = blockDim.x;
     const int lane = tid & 63;
+    int* const stk = stk_all + tid;
     const unsigned long long below = (1ull << lane) - 1ull;
     const int cap = sv.stack_cap;
     const int n = *n_ptr;
@@ -48,12 +50,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const Ray* __rest
     int i = -1;
     F3 o = f3(0, 0, 0), d = f3(0, 0, 0);
     SlabRay sr;
-    float wr = 0.0f;  // ray payload word (segment length / path slot bits)
+    TravState ts;
+    float wr = 0.0f;  // ray payload words: segment length / vertex depth, path slot
     float pw = 0.0f;
     float bt = 1e18f, bb1 = 0.0f, bb2 = 0.0f;
-    int bprim = -1, blf = -1, sp = 0;
-    int32_t leaf = 0;
-    bool has_leaf = false, done = false, overflow = false;
+    int bprim = -1, blf = -1;
+    bool done = false;
     uint32_t n_pf = 0, n_tt = 0;
     bool drained = false;  // wave-uniform: the global counter is exhausted
     int rb = 0, re = 0;    // wave-uniform: claimed, not yet assigned ray indices [rb, re)
@@ -73,10 +75,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const Ray* __rest
             const int take = min(n_idle, re - rb);
             if (i < 0) {
                 const int rank = __popcll(idle_mask & below);
-                const int k = rb + rank;
                 if (rank < take) {
-                    i = k;
-                    const Ray r = rays[k];
+                    i = rb + rank;
+                    const Ray r = rays[i];
                     o = f3(r.o.x, r.o.y, r.o.z);
                     d = f3(r.d.x, r.d.y, r.d.z);
                     sr = slab_ray(o, d);
@@ -85,14 +86,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const Ray* __rest
                     bt = ANY ? fminf(wr, 1e18f) : 1e18f;
                     bb1 = bb2 = 0.0f;
                     bprim = blf = -1;
-                    sp = 0;
-                    has_leaf = done = overflow = false;
+                    done = false;
                     n_pf = n_tt = 0;
-                    if (slab_fast(sv.root_min[0], sv.root_min[1], sv.root_min[2], sv.root_max[0], sv.root_max[1],
-                                  sv.root_max[2], sr)) {
-                        if (ref_is_leaf(sv.root_ref)) leaf = sv.root_ref, has_leaf = true;
-                        else stk[tid] = sv.root_ref, sp = 1;
-                    }
+                    trav_begin(sv, sr, stk, ts);
                 }
             }
             rb += take;
@@ -102,65 +98,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const Ray* __rest
             continue;
         }
         if (i >= 0) {
-            // ---- expand internal nodes until this lane holds a leaf (or runs dry)
-            while (sp > 0 && !has_leaf) {
-                --sp;
-                const int32_t e = stk[sp * bdim + tid];
-                if (ref_is_leaf(e)) {
-                    leaf = e;
-                    has_leaf = true;
-                    break;
-                }
-                const NodePair* np = sv.nodes + e;
-                const float4 a = np->a, b = np->b, c = np->c;
-                const int4 ch = np->d;
-                if (STATS) ++n_pf;
-                const bool hl = slab_fast(a.x, a.y, a.z, a.w, b.x, b.y, sr);
-                const bool hr = slab_fast(b.z, b.w, c.x, c.y, c.z, c.w, sr);
-                int32_t push0 = 0, push1 = 0;
-                int npush = 0;
-                if (hl) {
-                    if (ref_is_leaf(ch.x)) leaf = ch.x, has_leaf = true;
-                    else push0 = ch.x, npush = 1;
-                }
-                if (hr) {
-                    if (ref_is_leaf(ch.y) && !has_leaf) leaf = ch.y, has_leaf = true;
-                    else if (npush == 0) push0 = ch.y, npush = 1;
-                    else push1 = ch.y, npush = 2;
-                }
-                if (sp + npush > cap) {
-                    overflow = true;
-                    sp = 0;
-                    has_leaf = false;
-                    break;
-                }
-                if (npush >= 1) stk[sp * bdim + tid] = push0;
-                if (npush == 2) stk[(sp + 1) * bdim + tid] = push1;
-                sp += npush;
-            }
-            // ---- leaf loop, bvhnode.cpp:50-66, merged with the DFS tie rule
-            if (has_leaf) {
-                has_leaf = false;
-                const int first = ref_first(leaf), cnt = ref_count(leaf);
-                if (STATS) n_tt += cnt;
-                for (int k = 0; k < cnt; ++k) {
-                    const TriGeom g = sv.tris[first + k];
-                    float t, b1, b2;
-                    if (!tri_fast(g, o, d, bt, t, b1, b2)) continue;
-                    if (ANY) {
-                        if (double(t) + 2e-3 <= double(wr)) {
-                            bt = t; bb1 = b1; bb2 = b2; bprim = first + k;
-                            done = true;
-                            break;
-                        }
-                    } else if (t < bt || first > blf) {  // t <= bt here: equal t -> later leaf wins
-                        bt = t; bb1 = b1; bb2 = b2; bprim = first + k; blf = first;
-                    }
-                }
-            }
+            trav_expand<STATS>(sv, sr, stk, bdim, cap, ts, n_pf);
+            if (ts.has_leaf) done = trav_leaf<ANY, STATS>(sv, o, d, double(wr), ts, bt, bb1, bb2, bprim, blf, n_tt);
             // ---- finished: write the result, go idle
-            if ((sp == 0 && !has_leaf) || done || overflow) {
-                if (overflow) atomicOr(err, 1);
+            if ((ts.sp == 0 && !ts.has_leaf) || done || ts.overflow) {
+                if (ts.overflow) atomicOr(err, 1);
                 if (STATS) {
                     atomicAdd(&stats->rays, 1ull);
                     atomicAdd(&stats->pair_fetch, (unsigned long long)n_pf);

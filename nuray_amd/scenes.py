@@ -292,3 +292,25 @@ def terrain_spheres(n_cells: int = 512, n_spheres: int = 2000, width=1280, heigh
 def camera_array(cam):
     """{pos, euler, fov, aspect} -> float32[8] as the C ABI takes it."""
     return np.array(list(cam["pos"]) + list(cam["euler"]) + [cam["fov"], cam["aspect"]], np.float32)
+
+
+def theta_of(scene):
+    """Material parameter rows {r, g, b, alpha} of a scene (emitters included, ignored on set)."""
+    m = scene["mats"]
+    return np.ascontiguousarray(np.stack([m[:, 1], m[:, 2], m[:, 3], m[:, 4]], 1), np.float32)
+
+
+def adjoint_problem(seed_hidden=5, seed_start=6, width=256, height=256):
+    """The inverse-rendering problem: the 64-region GGX-reflect box room with hidden parameters
+    theta* (seed 5), and a start theta0 = theta* * (1 + U[-0.3, 0.3]) per parameter (seed 6).
+    Returns (scene at theta*, theta*, theta0); rows of the emitter are left unchanged."""
+    sc = box_room(patches=True, seed=seed_hidden)
+    sc["width"], sc["height"] = width, height
+    sc["camera"] = dict(sc["camera"], aspect=width / height)
+    theta_star = theta_of(sc)
+    rng = np.random.Generator(np.random.PCG64(seed_start))
+    pert = 1.0 + rng.uniform(-0.3, 0.3, theta_star.shape)
+    is_light = sc["mats"][:, 0] == LIGHT
+    theta0 = np.where(is_light[:, None], theta_star, theta_star * pert).astype(np.float32)
+    theta0[:, :3] = np.clip(theta0[:, :3], 0.0, None)
+    return sc, theta_star, theta0

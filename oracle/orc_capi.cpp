@@ -1,17 +1,12 @@
 // ORACLE -- test infrastructure only (see orc_common.h).
 //
 // C entry points of liboracle.so, loaded by tests/ and bench.py through ctypes.
-#include "orc_render.h"
+#include "orc_handle.h"
 #include <omp.h>
 
 using namespace orc;
 
 namespace {
-struct OScene {
-    Scene sc;
-    BVH bvh;
-    bool built = false;
-};
 OScene* S(void* h) { return static_cast<OScene*>(h); }
 
 void count_rec(const BVH& b, int i, const V3f& o, const V3f& d, TravStats& st) {

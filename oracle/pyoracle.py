@@ -56,6 +56,9 @@ def lib():
         L.orc_material_eval.argtypes = [vp, C.c_int, C.c_int, f32p, f32p, f32p, f32p, C.c_int, f32p, f32p, f32p]
         L.orc_slab.argtypes = [f32p, f32p, f32p, f32p]
         L.orc_tri_test.argtypes = [f32p, f32p, f32p, f32p]
+        L.orc_set_params.argtypes = [vp, f32p, C.c_int]
+        L.orc_adjoint.argtypes = [vp, f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_float, C.c_int,
+                                  C.c_int, C.c_int, f32p, f64p, C.c_int, C.c_int]
         L.orc_light_pick.argtypes = [vp, C.c_float]
         L.orc_light_p.argtypes = [vp]
         L.orc_light_p.restype = C.c_float
@@ -143,6 +146,20 @@ class Scene:
         wi_out = np.zeros((k, 3), np.float32); f = np.zeros((k, 3), np.float32); pdf = np.zeros(k, np.float32)
         lib().orc_material_eval(self.h, m, k, wo, n, wi_in, u_in, 1 if wi is None else 0, wi_out, f, pdf)
         return wi_out, f, pdf
+
+    def set_params(self, theta):
+        theta = _f32(theta)
+        lib().orc_set_params(self.h, theta.reshape(-1), len(theta))
+
+    def adjoint(self, cam, W, H, spp, dldi, n_mats, spp0=0, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0,
+                nthreads=0):
+        """FP64 gradient (n_mats, 4) of sum(dldi * mean image) for the current parameters."""
+        g = np.zeros(n_mats * 4, np.float64)
+        r = lib().orc_adjoint(self.h, _f32(cam), W, H, spp, spp0, seed, prr, mis, maxdepth, rrdepth,
+                              _f32(dldi).reshape(-1), g, n_mats, nthreads)
+        if r != 0:
+            raise RuntimeError("oracle adjoint failed")
+        return g.reshape(n_mats, 4)
 
     def light_pick(self, u):
         return lib().orc_light_pick(self.h, float(u))
