@@ -7,6 +7,9 @@ icospheres with GGX-reflect materials, four emissive quads and a 1024x512 sky (6
 triangles).  The scene is built and resident in HBM before the timed region; a step is one
 nr_render_ptnee call (all wavefront batches of the frame, FP64 accumulation included).
 
+The inverse-rendering iteration (render + loss + path-replay adjoint + Adam on the 256x256
+64-material box room) is timed after the frames and reported as "adjoint".
+
 Printed JSON line: value = Msamples/s over the timed steps, the traversal roofline (HIP events
 around every closest-hit launch; algorithmic bytes from the kernel's own node / triangle
 counters) and the CPU baseline (the test oracle's restatement of the reference estimator timed
@@ -53,6 +56,57 @@ def cpu_baseline(sc, spp_frame, target_s=12.0):
             "sample": f"the same 1280x720 frame at {k} spp (samples 1..{k}; {samples} camera samples in {dt:.1f} s)"}
 
 
+def adjoint_iteration(nr, scenes, iters=20, warmup=5):
+    """The inverse-rendering iteration on the 64-material box room: upload theta, 256x256 render
+    at 32 spp, L2 loss against a 1024-spp target, path-replay adjoint to the 256 parameters, Adam
+    step on the host.  Returns ms per iteration and its parts."""
+    sc, theta_star, theta0 = scenes.adjoint_problem(width=256, height=256)
+    W, H, M, cam = 256, 256, len(theta0), sc["camera"]
+    dev = nr.Scene(sc)
+    dev.set_params(theta_star)
+    tgt = dev.render(cam, W, H, 1024, seed=1234)["rgb"]
+    emit = sc["mats"][:, 0] == scenes.LIGHT
+    theta = theta0.astype(np.float64)
+    m = np.zeros_like(theta)
+    v = np.zeros_like(theta)
+
+    def step(it):
+        nonlocal theta, m, v
+        dev.set_params(theta.astype(np.float32))
+        dev.render(cam, W, H, 32, seed=it, fetch=False)
+        dev.loss_l2(tgt)
+        g = dev.adjoint(cam, W, H, 32, M, seed=it)
+        g[emit] = 0
+        m = 0.9 * m + 0.1 * g
+        v = 0.999 * v + 0.001 * g * g
+        upd = 1e-2 * (m / (1 - 0.9 ** it)) / (np.sqrt(v / (1 - 0.999 ** it)) + 1e-8)
+        theta = np.where(emit[:, None], theta, np.clip(theta - upd, [1e-3] * 3 + [0.02], 1.0))
+    for it in range(1, warmup + 1):
+        step(it)
+    nr.synchronize()
+    t0 = time.perf_counter()
+    for it in range(warmup + 1, warmup + iters + 1):
+        step(it)
+    nr.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / iters
+    # parts, timed separately after the loop
+    nr.synchronize()
+    t0 = time.perf_counter()
+    for it in range(iters):
+        dev.render(cam, W, H, 32, seed=it, fetch=False)
+    nr.synchronize()
+    render_ms = (time.perf_counter() - t0) * 1e3 / iters
+    dev.loss_l2(tgt)
+    t0 = time.perf_counter()
+    for it in range(iters):
+        dev.adjoint(cam, W, H, 32, M, seed=iters - 1)
+    nr.synchronize()
+    adjoint_ms = (time.perf_counter() - t0) * 1e3 / iters
+    return {"ms_per_iter": ms, "render_ms": render_ms, "adjoint_ms": adjoint_ms,
+            "workload": "box room, 64 GGX-reflect materials (256 parameters), 256x256 at 32 spp per "
+                        "iteration, L2 loss, path-replay adjoint, Adam lr 1e-2"}
+
+
 def measured_traffic():
     """Calibrated HBM bytes per closest-hit launch from the committed rocprofv3 counter summary."""
     path = os.path.join(ROOT, "profiles", "trace_closest_720p_traffic.txt")
@@ -75,8 +129,8 @@ def main():
                     help="only run the plain 1 GiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; a frame shards by pixels x samples "
-                                   "(see DESIGN.md) but no multi-GPU run is part of it", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "nuray_amd runs on one GPU: a frame shards by pixels x samples, but "
+                                   "multi-GPU runs are not implemented", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import nuray_amd as nr
@@ -131,6 +185,9 @@ def main():
                 "avg_launch_ms": closest_ms / max(launches, 1),
                 "rays_per_frame": cnt["rays"], "pair_fetch_per_ray": cnt["pair_fetch"] / max(cnt["rays"], 1),
                 "tri_tests_per_ray": cnt["tri_tests"] / max(cnt["rays"], 1)}
+    adj = adjoint_iteration(nr, scenes)
+    print(f"# adjoint iteration {adj['ms_per_iter']:.2f} ms (render {adj['render_ms']:.2f} ms, "
+          f"replay {adj['adjoint_ms']:.2f} ms)", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(sc, spp)
     out = {
         "metric": "Msamples/s at 720p", "value": value, "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
@@ -140,6 +197,7 @@ def main():
                                f"({len(sc['p'])} triangles) + 1024x512 sky", "global_batch": W * H * spp,
                    "seq_len": None, "parallelism": "single-gpu", "bvh_build_ms": info["build_ms"]},
         "roofline": roofline, "cpu_baseline": cpu, "hbm_copy_gbs": copy_gbs,
+        "adjoint": adj,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
