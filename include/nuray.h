@@ -133,6 +133,31 @@ int nr_loss_grad_l2(nr_scene* s, const float* target, double* loss, float* dL_di
 int nr_adjoint_ptnee(nr_scene* s, const nr_camera* cam, const nr_params* params, const float* dL_dimg,
                      double* grad);
 
+/* ---- Neural radiance cache.  Replaces NeuralRadianceCache (renderernrc.h:17-43) and its NNNode
+ * network (nnnode.cpp:1-208): here a 44 -> 5 x 64 (ReLU) -> 3 MLP on BF16 MFMA with FP32
+ * accumulation, trained with Adam on the relative L2 loss mean_s sum_c (y - t)^2 / (y^2 + 0.01).
+ * Weights: NR_NRC_NPARAM floats, W^T per layer (row = output unit, 64 inputs per row); layers 0..4
+ * are 64 x 64, layer 5 is 16 x 64 of which rows 0..2 are used; input 44 is a constant 1.
+ * Query rows: 16 floats {pos x, y, z, roughness alpha, wo theta/pi, wo phi/(2pi) + 1/2,
+ * n theta/pi, n phi/(2pi) + 1/2, albedo r, g, b, 0, 0, 0, 0, 0}; theta is measured from +y and
+ * phi = atan2(z, x).  Encoding: 12-frequency triangle waves of x, y, z (renderernrc.cpp:50-64,
+ * 108-110), the four angles, exp(-alpha) (renderernrc.cpp:115), albedo. */
+#define NR_NRC_NPARAM 21504
+typedef struct nr_nrc nr_nrc;
+/* He-uniform initial weights from `seed`; lr of Adam (beta1 0.9, beta2 0.99, eps 1e-8);
+ * max_batch training samples per step (<= 0: 65536). */
+nr_nrc* nr_nrc_create(uint32_t seed, float lr, int32_t max_batch);
+int nr_nrc_destroy(nr_nrc* c);
+int nr_nrc_set_weights(nr_nrc* c, const float* w, uint32_t n);  /* also resets Adam */
+int nr_nrc_get_weights(nr_nrc* c, float* w, uint32_t n);
+/* NeuralRadianceCache::eval without the clamp: rgb = n x 3 raw network outputs. */
+int nr_nrc_infer(nr_nrc* c, const float* queries, uint32_t n, float* rgb);
+/* NeuralRadianceCache::train over a batch: one Adam step; *loss (if non-NULL) = mean loss before
+ * the step. */
+int nr_nrc_train_step(nr_nrc* c, const float* queries, const float* targets, uint32_t n, double* loss);
+/* Gradient of the mean loss (NR_NRC_NPARAM floats) without an update. */
+int nr_nrc_gradient(nr_nrc* c, const float* queries, const float* targets, uint32_t n, float* grad, double* loss);
+
 /* No reference counterpart: a plain device-to-device copy of `bytes` bytes (16-byte vector loads
  * and stores), run `reps` times; *gbs = bytes read + written per second.  Calibrates the
  * hardware byte counters (a known number of HBM bytes per dispatch) and the achievable HBM rate. */

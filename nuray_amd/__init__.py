@@ -25,7 +25,11 @@ EXPORTS = [
     "nr_scene_set_texture_rgba8", "nr_scene_set_envmap", "nr_bvh_build_sah", "nr_bvh_info", "nr_bvh_export",
     "nr_bvh_build_host", "nr_intersect", "nr_render_ptnee", "nr_render_fetch", "nr_render_stats",
     "nr_set_profiling", "nr_bench_copy", "nr_scene_set_params", "nr_loss_grad_l2", "nr_adjoint_ptnee",
+    "nr_nrc_create", "nr_nrc_destroy", "nr_nrc_set_weights", "nr_nrc_get_weights", "nr_nrc_infer",
+    "nr_nrc_train_step", "nr_nrc_gradient",
 ]
+
+NRC_NPARAM = 5 * 4096 + 16 * 64
 
 
 class nr_camera(C.Structure):
@@ -76,6 +80,14 @@ def lib():
         L.nr_scene_set_params.argtypes = [vp, f32p, C.c_uint32]
         L.nr_loss_grad_l2.argtypes = [vp, f32p, C.POINTER(C.c_double), vp]
         L.nr_adjoint_ptnee.argtypes = [vp, C.POINTER(nr_camera), C.POINTER(nr_params), vp, f64p]
+        L.nr_nrc_create.restype = vp
+        L.nr_nrc_create.argtypes = [C.c_uint32, C.c_float, C.c_int32]
+        L.nr_nrc_destroy.argtypes = [vp]
+        L.nr_nrc_set_weights.argtypes = [vp, f32p, C.c_uint32]
+        L.nr_nrc_get_weights.argtypes = [vp, f32p, C.c_uint32]
+        L.nr_nrc_infer.argtypes = [vp, f32p, C.c_uint32, f32p]
+        L.nr_nrc_train_step.argtypes = [vp, f32p, f32p, C.c_uint32, C.POINTER(C.c_double)]
+        L.nr_nrc_gradient.argtypes = [vp, f32p, f32p, C.c_uint32, f32p, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -239,3 +251,50 @@ class Scene:
                 "shadow_ms", "shadow_rays", "shadow_pair_fetch", "shadow_tri_tests", "shade_ms", "shade_launches",
                 "shadow_launches", "reserved"]
         return dict(zip(keys, out.tolist()))
+
+
+class Nrc:
+    """The radiance-cache network (nr_nrc): 16-float query rows, RGB outputs."""
+
+    def __init__(self, seed=0, lr=1e-3, max_batch=1 << 16):
+        self.h = lib().nr_nrc_create(seed, lr, max_batch)
+        if not self.h:
+            raise NuRayError(lib().nr_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nr_nrc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def weights(self):
+        w = np.zeros(NRC_NPARAM, np.float32)
+        check(lib().nr_nrc_get_weights(self.h, w, NRC_NPARAM))
+        return w
+
+    def set_weights(self, w):
+        check(lib().nr_nrc_set_weights(self.h, _f32(w).reshape(-1), NRC_NPARAM))
+
+    def infer(self, q):
+        q = _f32(q).reshape(-1, 16)
+        out = np.zeros((len(q), 3), np.float32)
+        check(lib().nr_nrc_infer(self.h, q, len(q), out))
+        return out
+
+    def train_step(self, q, t):
+        q = _f32(q).reshape(-1, 16)
+        loss = C.c_double()
+        check(lib().nr_nrc_train_step(self.h, q, _f32(t).reshape(-1, 3), len(q), C.byref(loss)))
+        return loss.value
+
+    def gradient(self, q, t):
+        q = _f32(q).reshape(-1, 16)
+        g = np.zeros(NRC_NPARAM, np.float32)
+        loss = C.c_double()
+        check(lib().nr_nrc_gradient(self.h, q, _f32(t).reshape(-1, 3), len(q), g, C.byref(loss)))
+        return loss.value, g

@@ -4,6 +4,7 @@
 #include <stdexcept>
 #include <string>
 #include "adjoint.h"
+#include "nrc.h"
 #include "render.h"
 #include "scene.h"
 
@@ -19,6 +20,8 @@ struct nr_scene {
 
 namespace {
 thread_local std::string g_err;
+
+void check_hip_sync(hipStream_t s) { nr::check_hip(hipStreamSynchronize(s), "hipStreamSynchronize"); }
 
 template <class F>
 int guard(F&& f) {
@@ -263,6 +266,106 @@ int nr_adjoint_ptnee(nr_scene* s, const nr_camera* cam, const nr_params* params,
 int nr_bench_copy(uint64_t bytes, int32_t reps, double* gbs) {
     return guard([&] {
         *gbs = nr::bench_copy(size_t(bytes), reps);
+        return NR_OK;
+    });
+}
+
+// ---- neural radiance cache
+struct nr_nrc {
+    nr::Nrc* n;
+};
+
+nr_nrc* nr_nrc_create(uint32_t seed, float lr, int32_t max_batch) {
+    nr_nrc* out = nullptr;
+    guard([&] {
+        nr::NrcConfig c;
+        c.lr = lr;
+        if (max_batch > 0) c.max_batch = max_batch;
+        out = new nr_nrc{new nr::Nrc(seed, c)};
+        return NR_OK;
+    });
+    return out;
+}
+
+int nr_nrc_destroy(nr_nrc* h) {
+    return guard([&] {
+        if (h) {
+            delete h->n;
+            delete h;
+        }
+        return NR_OK;
+    });
+}
+
+int nr_nrc_set_weights(nr_nrc* h, const float* w, uint32_t n) {
+    return guard([&] {
+        if (n != uint32_t(NR_NRC_NPARAM)) throw std::runtime_error("nrc: weight count must be NR_NRC_NPARAM");
+        h->n->set_weights(w);
+        return NR_OK;
+    });
+}
+
+int nr_nrc_get_weights(nr_nrc* h, float* w, uint32_t n) {
+    return guard([&] {
+        if (n != uint32_t(NR_NRC_NPARAM)) throw std::runtime_error("nrc: weight count must be NR_NRC_NPARAM");
+        h->n->get_weights(w);
+        return NR_OK;
+    });
+}
+
+namespace {
+// Host query rows (16 floats) -> device staging.
+const nr::NrcQuery* stage_queries(nr::Nrc& n, const float* q, uint32_t count) {
+    n.q_tmp.ensure(sizeof(nr::NrcQuery) * size_t(count));
+    n.q_tmp.upload(q, sizeof(nr::NrcQuery) * size_t(count), n.stream);
+    return n.q_tmp.as<nr::NrcQuery>();
+}
+const float4* stage_targets(nr::Nrc& n, const float* t, uint32_t count) {
+    std::vector<float> t4(size_t(count) * 4, 0.0f);
+    for (size_t i = 0; i < count; ++i)
+        for (int c = 0; c < 3; ++c) t4[i * 4 + c] = t[i * 3 + c];
+    n.t_tmp.ensure(sizeof(float4) * size_t(count));
+    n.t_tmp.upload(t4.data(), sizeof(float4) * size_t(count), n.stream);
+    check_hip_sync(n.stream);
+    return n.t_tmp.as<float4>();
+}
+}  // namespace
+
+int nr_nrc_infer(nr_nrc* h, const float* queries, uint32_t n, float* rgb) {
+    return guard([&] {
+        if (n == 0) return NR_OK;
+        nr::Nrc& c = *h->n;
+        const nr::NrcQuery* q = stage_queries(c, queries, n);
+        c.o_tmp.ensure(sizeof(float4) * size_t(n));
+        c.infer(q, int(n), c.o_tmp.as<float4>());
+        std::vector<float> o4(size_t(n) * 4);
+        nr::check_hip(hipMemcpyAsync(o4.data(), c.o_tmp.ptr, sizeof(float4) * size_t(n), hipMemcpyDeviceToHost, c.stream),
+                      "nrc infer readback");
+        check_hip_sync(c.stream);
+        for (size_t i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) rgb[i * 3 + k] = o4[i * 4 + k];
+        return NR_OK;
+    });
+}
+
+int nr_nrc_train_step(nr_nrc* h, const float* queries, const float* targets, uint32_t n, double* loss) {
+    return guard([&] {
+        nr::Nrc& c = *h->n;
+        const nr::NrcQuery* q = stage_queries(c, queries, n);
+        const float4* t = stage_targets(c, targets, n);
+        const double l = c.train(q, t, int(n), loss != nullptr);
+        if (loss) *loss = l;
+        return NR_OK;
+    });
+}
+
+int nr_nrc_gradient(nr_nrc* h, const float* queries, const float* targets, uint32_t n, float* grad, double* loss) {
+    return guard([&] {
+        nr::Nrc& c = *h->n;
+        const nr::NrcQuery* q = stage_queries(c, queries, n);
+        const float4* t = stage_targets(c, targets, n);
+        const double l = c.train(q, t, int(n), true, grad);
+        if (loss) *loss = l;
         return NR_OK;
     });
 }

@@ -1,0 +1,46 @@
+// Neural radiance cache: host side of the MLP (parameters, Adam state, training and inference).
+#pragma once
+#include <cstdint>
+#include "nrc_mlp.h"
+#include "scene.h"
+
+namespace nr {
+
+struct NrcConfig {
+    float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-8f;
+    int max_batch = 1 << 16;   // training samples per step
+};
+
+class Nrc {
+public:
+    NrcConfig cfg;
+    int step = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevBuf w32, w16, m, v;     // FP32 master weights, BF16 copy, Adam moments
+    DevBuf act, gz;            // training: layer inputs and pre-activation gradients, [6][batch][64] bf16
+    DevBuf part;               // training: per-wave weight-gradient partials
+    DevBuf scal;               // float[4]: loss sum
+    DevBuf q_tmp, t_tmp, o_tmp;  // staging for host-buffer entry points
+
+    Nrc(uint32_t seed, const NrcConfig& c);
+    ~Nrc();
+    // He-uniform initialisation from a seeded splitmix64 stream (host), padded entries zero.
+    void init_weights(uint32_t seed);
+    void set_weights(const float* w);   // NRC_NPARAM floats (W^T per layer)
+    void get_weights(float* w);
+
+    // Device-buffer entry points (on `stream`).
+    void infer(const NrcQuery* q, int n, float4* out);
+    // Renderer form: radiance[slot] += beta * max(0, y) for each query (one query per slot).
+    void infer_accumulate(const NrcQuery* q, const int* count, int max_n, float4* radiance);
+    // One Adam step on n <= max_batch samples (targets rgb in .xyz).  grad_out (host, NRC_NPARAM)
+    // when non-null: only the mean-loss gradient is computed and copied out, no update.
+    // Returns the mean relative L2 loss when want_loss (synchronises), else 0.
+    double train(const NrcQuery* q, const float4* target, int n, bool want_loss, float* grad_out = nullptr);
+
+private:
+    int splits_for(int n) const;
+};
+
+}  // namespace nr

@@ -1,0 +1,402 @@
+// Neural radiance cache kernels for gfx950: inference, training forward + backward, split-K weight
+// gradient and Adam, all matrix work on BF16 MFMA with FP32 accumulation.
+//
+// The reference's cache (NeuralRadianceCache / NNNode*, renderernrc.h:17-43, nnnode.cpp:1-208) is a
+// per-sample scalar MLP trained by SGD; this one is the 44 -> 5 x 64 (ReLU) -> 3 network trained
+// with Adam on batches of 2^16 samples and the relative L2 loss
+//   L = mean_s sum_c (y_sc - t_sc)^2 / (y_sc^2 + 0.01)      (y in the denominator held constant)
+// whose output gradient has the reference's form (o - a) / (o^2 + 0.01) (nnnode.cpp:195-196).
+//
+// Kernels
+//   k_nrc_infer      persistent; weights in LDS once per block, one 64-sample tile per wave
+//   k_nrc_fwd_bwd    one wave per block (all six activation tiles of its 64 samples in LDS):
+//                    forward, loss gradient, backward; writes each layer's input X_l and
+//                    pre-activation gradient G_l (bf16, [sample][64]) for the weight gradient
+//   k_nrc_wgrad      dW_l^T = sum_s G_l[s]^T X_l[s]: K = samples, split over waves; per-wave
+//                    partials (no atomics)
+//   k_nrc_adam       sums the partials and applies Adam to the FP32 master weights, refreshes
+//                    the BF16 copy
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+#include "nrc.h"
+
+namespace nr {
+
+namespace {
+
+constexpr int INF_WAVES = 8;  // waves per inference block: 48 KB weights + 8 x 9 KB tiles in LDS
+
+template <bool ACC>
+__global__ __launch_bounds__(INF_WAVES * 64) void k_nrc_infer(const __bf16* __restrict__ w16,
+                                                              const NrcQuery* __restrict__ q,
+                                                              const int* __restrict__ count, int n_max,
+                                                              float4* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) __bf16 sW[NRC_LDS_W];
+    __shared__ __attribute__((aligned(16))) __bf16 sX[INF_WAVES][NRC_TILE];
+    nrc_load_weights(w16, sW);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = count ? min(*count, n_max) : n_max;
+    __bf16* X = sX[wave];
+    for (int chunk = blockIdx.x * INF_WAVES + wave; chunk * 64 < n; chunk += gridDim.x * INF_WAVES) {
+        const int s = chunk * 64 + lane;
+        NrcQuery rec;
+        const bool valid = s < n;
+        if (valid) rec = q[s];
+        else rec = NrcQuery{};
+        nrc_encode_row(rec, valid, X, lane);
+        wave_sync();
+        float4 y[4];
+        nrc_forward(sW, X, y, lane);
+        wave_sync();
+        if (lane < 16) {
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const int si = chunk * 64 + 16 * nt + lane;
+                if (si >= n) continue;
+                if (ACC) {
+                    const float4 qc = q[si].c, qd = q[si].d;
+                    const int slot = __float_as_int(qc.w);
+                    if (slot < 0) continue;
+                    float4 r = out[slot];
+                    r.x += qd.x * fmaxf(y[nt].x, 0.0f);
+                    r.y += qd.y * fmaxf(y[nt].y, 0.0f);
+                    r.z += qd.z * fmaxf(y[nt].z, 0.0f);
+                    out[slot] = r;
+                } else {
+                    out[si] = y[nt];
+                }
+            }
+        }
+    }
+}
+
+// Copies this wave's 64-row tile to global rows [s0, s0 + 64) of a [batch][64] bf16 array.
+__device__ __forceinline__ void tile_to_global(const __bf16* T, __bf16* g, int s0, int n, int lane) {
+    const int s = s0 + lane;
+    if (s >= n) return;
+    const uint4* src = reinterpret_cast<const uint4*>(T + lane * NRC_LD);
+    uint4* dst = reinterpret_cast<uint4*>(g + size_t(s) * 64);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) dst[c] = src[c];
+}
+
+__global__ __launch_bounds__(64) void k_nrc_fwd_bwd(const __bf16* __restrict__ w16, const NrcQuery* __restrict__ q,
+                                                    const float4* __restrict__ tgt, int n, float inv_n,
+                                                    __bf16* __restrict__ act, __bf16* __restrict__ gz,
+                                                    float* __restrict__ loss) {
+    __shared__ __attribute__((aligned(16))) __bf16 sW[NRC_LDS_W];
+    __shared__ __attribute__((aligned(16))) __bf16 sT[7][NRC_TILE];  // X_0..X_5, then G_5
+    nrc_load_weights(w16, sW);
+    __syncthreads();
+    const int lane = threadIdx.x, r = lane & 15, qd = lane >> 4;
+    const size_t layer_stride = size_t(n) * 64;
+    float loss_acc = 0.0f;
+    for (int chunk = blockIdx.x; chunk * 64 < n; chunk += gridDim.x) {
+        const int s0 = chunk * 64;
+        {
+            const int s = s0 + lane;
+            const bool valid = s < n;
+            NrcQuery rec = valid ? q[s] : NrcQuery{};
+            nrc_encode_row(rec, valid, sT[0], lane);
+        }
+        wave_sync();
+        tile_to_global(sT[0], act, s0, n, lane);
+        // forward, keeping every layer input
+        for (int l = 0; l < 5; ++l) {
+            f32x4 acc[4][4];
+            nrc_layer<4>(sW + l * 64 * NRC_LD, sT[l], acc, lane);
+            nrc_store_relu<4>(acc, sT[l + 1], lane);
+            wave_sync();
+            tile_to_global(sT[l + 1], act + (l + 1) * layer_stride, s0, n, lane);
+        }
+        f32x4 yo[1][4];
+        nrc_layer<1>(sW + 5 * 64 * NRC_LD, sT[5], yo, lane);
+        // output gradient into G_5 (columns 0..2; the rest zero)
+        {
+            bf16x8 z;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) z[j] = (__bf16)0.0f;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) *reinterpret_cast<bf16x8*>(sT[6] + lane * NRC_LD + c * 8) = z;
+        }
+        wave_sync();
+        if (qd == 0) {
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const int s = s0 + 16 * nt + r;
+                if (s >= n) continue;
+                const float4 t = tgt[s];
+                const float tv[3] = {t.x, t.y, t.z};
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const float y = yo[0][nt][c], den = y * y + 0.01f, e = y - tv[c];
+                    loss_acc += e * e / den;
+                    sT[6][(16 * nt + r) * NRC_LD + c] = (__bf16)(2.0f * e / den * inv_n);
+                }
+            }
+        }
+        wave_sync();
+        tile_to_global(sT[6], gz + 5 * layer_stride, s0, n, lane);
+        // backward: G_{l-1} = (W_l G_l) * [X_l > 0], written over X_l's tile
+        for (int l = 5; l >= 1; --l) {
+            const __bf16* Wl = sW + l * 64 * NRC_LD;
+            const __bf16* G = l == 5 ? sT[6] : sT[l + 1];
+            const int kin = l == 5 ? 1 : 2;    // k-steps of 32 over the layer's outputs
+            const int rows = nrc_layer_rows(l);
+            bf16x8 bg[4][2];
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks)
+                    bg[nt][ks] = *reinterpret_cast<const bf16x8*>(G + (16 * nt + r) * NRC_LD + 32 * ks + 8 * qd);
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                bf16x8 a[2];
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const int o = 32 * ks + 8 * qd + j;
+                        a[ks][j] = o < rows ? Wl[o * NRC_LD + 16 * mt + r] : (__bf16)0.0f;
+                    }
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) {
+                    f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+                    c = mfma16(a[0], bg[nt][0], c);
+                    if (kin == 2) c = mfma16(a[1], bg[nt][1], c);
+                    __bf16* xp = sT[l] + (16 * nt + r) * NRC_LD + 16 * mt + 4 * qd;
+                    const bf16x4 xm = *reinterpret_cast<const bf16x4*>(xp);
+                    bf16x4 gv;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) gv[i] = (__bf16)((float)xm[i] > 0.0f ? c[i] : 0.0f);
+                    *reinterpret_cast<bf16x4*>(xp) = gv;
+                }
+            }
+            wave_sync();
+            tile_to_global(sT[l], gz + (l - 1) * layer_stride, s0, n, lane);
+        }
+        wave_sync();
+    }
+    // wave reduction of the loss, one atomic per block
+    for (int off = 32; off > 0; off >>= 1) loss_acc += __shfl_xor(loss_acc, off);
+    if (lane == 0 && loss) atomicAdd(loss, loss_acc);
+}
+
+// Weight gradient of layer l over the samples of this wave's chunks of 32.
+template <int MT>
+__device__ __forceinline__ void wgrad_body(const __bf16* __restrict__ X, const __bf16* __restrict__ G, int n,
+                                           int first, int stride, __bf16* sG, __bf16* sX, float* __restrict__ part,
+                                           int lane) {
+    const int r = lane & 15, qd = lane >> 4;
+    f32x4 acc[MT][4];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int c0 = first; c0 * 32 < n; c0 += stride) {
+        // stage 32 rows of G and X: 256 16-byte pieces each, 4 per lane
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int piece = lane + 64 * k, row = piece >> 3, col = piece & 7;
+            const int s = c0 * 32 + row;
+            uint4 g = make_uint4(0, 0, 0, 0), x = make_uint4(0, 0, 0, 0);
+            if (s < n) {
+                g = reinterpret_cast<const uint4*>(G + size_t(s) * 64)[col];
+                x = reinterpret_cast<const uint4*>(X + size_t(s) * 64)[col];
+            }
+            *reinterpret_cast<uint4*>(sG + row * NRC_LD + col * 8) = g;
+            *reinterpret_cast<uint4*>(sX + row * NRC_LD + col * 8) = x;
+        }
+        wave_sync();
+        bf16x8 b[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) b[nt][j] = sX[(8 * qd + j) * NRC_LD + 16 * nt + r];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            bf16x8 a;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) a[j] = sG[(8 * qd + j) * NRC_LD + 16 * mt + r];
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = mfma16(a, b[nt], acc[mt][nt]);
+        }
+        wave_sync();
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) part[(16 * mt + 4 * qd + i) * 64 + 16 * nt + r] = acc[mt][nt][i];
+}
+
+// grid (splits, 6), 4 waves per block; partial index p = split * 4 + wave.
+__global__ __launch_bounds__(256) void k_nrc_wgrad(const __bf16* __restrict__ act, const __bf16* __restrict__ gz,
+                                                   int n, float* __restrict__ part) {
+    __shared__ __attribute__((aligned(16))) __bf16 sG[4][32 * NRC_LD];
+    __shared__ __attribute__((aligned(16))) __bf16 sX[4][32 * NRC_LD];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l = blockIdx.y;
+    const int P = gridDim.x * 4, p = blockIdx.x * 4 + wave;
+    const size_t ls = size_t(n) * 64;
+    float* dst = part + (size_t(l) * P + p) * 4096;
+    if (l < 5) wgrad_body<4>(act + l * ls, gz + l * ls, n, p, P, sG[wave], sX[wave], dst, lane);
+    else wgrad_body<1>(act + l * ls, gz + l * ls, n, p, P, sG[wave], sX[wave], dst, lane);
+}
+
+__global__ void k_nrc_adam(float* __restrict__ w32, __bf16* __restrict__ w16, float* __restrict__ m,
+                           float* __restrict__ v, const float* __restrict__ part, int P, float lr, float b1, float b2,
+                           float eps, float c1, float c2, float* __restrict__ grad_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NRC_NPARAM) return;
+    const int l = i < 5 * 4096 ? i / 4096 : 5, k = i - l * 4096;
+    const float* src = part + size_t(l) * P * 4096 + k;
+    float g = 0.0f;
+    for (int p = 0; p < P; ++p) g += src[size_t(p) * 4096];
+    if (grad_out) {
+        grad_out[i] = g;
+        return;
+    }
+    const float mi = b1 * m[i] + (1.0f - b1) * g;
+    const float vi = b2 * v[i] + (1.0f - b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    const float w = w32[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w32[i] = w;
+    w16[i] = (__bf16)w;
+}
+
+__global__ void k_to_bf16(const float* __restrict__ a, __bf16* __restrict__ b, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (__bf16)a[i];
+}
+
+int n_cus(int device) {
+    int cus = 0;
+    check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "CU count");
+    return cus;
+}
+
+uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+Nrc::Nrc(uint32_t seed, const NrcConfig& c) : cfg(c) {
+    if (cfg.max_batch <= 0 || cfg.max_batch > (1 << 22)) throw std::runtime_error("nrc: max_batch out of range");
+    check_hip(hipGetDevice(&device), "hipGetDevice");
+    check_hip(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    w32.ensure(sizeof(float) * NRC_NPARAM);
+    w16.ensure(sizeof(__bf16) * NRC_NPARAM);
+    m.ensure(sizeof(float) * NRC_NPARAM);
+    v.ensure(sizeof(float) * NRC_NPARAM);
+    act.ensure(sizeof(__bf16) * 6 * 64 * size_t(cfg.max_batch));
+    gz.ensure(sizeof(__bf16) * 6 * 64 * size_t(cfg.max_batch));
+    part.ensure(sizeof(float) * 6 * size_t(4 * 64) * 4096);  // at most 64 splits x 4 waves per layer
+    scal.ensure(sizeof(float) * 4);
+    init_weights(seed);
+}
+
+Nrc::~Nrc() {
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Nrc::init_weights(uint32_t seed) {
+    std::vector<float> w(NRC_NPARAM, 0.0f);
+    uint64_t st = 0x6E7263ull ^ (uint64_t(seed) << 20);
+    for (int l = 0; l < 6; ++l) {
+        const int fan_in = l == 0 ? NRC_IN + 1 : 64;
+        const int rows = l < 5 ? 64 : 3, cols = l == 0 ? NRC_IN + 1 : 64;
+        const double bound = std::sqrt(6.0 / fan_in) * (l == 5 ? 0.1 : 1.0);
+        for (int o = 0; o < rows; ++o)
+            for (int i = 0; i < cols; ++i) {
+                const double u = double(splitmix64(st) >> 11) * (1.0 / 9007199254740992.0);
+                w[nrc_layer_offset(l) + o * 64 + i] = float((2.0 * u - 1.0) * bound);
+            }
+    }
+    set_weights(w.data());
+}
+
+void Nrc::set_weights(const float* w) {
+    check_hip(hipMemcpyAsync(w32.ptr, w, sizeof(float) * NRC_NPARAM, hipMemcpyHostToDevice, stream), "nrc weights");
+    check_hip(hipMemsetAsync(m.ptr, 0, sizeof(float) * NRC_NPARAM, stream), "nrc adam m");
+    check_hip(hipMemsetAsync(v.ptr, 0, sizeof(float) * NRC_NPARAM, stream), "nrc adam v");
+    k_to_bf16<<<(NRC_NPARAM + 255) / 256, 256, 0, stream>>>(w32.as<float>(), w16.as<__bf16>(), NRC_NPARAM);
+    check_hip(hipGetLastError(), "k_to_bf16");
+    check_hip(hipStreamSynchronize(stream), "nrc set_weights");
+    step = 0;
+}
+
+void Nrc::get_weights(float* w) {
+    check_hip(hipMemcpyAsync(w, w32.ptr, sizeof(float) * NRC_NPARAM, hipMemcpyDeviceToHost, stream), "nrc weights");
+    check_hip(hipStreamSynchronize(stream), "nrc get_weights");
+}
+
+void Nrc::infer(const NrcQuery* q, int n, float4* out) {
+    if (n <= 0) return;
+    const int chunks = (n + 63) / 64;
+    const int blocks = std::min((chunks + INF_WAVES - 1) / INF_WAVES, n_cus(device));
+    k_nrc_infer<false><<<blocks, INF_WAVES * 64, 0, stream>>>(w16.as<__bf16>(), q, nullptr, n, out);
+    check_hip(hipGetLastError(), "k_nrc_infer");
+}
+
+void Nrc::infer_accumulate(const NrcQuery* q, const int* count, int max_n, float4* radiance) {
+    if (max_n <= 0) return;
+    const int chunks = (max_n + 63) / 64;
+    const int blocks = std::min((chunks + INF_WAVES - 1) / INF_WAVES, n_cus(device));
+    k_nrc_infer<true><<<blocks, INF_WAVES * 64, 0, stream>>>(w16.as<__bf16>(), q, count, max_n, radiance);
+    check_hip(hipGetLastError(), "k_nrc_infer<accumulate>");
+}
+
+int Nrc::splits_for(int n) const {
+    const int chunks = (n + 31) / 32;
+    return std::max(1, std::min(64, (chunks + 31) / 32));  // about 8 chunks of 32 per wave
+}
+
+double Nrc::train(const NrcQuery* q, const float4* target, int n, bool want_loss, float* grad_out) {
+    if (n <= 0) return 0.0;
+    if (n > cfg.max_batch) throw std::runtime_error("nrc: training batch larger than max_batch");
+    const int chunks = (n + 63) / 64;
+    const int cus = n_cus(device);
+    check_hip(hipMemsetAsync(scal.ptr, 0, sizeof(float) * 4, stream), "nrc loss clear");
+    k_nrc_fwd_bwd<<<std::min(chunks, 2 * cus), 64, 0, stream>>>(w16.as<__bf16>(), q, target, n, 1.0f / float(n),
+                                                               act.as<__bf16>(), gz.as<__bf16>(), scal.as<float>());
+    check_hip(hipGetLastError(), "k_nrc_fwd_bwd");
+    const int splits = splits_for(n);
+    k_nrc_wgrad<<<dim3(splits, 6), 256, 0, stream>>>(act.as<__bf16>(), gz.as<__bf16>(), n, part.as<float>());
+    check_hip(hipGetLastError(), "k_nrc_wgrad");
+    DevBuf gtmp;
+    float* gdev = nullptr;
+    if (grad_out) {
+        gtmp.ensure(sizeof(float) * NRC_NPARAM);
+        gdev = gtmp.as<float>();
+    } else {
+        ++step;
+    }
+    const float c1 = 1.0f - std::pow(cfg.beta1, float(std::max(step, 1)));
+    const float c2 = 1.0f - std::pow(cfg.beta2, float(std::max(step, 1)));
+    k_nrc_adam<<<(NRC_NPARAM + 255) / 256, 256, 0, stream>>>(w32.as<float>(), w16.as<__bf16>(), m.as<float>(),
+                                                            v.as<float>(), part.as<float>(), splits * 4, cfg.lr,
+                                                            cfg.beta1, cfg.beta2, cfg.eps, c1, c2, gdev);
+    check_hip(hipGetLastError(), "k_nrc_adam");
+    double loss = 0.0;
+    if (grad_out) {
+        check_hip(hipMemcpyAsync(grad_out, gdev, sizeof(float) * NRC_NPARAM, hipMemcpyDeviceToHost, stream), "grad");
+    }
+    if (want_loss || grad_out) {
+        float l = 0.0f;
+        check_hip(hipMemcpyAsync(&l, scal.ptr, sizeof(float), hipMemcpyDeviceToHost, stream), "loss");
+        check_hip(hipStreamSynchronize(stream), "nrc train");
+        loss = double(l) / n;
+    }
+    return loss;
+}
+
+}  // namespace nr

@@ -59,6 +59,10 @@ def lib():
         L.orc_set_params.argtypes = [vp, f32p, C.c_int]
         L.orc_adjoint.argtypes = [vp, f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_float, C.c_int,
                                   C.c_int, C.c_int, f32p, f64p, C.c_int, C.c_int]
+        L.orc_nrc_encode.argtypes = [f32p, C.c_int, f32p]
+        L.orc_nrc_forward.argtypes = [f32p, f32p, C.c_int, C.c_int, f64p]
+        L.orc_nrc_grad.argtypes = [f32p, f32p, f32p, C.c_int, C.c_int, f64p]
+        L.orc_nrc_grad.restype = C.c_double
         L.orc_light_pick.argtypes = [vp, C.c_float]
         L.orc_light_p.argtypes = [vp]
         L.orc_light_p.restype = C.c_float
@@ -193,3 +197,29 @@ def camera_rays(cam, px, py, W, H):
     out = np.zeros((len(px), 3), np.float32)
     lib().orc_camera_rays(_f32(cam), len(px), px, py, float(W), float(H), out)
     return out
+
+
+# ---- radiance-cache network (orc_nrc.cpp)
+NRC_NPARAM = 5 * 4096 + 16 * 64
+
+
+def nrc_encode(q):
+    q = _f32(q).reshape(-1, 16)
+    out = np.zeros((len(q), 64), np.float32)
+    lib().orc_nrc_encode(q, len(q), out)
+    return out
+
+
+def nrc_forward(w, q, bf16=False):
+    q = _f32(q).reshape(-1, 16)
+    y = np.zeros((len(q), 3), np.float64)
+    lib().orc_nrc_forward(_f32(w), q, len(q), int(bf16), y)
+    return y
+
+
+def nrc_grad(w, q, t, bf16=False):
+    """(mean relative L2 loss, gradient of NRC_NPARAM doubles)."""
+    q = _f32(q).reshape(-1, 16)
+    g = np.zeros(NRC_NPARAM, np.float64)
+    loss = lib().orc_nrc_grad(_f32(w), q, _f32(t).reshape(-1, 3), len(q), int(bf16), g)
+    return loss, g

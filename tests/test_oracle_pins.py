@@ -461,3 +461,74 @@ def test_adjoint_roughness_expectation_on_ggx_plane(po):
     m, se = np.mean(runs), np.std(runs, ddof=1) / math.sqrt(len(runs))
     assert abs(m - dexp) < 5 * se + 2e-3 * abs(dexp), (m, se, dexp)
     assert abs(dexp) > 20 * se   # the check has power: the derivative is well resolved
+
+
+# ------------------------------------------------------------------ radiance-cache network
+def test_nrc_encoding_known_values(po):
+    """Triangle wave of renderernrc.cpp:50-53 at points with known values: tri(0) = 1,
+    tri(1) = -1, tri(1/2) = 0, period 2, and C fmod's sign for negative x: tri(-1/2) = 2."""
+    import nrcdata
+    q = np.zeros((4, 16), np.float32)
+    q[:, 0] = [0.0, 1.0, -0.5, 6.5]
+    q[:, 3] = [0.0, 1.0, 0.5, 2.0]
+    q[:, 4:8] = 0.25
+    q[:, 8:11] = 0.7
+    f = po.nrc_encode(q)
+    assert list(f[:, 0]) == [1.0, -1.0, 2.0, 0.0]          # x / 1: 0, 1, -0.5, 6.5 (fmod 0.5)
+    assert list(f[:, 1]) == [1.0, 0.0, 1.5, -0.5]          # x / 2: 0, 0.5, -0.25, 3.25 (fmod 1.25)
+    assert np.all(f[:, 36:40] == 0.25) and np.all(f[:, 41:44] == np.float32(0.7))
+    assert np.allclose(f[:, 40], np.exp(-q[:, 3]), rtol=1e-7)
+    assert np.all(f[:, 44] == 1.0) and not f[:, 45:].any()
+    assert np.abs(f[:, 0:36]).max() <= 5.0
+    # every frequency of a positive coordinate lies in [-1, 1]
+    g = po.nrc_encode(nrcdata.queries(1000, seed=3))
+    assert np.abs(g[:, 0:36]).max() <= 1.0
+
+
+def test_nrc_forward_identity_network(po):
+    """Weights that route the albedo inputs straight to the outputs (W^T rows = output units):
+    the exact network returns the albedo, the BF16 network its BF16 rounding."""
+    import nrcdata
+    q = nrcdata.queries(500, seed=1)
+    w = nrcdata.identity_weights(po.NRC_NPARAM)
+    y = po.nrc_forward(w, q)
+    assert np.array_equal(y, q[:, 8:11].astype(np.float64))
+    yb = po.nrc_forward(w, q, bf16=True)
+    a = q[:, 8:11].astype(np.float32)
+    ab = (a.view(np.uint32) + 0x7FFF + ((a.view(np.uint32) >> 16) & 1)) & 0xFFFF0000
+    assert np.array_equal(yb, ab.view(np.float32).astype(np.float64))
+
+
+def test_nrc_oracle_gradient_central_difference(po):
+    """The FP64 network's loss gradient against central differences.  The relative L2 loss holds
+    its denominator y^2 + 0.01 constant (stop-gradient), so the differenced function is
+    mean sum (y(w) - t)^2 / (y(w0)^2 + 0.01) with y(w0) at the base weights."""
+    import nrcdata
+    rng = np.random.Generator(np.random.PCG64(4))
+    w = np.zeros(po.NRC_NPARAM, np.float32)
+    for l in range(6):
+        rows = 64 if l < 5 else 3
+        cols = 45 if l == 0 else 64
+        n_rows = 64 if l < 5 else 16
+        w[l * 4096: l * 4096 + n_rows * 64].reshape(n_rows, 64)[:rows, :cols] = \
+            rng.uniform(-1, 1, (rows, cols)) * np.sqrt(6 / cols)
+    q = nrcdata.queries(64, seed=2)
+    t = nrcdata.smooth_target(q).astype(np.float64)
+    loss, g = po.nrc_grad(w, q, t)
+    y0 = po.nrc_forward(w, q)
+    den = y0 ** 2 + 0.01
+    assert abs(loss - ((y0 - t) ** 2 / den).sum(1).mean()) < 1e-12 * max(loss, 1.0)
+
+    def surrogate(wv):
+        y = po.nrc_forward(wv, q)
+        return ((y - t) ** 2 / den).sum(1).mean()
+    idx = [5, 64 * 3 + 44, 4096 + 70, 2 * 4096 + 64 * 10 + 9, 4 * 4096 + 64 * 63 + 1, 5 * 4096 + 2 * 64 + 17,
+           5 * 4096 + 5]
+    for i in idx:
+        h = 1e-5 * max(1.0, abs(float(w[i])))   # small enough that no ReLU changes side
+        wp, wm = w.copy(), w.copy()
+        wp[i] = np.float32(float(w[i]) + h)
+        wm[i] = np.float32(float(w[i]) - h)
+        fd = (surrogate(wp) - surrogate(wm)) / (float(wp[i]) - float(wm[i]))
+        assert abs(fd - g[i]) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(g).max()), (i, fd, g[i])
+    assert g[5 * 4096 + 3 * 64 + 5] == 0.0 and g[45] == 0.0   # unused output row, zero input
