@@ -158,6 +158,24 @@ int nr_nrc_train_step(nr_nrc* c, const float* queries, const float* targets, uin
 /* Gradient of the mean loss (NR_NRC_NPARAM floats) without an update. */
 int nr_nrc_gradient(nr_nrc* c, const float* queries, const float* targets, uint32_t n, float* grad, double* loss);
 
+/* RendererNRC::render (renderernrc.cpp:185-281), as the radiance-cache method: the frame's
+ * p.spp samples per pixel follow ptnee until the path spread a(x1..xn) exceeds spread * a0
+ * (a0 = the primary vertex's footprint), where the cache's answer beta * cache(x, wo) ends the path;
+ * train_paths extra paths from random pixels record up to train_maxlen vertices (the last answered
+ * by the cache) whose observed radiance trains the cache once per frame (batch <= the cache's
+ * max_batch).  use_cache = 0 renders plain ptnee; train = 0 skips the training step.  Results
+ * accumulate like nr_render_ptnee (params->accumulate) and are read with nr_render_fetch.
+ * stats (optional, 4 doubles): training loss, cache queries, recorded vertices, vertices trained. */
+typedef struct nr_nrc_params {
+    int32_t train_paths;
+    int32_t train_maxlen;
+    float spread;
+    int32_t use_cache;
+    int32_t train;
+} nr_nrc_params;
+int nr_render_nrc(nr_scene* s, nr_nrc* c, const nr_camera* cam, const nr_params* params, const nr_nrc_params* np,
+                  double* stats);
+
 /* No reference counterpart: a plain device-to-device copy of `bytes` bytes (16-byte vector loads
  * and stores), run `reps` times; *gbs = bytes read + written per second.  Calibrates the
  * hardware byte counters (a known number of HBM bytes per dispatch) and the achievable HBM rate. */

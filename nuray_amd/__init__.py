@@ -26,7 +26,7 @@ EXPORTS = [
     "nr_bvh_build_host", "nr_intersect", "nr_render_ptnee", "nr_render_fetch", "nr_render_stats",
     "nr_set_profiling", "nr_bench_copy", "nr_scene_set_params", "nr_loss_grad_l2", "nr_adjoint_ptnee",
     "nr_nrc_create", "nr_nrc_destroy", "nr_nrc_set_weights", "nr_nrc_get_weights", "nr_nrc_infer",
-    "nr_nrc_train_step", "nr_nrc_gradient",
+    "nr_nrc_train_step", "nr_nrc_gradient", "nr_render_nrc",
 ]
 
 NRC_NPARAM = 5 * 4096 + 16 * 64
@@ -40,6 +40,11 @@ class nr_params(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("spp_offset", C.c_int32),
                 ("seed", C.c_uint32), ("prr", C.c_float), ("mis", C.c_int32), ("maxdepth", C.c_int32),
                 ("rrdepth", C.c_int32), ("batch_spp", C.c_int32), ("accumulate", C.c_int32)]
+
+
+class nr_nrc_params(C.Structure):
+    _fields_ = [("train_paths", C.c_int32), ("train_maxlen", C.c_int32), ("spread", C.c_float),
+                ("use_cache", C.c_int32), ("train", C.c_int32)]
 
 
 PROGRESS = C.CFUNCTYPE(None, C.c_float, C.c_void_p)
@@ -88,6 +93,7 @@ def lib():
         L.nr_nrc_infer.argtypes = [vp, f32p, C.c_uint32, f32p]
         L.nr_nrc_train_step.argtypes = [vp, f32p, f32p, C.c_uint32, C.POINTER(C.c_double)]
         L.nr_nrc_gradient.argtypes = [vp, f32p, f32p, C.c_uint32, f32p, C.POINTER(C.c_double)]
+        L.nr_render_nrc.argtypes = [vp, vp, C.POINTER(nr_camera), C.POINTER(nr_params), C.POINTER(nr_nrc_params), vp]
         _lib = L
     return _lib
 
@@ -240,6 +246,21 @@ class Scene:
         dl = None if dl_dimg is None else _f32(dl_dimg).reshape(-1)
         check(lib().nr_adjoint_ptnee(self.h, C.byref(c), C.byref(p), None if dl is None else dl.ctypes.data, grad))
         return grad.reshape(n_mats, 4)
+
+    def render_nrc(self, cache, cam, W, H, spp=1, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0, spp_offset=0,
+                   accumulate=False, train_paths=16384, train_maxlen=5, spread=0.01, use_cache=True, train=True,
+                   fetch=False):
+        """One radiance-cache frame (nr_render_nrc).  Returns the frame statistics (and the fetched
+        image when fetch=True)."""
+        p = nr_params(W, H, spp, spp_offset, seed, prr, mis, maxdepth, rrdepth, 0, int(accumulate))
+        c = make_camera(cam)
+        npar = nr_nrc_params(train_paths, train_maxlen, spread, int(use_cache), int(train))
+        st = np.zeros(4, np.float64)
+        check(lib().nr_render_nrc(self.h, cache.h, C.byref(c), C.byref(p), C.byref(npar), st.ctypes.data))
+        stats = dict(loss=st[0], queries=int(st[1]), records=int(st[2]), trained=int(st[3]))
+        if fetch:
+            return stats, self.fetch(W, H)
+        return stats
 
     def set_profiling(self, mode):
         check(lib().nr_set_profiling(self.h, mode))

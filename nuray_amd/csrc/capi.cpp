@@ -370,4 +370,31 @@ int nr_nrc_gradient(nr_nrc* h, const float* queries, const float* targets, uint3
     });
 }
 
+int nr_render_nrc(nr_scene* s, nr_nrc* c, const nr_camera* cam, const nr_params* params, const nr_nrc_params* np,
+                  double* stats) {
+    return guard([&] {
+        if (!s || !c || !cam || !params || !np) throw std::runtime_error("null argument");
+        nr::CamView cv = nr::make_camera(cam->pos, cam->euler, cam->fov, cam->aspect);
+        nr::NrcRenderParams rp;
+        rp.train_paths = np->train_paths;
+        rp.train_maxlen = np->train_maxlen;
+        rp.c_spread = np->spread;
+        rp.use_cache = np->use_cache;
+        rp.train = np->train;
+        nr::NrcFrameStats fs;
+        nr::RenderStats st;
+        nr::render_nrc(*s->sc, *c->n, cv, to_params(params), rp, &fs, &st);
+        s->stats[0] = st.ms;
+        s->stats[1] = double(st.samples);
+        s->stats[2] = st.iterations;
+        if (stats) {
+            stats[0] = fs.train_loss;
+            stats[1] = double(fs.queries);
+            stats[2] = double(fs.records);
+            stats[3] = double(fs.trained);
+        }
+        return NR_OK;
+    });
+}
+
 }  // extern "C"

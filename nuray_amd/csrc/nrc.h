@@ -30,17 +30,24 @@ public:
     void set_weights(const float* w);   // NRC_NPARAM floats (W^T per layer)
     void get_weights(float* w);
 
-    // Device-buffer entry points (on `stream`).
-    void infer(const NrcQuery* q, int n, float4* out);
-    // Renderer form: radiance[slot] += beta * max(0, y) for each query (one query per slot).
-    void infer_accumulate(const NrcQuery* q, const int* count, int max_n, float4* radiance);
+    // Device-buffer entry points, on stream s (nullptr: the cache's own stream).
+    void infer(const NrcQuery* q, int n, float4* out, hipStream_t s = nullptr);
+    // Renderer form: radiance[slot] += beta * max(0, y) for queries [0, min(*count, max_n)), with
+    // slot = bits of q.c.w and beta = q.d.xyz (one query per slot).
+    void infer_accumulate(const NrcQuery* q, const int* count, int max_n, float4* radiance, hipStream_t s = nullptr);
     // One Adam step on n <= max_batch samples (targets rgb in .xyz).  grad_out (host, NRC_NPARAM)
     // when non-null: only the mean-loss gradient is computed and copied out, no update.
     // Returns the mean relative L2 loss when want_loss (synchronises), else 0.
-    double train(const NrcQuery* q, const float4* target, int n, bool want_loss, float* grad_out = nullptr);
+    double train(const NrcQuery* q, const float4* target, int n, bool want_loss, float* grad_out = nullptr,
+                 hipStream_t s = nullptr);
 
 private:
     int splits_for(int n) const;
 };
+
+// Training targets of recorded path vertices: target = (L_path - L_before) / beta, per channel,
+// where L_path = radiance[slot] at the end of the frame, L_before = lb[i] and beta = rec[i].d.
+void nrc_targets(const NrcQuery* rec, const float4* lb, const float4* radiance, int n, float4* target,
+                 hipStream_t s);
 
 }  // namespace nr

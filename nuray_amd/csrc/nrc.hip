@@ -269,6 +269,16 @@ __global__ void k_nrc_adam(float* __restrict__ w32, __bf16* __restrict__ w16, fl
     w16[i] = (__bf16)w;
 }
 
+__global__ void k_nrc_targets(const NrcQuery* __restrict__ rec, const float4* __restrict__ lb,
+                              const float4* __restrict__ radiance, int n, float4* __restrict__ target) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const NrcQuery r = rec[i];
+    const float4 L = radiance[__float_as_int(r.c.w)], b = lb[i];
+    auto q = [](float l, float l0, float beta) { return beta > 1e-8f ? (l - l0) / beta : 0.0f; };
+    target[i] = make_float4(q(L.x, b.x, r.d.x), q(L.y, b.y, r.d.y), q(L.z, b.z, r.d.z), 0.0f);
+}
+
 __global__ void k_to_bf16(const float* __restrict__ a, __bf16* __restrict__ b, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (__bf16)a[i];
@@ -288,6 +298,13 @@ uint64_t splitmix64(uint64_t& s) {
 }
 
 }  // namespace
+
+void nrc_targets(const NrcQuery* rec, const float4* lb, const float4* radiance, int n, float4* target,
+                 hipStream_t s) {
+    if (n <= 0) return;
+    k_nrc_targets<<<(n + 255) / 256, 256, 0, s>>>(rec, lb, radiance, n, target);
+    check_hip(hipGetLastError(), "k_nrc_targets");
+}
 
 Nrc::Nrc(uint32_t seed, const NrcConfig& c) : cfg(c) {
     if (cfg.max_batch <= 0 || cfg.max_batch > (1 << 22)) throw std::runtime_error("nrc: max_batch out of range");
@@ -339,19 +356,21 @@ void Nrc::get_weights(float* w) {
     check_hip(hipStreamSynchronize(stream), "nrc get_weights");
 }
 
-void Nrc::infer(const NrcQuery* q, int n, float4* out) {
+void Nrc::infer(const NrcQuery* q, int n, float4* out, hipStream_t s) {
+    if (!s) s = stream;
     if (n <= 0) return;
     const int chunks = (n + 63) / 64;
     const int blocks = std::min((chunks + INF_WAVES - 1) / INF_WAVES, n_cus(device));
-    k_nrc_infer<false><<<blocks, INF_WAVES * 64, 0, stream>>>(w16.as<__bf16>(), q, nullptr, n, out);
+    k_nrc_infer<false><<<blocks, INF_WAVES * 64, 0, s>>>(w16.as<__bf16>(), q, nullptr, n, out);
     check_hip(hipGetLastError(), "k_nrc_infer");
 }
 
-void Nrc::infer_accumulate(const NrcQuery* q, const int* count, int max_n, float4* radiance) {
+void Nrc::infer_accumulate(const NrcQuery* q, const int* count, int max_n, float4* radiance, hipStream_t s) {
+    if (!s) s = stream;
     if (max_n <= 0) return;
     const int chunks = (max_n + 63) / 64;
     const int blocks = std::min((chunks + INF_WAVES - 1) / INF_WAVES, n_cus(device));
-    k_nrc_infer<true><<<blocks, INF_WAVES * 64, 0, stream>>>(w16.as<__bf16>(), q, count, max_n, radiance);
+    k_nrc_infer<true><<<blocks, INF_WAVES * 64, 0, s>>>(w16.as<__bf16>(), q, count, max_n, radiance);
     check_hip(hipGetLastError(), "k_nrc_infer<accumulate>");
 }
 
@@ -360,17 +379,18 @@ int Nrc::splits_for(int n) const {
     return std::max(1, std::min(64, (chunks + 31) / 32));  // about 8 chunks of 32 per wave
 }
 
-double Nrc::train(const NrcQuery* q, const float4* target, int n, bool want_loss, float* grad_out) {
+double Nrc::train(const NrcQuery* q, const float4* target, int n, bool want_loss, float* grad_out, hipStream_t s) {
+    if (!s) s = stream;
     if (n <= 0) return 0.0;
     if (n > cfg.max_batch) throw std::runtime_error("nrc: training batch larger than max_batch");
     const int chunks = (n + 63) / 64;
     const int cus = n_cus(device);
-    check_hip(hipMemsetAsync(scal.ptr, 0, sizeof(float) * 4, stream), "nrc loss clear");
-    k_nrc_fwd_bwd<<<std::min(chunks, 2 * cus), 64, 0, stream>>>(w16.as<__bf16>(), q, target, n, 1.0f / float(n),
+    check_hip(hipMemsetAsync(scal.ptr, 0, sizeof(float) * 4, s), "nrc loss clear");
+    k_nrc_fwd_bwd<<<std::min(chunks, 2 * cus), 64, 0, s>>>(w16.as<__bf16>(), q, target, n, 1.0f / float(n),
                                                                act.as<__bf16>(), gz.as<__bf16>(), scal.as<float>());
     check_hip(hipGetLastError(), "k_nrc_fwd_bwd");
     const int splits = splits_for(n);
-    k_nrc_wgrad<<<dim3(splits, 6), 256, 0, stream>>>(act.as<__bf16>(), gz.as<__bf16>(), n, part.as<float>());
+    k_nrc_wgrad<<<dim3(splits, 6), 256, 0, s>>>(act.as<__bf16>(), gz.as<__bf16>(), n, part.as<float>());
     check_hip(hipGetLastError(), "k_nrc_wgrad");
     DevBuf gtmp;
     float* gdev = nullptr;
@@ -382,18 +402,18 @@ double Nrc::train(const NrcQuery* q, const float4* target, int n, bool want_loss
     }
     const float c1 = 1.0f - std::pow(cfg.beta1, float(std::max(step, 1)));
     const float c2 = 1.0f - std::pow(cfg.beta2, float(std::max(step, 1)));
-    k_nrc_adam<<<(NRC_NPARAM + 255) / 256, 256, 0, stream>>>(w32.as<float>(), w16.as<__bf16>(), m.as<float>(),
+    k_nrc_adam<<<(NRC_NPARAM + 255) / 256, 256, 0, s>>>(w32.as<float>(), w16.as<__bf16>(), m.as<float>(),
                                                             v.as<float>(), part.as<float>(), splits * 4, cfg.lr,
                                                             cfg.beta1, cfg.beta2, cfg.eps, c1, c2, gdev);
     check_hip(hipGetLastError(), "k_nrc_adam");
     double loss = 0.0;
     if (grad_out) {
-        check_hip(hipMemcpyAsync(grad_out, gdev, sizeof(float) * NRC_NPARAM, hipMemcpyDeviceToHost, stream), "grad");
+        check_hip(hipMemcpyAsync(grad_out, gdev, sizeof(float) * NRC_NPARAM, hipMemcpyDeviceToHost, s), "grad");
     }
     if (want_loss || grad_out) {
         float l = 0.0f;
-        check_hip(hipMemcpyAsync(&l, scal.ptr, sizeof(float), hipMemcpyDeviceToHost, stream), "loss");
-        check_hip(hipStreamSynchronize(stream), "nrc train");
+        check_hip(hipMemcpyAsync(&l, scal.ptr, sizeof(float), hipMemcpyDeviceToHost, s), "loss");
+        check_hip(hipStreamSynchronize(s), "nrc train");
         loss = double(l) / n;
     }
     return loss;

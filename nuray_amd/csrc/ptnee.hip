@@ -19,6 +19,7 @@
 #include <cstring>
 #include <vector>
 #include "nr_material.h"
+#include "nrc.h"
 #include "render.h"
 
 namespace nr {
@@ -77,7 +78,7 @@ constexpr int BLK = 256;
 constexpr int LDS_LIGHTS = 2048;  // light CDFs up to this size are searched in LDS
 
 // Counter slots
-enum { C_RAY0 = 0, C_RAY1 = 1, C_SHADOW = 2, C_WORK0 = 3, C_WORK1 = 4, C_ERR = 5 };
+enum { C_RAY0 = 0, C_RAY1 = 1, C_SHADOW = 2, C_WORK0 = 3, C_WORK1 = 4, C_ERR = 5, C_QUERY = 6, C_REC = 7 };
 
 __global__ void k_zero_doubles(double* a, double* b, size_t n) {
     for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
@@ -100,13 +101,9 @@ __global__ void k_set_counters(int* c, int ray_slot, int ray_value, int clear_sl
     }
 }
 
-// Camera rays: rendererptls.cpp:172 jitter (x + U, y + U), Camera::generateRay (camera.cpp:24-32).
-__global__ void k_raygen(CamView cam, int W, int H, int npix, int P, uint32_t seed, uint32_t sample0, Ray* rays,
-                         float4* beta, float4* radiance) {
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= P) return;
-    const int pixel = slot % npix;
-    const uint32_t sample = sample0 + uint32_t(slot / npix);
+// Camera ray of one sample: rendererptls.cpp:172 jitter (x + U, y + U), Camera::generateRay
+// (camera.cpp:24-32).
+__device__ __forceinline__ F3 camera_dir(const CamView& cam, int W, int H, int pixel, uint32_t seed, uint32_t sample) {
     const int x = pixel % W, y = pixel / W;
     const float4 u = rng4(seed, uint32_t(pixel), sample, 0u, 0u);
     const float ix = float(x) + u.x, iy = float(y) + u.y;
@@ -115,12 +112,69 @@ __global__ void k_raygen(CamView cam, int W, int H, int npix, int P, uint32_t se
     const F3 gaze = f3(cam.gaze[0], cam.gaze[1], cam.gaze[2]);
     const F3 hand = f3(cam.hand[0], cam.hand[1], cam.hand[2]);
     const F3 up = f3(cam.up[0], cam.up[1], cam.up[2]);
-    const F3 d = normalized(sub(add(gaze, mul(mul(mul(hand, cam.fw), cx), 0.5f)), mul(mul(mul(up, cam.fh), cy), 0.5f)));
+    return normalized(sub(add(gaze, mul(mul(mul(hand, cam.fw), cx), 0.5f)), mul(mul(mul(up, cam.fh), cy), 0.5f)));
+}
+
+__global__ void k_raygen(CamView cam, int W, int H, int npix, int P, uint32_t seed, uint32_t sample0, Ray* rays,
+                         float4* beta, float4* radiance) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= P) return;
+    const int pixel = slot % npix;
+    const F3 d = camera_dir(cam, W, H, pixel, seed, sample0 + uint32_t(slot / npix));
     rays[slot].o = make_float4(cam.pos[0], cam.pos[1], cam.pos[2], __int_as_float(0));
     rays[slot].d = make_float4(d.x, d.y, d.z, __int_as_float(slot));
     beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
     radiance[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
+
+// Sample index of training path i of a frame: its own Philox counters, disjoint from the render
+// samples (bit 31 set).
+__device__ __forceinline__ uint32_t train_sample(uint32_t frame_sample0, int T, int i) {
+    return 0x80000000u | (frame_sample0 * uint32_t(T) + uint32_t(i));
+}
+
+// Radiance-cache render: slots [0, P) are the frame's camera samples as in k_raygen, slots
+// [P, P + T) training paths from pixels drawn uniformly; every path starts with an empty
+// spread state.
+__global__ void k_raygen_nrc(CamView cam, int W, int H, int npix, int P, int T, uint32_t seed, uint32_t sample0,
+                             Ray* rays, float4* beta, float4* radiance, float4* state) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= P + T) return;
+    int pixel;
+    uint32_t sample;
+    if (slot < P) {
+        pixel = slot % npix;
+        sample = sample0 + uint32_t(slot / npix);
+    } else {
+        sample = train_sample(sample0, T, slot - P);
+        const uint4 h = philox(make_uint4(uint32_t(slot - P), sample0, 0x6E7263u, 0u), seed, 0x74726E67u);
+        pixel = int(h.x % uint32_t(npix));
+    }
+    const F3 d = camera_dir(cam, W, H, pixel, seed, sample);
+    rays[slot].o = make_float4(cam.pos[0], cam.pos[1], cam.pos[2], __int_as_float(0));
+    rays[slot].d = make_float4(d.x, d.y, d.z, __int_as_float(slot));
+    beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    radiance[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    state[slot] = make_float4(0.0f, 0.0f, 1.0f, __int_as_float(pixel));
+}
+
+// Radiance-cache part of the shade step.  Path state beside each queued ray: {sum of sqrt spread
+// increments, sqrt(c a0), pdf of the direction that led here, pixel (int bits)}.
+struct NrcShade {
+    const float4* sin;
+    float4* sout;
+    NrcQuery* queries;   // cache queries: the path ends here and adds beta * cache(x, wo)
+    NrcQuery* recs;      // training vertices (training paths only)
+    float4* rec_lb;      // radiance of the path before the recorded vertex
+    int* n_queries;
+    int* n_recs;
+    int query_cap, rec_cap;
+    int P_render;        // slots >= P_render are training paths
+    int T;
+    float c_spread;
+    int train_maxlen;
+    int use_cache;
+};
 
 struct ShadeArgs {
     SceneView sv;
@@ -175,6 +229,37 @@ __device__ __forceinline__ void block_append2(BlockAppend& sa, int parity, bool 
     p1 = o1 + __popcll(m1 & below);
 }
 
+// The same for four queues (radiance-cache shading: shadow, extension, cache query, training record).
+struct BlockAppend4 {
+    int cnt[2][4][NWAVE];
+    int base[2][4];
+};
+__device__ __forceinline__ void block_append4(BlockAppend4& sa, int parity, const bool (&e)[4], int* const (&c)[4],
+                                              int (&pos)[4]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned long long m[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) m[k] = __ballot(e[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sa.cnt[parity][k][wave] = __popcll(m[k]);
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int k = threadIdx.x;
+        int t = 0;
+        for (int w = 0; w < NWAVE; ++w) t += sa.cnt[parity][k][w];
+        sa.base[parity][k] = t ? atomicAdd(c[k], t) : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int o = sa.base[parity][k];
+        for (int w = 0; w < wave; ++w) o += sa.cnt[parity][k][w];
+        pos[k] = o + __popcll(m[k] & below);
+    }
+}
+
 // LightSampler::sampleLight over an LDS copy of the CDF (lower_bound, lightsampler.cpp:55-56).
 __device__ __forceinline__ int pick_light_lds(const float* cdf, int n, float u) {
     int lo = 0, hi = n;
@@ -186,8 +271,8 @@ __device__ __forceinline__ int pick_light_lds(const float* cdf, int n, float u) 
     return lo < n ? lo : n - 1;
 }
 
-template <unsigned F>
-__device__ __forceinline__ void shade_body(const ShadeArgs& a) {
+template <unsigned F, bool NRC>
+__device__ __forceinline__ void shade_body(const ShadeArgs& a, const NrcShade& nrc) {
     __shared__ float s_cdf[LDS_LIGHTS];
     const SceneView& sv = a.sv;
     const bool cdf_lds = sv.n_lights <= LDS_LIGHTS;
@@ -195,15 +280,18 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a) {
         for (int k = threadIdx.x; k < sv.n_lights; k += blockDim.x) s_cdf[k] = sv.light_cdf[k];
     __syncthreads();
     __shared__ BlockAppend s_app;
+    __shared__ BlockAppend4 s_app4;
     const int n = *a.n_in;
     const int stride = gridDim.x * blockDim.x;
     int parity = 0;
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride, parity ^= 1) {
         const int j = base + int(threadIdx.x);
         const bool active = j < n;
-        bool emit_sh = false, emit_next = false;
+        bool emit_sh = false, emit_next = false, emit_q = false, emit_rec = false;
         Ray sh_ray, next_ray;
         float4 sh_c = make_float4(0, 0, 0, 0), next_b = make_float4(0, 0, 0, 0);
+        float4 ns = make_float4(0, 0, 0, 0), rec_l = make_float4(0, 0, 0, 0);
+        NrcQuery qrec;
         if (active) {
             const Ray r = a.rin[j];
             const HitRec h = a.hits[j];
@@ -213,6 +301,7 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a) {
             const F3 beta = f3(bf.x, bf.y, bf.z);
             const float fac = bf.w;
             const F3 o = f3(r.o.x, r.o.y, r.o.z), d = f3(r.d.x, r.d.y, r.d.z);
+            if (NRC) ns = nrc.sin[j];
             F3 Le = f3(0.0f, 0.0f, 0.0f);  // radiance this vertex adds directly
             if (h.prim < 0) {
                 Le = mulv(beta, env_radiance(sv, d));
@@ -230,11 +319,48 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a) {
                     const float tv = w0 * ts.s2.z + b1 * ts.s3.x + b2 * ts.s3.z;
                     const F3 hp = add(o, mul(d, h.t));
                     const F3 wo = neg(d);
-                    const int pixel = slot % a.npix;
-                    const uint32_t sample = a.sample0 + uint32_t(slot / a.npix);
+                    const int pixel = NRC ? __float_as_int(ns.w) : slot % a.npix;
+                    const bool train = NRC && slot >= nrc.P_render;
+                    const uint32_t sample = train ? train_sample(a.sample0, nrc.T, slot - nrc.P_render)
+                                                  : a.sample0 + uint32_t(slot / a.npix);
+                    bool stop = false;  // the cache answers for this vertex and everything after it
+                    if (NRC) {
+                        // Path spread (Mueller et al. 2021): a0 = |x0 - x1|^2 / (4 pi cos1) at the primary
+                        // vertex, then sqrt(|x_i-1 - x_i|^2 / (p_i-1 |cos_i|)) summed; the render path
+                        // stops when the sum exceeds sqrt(c a0).
+                        const float cs = fabsf(dot(nrm, d));
+                        const float t2 = h.t * h.t;
+                        if (depth == 0) {
+                            ns.x = 0.0f;
+                            ns.y = sqrtf(nrc.c_spread * t2 / (4.0f * 3.14159265f * fmaxf(cs, 1e-6f)));
+                        } else {
+                            ns.x += sqrtf(t2 / fmaxf(ns.z * cs, 1e-20f));
+                        }
+                        bool query = false;
+                        if (!train) query = nrc.use_cache && depth >= 1 && ns.x > ns.y;
+                        else query = depth + 1 >= nrc.train_maxlen;
+                        if (query || train) {
+                            float th, ph, nth, nph;
+                            nrc_spherical(wo.x, wo.y, wo.z, th, ph);
+                            nrc_spherical(nrm.x, nrm.y, nrm.z, nth, nph);
+                            const F3 alb = mat_albedo<F>(sv, m, tu, tv);
+                            qrec.a = make_float4(hp.x, hp.y, hp.z, m.type == MAT_LAMBERT ? 0.0f : m.alpha);
+                            qrec.b = make_float4(th, ph, nth, nph);
+                            qrec.c = make_float4(alb.x, alb.y, alb.z, __int_as_float(slot));
+                            qrec.d = make_float4(beta.x, beta.y, beta.z, 0.0f);
+                        }
+                        if (train && !query) {  // the tail vertex answers from the cache: no target
+                            emit_rec = true;
+                            rec_l = a.radiance[slot];
+                        }
+                        if (query) {
+                            emit_q = true;
+                            stop = true;
+                        }
+                    }
                     const float4 ua = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 0u);
                     // NEE direction (rendererptls.cpp:50-69): light pick, point on it, wl
-                    const bool do_nee = mat_needs_nee(m) && sv.n_lights > 0;
+                    const bool do_nee = !stop && mat_needs_nee(m) && sv.n_lights > 0;
                     F3 wl = f3(0, 0, 0), lv = f3(0, 0, 0), ke = f3(0, 0, 0);
                     float lcos = 0.0f, ncos = 0.0f;
                     if (do_nee) {
@@ -255,10 +381,10 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a) {
                         ke = f3(lr.ke.x, lr.ke.y, lr.ke.z);
                     }
                     // Russian roulette and the continuation direction (rendererptls.cpp:80-88)
-                    bool do_next = true;
+                    bool do_next = !stop;
                     float prr = 1.0f;
                     if (depth >= a.rrdepth) {
-                        do_next = !(ua.w > a.prr);
+                        do_next = do_next && !(ua.w > a.prr);
                         prr = a.prr;
                     }
                     if (a.maxdepth > 0 && depth + 1 >= a.maxdepth) do_next = false;
@@ -307,6 +433,7 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a) {
                             next_ray.o = make_float4(no.x, no.y, no.z, __int_as_float(depth + 1));
                             next_ray.d = make_float4(wi.x, wi.y, wi.z, __int_as_float(slot));
                             next_b = make_float4(nb.x, nb.y, nb.z, fac_next);
+                            if (NRC) ns.z = p_i;
                         }
                     }
                 }
@@ -319,24 +446,48 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a) {
                 a.radiance[slot] = L;
             }
         }
-        int ps, pn;
-        block_append2(s_app, parity, emit_sh, emit_next, a.n_shadow, a.n_out, ps, pn);
-        if (emit_sh) {
-            a.shadow[ps] = sh_ray;
-            a.contrib[ps] = sh_c;
-        }
-        if (emit_next) {
-            a.rout[pn] = next_ray;
-            a.bout[pn] = next_b;
+        if (NRC) {
+            const bool e[4] = {emit_sh, emit_next, emit_q, emit_rec};
+            int* const c[4] = {a.n_shadow, a.n_out, nrc.n_queries, nrc.n_recs};
+            int pos[4];
+            block_append4(s_app4, parity, e, c, pos);
+            if (emit_sh) {
+                a.shadow[pos[0]] = sh_ray;
+                a.contrib[pos[0]] = sh_c;
+            }
+            if (emit_next) {
+                a.rout[pos[1]] = next_ray;
+                a.bout[pos[1]] = next_b;
+                nrc.sout[pos[1]] = ns;
+            }
+            if (emit_q && pos[2] < nrc.query_cap) nrc.queries[pos[2]] = qrec;
+            if (emit_rec && pos[3] < nrc.rec_cap) {
+                nrc.recs[pos[3]] = qrec;
+                nrc.rec_lb[pos[3]] = rec_l;
+            }
+        } else {
+            int ps, pn;
+            block_append2(s_app, parity, emit_sh, emit_next, a.n_shadow, a.n_out, ps, pn);
+            if (emit_sh) {
+                a.shadow[ps] = sh_ray;
+                a.contrib[ps] = sh_c;
+            }
+            if (emit_next) {
+                a.rout[pn] = next_ray;
+                a.bout[pn] = next_b;
+            }
         }
     }
 }
 
 // Lambert / GGX-reflect / light scenes without textures: 96 VGPRs, 5 waves per SIMD, no spills.
 __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_shade_basic(ShadeArgs a) {
-    shade_body<0u>(a);
+    shade_body<0u, false>(a, NrcShade{});
 }
-__global__ __launch_bounds__(BLK) void k_shade_full(ShadeArgs a) { shade_body<F_ALL>(a); }
+__global__ __launch_bounds__(BLK) void k_shade_full(ShadeArgs a) { shade_body<F_ALL, false>(a, NrcShade{}); }
+// Radiance-cache rendering (spread state, cache queries, training records).
+__global__ __launch_bounds__(BLK) void k_shade_nrc_basic(ShadeArgs a, NrcShade n) { shade_body<0u, true>(a, n); }
+__global__ __launch_bounds__(BLK) void k_shade_nrc_full(ShadeArgs a, NrcShade n) { shade_body<F_ALL, true>(a, n); }
 
 // Per-pixel reduction of the batch's kb samples: sums of max(0, L) and its square in FP64
 // (rendererptls.cpp:173-175: max(0, trace(...)) per component, which also maps NaN to 0).
@@ -485,6 +636,91 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
     if (ts && tstats) check_hip(hipMemcpy(tstats, ts, 2 * sizeof(TraceStats), hipMemcpyDeviceToHost), "stats");
     if (st) *st = rs;
     return status;
+}
+
+int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, const NrcRenderParams& np,
+               NrcFrameStats* fs, RenderStats* st) {
+    if (!sc.built) throw std::runtime_error("scene not built (call nr_bvh_build_sah)");
+    if (p.W <= 0 || p.H <= 0 || p.spp <= 0) throw std::runtime_error("bad image size or spp");
+    if (np.train_paths < 0 || np.train_maxlen < 1) throw std::runtime_error("bad radiance-cache parameters");
+    const int npix = p.W * p.H;
+    const size_t P = size_t(p.spp) * npix, T = size_t(np.train_paths);
+    if (P + T > (size_t(64) << 20)) throw std::runtime_error("radiance-cache frame larger than one wavefront batch");
+    if (!sc.target) sc.target = new RenderTarget();
+    RenderTarget& t = *sc.target;
+    const size_t cap = P + T, rec_cap = std::max<size_t>(1, T * size_t(np.train_maxlen));
+    t.ensure(p.W, p.H, cap);
+    for (int k = 0; k < 2; ++k) t.nstate[k].ensure(sizeof(float4) * cap);
+    t.nrc_queries.ensure(sizeof(NrcQuery) * cap);
+    t.nrc_recs.ensure(sizeof(NrcQuery) * rec_cap);
+    t.nrc_lb.ensure(sizeof(float4) * rec_cap);
+    t.nrc_tgt.ensure(sizeof(float4) * rec_cap);
+    t.adj_valid = false;
+    hipStream_t s = sc.stream;
+    int* cnt = t.counters.as<int>();
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(64), 0, s, cnt);
+    if (!p.accumulate) {
+        hipLaunchKernelGGL(k_zero_doubles, dim3(1024), dim3(BLK), 0, s, t.sum.as<double>(), t.sum2.as<double>(),
+                           size_t(npix) * 3);
+        t.spp_total = 0;
+    }
+    const int grid_shade = device_cu_count() * 8;
+    auto t0 = std::chrono::steady_clock::now();
+    RenderStats rs;
+    hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, int(C_RAY0), int(cap), -1);
+    hipLaunchKernelGGL(k_raygen_nrc, dim3((cap + BLK - 1) / BLK), dim3(BLK), 0, s, cam, p.W, p.H, npix, int(P), int(T),
+                       p.seed, uint32_t(p.spp_offset), t.rays[0].as<Ray>(), t.beta[0].as<float4>(),
+                       t.radiance.as<float4>(), t.nstate[0].as<float4>());
+    for (int it = 0;; ++it) {
+        const int cur = it & 1, nxt = cur ^ 1;
+        hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, -1, 0, int(C_RAY0 + nxt));
+        launch_closest(sc.view, t.rays[cur].as<Ray>(), cnt + C_RAY0 + cur, t.hits.as<HitRec>(), cnt + C_WORK0,
+                       cnt + C_ERR, nullptr, s);
+        ShadeArgs a{sc.view, npix, p.seed, uint32_t(p.spp_offset), p.prr, p.mis, p.maxdepth, p.rrdepth,
+                    t.rays[cur].as<Ray>(), t.beta[cur].as<float4>(), t.hits.as<HitRec>(), cnt + C_RAY0 + cur,
+                    t.radiance.as<float4>(), t.rays[nxt].as<Ray>(), t.beta[nxt].as<float4>(), cnt + C_RAY0 + nxt,
+                    t.shadow.as<Ray>(), t.contrib.as<float4>(), cnt + C_SHADOW};
+        NrcShade ns{t.nstate[cur].as<float4>(), t.nstate[nxt].as<float4>(), t.nrc_queries.as<NrcQuery>(),
+                    t.nrc_recs.as<NrcQuery>(), t.nrc_lb.as<float4>(), cnt + C_QUERY, cnt + C_REC, int(cap),
+                    int(rec_cap), int(P), int(T), np.c_spread, np.train_maxlen, np.use_cache};
+        if (sc.features == 0) hipLaunchKernelGGL(k_shade_nrc_basic, dim3(grid_shade), dim3(BLK), 0, s, a, ns);
+        else hipLaunchKernelGGL(k_shade_nrc_full, dim3(grid_shade), dim3(BLK), 0, s, a, ns);
+        launch_shadow(sc.view, t.shadow.as<Ray>(), cnt + C_SHADOW, nullptr, nullptr, t.contrib.as<float4>(),
+                      t.radiance.as<float4>(), cnt + C_WORK1, cnt + C_ERR, nullptr, s);
+        rs.iterations++;
+        if (p.maxdepth > 0) {
+            if (it + 1 >= p.maxdepth) break;
+        } else {
+            check_hip(hipMemcpyAsync(t.pinned, cnt + C_RAY0 + nxt, sizeof(int), hipMemcpyDeviceToHost, s), "cnt");
+            check_hip(hipStreamSynchronize(s), "sync");
+            if (*t.pinned == 0) break;
+        }
+    }
+    // the cache answers every stopped path (render paths and training-path tails)
+    nrc.infer_accumulate(t.nrc_queries.as<NrcQuery>(), cnt + C_QUERY, int(cap), t.radiance.as<float4>(), s);
+    hipLaunchKernelGGL(k_finalize, dim3((npix + BLK - 1) / BLK), dim3(BLK), 0, s, t.radiance.as<float4>(), npix,
+                       p.spp, t.sum.as<double>(), t.sum2.as<double>());
+    check_hip(hipGetLastError(), "nrc frame launch");
+    t.spp_total += p.spp;
+    rs.samples = (long long)P;
+    check_hip(hipMemcpyAsync(t.pinned, cnt, sizeof(int) * 8, hipMemcpyDeviceToHost, s), "counters");
+    check_hip(hipStreamSynchronize(s), "nrc frame sync");
+    if (t.pinned[C_ERR]) throw std::runtime_error("traversal stack overflow");
+    NrcFrameStats f;
+    f.queries = t.pinned[C_QUERY];
+    f.records = std::min<long long>(t.pinned[C_REC], (long long)rec_cap);
+    if (np.train && f.records > 0) {
+        const int n = int(std::min<long long>(f.records, nrc.cfg.max_batch));
+        nrc_targets(t.nrc_recs.as<NrcQuery>(), t.nrc_lb.as<float4>(), t.radiance.as<float4>(), n, t.nrc_tgt.as<float4>(),
+                    s);
+        f.train_loss = nrc.train(t.nrc_recs.as<NrcQuery>(), t.nrc_tgt.as<float4>(), n, fs != nullptr, nullptr, s);
+        f.trained = n;
+    }
+    check_hip(hipStreamSynchronize(s), "nrc train sync");
+    rs.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (fs) *fs = f;
+    if (st) *st = rs;
+    return 0;
 }
 
 void render_fetch(Scene& sc, float* rgb, double* sum, double* sum2, uint8_t* rgb8) {

@@ -52,9 +52,27 @@ struct RenderTarget {
     int adj_spp = 0, adj_spp0 = 0;
     uint32_t adj_seed = 0;
     DevBuf adj_target, adj_dldi, adj_loss, adj_grad;
+    // radiance-cache rendering: spread state per queued ray, cache queries, training vertices
+    DevBuf nstate[2], nrc_queries, nrc_recs, nrc_lb, nrc_tgt;
     ~RenderTarget();
     void ensure(int W, int H, size_t cap);
 };
+
+// RendererNRC::render parameters (renderernrc.cpp:185-281), with the spread rule of the
+// radiance-cache method in place of the reference's fixed depth limit.
+struct NrcRenderParams {
+    int train_paths = 16384;  // training paths per frame (pixels drawn uniformly)
+    int train_maxlen = 5;     // vertices of a training path; the last one is answered by the cache
+    float c_spread = 0.01f;   // a render path stops at x_n when a(x1..xn) > c a0
+    int use_cache = 1;        // 0: render paths are plain ptnee (training still runs)
+    int train = 1;            // 0: no training step this frame
+};
+struct NrcFrameStats {
+    double train_loss = 0;
+    long long queries = 0, records = 0, trained = 0;
+};
+
+class Nrc;
 
 typedef void (*ProgressFn)(float, void*);
 
@@ -62,6 +80,11 @@ typedef void (*ProgressFn)(float, void*);
 int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressFn cb, void* user,
                  volatile int* cancel, RenderStats* st, bool trace_stats = false, TraceStats* tstats = nullptr,
                  KernelTimes* kt = nullptr);
+// One radiance-cache frame: p.spp samples per pixel whose paths stop at the spread rule and take
+// the cache's answer there, plus the training paths, then one training step on their vertices.
+// Accumulates into the same sums as render_ptnee (p.accumulate).
+int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, const NrcRenderParams& np,
+               NrcFrameStats* fs, RenderStats* st);
 // Copies the accumulated result: linear mean (float W*H*3), FP64 sums, gamma-2.2 RGB8.
 void render_fetch(Scene& sc, float* rgb, double* sum, double* sum2, uint8_t* rgb8);
 
