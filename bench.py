@@ -8,7 +8,8 @@ triangles).  The scene is built and resident in HBM before the timed region; a s
 nr_render_ptnee call (all wavefront batches of the frame, FP64 accumulation included).
 
 The inverse-rendering iteration (render + loss + path-replay adjoint + Adam on the 256x256
-64-material box room) is timed after the frames and reported as "adjoint".
+64-material box room) is timed after the frames and reported as "adjoint"; the time to RMSE 1e-2
+of plain ptnee and of the radiance cache at 1920x1080, 1 spp per frame, as "nrc".
 
 Printed JSON line: value = Msamples/s over the timed steps, the traversal roofline (HIP events
 around every closest-hit launch; algorithmic bytes from the kernel's own node / triangle
@@ -107,6 +108,74 @@ def adjoint_iteration(nr, scenes, iters=20, warmup=5):
                         "iteration, L2 loss, path-replay adjoint, Adam lr 1e-2"}
 
 
+def nrc_time_to_rmse(nr, scenes, eps=1e-2, ref_spp=4096, warm=256, k_small=16, k_large=512):
+    """Time to RMSE `eps` against a high-spp reference, plain ptnee vs the radiance cache, on the
+    1920x1080 heightfield + spheres scene at 1 spp per frame (maxdepth 10, RR from vertex 3).
+
+    Reference: ptnee at ref_spp spp (FP64 sums); the reference's own variance (from its sums of
+    squares) is subtracted from every measured MSE, so the figures estimate the error against the
+    expectation.  Plain PT is unbiased: MSE(K frames) = s2 / K, s2 measured on a 256-frame
+    accumulation, time = frame_ms * s2 / eps^2.  Radiance cache: `warm` frames of online training
+    from a fresh network (timed, not accumulated), then frames that keep training accumulated from
+    scratch; MSE(K) = b2 + s2 / K fitted at K = k_small and k_large, time = warm-up +
+    frame_ms * s2 / (eps^2 - b2) (never, if b2 >= eps^2)."""
+    W, H = 1920, 1080
+    sc = scenes.terrain_spheres(width=W, height=H)
+    cam = sc["camera"]
+    kw = dict(maxdepth=10, rrdepth=3)
+    dev = nr.Scene(sc)
+    t0 = time.perf_counter()
+    r = dev.render(cam, W, H, ref_spp, seed=999, **kw)
+    ref = r["sum"] / ref_spp
+    var_ref = float(np.maximum(r["sum2"] / ref_spp - ref ** 2, 0.0).mean() / ref_spp)
+    ref_s = time.perf_counter() - t0
+
+    def mse():
+        return float(((dev.fetch(W, H)["sum"] / dev_spp[0] - ref) ** 2).mean()) - var_ref
+    dev_spp = [1]
+    # plain ptnee
+    K = 256
+    nr.synchronize()
+    t0 = time.perf_counter()
+    for k in range(K):
+        dev.render(cam, W, H, 1, seed=5, spp_offset=k, accumulate=k > 0, fetch=False, **kw)
+    nr.synchronize()
+    pt_ms = (time.perf_counter() - t0) * 1e3 / K
+    dev_spp[0] = K
+    pt_s2 = mse() * K
+    pt_time = pt_ms * 1e-3 * pt_s2 / eps ** 2
+    # radiance cache
+    cache = nr.Nrc(seed=1)
+    nr.synchronize()
+    t0 = time.perf_counter()
+    for k in range(warm):
+        dev.render_nrc(cache, cam, W, H, spp=1, seed=3, spp_offset=k, **kw)
+    nr.synchronize()
+    warm_s = time.perf_counter() - t0
+    fit = {}
+    frame = 0
+    for kk in (k_small, k_large):
+        nr.synchronize()
+        t0 = time.perf_counter()
+        for k in range(kk):
+            st = dev.render_nrc(cache, cam, W, H, spp=1, seed=11, spp_offset=warm + frame, accumulate=k > 0, **kw)
+            frame += 1
+        nr.synchronize()
+        fit[kk] = ((time.perf_counter() - t0) * 1e3 / kk, None)
+        dev_spp[0] = kk
+        fit[kk] = (fit[kk][0], mse())
+    s2 = (fit[k_small][1] - fit[k_large][1]) / (1.0 / k_small - 1.0 / k_large)
+    b2 = fit[k_large][1] - s2 / k_large
+    nrc_ms = fit[k_large][0]
+    nrc_time = warm_s + nrc_ms * 1e-3 * s2 / (eps ** 2 - b2) if b2 < eps ** 2 else None
+    return {"eps": eps, "pt_s": pt_time, "nrc_s": nrc_time, "speedup": (pt_time / nrc_time) if nrc_time else None,
+            "pt_frame_ms": pt_ms, "pt_var_per_frame": pt_s2, "nrc_frame_ms": nrc_ms, "nrc_var_per_frame": s2,
+            "nrc_bias2": b2, "nrc_warmup_s": warm_s, "nrc_queries_per_frame": st["queries"],
+            "nrc_trained_per_frame": st["trained"], "reference_spp": ref_spp, "reference_s": ref_s,
+            "workload": "1920x1080, 1 spp per frame, heightfield + spheres scene, maxdepth 10; time to RMSE "
+                        "extrapolated from measured per-frame variance and cache bias"}
+
+
 def measured_traffic():
     """Calibrated HBM bytes per closest-hit launch from the committed rocprofv3 counter summary."""
     path = os.path.join(ROOT, "profiles", "trace_closest_720p_traffic.txt")
@@ -125,6 +194,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-nrc", action="store_true", help="skip the time-to-RMSE measurement (about 40 s)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the plain 1 GiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -188,6 +258,10 @@ def main():
     adj = adjoint_iteration(nr, scenes)
     print(f"# adjoint iteration {adj['ms_per_iter']:.2f} ms (render {adj['render_ms']:.2f} ms, "
           f"replay {adj['adjoint_ms']:.2f} ms)", file=sys.stderr, flush=True)
+    nrc = None if args.no_nrc else nrc_time_to_rmse(nr, scenes)
+    if nrc:
+        print(f"# time to RMSE 1e-2 at 1080p: PT {nrc['pt_s']:.1f} s, radiance cache "
+              f"{nrc['nrc_s'] if nrc['nrc_s'] is None else round(nrc['nrc_s'], 1)} s", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(sc, spp)
     out = {
         "metric": "Msamples/s at 720p", "value": value, "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
@@ -197,7 +271,7 @@ def main():
                                f"({len(sc['p'])} triangles) + 1024x512 sky", "global_batch": W * H * spp,
                    "seq_len": None, "parallelism": "single-gpu", "bvh_build_ms": info["build_ms"]},
         "roofline": roofline, "cpu_baseline": cpu, "hbm_copy_gbs": copy_gbs,
-        "adjoint": adj,
+        "adjoint": adj, "nrc": nrc,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]

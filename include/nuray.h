@@ -145,7 +145,8 @@ int nr_adjoint_ptnee(nr_scene* s, const nr_camera* cam, const nr_params* params,
 #define NR_NRC_NPARAM 21504
 typedef struct nr_nrc nr_nrc;
 /* He-uniform initial weights from `seed`; lr of Adam (beta1 0.9, beta2 0.99, eps 1e-8);
- * max_batch training samples per step (<= 0: 65536). */
+ * max_batch training samples per step (<= 0: 65536).  Inference (nr_nrc_infer and rendering)
+ * uses the bias-corrected exponential moving average (decay 0.99) of the trained weights. */
 nr_nrc* nr_nrc_create(uint32_t seed, float lr, int32_t max_batch);
 int nr_nrc_destroy(nr_nrc* c);
 int nr_nrc_set_weights(nr_nrc* c, const float* w, uint32_t n);  /* also resets Adam */
@@ -172,6 +173,8 @@ typedef struct nr_nrc_params {
     float spread;
     int32_t use_cache;
     int32_t train;
+    int32_t train_steps; /* Adam steps per frame over disjoint parts of the recorded vertices */
+    int32_t min_depth;   /* render paths query the cache only from this vertex index on (>= 1) */
 } nr_nrc_params;
 int nr_render_nrc(nr_scene* s, nr_nrc* c, const nr_camera* cam, const nr_params* params, const nr_nrc_params* np,
                   double* stats);

@@ -44,7 +44,8 @@ class nr_params(C.Structure):
 
 class nr_nrc_params(C.Structure):
     _fields_ = [("train_paths", C.c_int32), ("train_maxlen", C.c_int32), ("spread", C.c_float),
-                ("use_cache", C.c_int32), ("train", C.c_int32)]
+                ("use_cache", C.c_int32), ("train", C.c_int32), ("train_steps", C.c_int32),
+                ("min_depth", C.c_int32)]
 
 
 PROGRESS = C.CFUNCTYPE(None, C.c_float, C.c_void_p)
@@ -248,13 +249,13 @@ class Scene:
         return grad.reshape(n_mats, 4)
 
     def render_nrc(self, cache, cam, W, H, spp=1, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0, spp_offset=0,
-                   accumulate=False, train_paths=16384, train_maxlen=5, spread=0.01, use_cache=True, train=True,
-                   fetch=False):
+                   accumulate=False, train_paths=65536, train_maxlen=5, spread=0.01, use_cache=True, train=True,
+                   train_steps=4, min_depth=1, fetch=False):
         """One radiance-cache frame (nr_render_nrc).  Returns the frame statistics (and the fetched
         image when fetch=True)."""
         p = nr_params(W, H, spp, spp_offset, seed, prr, mis, maxdepth, rrdepth, 0, int(accumulate))
         c = make_camera(cam)
-        npar = nr_nrc_params(train_paths, train_maxlen, spread, int(use_cache), int(train))
+        npar = nr_nrc_params(train_paths, train_maxlen, spread, int(use_cache), int(train), train_steps, min_depth)
         st = np.zeros(4, np.float64)
         check(lib().nr_render_nrc(self.h, cache.h, C.byref(c), C.byref(p), C.byref(npar), st.ctypes.data))
         stats = dict(loss=st[0], queries=int(st[1]), records=int(st[2]), trained=int(st[3]))

@@ -381,6 +381,8 @@ int nr_render_nrc(nr_scene* s, nr_nrc* c, const nr_camera* cam, const nr_params*
         rp.c_spread = np->spread;
         rp.use_cache = np->use_cache;
         rp.train = np->train;
+        rp.train_steps = np->train_steps;
+        rp.min_depth = np->min_depth;
         nr::NrcFrameStats fs;
         nr::RenderStats st;
         nr::render_nrc(*s->sc, *c->n, cv, to_params(params), rp, &fs, &st);

@@ -8,6 +8,7 @@ namespace nr {
 
 struct NrcConfig {
     float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-8f;
+    float ema = 0.99f;         // inference uses the bias-corrected exponential average of the weights
     int max_batch = 1 << 16;   // training samples per step
 };
 
@@ -17,7 +18,8 @@ public:
     int step = 0;
     int device = 0;
     hipStream_t stream = nullptr;
-    DevBuf w32, w16, m, v;     // FP32 master weights, BF16 copy, Adam moments
+    DevBuf w32, w16, m, v;     // FP32 master weights, BF16 copy (training), Adam moments
+    DevBuf wema, w16inf;       // FP32 weight average, BF16 inference weights
     DevBuf act, gz;            // training: layer inputs and pre-activation gradients, [6][batch][64] bf16
     DevBuf part;               // training: per-wave weight-gradient partials
     DevBuf scal;               // float[4]: loss sum
@@ -49,5 +51,10 @@ private:
 // where L_path = radiance[slot] at the end of the frame, L_before = lb[i] and beta = rec[i].d.
 void nrc_targets(const NrcQuery* rec, const float4* lb, const float4* radiance, int n, float4* target,
                  hipStream_t s);
+
+// Splits n records into `steps` batches of interleaved elements (batch k = records k, k + steps, ...),
+// stored one after another in out / tout; batch k starts at k * (n / steps) + min(k, n % steps).
+void nrc_interleave(const NrcQuery* in, const float4* tin, int n, int steps, NrcQuery* out, float4* tout,
+                    hipStream_t s);
 
 }  // namespace nr

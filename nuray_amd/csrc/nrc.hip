@@ -248,8 +248,9 @@ __global__ __launch_bounds__(256) void k_nrc_wgrad(const __bf16* __restrict__ ac
 }
 
 __global__ void k_nrc_adam(float* __restrict__ w32, __bf16* __restrict__ w16, float* __restrict__ m,
-                           float* __restrict__ v, const float* __restrict__ part, int P, float lr, float b1, float b2,
-                           float eps, float c1, float c2, float* __restrict__ grad_out) {
+                           float* __restrict__ v, float* __restrict__ wema, __bf16* __restrict__ w16inf,
+                           const float* __restrict__ part, int P, float lr, float b1, float b2, float eps, float c1,
+                           float c2, float ema, float ce, float* __restrict__ grad_out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= NRC_NPARAM) return;
     const int l = i < 5 * 4096 ? i / 4096 : 5, k = i - l * 4096;
@@ -267,6 +268,13 @@ __global__ void k_nrc_adam(float* __restrict__ w32, __bf16* __restrict__ w16, fl
     const float w = w32[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
     w32[i] = w;
     w16[i] = (__bf16)w;
+    if (ema > 0.0f) {
+        const float e = ema * wema[i] + (1.0f - ema) * w;
+        wema[i] = e;
+        w16inf[i] = (__bf16)(e / ce);
+    } else {
+        w16inf[i] = (__bf16)w;
+    }
 }
 
 __global__ void k_nrc_targets(const NrcQuery* __restrict__ rec, const float4* __restrict__ lb,
@@ -277,6 +285,16 @@ __global__ void k_nrc_targets(const NrcQuery* __restrict__ rec, const float4* __
     const float4 L = radiance[__float_as_int(r.c.w)], b = lb[i];
     auto q = [](float l, float l0, float beta) { return beta > 1e-8f ? (l - l0) / beta : 0.0f; };
     target[i] = make_float4(q(L.x, b.x, r.d.x), q(L.y, b.y, r.d.y), q(L.z, b.z, r.d.z), 0.0f);
+}
+
+__global__ void k_nrc_interleave(const NrcQuery* __restrict__ in, const float4* __restrict__ tin, int n, int steps,
+                                 NrcQuery* __restrict__ out, float4* __restrict__ tout) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = i % steps, j = i / steps, base = n / steps, rem = n % steps;
+    const int o = k * base + min(k, rem) + j;
+    out[o] = in[i];
+    tout[o] = tin[i];
 }
 
 __global__ void k_to_bf16(const float* __restrict__ a, __bf16* __restrict__ b, int n) {
@@ -306,6 +324,13 @@ void nrc_targets(const NrcQuery* rec, const float4* lb, const float4* radiance, 
     check_hip(hipGetLastError(), "k_nrc_targets");
 }
 
+void nrc_interleave(const NrcQuery* in, const float4* tin, int n, int steps, NrcQuery* out, float4* tout,
+                    hipStream_t s) {
+    if (n <= 0) return;
+    k_nrc_interleave<<<(n + 255) / 256, 256, 0, s>>>(in, tin, n, steps, out, tout);
+    check_hip(hipGetLastError(), "k_nrc_interleave");
+}
+
 Nrc::Nrc(uint32_t seed, const NrcConfig& c) : cfg(c) {
     if (cfg.max_batch <= 0 || cfg.max_batch > (1 << 22)) throw std::runtime_error("nrc: max_batch out of range");
     check_hip(hipGetDevice(&device), "hipGetDevice");
@@ -314,6 +339,8 @@ Nrc::Nrc(uint32_t seed, const NrcConfig& c) : cfg(c) {
     w16.ensure(sizeof(__bf16) * NRC_NPARAM);
     m.ensure(sizeof(float) * NRC_NPARAM);
     v.ensure(sizeof(float) * NRC_NPARAM);
+    wema.ensure(sizeof(float) * NRC_NPARAM);
+    w16inf.ensure(sizeof(__bf16) * NRC_NPARAM);
     act.ensure(sizeof(__bf16) * 6 * 64 * size_t(cfg.max_batch));
     gz.ensure(sizeof(__bf16) * 6 * 64 * size_t(cfg.max_batch));
     part.ensure(sizeof(float) * 6 * size_t(4 * 64) * 4096);  // at most 64 splits x 4 waves per layer
@@ -345,7 +372,10 @@ void Nrc::set_weights(const float* w) {
     check_hip(hipMemcpyAsync(w32.ptr, w, sizeof(float) * NRC_NPARAM, hipMemcpyHostToDevice, stream), "nrc weights");
     check_hip(hipMemsetAsync(m.ptr, 0, sizeof(float) * NRC_NPARAM, stream), "nrc adam m");
     check_hip(hipMemsetAsync(v.ptr, 0, sizeof(float) * NRC_NPARAM, stream), "nrc adam v");
+    check_hip(hipMemsetAsync(wema.ptr, 0, sizeof(float) * NRC_NPARAM, stream), "nrc ema");
     k_to_bf16<<<(NRC_NPARAM + 255) / 256, 256, 0, stream>>>(w32.as<float>(), w16.as<__bf16>(), NRC_NPARAM);
+    check_hip(hipMemcpyAsync(w16inf.ptr, w16.ptr, sizeof(__bf16) * NRC_NPARAM, hipMemcpyDeviceToDevice, stream),
+              "nrc inference weights");
     check_hip(hipGetLastError(), "k_to_bf16");
     check_hip(hipStreamSynchronize(stream), "nrc set_weights");
     step = 0;
@@ -361,7 +391,7 @@ void Nrc::infer(const NrcQuery* q, int n, float4* out, hipStream_t s) {
     if (n <= 0) return;
     const int chunks = (n + 63) / 64;
     const int blocks = std::min((chunks + INF_WAVES - 1) / INF_WAVES, n_cus(device));
-    k_nrc_infer<false><<<blocks, INF_WAVES * 64, 0, s>>>(w16.as<__bf16>(), q, nullptr, n, out);
+    k_nrc_infer<false><<<blocks, INF_WAVES * 64, 0, s>>>(w16inf.as<__bf16>(), q, nullptr, n, out);
     check_hip(hipGetLastError(), "k_nrc_infer");
 }
 
@@ -370,7 +400,7 @@ void Nrc::infer_accumulate(const NrcQuery* q, const int* count, int max_n, float
     if (max_n <= 0) return;
     const int chunks = (max_n + 63) / 64;
     const int blocks = std::min((chunks + INF_WAVES - 1) / INF_WAVES, n_cus(device));
-    k_nrc_infer<true><<<blocks, INF_WAVES * 64, 0, s>>>(w16.as<__bf16>(), q, count, max_n, radiance);
+    k_nrc_infer<true><<<blocks, INF_WAVES * 64, 0, s>>>(w16inf.as<__bf16>(), q, count, max_n, radiance);
     check_hip(hipGetLastError(), "k_nrc_infer<accumulate>");
 }
 
@@ -402,9 +432,11 @@ double Nrc::train(const NrcQuery* q, const float4* target, int n, bool want_loss
     }
     const float c1 = 1.0f - std::pow(cfg.beta1, float(std::max(step, 1)));
     const float c2 = 1.0f - std::pow(cfg.beta2, float(std::max(step, 1)));
+    const float ce = 1.0f - std::pow(cfg.ema, float(std::max(step, 1)));
     k_nrc_adam<<<(NRC_NPARAM + 255) / 256, 256, 0, s>>>(w32.as<float>(), w16.as<__bf16>(), m.as<float>(),
-                                                            v.as<float>(), part.as<float>(), splits * 4, cfg.lr,
-                                                            cfg.beta1, cfg.beta2, cfg.eps, c1, c2, gdev);
+                                                            v.as<float>(), wema.as<float>(), w16inf.as<__bf16>(),
+                                                            part.as<float>(), splits * 4, cfg.lr, cfg.beta1,
+                                                            cfg.beta2, cfg.eps, c1, c2, cfg.ema, ce, gdev);
     check_hip(hipGetLastError(), "k_nrc_adam");
     double loss = 0.0;
     if (grad_out) {

@@ -174,6 +174,7 @@ struct NrcShade {
     float c_spread;
     int train_maxlen;
     int use_cache;
+    int min_depth;
 };
 
 struct ShadeArgs {
@@ -337,7 +338,7 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a, const NrcShade& n
                             ns.x += sqrtf(t2 / fmaxf(ns.z * cs, 1e-20f));
                         }
                         bool query = false;
-                        if (!train) query = nrc.use_cache && depth >= 1 && ns.x > ns.y;
+                        if (!train) query = nrc.use_cache && depth >= nrc.min_depth && ns.x > ns.y;
                         else query = depth + 1 >= nrc.train_maxlen;
                         if (query || train) {
                             float th, ph, nth, nph;
@@ -682,7 +683,8 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
                     t.shadow.as<Ray>(), t.contrib.as<float4>(), cnt + C_SHADOW};
         NrcShade ns{t.nstate[cur].as<float4>(), t.nstate[nxt].as<float4>(), t.nrc_queries.as<NrcQuery>(),
                     t.nrc_recs.as<NrcQuery>(), t.nrc_lb.as<float4>(), cnt + C_QUERY, cnt + C_REC, int(cap),
-                    int(rec_cap), int(P), int(T), np.c_spread, np.train_maxlen, np.use_cache};
+                    int(rec_cap), int(P), int(T), np.c_spread, np.train_maxlen, np.use_cache,
+                    std::max(1, np.min_depth)};
         if (sc.features == 0) hipLaunchKernelGGL(k_shade_nrc_basic, dim3(grid_shade), dim3(BLK), 0, s, a, ns);
         else hipLaunchKernelGGL(k_shade_nrc_full, dim3(grid_shade), dim3(BLK), 0, s, a, ns);
         launch_shadow(sc.view, t.shadow.as<Ray>(), cnt + C_SHADOW, nullptr, nullptr, t.contrib.as<float4>(),
@@ -713,7 +715,21 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
         const int n = int(std::min<long long>(f.records, nrc.cfg.max_batch));
         nrc_targets(t.nrc_recs.as<NrcQuery>(), t.nrc_lb.as<float4>(), t.radiance.as<float4>(), n, t.nrc_tgt.as<float4>(),
                     s);
-        f.train_loss = nrc.train(t.nrc_recs.as<NrcQuery>(), t.nrc_tgt.as<float4>(), n, fs != nullptr, nullptr, s);
+        // vertices are recorded bounce by bounce: interleave them so every batch mixes all depths
+        const int steps = std::max(1, std::min(np.train_steps, n));
+        t.nrc_batch.ensure(sizeof(NrcQuery) * size_t(n));
+        t.nrc_batch_tgt.ensure(sizeof(float4) * size_t(n));
+        nrc_interleave(t.nrc_recs.as<NrcQuery>(), t.nrc_tgt.as<float4>(), n, steps, t.nrc_batch.as<NrcQuery>(),
+                       t.nrc_batch_tgt.as<float4>(), s);
+        double loss = 0.0;
+        for (int k = 0, b0 = 0; k < steps; ++k) {
+            const int cnt = n / steps + (k < n % steps ? 1 : 0);
+            const bool last = k + 1 == steps;
+            loss = nrc.train(t.nrc_batch.as<NrcQuery>() + b0, t.nrc_batch_tgt.as<float4>() + b0, cnt,
+                             last && fs != nullptr, nullptr, s);
+            b0 += cnt;
+        }
+        f.train_loss = loss;  // of the last step
         f.trained = n;
     }
     check_hip(hipStreamSynchronize(s), "nrc train sync");

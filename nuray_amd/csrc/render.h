@@ -53,7 +53,7 @@ struct RenderTarget {
     uint32_t adj_seed = 0;
     DevBuf adj_target, adj_dldi, adj_loss, adj_grad;
     // radiance-cache rendering: spread state per queued ray, cache queries, training vertices
-    DevBuf nstate[2], nrc_queries, nrc_recs, nrc_lb, nrc_tgt;
+    DevBuf nstate[2], nrc_queries, nrc_recs, nrc_lb, nrc_tgt, nrc_batch, nrc_batch_tgt;
     ~RenderTarget();
     void ensure(int W, int H, size_t cap);
 };
@@ -61,11 +61,13 @@ struct RenderTarget {
 // RendererNRC::render parameters (renderernrc.cpp:185-281), with the spread rule of the
 // radiance-cache method in place of the reference's fixed depth limit.
 struct NrcRenderParams {
-    int train_paths = 16384;  // training paths per frame (pixels drawn uniformly)
+    int train_paths = 65536;  // training paths per frame (pixels drawn uniformly)
     int train_maxlen = 5;     // vertices of a training path; the last one is answered by the cache
     float c_spread = 0.01f;   // a render path stops at x_n when a(x1..xn) > c a0
     int use_cache = 1;        // 0: render paths are plain ptnee (training still runs)
     int train = 1;            // 0: no training step this frame
+    int train_steps = 4;      // Adam steps per frame over disjoint parts of the recorded vertices
+    int min_depth = 1;        // render paths query the cache only from this vertex index on
 };
 struct NrcFrameStats {
     double train_loss = 0;
