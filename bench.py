@@ -176,6 +176,25 @@ def nrc_time_to_rmse(nr, scenes, eps=1e-2, ref_spp=4096, warm=256, k_small=16, k
                         "extrapolated from measured per-frame variance and cache bias"}
 
 
+BF16_PEAK_TFLOPS = 2500.0
+NRC_FWD_FLOP = 2 * (44 * 64 + 4 * 64 * 64 + 64 * 3)     # 38,784 per query
+NRC_TRAIN_FLOP = 3 * NRC_FWD_FLOP                         # forward, input gradient, weight gradient
+
+
+def nrc_kernels(nr, n_query=1920 * 1080, reps=20):
+    """Radiance-cache MLP kernels on device-resident synthetic queries: one inference launch over
+    a 1080p frame's worth of queries and one training step of 65,536 samples, against the BF16
+    MFMA peak (algorithmic FLOPs: 2 * (44*64 + 4*64*64 + 64*3) per query, 3x that per training
+    sample)."""
+    cache = nr.Nrc(seed=3)
+    infer_ms, train_ms = cache.bench(n_query, reps)
+    inf_tf = n_query * NRC_FWD_FLOP / (infer_ms * 1e-3) / 1e12
+    tr_tf = 65536 * NRC_TRAIN_FLOP / (train_ms * 1e-3) / 1e12
+    return {"infer_queries": n_query, "infer_ms": infer_ms, "infer_tflops": inf_tf,
+            "infer_frac_bf16_peak": inf_tf / BF16_PEAK_TFLOPS, "train_samples": 65536, "train_step_ms": train_ms,
+            "train_tflops": tr_tf, "train_frac_bf16_peak": tr_tf / BF16_PEAK_TFLOPS}
+
+
 def measured_traffic():
     """Calibrated HBM bytes per closest-hit launch from the committed rocprofv3 counter summary."""
     path = os.path.join(ROOT, "profiles", "trace_closest_720p_traffic.txt")
@@ -195,6 +214,8 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-nrc", action="store_true", help="skip the time-to-RMSE measurement (about 40 s)")
+    ap.add_argument("--mlp-only", action="store_true",
+                    help="only time the radiance-cache MLP kernels (inference + training step) and exit")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the plain 1 GiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -210,6 +231,9 @@ def main():
     copy_gbs = nr.bench_copy(1 << 30, 5)
     if args.copy_only:
         print(json.dumps({"copy_bytes": 1 << 30, "reps": 5, "copy_gbs": copy_gbs}), flush=True)
+        return
+    if args.mlp_only:
+        print(json.dumps(nrc_kernels(nr)), flush=True)
         return
     sc = scenes.terrain_spheres()
     W, H, spp = sc["width"], sc["height"], args.spp
@@ -259,6 +283,9 @@ def main():
     print(f"# adjoint iteration {adj['ms_per_iter']:.2f} ms (render {adj['render_ms']:.2f} ms, "
           f"replay {adj['adjoint_ms']:.2f} ms)", file=sys.stderr, flush=True)
     nrc = None if args.no_nrc else nrc_time_to_rmse(nr, scenes)
+    mlp = nrc_kernels(nr)
+    print(f"# cache MLP: inference {mlp['infer_ms']:.3f} ms / 1080p frame ({mlp['infer_tflops']:.0f} TFLOP/s), "
+          f"training step {mlp['train_step_ms']:.3f} ms ({mlp['train_tflops']:.0f} TFLOP/s)", file=sys.stderr, flush=True)
     if nrc:
         print(f"# time to RMSE 1e-2 at 1080p: PT {nrc['pt_s']:.1f} s, radiance cache "
               f"{nrc['nrc_s'] if nrc['nrc_s'] is None else round(nrc['nrc_s'], 1)} s", file=sys.stderr, flush=True)
@@ -271,7 +298,7 @@ def main():
                                f"({len(sc['p'])} triangles) + 1024x512 sky", "global_batch": W * H * spp,
                    "seq_len": None, "parallelism": "single-gpu", "bvh_build_ms": info["build_ms"]},
         "roofline": roofline, "cpu_baseline": cpu, "hbm_copy_gbs": copy_gbs,
-        "adjoint": adj, "nrc": nrc,
+        "adjoint": adj, "nrc": nrc, "nrc_mlp": mlp,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]

@@ -159,6 +159,12 @@ int nr_nrc_train_step(nr_nrc* c, const float* queries, const float* targets, uin
 /* Gradient of the mean loss (NR_NRC_NPARAM floats) without an update. */
 int nr_nrc_gradient(nr_nrc* c, const float* queries, const float* targets, uint32_t n, float* grad, double* loss);
 
+/* No reference counterpart: times the cache kernels on n synthetic device-resident queries,
+ * `reps` launches each (HIP events): *infer_ms per inference launch over n queries, *train_ms
+ * per training step (forward + backward + weight gradient + Adam) on min(n, max_batch) samples.
+ * Changes the weights. */
+int nr_nrc_bench(nr_nrc* c, uint32_t n, int32_t reps, double* infer_ms, double* train_ms);
+
 /* RendererNRC::render (renderernrc.cpp:185-281), as the radiance-cache method: the frame's
  * p.spp samples per pixel follow ptnee until the path spread a(x1..xn) exceeds spread * a0
  * (a0 = the primary vertex's footprint), where the cache's answer beta * cache(x, wo) ends the path;

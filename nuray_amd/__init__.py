@@ -26,7 +26,7 @@ EXPORTS = [
     "nr_bvh_build_host", "nr_intersect", "nr_render_ptnee", "nr_render_fetch", "nr_render_stats",
     "nr_set_profiling", "nr_bench_copy", "nr_scene_set_params", "nr_loss_grad_l2", "nr_adjoint_ptnee",
     "nr_nrc_create", "nr_nrc_destroy", "nr_nrc_set_weights", "nr_nrc_get_weights", "nr_nrc_infer",
-    "nr_nrc_train_step", "nr_nrc_gradient", "nr_render_nrc",
+    "nr_nrc_train_step", "nr_nrc_gradient", "nr_render_nrc", "nr_nrc_bench",
 ]
 
 NRC_NPARAM = 5 * 4096 + 16 * 64
@@ -94,6 +94,7 @@ def lib():
         L.nr_nrc_infer.argtypes = [vp, f32p, C.c_uint32, f32p]
         L.nr_nrc_train_step.argtypes = [vp, f32p, f32p, C.c_uint32, C.POINTER(C.c_double)]
         L.nr_nrc_gradient.argtypes = [vp, f32p, f32p, C.c_uint32, f32p, C.POINTER(C.c_double)]
+        L.nr_nrc_bench.argtypes = [vp, C.c_uint32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.nr_render_nrc.argtypes = [vp, vp, C.POINTER(nr_camera), C.POINTER(nr_params), C.POINTER(nr_nrc_params), vp]
         _lib = L
     return _lib
@@ -313,6 +314,12 @@ class Nrc:
         loss = C.c_double()
         check(lib().nr_nrc_train_step(self.h, q, _f32(t).reshape(-1, 3), len(q), C.byref(loss)))
         return loss.value
+
+    def bench(self, n, reps=20):
+        """(ms per inference launch over n synthetic queries, ms per training step)."""
+        a, b = C.c_double(), C.c_double()
+        check(lib().nr_nrc_bench(self.h, n, reps, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def gradient(self, q, t):
         q = _f32(q).reshape(-1, 16)

@@ -370,6 +370,15 @@ int nr_nrc_gradient(nr_nrc* h, const float* queries, const float* targets, uint3
     });
 }
 
+int nr_nrc_bench(nr_nrc* c, uint32_t n, int32_t reps, double* infer_ms, double* train_ms) {
+    return guard([&] {
+        if (n == 0 || reps <= 0) throw std::runtime_error("nrc bench: n and reps must be positive");
+        const double ms = c->n->bench(int(n), reps, train_ms);
+        if (infer_ms) *infer_ms = ms;
+        return NR_OK;
+    });
+}
+
 int nr_render_nrc(nr_scene* s, nr_nrc* c, const nr_camera* cam, const nr_params* params, const nr_nrc_params* np,
                   double* stats) {
     return guard([&] {

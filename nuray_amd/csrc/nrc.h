@@ -43,6 +43,11 @@ public:
     double train(const NrcQuery* q, const float4* target, int n, bool want_loss, float* grad_out = nullptr,
                  hipStream_t s = nullptr);
 
+    // Timing on synthetic device-resident queries: ms per inference launch over n queries and, in
+    // *train_ms, per training step of min(n, max_batch) samples (HIP events, `reps` launches each).
+    // Trains the network: use a scratch cache.
+    double bench(int n, int reps, double* train_ms);
+
 private:
     int splits_for(int n) const;
 };

@@ -9,8 +9,8 @@
 //
 // Kernels
 //   k_nrc_infer      persistent; weights in LDS once per block, one 64-sample tile per wave
-//   k_nrc_fwd_bwd    one wave per block (all six activation tiles of its 64 samples in LDS):
-//                    forward, loss gradient, backward; writes each layer's input X_l and
+//   k_nrc_fwd_bwd    one 64-sample tile per wave, in place: forward, loss gradient, backward
+//                    (ReLU masks kept in registers); writes each layer's input X_l and
 //                    pre-activation gradient G_l (bf16, [sample][64]) for the weight gradient
 //   k_nrc_wgrad      dW_l^T = sum_s G_l[s]^T X_l[s]: K = samples, split over waves; per-wave
 //                    partials (no atomics)
@@ -83,44 +83,66 @@ __device__ __forceinline__ void tile_to_global(const __bf16* T, __bf16* g, int s
     for (int c = 0; c < 8; ++c) dst[c] = src[c];
 }
 
-__global__ __launch_bounds__(64) void k_nrc_fwd_bwd(const __bf16* __restrict__ w16, const NrcQuery* __restrict__ q,
-                                                    const float4* __restrict__ tgt, int n, float inv_n,
-                                                    __bf16* __restrict__ act, __bf16* __restrict__ gz,
-                                                    float* __restrict__ loss) {
+constexpr int TRAIN_WAVES = 8;  // 48 KB weights + one 9 KB tile per wave in LDS
+
+// Forward, loss gradient and backward of 64 samples per wave, in one LDS tile per wave: the
+// forward runs in place, and the ReLU masks of every hidden layer stay in registers (a lane's
+// backward results sit at exactly the tile positions of its forward results), so the backward
+// also runs in place.  Layer inputs X_l and pre-activation gradients G_l go to global memory for
+// the weight gradient.
+__global__ __launch_bounds__(TRAIN_WAVES * 64) void k_nrc_fwd_bwd(const __bf16* __restrict__ w16,
+                                                                  const NrcQuery* __restrict__ q,
+                                                                  const float4* __restrict__ tgt, int n, float inv_n,
+                                                                  __bf16* __restrict__ act, __bf16* __restrict__ gz,
+                                                                  float* __restrict__ loss) {
     __shared__ __attribute__((aligned(16))) __bf16 sW[NRC_LDS_W];
-    __shared__ __attribute__((aligned(16))) __bf16 sT[7][NRC_TILE];  // X_0..X_5, then G_5
+    __shared__ __attribute__((aligned(16))) __bf16 sT[TRAIN_WAVES][NRC_TILE];
     nrc_load_weights(w16, sW);
     __syncthreads();
-    const int lane = threadIdx.x, r = lane & 15, qd = lane >> 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, qd = lane >> 4;
+    __bf16* T = sT[wave];
     const size_t layer_stride = size_t(n) * 64;
     float loss_acc = 0.0f;
-    for (int chunk = blockIdx.x; chunk * 64 < n; chunk += gridDim.x) {
+    for (int chunk = blockIdx.x * TRAIN_WAVES + wave; chunk * 64 < n; chunk += gridDim.x * TRAIN_WAVES) {
         const int s0 = chunk * 64;
         {
             const int s = s0 + lane;
             const bool valid = s < n;
             NrcQuery rec = valid ? q[s] : NrcQuery{};
-            nrc_encode_row(rec, valid, sT[0], lane);
+            nrc_encode_row(rec, valid, T, lane);
         }
         wave_sync();
-        tile_to_global(sT[0], act, s0, n, lane);
-        // forward, keeping every layer input
+        tile_to_global(T, act, s0, n, lane);
+        // forward in place; bit (16 mt + 4 nt + i) of mask[l]: output of layer l at this lane's
+        // fragment position is positive
+        uint64_t mask[5];
         for (int l = 0; l < 5; ++l) {
             f32x4 acc[4][4];
-            nrc_layer<4>(sW + l * 64 * NRC_LD, sT[l], acc, lane);
-            nrc_store_relu<4>(acc, sT[l + 1], lane);
+            nrc_layer<4>(sW + l * 64 * NRC_LD, T, acc, lane);
+            uint64_t m = 0;
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if ((float)(__bf16)fmaxf(acc[mt][nt][i], 0.0f) > 0.0f) m |= 1ull << (16 * mt + 4 * nt + i);
+            mask[l] = m;
             wave_sync();
-            tile_to_global(sT[l + 1], act + (l + 1) * layer_stride, s0, n, lane);
+            nrc_store_relu<4>(acc, T, lane);
+            wave_sync();
+            tile_to_global(T, act + (l + 1) * layer_stride, s0, n, lane);
         }
         f32x4 yo[1][4];
-        nrc_layer<1>(sW + 5 * 64 * NRC_LD, sT[5], yo, lane);
-        // output gradient into G_5 (columns 0..2; the rest zero)
+        nrc_layer<1>(sW + 5 * 64 * NRC_LD, T, yo, lane);
+        wave_sync();
+        // output gradient G_5 over the tile (columns 0..2; the rest zero)
         {
             bf16x8 z;
 #pragma unroll
             for (int j = 0; j < 8; ++j) z[j] = (__bf16)0.0f;
 #pragma unroll
-            for (int c = 0; c < 8; ++c) *reinterpret_cast<bf16x8*>(sT[6] + lane * NRC_LD + c * 8) = z;
+            for (int c = 0; c < 8; ++c) *reinterpret_cast<bf16x8*>(T + lane * NRC_LD + c * 8) = z;
         }
         wave_sync();
         if (qd == 0) {
@@ -134,16 +156,15 @@ __global__ __launch_bounds__(64) void k_nrc_fwd_bwd(const __bf16* __restrict__ w
                 for (int c = 0; c < 3; ++c) {
                     const float y = yo[0][nt][c], den = y * y + 0.01f, e = y - tv[c];
                     loss_acc += e * e / den;
-                    sT[6][(16 * nt + r) * NRC_LD + c] = (__bf16)(2.0f * e / den * inv_n);
+                    T[(16 * nt + r) * NRC_LD + c] = (__bf16)(2.0f * e / den * inv_n);
                 }
             }
         }
         wave_sync();
-        tile_to_global(sT[6], gz + 5 * layer_stride, s0, n, lane);
-        // backward: G_{l-1} = (W_l G_l) * [X_l > 0], written over X_l's tile
+        tile_to_global(T, gz + 5 * layer_stride, s0, n, lane);
+        // backward in place: G_{l-1} = (W_l G_l) * [Z_{l-1} > 0]
         for (int l = 5; l >= 1; --l) {
             const __bf16* Wl = sW + l * 64 * NRC_LD;
-            const __bf16* G = l == 5 ? sT[6] : sT[l + 1];
             const int kin = l == 5 ? 1 : 2;    // k-steps of 32 over the layer's outputs
             const int rows = nrc_layer_rows(l);
             bf16x8 bg[4][2];
@@ -151,7 +172,9 @@ __global__ __launch_bounds__(64) void k_nrc_fwd_bwd(const __bf16* __restrict__ w
             for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
                 for (int ks = 0; ks < 2; ++ks)
-                    bg[nt][ks] = *reinterpret_cast<const bf16x8*>(G + (16 * nt + r) * NRC_LD + 32 * ks + 8 * qd);
+                    bg[nt][ks] = *reinterpret_cast<const bf16x8*>(T + (16 * nt + r) * NRC_LD + 32 * ks + 8 * qd);
+            wave_sync();
+            const uint64_t m = mask[l - 1];
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) {
                 bf16x8 a[2];
@@ -167,20 +190,19 @@ __global__ __launch_bounds__(64) void k_nrc_fwd_bwd(const __bf16* __restrict__ w
                     f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
                     c = mfma16(a[0], bg[nt][0], c);
                     if (kin == 2) c = mfma16(a[1], bg[nt][1], c);
-                    __bf16* xp = sT[l] + (16 * nt + r) * NRC_LD + 16 * mt + 4 * qd;
-                    const bf16x4 xm = *reinterpret_cast<const bf16x4*>(xp);
                     bf16x4 gv;
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) gv[i] = (__bf16)((float)xm[i] > 0.0f ? c[i] : 0.0f);
-                    *reinterpret_cast<bf16x4*>(xp) = gv;
+                    for (int i = 0; i < 4; ++i)
+                        gv[i] = (__bf16)(((m >> (16 * mt + 4 * nt + i)) & 1ull) ? c[i] : 0.0f);
+                    *reinterpret_cast<bf16x4*>(T + (16 * nt + r) * NRC_LD + 16 * mt + 4 * qd) = gv;
                 }
             }
             wave_sync();
-            tile_to_global(sT[l], gz + (l - 1) * layer_stride, s0, n, lane);
+            tile_to_global(T, gz + (l - 1) * layer_stride, s0, n, lane);
         }
         wave_sync();
     }
-    // wave reduction of the loss, one atomic per block
+    // wave reduction of the loss, one atomic per wave
     for (int off = 32; off > 0; off >>= 1) loss_acc += __shfl_xor(loss_acc, off);
     if (lane == 0 && loss) atomicAdd(loss, loss_acc);
 }
@@ -297,6 +319,28 @@ __global__ void k_nrc_interleave(const NrcQuery* __restrict__ in, const float4* 
     tout[o] = tin[i];
 }
 
+// Synthetic queries for timing: positions in a 1000-unit box, angles and albedo in [0, 1).
+__global__ void k_nrc_fill(NrcQuery* q, float4* t, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t h = uint32_t(i) * 0x9E3779B9u + 0x7F4A7C15u;
+    auto u = [&h]() {
+        h ^= h >> 16;
+        h *= 0x85EBCA6Bu;
+        h ^= h >> 13;
+        h *= 0xC2B2AE35u;
+        h ^= h >> 16;
+        return float(h >> 8) * (1.0f / 16777216.0f);
+    };
+    NrcQuery r;
+    r.a = make_float4(1000.0f * u(), 200.0f * u(), 1000.0f * u(), 0.05f + 0.75f * u());
+    r.b = make_float4(u(), u(), u(), u());
+    r.c = make_float4(u(), u(), u(), __int_as_float(-1));
+    r.d = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    q[i] = r;
+    t[i] = make_float4(u(), u(), u(), 0.0f);
+}
+
 __global__ void k_to_bf16(const float* __restrict__ a, __bf16* __restrict__ b, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (__bf16)a[i];
@@ -322,6 +366,39 @@ void nrc_targets(const NrcQuery* rec, const float4* lb, const float4* radiance, 
     if (n <= 0) return;
     k_nrc_targets<<<(n + 255) / 256, 256, 0, s>>>(rec, lb, radiance, n, target);
     check_hip(hipGetLastError(), "k_nrc_targets");
+}
+
+double Nrc::bench(int n, int reps, double* train_ms) {
+    DevBuf q, t, o;
+    q.ensure(sizeof(NrcQuery) * size_t(n));
+    t.ensure(sizeof(float4) * size_t(n));
+    o.ensure(sizeof(float4) * size_t(n));
+    k_nrc_fill<<<(n + 255) / 256, 256, 0, stream>>>(q.as<NrcQuery>(), t.as<float4>(), n);
+    check_hip(hipGetLastError(), "k_nrc_fill");
+    hipEvent_t e0, e1;
+    check_hip(hipEventCreate(&e0), "event");
+    check_hip(hipEventCreate(&e1), "event");
+    infer(q.as<NrcQuery>(), n, o.as<float4>());
+    check_hip(hipEventRecord(e0, stream), "event");
+    for (int r = 0; r < reps; ++r) infer(q.as<NrcQuery>(), n, o.as<float4>());
+    check_hip(hipEventRecord(e1, stream), "event");
+    check_hip(hipEventSynchronize(e1), "event");
+    float ms = 0.0f;
+    check_hip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    const double infer_ms = double(ms) / reps;
+    if (train_ms) {
+        const int nb = std::min(n, cfg.max_batch);
+        train(q.as<NrcQuery>(), t.as<float4>(), nb, false);
+        check_hip(hipEventRecord(e0, stream), "event");
+        for (int r = 0; r < reps; ++r) train(q.as<NrcQuery>(), t.as<float4>(), nb, false);
+        check_hip(hipEventRecord(e1, stream), "event");
+        check_hip(hipEventSynchronize(e1), "event");
+        check_hip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        *train_ms = double(ms) / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return infer_ms;
 }
 
 void nrc_interleave(const NrcQuery* in, const float4* tin, int n, int steps, NrcQuery* out, float4* tout,
@@ -406,7 +483,7 @@ void Nrc::infer_accumulate(const NrcQuery* q, const int* count, int max_n, float
 
 int Nrc::splits_for(int n) const {
     const int chunks = (n + 31) / 32;
-    return std::max(1, std::min(64, (chunks + 31) / 32));  // about 8 chunks of 32 per wave
+    return std::max(1, std::min(64, (chunks + 127) / 128));  // about 32 chunks of 32 per wave
 }
 
 double Nrc::train(const NrcQuery* q, const float4* target, int n, bool want_loss, float* grad_out, hipStream_t s) {
@@ -416,7 +493,8 @@ double Nrc::train(const NrcQuery* q, const float4* target, int n, bool want_loss
     const int chunks = (n + 63) / 64;
     const int cus = n_cus(device);
     check_hip(hipMemsetAsync(scal.ptr, 0, sizeof(float) * 4, s), "nrc loss clear");
-    k_nrc_fwd_bwd<<<std::min(chunks, 2 * cus), 64, 0, s>>>(w16.as<__bf16>(), q, target, n, 1.0f / float(n),
+    k_nrc_fwd_bwd<<<std::min((chunks + TRAIN_WAVES - 1) / TRAIN_WAVES, cus), TRAIN_WAVES * 64, 0, s>>>(
+        w16.as<__bf16>(), q, target, n, 1.0f / float(n),
                                                                act.as<__bf16>(), gz.as<__bf16>(), scal.as<float>());
     check_hip(hipGetLastError(), "k_nrc_fwd_bwd");
     const int splits = splits_for(n);
