@@ -314,3 +314,68 @@ def adjoint_problem(seed_hidden=5, seed_start=6, width=256, height=256):
     theta0 = np.where(is_light[:, None], theta_star, theta_star * pert).astype(np.float32)
     theta0[:, :3] = np.clip(theta0[:, :3], 0.0, None)
     return sc, theta_star, theta0
+
+
+def _checker_rgba(w, h, seed):
+    """RGBA8 checkerboard with a colour gradient (seeded tints)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    tint = 0.3 + 0.7 * rng.random((2, 3))
+    y, x = np.mgrid[0:h, 0:w]
+    chk = ((x // 8 + y // 8) % 2).astype(np.float64)
+    grad = (x / max(w - 1, 1))[..., None]
+    rgb = (chk[..., None] * tint[0] + (1 - chk[..., None]) * tint[1]) * (0.5 + 0.5 * grad)
+    out = np.zeros((h, w, 4), np.uint8)
+    out[..., :3] = np.floor(rgb * 255.0 + 0.5).astype(np.uint8)
+    out[..., 3] = 255
+    return out
+
+
+def material_zoo(width=160, height=120):
+    """The box room with every material feature of the path: a textured Lambert floor (uv 0..4,
+    wrapping), a textured GGX-reflect back wall, Lambert side walls, two translucent GGX dielectric
+    icospheres with smooth normals (alpha 0.2 / ior 1.5 and alpha 0.5 / ior 1.3), a GGX-reflect tall
+    box, the ceiling light, and a 64x32 sky through the open front."""
+    base = box_room()
+    tris, n9, uv6, mat = [], [], [], []
+    white, red, green = (0.725, 0.71, 0.68), (0.63, 0.065, 0.05), (0.14, 0.45, 0.09)
+    mats = [mat_row(LAMBERT, white, tex=0),            # 0 floor, textured
+            mat_row(GGXREFL, (1, 1, 1), 0.3, tex=1),   # 1 back wall, textured F0
+            mat_row(LAMBERT, red), mat_row(LAMBERT, green), mat_row(LAMBERT, white),  # 2, 3, 4
+            mat_row(LIGHT, (17.0, 12.0, 4.0)),         # 5
+            mat_row(GGX, (0.95, 0.95, 0.95), 0.2, ior=1.5),   # 6 glass-like sphere
+            mat_row(GGX, (0.9, 0.8, 0.7), 0.5, ior=1.3),      # 7 rough dielectric sphere
+            mat_row(GGXREFL, (0.8, 0.7, 0.5), 0.15)]          # 8 tall box
+    R, H = 100.0, 200.0
+
+    def add_quad(corners, nrm, m, uvq=None):
+        tl = _quad(*corners, np.array(nrm, float))
+        uvq = uvq if uvq is not None else [(0, 0), (0, 0), (0, 0), (0, 0)]
+        # _quad may swap p1 / p3 to fix the winding: recover the corner order from the points
+        cs = [np.asarray(c, np.float64) for c in corners]
+        for t in tl:
+            pts = t.reshape(3, 3)
+            idx = [int(np.argmin([np.abs(pp - c).sum() for c in cs])) for pp in pts]
+            tris.append(t)
+            n9.append(np.zeros(9))
+            uv6.append(np.concatenate([np.asarray(uvq[i], np.float64) for i in idx]))
+            mat.append(m)
+    add_quad(((-R, 0, R), (R, 0, R), (R, 0, -R), (-R, 0, -R)), (0, 1, 0), 0, [(0, 0), (4, 0), (4, 4), (0, 4)])
+    add_quad(((-R, 0, -R), (R, 0, -R), (R, H, -R), (-R, H, -R)), (0, 0, 1), 1, [(0, 0), (1, 0), (1, 1), (0, 1)])
+    add_quad(((-R, 0, R), (-R, 0, -R), (-R, H, -R), (-R, H, R)), (1, 0, 0), 2)
+    add_quad(((R, 0, -R), (R, 0, R), (R, H, R), (R, H, -R)), (-1, 0, 0), 3)
+    add_quad(((-R, H, R), (R, H, R), (R, H, -R), (-R, H, -R)), (0, -1, 0), 4)
+    add_quad(((-25, 199, -25), (25, 199, -25), (25, 199, 25), (-25, 199, 25)), (0, -1, 0), 5)
+    for axis, sign, tl in _box((-35.0, 60.0, -35.0), (60.0, 120.0, 60.0), -18.0):
+        for t in tl:
+            tris.append(t); n9.append(np.zeros(9)); uv6.append(np.zeros(6)); mat.append(8)
+    sv, sf = _icosphere80()
+    for (c, r, m) in (((40.0, 30.0, 30.0), 30.0, 6), ((-50.0, 25.0, 60.0), 25.0, 7)):
+        for f in sf:
+            pts = sv[f] * r + np.asarray(c)
+            tris.append(pts.reshape(-1)); n9.append(sv[f].reshape(-1)); uv6.append(np.zeros(6)); mat.append(m)
+    sky = _sky(64, 32) * 0.5
+    return dict(p=np.asarray(tris).astype(np.float32), n=np.asarray(n9).astype(np.float32),
+                uv=np.asarray(uv6).astype(np.float32), mat=np.asarray(mat, np.int32),
+                mats=np.asarray(mats, np.float32), textures=[_checker_rgba(64, 64, 1), _checker_rgba(32, 32, 2)],
+                camera=dict(base["camera"], aspect=width / height), width=width, height=height,
+                envmap=dict(rgb=sky, gain=1.0), name="material_zoo")

@@ -10,8 +10,8 @@
 // no fast-math so that float arithmetic follows IEEE binary32 step by step, as the
 // reference's x86-64 SSE build does (projects/nuRay.pro:13: -O3, no -mfma).
 //
-// Pinning status per function is listed in the header comment of each function and in the
-// oracle section of DESIGN.md.
+// The checks that pin each function against the reference's behaviour live in
+// tests/test_oracle_pins.py.
 #pragma once
 #include <cmath>
 #include <cstdint>

@@ -122,8 +122,9 @@ struct Scene {
 
 // ---------------------------------------------------------------- materials
 // Shading frame shared by MatLambert (matlambert.cpp:11-13) and MatGGXRefl (matggxrefl.cpp:20-22).
-// The reference's `abs` is read as the floating-point absolute value (a g++ build
-// without Qt headers in the material TUs binds int abs; the published renders show float abs).
+// The reference's `abs` is read as the floating-point absolute value: a g++ build without the
+// Qt headers in the material translation units would bind int abs, but the reference's published
+// renders are only consistent with the floating-point one.
 template <class T>
 inline void frame(const V3<T>& n, V3<T>& ax1, V3<T>& ax2) {
     V3<T> ax0 = double(std::fabs(n.x)) < 0.8 ? V3<T>(1, 0, 0) : V3<T>(0, 1, 0);

@@ -135,7 +135,7 @@ def _adam(theta, g, m, v, it, lr=1e-2, b1=0.9, b2=0.999):
 def test_adam_inverse_rendering_256(nr, scenes):
     """The inverse-rendering workload: 256x256 target at 1024 spp from hidden parameters, start
     perturbed by +-30 %, per iteration a 32-spp render, L2 loss, path-replay adjoint to all
-    256 parameters and Adam (lr 1e-2), 200 iterations.  Progress is judged on the converged
+    256 parameters and Adam (lr 1e-2), 200 iterations.  Progress is measured on the converged
     image: the excess MSE of a 1024-spp render over the noise floor must fall by 4x."""
     sc, theta_star, theta0 = _problem(scenes, 256, 256)
     W, H = 256, 256

@@ -1,7 +1,7 @@
 """Traversal parity on the GPU: libnuray's nr_intersect against the oracle's restatement of
 BVH::intersection (bvh.cpp:99-137) on identical ray batches.
 
-Bar: hit triangle index identical, t / b1 / b2 bit-identical (the north-star tolerance for t is
+Bar: hit triangle index identical, t / b1 / b2 bit-identical (the required tolerance for t is
 1e-5 relative; it is checked too, but the kernel reproduces the reference arithmetic exactly)."""
 import numpy as np
 import pytest
