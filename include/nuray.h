@@ -75,6 +75,10 @@ nr_scene* nr_scene_create(const float* p, const float* n, const float* uv, const
 int nr_scene_destroy(nr_scene* s);
 /* Texture::load result (texture.cpp:5-31): RGBA8 texels, row-major, first row at v = 1. */
 int nr_scene_set_texture_rgba8(nr_scene* s, int32_t id, int32_t w, int32_t h, const uint8_t* rgba);
+/* Linear RGB float albedo texture (3 floats per texel, first row at v = 1): the form to optimise
+ * with nr_adjoint_ptnee_tex.  Setting a texture of the same size and format on a built scene
+ * rewrites it in place (no rebuild).  No reference counterpart (the reference's textures are 8-bit). */
+int nr_scene_set_texture_rgb32f(nr_scene* s, int32_t id, int32_t w, int32_t h, const float* rgb);
 /* Renderer::setEnvmap / Envmap(texture, gain) (renderer.h:30, envmap.cpp:17-24): linear RGB float
  * texels, row-major, looked up with the reference's bilinear rule. */
 int nr_scene_set_envmap(nr_scene* s, int32_t w, int32_t h, const float* rgb, float gain);
@@ -128,10 +132,18 @@ int nr_scene_set_params(nr_scene* s, const float* theta, uint32_t n_mats);
 int nr_loss_grad_l2(nr_scene* s, const float* target, double* loss, float* dL_dimg);
 /* Path-replay backpropagation: grad (n_mats x 4 doubles, same layout as theta) of the loss whose
  * image gradient is dL_dimg (W*H*3 floats, NULL = the one nr_loss_grad_l2 left on the device),
- * for the last nr_render_ptnee call, which must have used the same camera and params and run as a
- * single wavefront batch.  Lambert and GGX-reflect materials. */
+ * for the samples of cam / params (the last render's when they match).  When the last
+ * nr_render_ptnee ran those samples as one wavefront batch its per-sample radiance is reused;
+ * otherwise each sample is replayed forward first (recompute-only: no per-path storage, any
+ * sample count).  Lambert and GGX-reflect materials, with or without albedo textures (a textured
+ * material's Kd row gets no gradient). */
 int nr_adjoint_ptnee(nr_scene* s, const nr_camera* cam, const nr_params* params, const float* dL_dimg,
                      double* grad);
+/* The same, plus the gradient with respect to the linear texel values of albedo texture tex_id
+ * (w * h * 3 doubles, texel order of the upload), scattered with the bilinear weights of
+ * Texture::pixelBi (texture.cpp:54-72); texels outside the image get none. */
+int nr_adjoint_ptnee_tex(nr_scene* s, const nr_camera* cam, const nr_params* params, const float* dL_dimg,
+                         double* grad, int32_t tex_id, double* grad_texels);
 
 /* ---- Neural radiance cache.  Replaces NeuralRadianceCache (renderernrc.h:17-43) and its NNNode
  * network (nnnode.cpp:1-208): here a 44 -> 5 x 64 (ReLU) -> 3 MLP on BF16 MFMA with FP32

@@ -26,7 +26,8 @@ EXPORTS = [
     "nr_bvh_build_host", "nr_intersect", "nr_render_ptnee", "nr_render_fetch", "nr_render_stats",
     "nr_set_profiling", "nr_bench_copy", "nr_scene_set_params", "nr_loss_grad_l2", "nr_adjoint_ptnee",
     "nr_nrc_create", "nr_nrc_destroy", "nr_nrc_set_weights", "nr_nrc_get_weights", "nr_nrc_infer",
-    "nr_nrc_train_step", "nr_nrc_gradient", "nr_render_nrc", "nr_nrc_bench",
+    "nr_nrc_train_step", "nr_nrc_gradient", "nr_render_nrc", "nr_nrc_bench", "nr_scene_set_texture_rgb32f",
+    "nr_adjoint_ptnee_tex",
 ]
 
 NRC_NPARAM = 5 * 4096 + 16 * 64
@@ -86,6 +87,8 @@ def lib():
         L.nr_scene_set_params.argtypes = [vp, f32p, C.c_uint32]
         L.nr_loss_grad_l2.argtypes = [vp, f32p, C.POINTER(C.c_double), vp]
         L.nr_adjoint_ptnee.argtypes = [vp, C.POINTER(nr_camera), C.POINTER(nr_params), vp, f64p]
+        L.nr_scene_set_texture_rgb32f.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, f32p]
+        L.nr_adjoint_ptnee_tex.argtypes = [vp, C.POINTER(nr_camera), C.POINTER(nr_params), vp, f64p, C.c_int32, vp]
         L.nr_nrc_create.restype = vp
         L.nr_nrc_create.argtypes = [C.c_uint32, C.c_float, C.c_int32]
         L.nr_nrc_destroy.argtypes = [vp]
@@ -163,8 +166,12 @@ class Scene:
         if not self.h:
             raise NuRayError(L.nr_last_error().decode())
         for tid, tex in enumerate(sc.get("textures") or []):
-            t = np.ascontiguousarray(tex, np.uint8)
-            check(L.nr_scene_set_texture_rgba8(self.h, tid, t.shape[1], t.shape[0], t))
+            if np.asarray(tex).dtype == np.float32:   # linear RGB float texture
+                t = _f32(tex)
+                check(L.nr_scene_set_texture_rgb32f(self.h, tid, t.shape[1], t.shape[0], t))
+            else:
+                t = np.ascontiguousarray(tex, np.uint8)
+                check(L.nr_scene_set_texture_rgba8(self.h, tid, t.shape[1], t.shape[0], t))
         env = sc.get("envmap")
         if env is not None:
             rgb = _f32(env["rgb"])
@@ -241,13 +248,25 @@ class Scene:
         return (loss.value, g) if want_grad else loss.value
 
     def adjoint(self, cam, W, H, spp, n_mats, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0, spp_offset=0,
-                dl_dimg=None):
+                dl_dimg=None, tex_id=-1, tex_shape=None):
+        """Material gradient (n_mats, 4); with tex_id >= 0 also the texel gradient (h, w, 3) of that
+        texture (tex_shape = (h, w))."""
         p = nr_params(W, H, spp, spp_offset, seed, prr, mis, maxdepth, rrdepth, 0, 0)
         c = make_camera(cam)
         grad = np.zeros(n_mats * 4, np.float64)
         dl = None if dl_dimg is None else _f32(dl_dimg).reshape(-1)
-        check(lib().nr_adjoint_ptnee(self.h, C.byref(c), C.byref(p), None if dl is None else dl.ctypes.data, grad))
-        return grad.reshape(n_mats, 4)
+        if tex_id < 0:
+            check(lib().nr_adjoint_ptnee(self.h, C.byref(c), C.byref(p), None if dl is None else dl.ctypes.data,
+                                         grad))
+            return grad.reshape(n_mats, 4)
+        gt = np.zeros((tex_shape[0], tex_shape[1], 3), np.float64)
+        check(lib().nr_adjoint_ptnee_tex(self.h, C.byref(c), C.byref(p), None if dl is None else dl.ctypes.data,
+                                         grad, tex_id, gt.ctypes.data))
+        return grad.reshape(n_mats, 4), gt
+
+    def set_texture_rgb32f(self, tex_id, rgb):
+        rgb = _f32(rgb)
+        check(lib().nr_scene_set_texture_rgb32f(self.h, tex_id, rgb.shape[1], rgb.shape[0], rgb))
 
     def render_nrc(self, cache, cam, W, H, spp=1, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0, spp_offset=0,
                    accumulate=False, train_paths=65536, train_maxlen=5, spread=0.01, use_cache=True, train=True,

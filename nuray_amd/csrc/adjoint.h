@@ -9,8 +9,10 @@ void set_material_params(Scene& sc, const float* theta, int n_mats);
 // Mean squared error of the last render's mean image against `target` (W*H*3); keeps
 // dLoss/dImage on the device (and copies it to dldi_host if non-null).
 double loss_l2(Scene& sc, const float* target, float* dldi_host);
-// Gradient of the loss with respect to every material's {r, g, b, alpha} (materials x 4 doubles),
-// replaying the paths of the last forward render (which must have run as one batch).
-void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const float* dldi_host, double* grad);
+// Gradient of the loss with respect to every material's {r, g, b, alpha} (materials x 4 doubles)
+// and, for tex_id >= 0, with respect to the linear texel values of that albedo texture
+// (w * h * 3 doubles), by replaying the samples described by cam / p.
+void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const float* dldi_host, double* grad,
+                   int tex_id = -1, double* gtex = nullptr);
 
 }  // namespace nr

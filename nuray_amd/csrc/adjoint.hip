@@ -14,11 +14,13 @@
 // factor `fac` carried to the next emitter hit depend on alpha through the reference's pdf
 // (matggxrefl.cpp:54-64) and are differentiated as well.
 //
-// Replay: the forward render keeps every sample's radiance L (path slot layout).  The backward
-// kernel re-derives each path from the same Philox counters and the same exact traversal, and
-// peels each vertex's own contribution off L, so the radiance still to come through the next
-// bounce is known at every vertex without storing any path.  Per-block gradients accumulate in
-// FP32 in LDS; blocks add into an FP64 buffer (the only FP64 arithmetic on this path).
+// Replay: the backward kernel re-derives each path from the same Philox counters and the same
+// exact traversal, and peels each vertex's own contribution off the sample's radiance L, so the
+// radiance still to come through the next bounce is known at every vertex without storing any
+// path.  L is the forward render's per-sample radiance when that render ran as one batch with
+// the same samples; otherwise a forward replay recomputes it first (recompute-only mode, any
+// number of samples).  Material gradients accumulate per block in FP32 in LDS, then into FP64;
+// texel gradients (albedo textures) go straight to FP64 atomics at the four bilinear corners.
 #include <cstring>
 #include "adjoint.h"
 #include "nr_material.h"
@@ -52,7 +54,7 @@ __device__ __forceinline__ void ggx_g_and_deriv(float a, float c, float& G, floa
     const float dS = kap * (1.0f - c * c) / (2.0f * S);
     dG = -2.0f * c / ((c + S) * (c + S)) * dS;
 }
-__device__ __forceinline__ GgxParts ggx_parts(const Mat& m, F3 wo, F3 n, F3 wi) {
+__device__ __forceinline__ GgxParts ggx_parts(const Mat& m, F3 kd, F3 wo, F3 n, F3 wi) {
     GgxParts r;
     r.f = r.df_dkd = r.df_da = f3(0, 0, 0);
     r.pdf = bsdf_pdf<0u>(m, wo, n, wi);
@@ -75,19 +77,21 @@ __device__ __forceinline__ GgxParts ggx_parts(const Mat& m, F3 wo, F3 n, F3 wi) 
     const float den = 4.0f * (co + 1e-6f) * (ci + 1e-6f);
     const float s = D * Gl * Gv / 4.0f / (co + 1e-6f) / (ci + 1e-6f);
     const float ds = (dD * Gl * Gv + D * dGl * Gv + D * Gl * dGv) / den;
-    const F3 F = f3(m.kd.x + (1.0f - m.kd.x) * fr, m.kd.y + (1.0f - m.kd.y) * fr, m.kd.z + (1.0f - m.kd.z) * fr);
+    const F3 F = f3(kd.x + (1.0f - kd.x) * fr, kd.y + (1.0f - kd.y) * fr, kd.z + (1.0f - kd.z) * fr);
     r.f = mul(F, s);
     r.df_dkd = f3(s * (1.0f - fr), s * (1.0f - fr), s * (1.0f - fr));
     r.df_da = mul(F, ds);
     return r;
 }
 
-// Per-material parts for the two models the adjoint supports.
+// Per-material parts for the two models the adjoint supports; the albedo (Kd or F0) comes from
+// the material row or its texture.
+template <unsigned F>
 __device__ __forceinline__ GgxParts mat_parts(const SceneView& sv, const Mat& m, F3 wo, F3 n, F3 wi, float u, float v) {
-    if (m.type == MAT_GGXREFL) return ggx_parts(m, wo, n, wi);
+    if (m.type == MAT_GGXREFL) return ggx_parts(m, mat_albedo<F>(sv, m, u, v), wo, n, wi);
     GgxParts r;
-    r.f = bsdf_eval<0u>(sv, m, wo, n, wi, u, v);
-    r.pdf = bsdf_pdf<0u>(m, wo, n, wi);
+    r.f = bsdf_eval<F>(sv, m, wo, n, wi, u, v);
+    r.pdf = bsdf_pdf<F>(m, wo, n, wi);
     const bool same = !(dot(wi, n) * dot(wo, n) < 0.0f);
     const float inv = same ? 1.0f / 3.14159f : 0.0f;  // Lambert: f = Kd / pi
     r.df_dkd = f3(inv, inv, inv);
@@ -113,12 +117,15 @@ struct AdjArgs {
     uint32_t seed, sample0;
     float prr;
     int mis, maxdepth, rrdepth;
-    const float4* radiance;  // per-sample radiance of the forward pass (slot layout)
+    const float4* radiance;  // per-sample radiance of the forward pass (slot layout), or null: recompute it
+    int slot0, slot1;        // samples [slot0, slot1) of this launch
     const float* dldi;       // dLoss / dImage, W*H*3
     float inv_spp;
     int n_params;            // materials * 4
     double* grad;            // n_params
     int* err;
+    int tex_id;              // texture whose texel gradient is wanted (-1: none)
+    double* gtex;            // w * h * 3 texel gradient (linear texel values)
 };
 
 __device__ __forceinline__ void gadd(float* lds, double* glob, bool use_lds, int idx, float v) {
@@ -127,97 +134,129 @@ __device__ __forceinline__ void gadd(float* lds, double* glob, bool use_lds, int
     else atomicAdd(glob + idx, double(v));
 }
 
-__global__ __launch_bounds__(ABLK) void k_adjoint(AdjArgs a) {
-    extern __shared__ char smem[];
+// Texture::pixelUV / pixelBi (texture.cpp:54-72) run backwards: the albedo gradient goes to the
+// four bilinear corner texels with their weights; texels outside the image (read as black) get none.
+__device__ __forceinline__ void scatter_texels(const AdjArgs& a, const SceneView& sv, int tex, float u, float v, F3 dalb) {
+    if (!a.gtex || tex != a.tex_id) return;
+    const TexDesc t = sv.tex[tex];
+    const float fu = fmodf(fmodf(u, 1.0f) + 1.0f, 1.0f);
+    const float fv = fmodf(fmodf(1.0f - v, 1.0f) + 1.0f, 1.0f);
+    const float x = fu * float(t.w), y = fv * float(t.h);
+    const int x0 = int(floorf(x)), y0 = int(floorf(y));
+    const int x1 = x0 + 1, y1 = y0 + 1;
+    const int xs[4] = {x1, x1, x0, x0}, ys[4] = {y1, y0, y1, y0};
+    const float ws[4] = {(x - float(x0)) * (y - float(y0)), (x - float(x0)) * (float(y1) - y),
+                         (float(x1) - x) * (y - float(y0)), (float(x1) - x) * (float(y1) - y)};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (xs[k] < 0 || ys[k] < 0 || xs[k] >= t.w || ys[k] >= t.h || ws[k] == 0.0f) continue;
+        double* g = a.gtex + (size_t(ys[k]) * t.w + xs[k]) * 3;
+        if (dalb.x != 0.0f) atomicAdd(g + 0, double(dalb.x * ws[k]));
+        if (dalb.y != 0.0f) atomicAdd(g + 1, double(dalb.y * ws[k]));
+        if (dalb.z != 0.0f) atomicAdd(g + 2, double(dalb.z * ws[k]));
+    }
+}
+
+// One sample's path, replayed from its Philox counters.  GRAD = false: the sample's radiance
+// (the forward estimator, accumulated in vertex order as the wavefront does).  GRAD = true: the
+// parameter gradient of g . max(0, L), with Lt the sample's radiance.
+template <unsigned F, bool GRAD>
+__device__ __forceinline__ F3 replay(const AdjArgs& a, int slot, F3 Lt, F3 g, int* stk, float* gacc, bool use_lds,
+                                     bool& overflow) {
     const SceneView& sv = a.sv;
-    int* stk = reinterpret_cast<int*>(smem) + threadIdx.x;
-    float* gacc = reinterpret_cast<float*>(smem + size_t(sv.stack_cap) * ABLK * sizeof(int));
-    const bool use_lds = a.n_params <= LDS_PARAMS;
-    if (use_lds)
-        for (int k = threadIdx.x; k < a.n_params; k += blockDim.x) gacc[k] = 0.0f;
-    __syncthreads();
-    bool overflow = false;
-    for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < a.P; slot += gridDim.x * blockDim.x) {
-        const int pixel = slot % a.npix;
-        const uint32_t sample = a.sample0 + uint32_t(slot / a.npix);
-        const float4 Lt = a.radiance[slot];
-        // max(0, L) per channel at the camera (rendererptls.cpp:173): zero slope where L <= 0 or NaN
-        const F3 g = f3(Lt.x > 0.0f ? a.dldi[3 * pixel] * a.inv_spp : 0.0f,
-                        Lt.y > 0.0f ? a.dldi[3 * pixel + 1] * a.inv_spp : 0.0f,
-                        Lt.z > 0.0f ? a.dldi[3 * pixel + 2] * a.inv_spp : 0.0f);
-        if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) continue;
-        // camera ray, identical to k_raygen
-        const int x = pixel % a.W, y = pixel / a.W;
-        const float4 u0 = rng4(a.seed, uint32_t(pixel), sample, 0u, 0u);
-        const float cx = (float(x) + u0.x) / float(a.W) * 2.0f - 1.0f;
-        const float cy = (float(y) + u0.y) / float(a.H) * 2.0f - 1.0f;
-        const CamView& cam = a.cam;
-        F3 d = normalized(sub(add(f3(cam.gaze[0], cam.gaze[1], cam.gaze[2]),
-                                  mul(mul(mul(f3(cam.hand[0], cam.hand[1], cam.hand[2]), cam.fw), cx), 0.5f)),
-                              mul(mul(mul(f3(cam.up[0], cam.up[1], cam.up[2]), cam.fh), cy), 0.5f)));
-        F3 o = f3(cam.pos[0], cam.pos[1], cam.pos[2]);
-        F3 beta = f3(1.0f, 1.0f, 1.0f);
-        F3 Lrem = f3(Lt.x, Lt.y, Lt.z);  // radiance of this sample not yet accounted for
-        float fac = 1.0f, dfac = 0.0f;   // MIS factor for the next emitter hit and d/d alpha of it
-        int prev_mat = -1;
-        for (int depth = 0;; ++depth) {
-            const HitRec h = trace_ray<false>(sv, o, d, 0.0f, stk, ABLK, overflow);
-            if (h.prim < 0) break;  // environment: no parameter dependence
-            const TriShade ts = sv.shade[h.prim];
-            const float b1 = h.b1, b2 = h.b2, w0 = 1.0f - b1 - b2;
-            const F3 nrm = normalized(add(add(mul(f3(ts.s0.x, ts.s0.y, ts.s0.z), w0), mul(f3(ts.s0.w, ts.s1.x, ts.s1.y), b1)),
-                                          mul(f3(ts.s1.z, ts.s1.w, ts.s2.x), b2)));
-            const int mid = __float_as_int(ts.s3.w);
-            const Mat m = load_mat(sv, mid);
-            if (m.type == MAT_LIGHT) {
-                // Le * fac: fac depends on the previous vertex's roughness through its NEE MIS weight
-                if (prev_mat >= 0 && dfac != 0.0f) {
-                    const float v = g.x * beta.x * m.kd.x * dfac + g.y * beta.y * m.kd.y * dfac + g.z * beta.z * m.kd.z * dfac;
-                    gadd(gacc, a.grad, use_lds, 4 * prev_mat + 3, v);
-                }
-                break;
+    const int pixel = slot % a.npix;
+    const uint32_t sample = a.sample0 + uint32_t(slot / a.npix);
+    // camera ray, identical to k_raygen
+    const int x = pixel % a.W, y = pixel / a.W;
+    const float4 u0 = rng4(a.seed, uint32_t(pixel), sample, 0u, 0u);
+    const float cx = (float(x) + u0.x) / float(a.W) * 2.0f - 1.0f;
+    const float cy = (float(y) + u0.y) / float(a.H) * 2.0f - 1.0f;
+    const CamView& cam = a.cam;
+    F3 d = normalized(sub(add(f3(cam.gaze[0], cam.gaze[1], cam.gaze[2]),
+                              mul(mul(mul(f3(cam.hand[0], cam.hand[1], cam.hand[2]), cam.fw), cx), 0.5f)),
+                          mul(mul(mul(f3(cam.up[0], cam.up[1], cam.up[2]), cam.fh), cy), 0.5f)));
+    F3 o = f3(cam.pos[0], cam.pos[1], cam.pos[2]);
+    F3 beta = f3(1.0f, 1.0f, 1.0f);
+    F3 L = f3(0.0f, 0.0f, 0.0f);     // forward: radiance so far
+    F3 Lrem = Lt;                    // gradient: radiance of this sample not yet accounted for
+    float fac = 1.0f, dfac = 0.0f;   // MIS factor for the next emitter hit and d/d alpha of it
+    int prev_mat = -1;
+    auto add_albedo = [&](const Mat& m, int mid, float tu, float tv, F3 v) {
+        if (m.tex < 0) {
+            gadd(gacc, a.grad, use_lds, 4 * mid + 0, v.x);
+            gadd(gacc, a.grad, use_lds, 4 * mid + 1, v.y);
+            gadd(gacc, a.grad, use_lds, 4 * mid + 2, v.z);
+        } else if (F & F_TEX) {
+            scatter_texels(a, sv, m.tex, tu, tv, v);
+        }
+    };
+    for (int depth = 0;; ++depth) {
+        const HitRec h = trace_ray<false>(sv, o, d, 0.0f, stk, ABLK, overflow);
+        if (h.prim < 0) {  // environment: no parameter dependence
+            if (!GRAD) {
+                const F3 e = mulv(beta, env_radiance(sv, d));
+                L = add(L, e);
             }
-            if (dot(nrm, d) > 0.0f && !mat_transmissive(m)) break;
-            const float tu = w0 * ts.s2.y + b1 * ts.s2.w + b2 * ts.s3.y;
-            const float tv = w0 * ts.s2.z + b1 * ts.s3.x + b2 * ts.s3.z;
-            const F3 hp = add(o, mul(d, h.t));
-            const F3 wo = neg(d);
-            const float4 ua = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 0u);
-            float fac_next = 1.0f, dfac_next = 0.0f;
-            if (mat_needs_nee(m) && sv.n_lights > 0) {
-                const LightRec lr = sv.lights[pick_light(sv, ua.x)];
-                float r1 = ua.y, r2 = ua.z;
-                if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
-                const float lw = 1.0f - r1 - r2;
-                const F3 lp = add(add(mul(f3(lr.p0.x, lr.p0.y, lr.p0.z), lw), mul(f3(lr.p1.x, lr.p1.y, lr.p1.z), r1)),
-                                  mul(f3(lr.p2.x, lr.p2.y, lr.p2.z), r2));
-                const F3 ln = normalized(add(add(mul(f3(lr.n0.x, lr.n0.y, lr.n0.z), lw), mul(f3(lr.n1.x, lr.n1.y, lr.n1.z), r1)),
-                                             mul(f3(lr.n2.x, lr.n2.y, lr.n2.z), r2)));
-                const F3 lv = sub(lp, hp);
-                const F3 wl = normalized(lv);
-                const float lcos = dot(ln, neg(wl));
-                const float ncos = dot(nrm, normalized(sub(lp, hp)));
-                const GgxParts pl = mat_parts(sv, m, wo, nrm, wl, tu, tv);
-                const float lpdf = sv.light_p;
-                const float lop = lpdf * dot(lv, lv) / ((1e-6f < lcos) ? lcos : 1e-6f);
-                float mis = lop / (lop + pl.pdf);
-                float dmis = -lop / ((lop + pl.pdf) * (lop + pl.pdf)) * pl.dpdf_da;
-                if (a.mis == 0) mis = 0.5f, dmis = 0.0f;
-                fac_next = 1.0f - mis;
-                dfac_next = -dmis;
-                const F3 Ll = mul(divs(f3(lr.ke.x, lr.ke.y, lr.ke.z), dot(lv, lv)), (0.0f < lcos ? lcos : 0.0f) / lpdf);
-                const float cn = (0.0f < ncos ? ncos : 0.0f);
-                const F3 c = mul(mulv(mulv(beta, Ll), pl.f), cn * mis);
-                if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
-                    const F3 so = add(hp, mul(wl, 1e-3f));
-                    const HitRec sh = trace_ray<true>(sv, so, wl, norm(lv), stk, ABLK, overflow);
-                    if (sh.prim < 0) {  // visible: d(beta Ll f cos mis)
+            break;
+        }
+        const TriShade ts = sv.shade[h.prim];
+        const float b1 = h.b1, b2 = h.b2, w0 = 1.0f - b1 - b2;
+        const F3 nrm = normalized(add(add(mul(f3(ts.s0.x, ts.s0.y, ts.s0.z), w0), mul(f3(ts.s0.w, ts.s1.x, ts.s1.y), b1)),
+                                      mul(f3(ts.s1.z, ts.s1.w, ts.s2.x), b2)));
+        const int mid = __float_as_int(ts.s3.w);
+        const Mat m = load_mat(sv, mid);
+        if (m.type == MAT_LIGHT) {
+            if (!GRAD) {
+                L = add(L, mulv(beta, mul(m.kd, fac)));
+            } else if (prev_mat >= 0 && dfac != 0.0f) {
+                // Le * fac: fac depends on the previous vertex's roughness through its NEE MIS weight
+                const float v = g.x * beta.x * m.kd.x * dfac + g.y * beta.y * m.kd.y * dfac + g.z * beta.z * m.kd.z * dfac;
+                gadd(gacc, a.grad, use_lds, 4 * prev_mat + 3, v);
+            }
+            break;
+        }
+        if (dot(nrm, d) > 0.0f && !mat_transmissive(m)) break;
+        const float tu = w0 * ts.s2.y + b1 * ts.s2.w + b2 * ts.s3.y;
+        const float tv = w0 * ts.s2.z + b1 * ts.s3.x + b2 * ts.s3.z;
+        const F3 hp = add(o, mul(d, h.t));
+        const F3 wo = neg(d);
+        const float4 ua = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 0u);
+        float fac_next = 1.0f, dfac_next = 0.0f;
+        if (mat_needs_nee(m) && sv.n_lights > 0) {
+            const LightRec lr = sv.lights[pick_light(sv, ua.x)];
+            float r1 = ua.y, r2 = ua.z;
+            if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
+            const float lw = 1.0f - r1 - r2;
+            const F3 lp = add(add(mul(f3(lr.p0.x, lr.p0.y, lr.p0.z), lw), mul(f3(lr.p1.x, lr.p1.y, lr.p1.z), r1)),
+                              mul(f3(lr.p2.x, lr.p2.y, lr.p2.z), r2));
+            const F3 ln = normalized(add(add(mul(f3(lr.n0.x, lr.n0.y, lr.n0.z), lw), mul(f3(lr.n1.x, lr.n1.y, lr.n1.z), r1)),
+                                         mul(f3(lr.n2.x, lr.n2.y, lr.n2.z), r2)));
+            const F3 lv = sub(lp, hp);
+            const F3 wl = normalized(lv);
+            const float lcos = dot(ln, neg(wl));
+            const float ncos = dot(nrm, normalized(sub(lp, hp)));
+            const GgxParts pl = mat_parts<F>(sv, m, wo, nrm, wl, tu, tv);
+            const float lpdf = sv.light_p;
+            const float lop = lpdf * dot(lv, lv) / ((1e-6f < lcos) ? lcos : 1e-6f);
+            float mis = lop / (lop + pl.pdf);
+            float dmis = -lop / ((lop + pl.pdf) * (lop + pl.pdf)) * pl.dpdf_da;
+            if (a.mis == 0) mis = 0.5f, dmis = 0.0f;
+            fac_next = 1.0f - mis;
+            dfac_next = -dmis;
+            const F3 Ll = mul(divs(f3(lr.ke.x, lr.ke.y, lr.ke.z), dot(lv, lv)), (0.0f < lcos ? lcos : 0.0f) / lpdf);
+            const float cn = (0.0f < ncos ? ncos : 0.0f);
+            const F3 c = mul(mulv(mulv(beta, Ll), pl.f), cn * mis);
+            if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
+                const F3 so = add(hp, mul(wl, 1e-3f));
+                const HitRec sh = trace_ray<true>(sv, so, wl, norm(lv), stk, ABLK, overflow);
+                if (sh.prim < 0) {  // visible: d(beta Ll f cos mis)
+                    if (!GRAD) {
+                        L = add(L, c);
+                    } else {
                         const F3 k = mul(mulv(beta, Ll), cn);
-                        if (m.tex < 0) {
-                            gadd(gacc, a.grad, use_lds, 4 * mid + 0, g.x * k.x * pl.df_dkd.x * mis);
-                            gadd(gacc, a.grad, use_lds, 4 * mid + 1, g.y * k.y * pl.df_dkd.y * mis);
-                            gadd(gacc, a.grad, use_lds, 4 * mid + 2, g.z * k.z * pl.df_dkd.z * mis);
-                        }
+                        add_albedo(m, mid, tu, tv,
+                                   f3(g.x * k.x * pl.df_dkd.x * mis, g.y * k.y * pl.df_dkd.y * mis,
+                                      g.z * k.z * pl.df_dkd.z * mis));
                         const float va = g.x * k.x * (pl.df_da.x * mis + pl.f.x * dmis) +
                                          g.y * k.y * (pl.df_da.y * mis + pl.f.y * dmis) +
                                          g.z * k.z * (pl.df_da.z * mis + pl.f.z * dmis);
@@ -226,46 +265,80 @@ __global__ __launch_bounds__(ABLK) void k_adjoint(AdjArgs a) {
                     }
                 }
             }
-            bool cont = true;
-            float prr = 1.0f;
-            if (depth >= a.rrdepth) {
-                cont = !(ua.w > a.prr);
-                prr = a.prr;
-            }
-            if (a.maxdepth > 0 && depth + 1 >= a.maxdepth) cont = false;
-            if (!cont) break;
-            const float4 ub = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 1u);
-            const F3 wi = bsdf_sample<0u>(m, ub.x, ub.y, ub.z, wo, nrm);
-            const GgxParts pi = mat_parts(sv, m, wo, nrm, wi, tu, tv);
-            const float cw = fabsf(dot(wi, nrm));
-            const F3 nb = f3(beta.x * cw * pi.f.x / pi.pdf / prr, beta.y * cw * pi.f.y / pi.pdf / prr,
-                             beta.z * cw * pi.f.z / pi.pdf / prr);
+        }
+        bool cont = true;
+        float prr = 1.0f;
+        if (depth >= a.rrdepth) {
+            cont = !(ua.w > a.prr);
+            prr = a.prr;
+        }
+        if (a.maxdepth > 0 && depth + 1 >= a.maxdepth) cont = false;
+        if (!cont) break;
+        const float4 ub = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 1u);
+        const F3 wi = bsdf_sample<F>(m, ub.x, ub.y, ub.z, wo, nrm);
+        const GgxParts pi = mat_parts<F>(sv, m, wo, nrm, wi, tu, tv);
+        const float cw = fabsf(dot(wi, nrm));
+        const F3 nb = f3(beta.x * cw * pi.f.x / pi.pdf / prr, beta.y * cw * pi.f.y / pi.pdf / prr,
+                         beta.z * cw * pi.f.z / pi.pdf / prr);
+        if (GRAD) {
             // everything still in Lrem arrives through this bounce: d log T = d log f - d log pdf + score
             const float score = m.type == MAT_GGXREFL ? ggx_score(m.alpha, wo, nrm, wi) : 0.0f;
             const float dlp = pi.pdf > 0.0f ? pi.dpdf_da / pi.pdf : 0.0f;
             float va = 0.0f;
+            F3 dal = f3(0.0f, 0.0f, 0.0f);
             if (pi.f.x != 0.0f) {
-                if (m.tex < 0) gadd(gacc, a.grad, use_lds, 4 * mid + 0, g.x * Lrem.x * pi.df_dkd.x / pi.f.x);
+                dal.x = g.x * Lrem.x * pi.df_dkd.x / pi.f.x;
                 va += g.x * Lrem.x * (pi.df_da.x / pi.f.x - dlp + score);
             }
             if (pi.f.y != 0.0f) {
-                if (m.tex < 0) gadd(gacc, a.grad, use_lds, 4 * mid + 1, g.y * Lrem.y * pi.df_dkd.y / pi.f.y);
+                dal.y = g.y * Lrem.y * pi.df_dkd.y / pi.f.y;
                 va += g.y * Lrem.y * (pi.df_da.y / pi.f.y - dlp + score);
             }
             if (pi.f.z != 0.0f) {
-                if (m.tex < 0) gadd(gacc, a.grad, use_lds, 4 * mid + 2, g.z * Lrem.z * pi.df_dkd.z / pi.f.z);
+                dal.z = g.z * Lrem.z * pi.df_dkd.z / pi.f.z;
                 va += g.z * Lrem.z * (pi.df_da.z / pi.f.z - dlp + score);
             }
+            add_albedo(m, mid, tu, tv, dal);
             if (m.type == MAT_GGXREFL) gadd(gacc, a.grad, use_lds, 4 * mid + 3, va);
-            if (nb.x == 0.0f && nb.y == 0.0f && nb.z == 0.0f) break;
-            beta = nb;
-            fac = fac_next;
-            dfac = dfac_next;
-            prev_mat = mid;
-            o = add(hp, mul(wi, 1e-3f));
-            d = wi;
         }
-        (void)fac;
+        if (nb.x == 0.0f && nb.y == 0.0f && nb.z == 0.0f) break;
+        beta = nb;
+        fac = fac_next;
+        dfac = dfac_next;
+        prev_mat = mid;
+        o = add(hp, mul(wi, 1e-3f));
+        d = wi;
+    }
+    return L;
+}
+
+// Persistent grid over samples [slot0, slot1).  Stored radiance (the last single-batch render)
+// or, without it, a forward replay per sample first (recompute-only: nothing per path is kept).
+template <unsigned F>
+__global__ __launch_bounds__(ABLK) void k_adjoint(AdjArgs a) {
+    extern __shared__ char smem[];
+    int* stk = reinterpret_cast<int*>(smem) + threadIdx.x;
+    float* gacc = reinterpret_cast<float*>(smem + size_t(a.sv.stack_cap) * ABLK * sizeof(int));
+    const bool use_lds = a.n_params <= LDS_PARAMS;
+    if (use_lds)
+        for (int k = threadIdx.x; k < a.n_params; k += blockDim.x) gacc[k] = 0.0f;
+    __syncthreads();
+    bool overflow = false;
+    for (int slot = a.slot0 + blockIdx.x * blockDim.x + threadIdx.x; slot < a.slot1; slot += gridDim.x * blockDim.x) {
+        const int pixel = slot % a.npix;
+        F3 Lt;
+        if (a.radiance) {
+            const float4 r = a.radiance[slot];
+            Lt = f3(r.x, r.y, r.z);
+        } else {
+            Lt = replay<F, false>(a, slot, f3(0, 0, 0), f3(0, 0, 0), stk, gacc, use_lds, overflow);
+        }
+        // max(0, L) per channel at the camera (rendererptls.cpp:173): zero slope where L <= 0 or NaN
+        const F3 g = f3(Lt.x > 0.0f ? a.dldi[3 * pixel] * a.inv_spp : 0.0f,
+                        Lt.y > 0.0f ? a.dldi[3 * pixel + 1] * a.inv_spp : 0.0f,
+                        Lt.z > 0.0f ? a.dldi[3 * pixel + 2] * a.inv_spp : 0.0f);
+        if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) continue;
+        replay<F, true>(a, slot, Lt, g, stk, gacc, use_lds, overflow);
     }
     if (overflow) atomicOr(a.err, 1);
     __syncthreads();
@@ -341,36 +414,72 @@ double loss_l2(Scene& sc, const float* target, float* dldi_host) {
     return loss;
 }
 
-void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const float* dldi_host, double* grad_host) {
-    if (!sc.target || !sc.target->adj_valid) throw std::runtime_error("adjoint needs a single-batch forward render first");
+void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const float* dldi_host, double* grad_host,
+                   int tex_id, double* gtex_host) {
+    if (!sc.built) throw std::runtime_error("scene not built");
+    if (sc.features & F_GGX) throw std::runtime_error("the adjoint supports Lambert and GGX-reflect materials");
+    if (p.W <= 0 || p.H <= 0 || p.spp <= 0) throw std::runtime_error("bad image size or spp");
+    if (!sc.target) sc.target = new RenderTarget();
     RenderTarget& t = *sc.target;
-    if (t.W != p.W || t.H != p.H || t.adj_spp != p.spp || t.adj_seed != p.seed || t.adj_spp0 != p.spp_offset)
-        throw std::runtime_error("adjoint parameters differ from the last forward render");
-    if (sc.features & 1u) throw std::runtime_error("the adjoint supports Lambert and GGX-reflect materials");
+    // the stored per-sample radiance serves when it belongs to exactly these samples
+    const bool stored = t.adj_valid && t.W == p.W && t.H == p.H && t.adj_spp == p.spp && t.adj_seed == p.seed &&
+                        t.adj_spp0 == p.spp_offset;
     const size_t n3 = size_t(p.W) * p.H * 3;
-    if (dldi_host) t.adj_dldi.upload(dldi_host, n3 * sizeof(float), sc.stream);
-    else if (!t.adj_dldi.ptr) throw std::runtime_error("no dL/dimage: pass one or call nr_loss_grad_l2");
+    if (dldi_host) {
+        t.adj_dldi.ensure(n3 * sizeof(float));
+        t.adj_dldi.upload(dldi_host, n3 * sizeof(float), sc.stream);
+    } else if (!t.adj_dldi.ptr || t.W != p.W || t.H != p.H) {
+        throw std::runtime_error("no dL/dimage: pass one or call nr_loss_grad_l2");
+    }
     const int n_params = int(sc.mats.size() / 8) * 4;
     t.adj_grad.ensure(sizeof(double) * n_params);
     hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, sc.stream, t.adj_grad.as<double>(), n_params);
-    int* cnt = t.counters.as<int>();
+    double* gtex = nullptr;
+    size_t ntex = 0;
+    if (tex_id >= 0 && gtex_host) {
+        if (tex_id >= int(sc.textures.size()) || sc.textures[tex_id].w == 0)
+            throw std::runtime_error("texel gradient of a texture that does not exist");
+        ntex = size_t(sc.textures[tex_id].w) * sc.textures[tex_id].h * 3;
+        t.adj_gtex.ensure(sizeof(double) * ntex);
+        hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, sc.stream, t.adj_gtex.as<double>(), int(ntex));
+        gtex = t.adj_gtex.as<double>();
+    }
+    int* cnt = t.counters.ptr ? t.counters.as<int>() : nullptr;
+    if (!cnt) {
+        t.counters.ensure(sizeof(int) * 16);
+        cnt = t.counters.as<int>();
+    }
     const int zero = 0;
     check_hip(hipMemcpyAsync(cnt + 15, &zero, sizeof(int), hipMemcpyHostToDevice, sc.stream), "H2D");
-    AdjArgs a{sc.view, cam, p.W, p.H, p.W * p.H, p.W * p.H * p.spp, p.seed, uint32_t(p.spp_offset), p.prr, p.mis,
-              p.maxdepth, p.rrdepth, t.radiance.as<float4>(), t.adj_dldi.as<float>(), 1.0f / float(p.spp), n_params,
-              t.adj_grad.as<double>(), cnt + 15};
+    const long long P = (long long)p.W * p.H * p.spp;
+    if (P > 0x7FFFFFFFll) throw std::runtime_error("adjoint: more than 2^31 samples");
+    AdjArgs a{sc.view, cam, p.W, p.H, p.W * p.H, int(P), p.seed, uint32_t(p.spp_offset), p.prr, p.mis, p.maxdepth,
+              p.rrdepth, stored ? t.radiance.as<float4>() : nullptr, 0, int(P), t.adj_dldi.as<float>(),
+              1.0f / float(p.spp), n_params, t.adj_grad.as<double>(), cnt + 15, tex_id, gtex};
     const size_t shmem = size_t(sc.view.stack_cap) * ABLK * sizeof(int) +
                          (n_params <= LDS_PARAMS ? size_t(n_params) * sizeof(float) : 0);
     if (shmem > 160 * 1024) throw std::runtime_error("adjoint LDS budget exceeded");
+    const bool tex = (sc.features & F_TEX) != 0;
     int per_cu = 0;
-    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adjoint, ABLK, shmem), "occupancy");
+    check_hip(tex ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adjoint<F_TEX>, ABLK, shmem)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adjoint<0u>, ABLK, shmem),
+              "occupancy");
     const int grid = std::max(1, per_cu) * device_cu_count();
-    hipLaunchKernelGGL(k_adjoint, dim3(grid), dim3(ABLK), shmem, sc.stream, a);
-    check_hip(hipGetLastError(), "k_adjoint launch");
+    // launches of at most 16 M samples, so that no single dispatch runs for long
+    const long long chunk = 16ll << 20;
+    for (long long s0 = 0; s0 < P; s0 += chunk) {
+        a.slot0 = int(s0);
+        a.slot1 = int(std::min(P, s0 + chunk));
+        if (tex) hipLaunchKernelGGL(k_adjoint<F_TEX>, dim3(grid), dim3(ABLK), shmem, sc.stream, a);
+        else hipLaunchKernelGGL(k_adjoint<0u>, dim3(grid), dim3(ABLK), shmem, sc.stream, a);
+        check_hip(hipGetLastError(), "k_adjoint launch");
+    }
     int err = 0;
     check_hip(hipMemcpyAsync(&err, cnt + 15, sizeof(int), hipMemcpyDeviceToHost, sc.stream), "D2H");
     check_hip(hipMemcpyAsync(grad_host, t.adj_grad.ptr, sizeof(double) * n_params, hipMemcpyDeviceToHost, sc.stream),
               "D2H");
+    if (gtex)
+        check_hip(hipMemcpyAsync(gtex_host, gtex, sizeof(double) * ntex, hipMemcpyDeviceToHost, sc.stream), "D2H");
     check_hip(hipStreamSynchronize(sc.stream), "adjoint sync");
     if (err) throw std::runtime_error("traversal stack overflow in the adjoint");
 }

@@ -83,6 +83,14 @@ int nr_scene_set_texture_rgba8(nr_scene* s, int32_t id, int32_t w, int32_t h, co
     });
 }
 
+int nr_scene_set_texture_rgb32f(nr_scene* s, int32_t id, int32_t w, int32_t h, const float* rgb) {
+    return guard([&] {
+        if (s->sc->env_tex >= 0 && id == s->sc->env_tex) throw std::runtime_error("texture id is the environment map");
+        s->sc->add_texture(id, w, h, 1, rgb);
+        return NR_OK;
+    });
+}
+
 int nr_scene_set_envmap(nr_scene* s, int32_t w, int32_t h, const float* rgb, float gain) {
     return guard([&] {
         // the environment map lives in the texture pool after the albedo textures
@@ -259,6 +267,15 @@ int nr_adjoint_ptnee(nr_scene* s, const nr_camera* cam, const nr_params* params,
     return guard([&] {
         nr::CamView cv = nr::make_camera(cam->pos, cam->euler, cam->fov, cam->aspect);
         nr::adjoint_ptnee(*s->sc, cv, to_params(params), dL_dimg, grad);
+        return NR_OK;
+    });
+}
+
+int nr_adjoint_ptnee_tex(nr_scene* s, const nr_camera* cam, const nr_params* params, const float* dL_dimg,
+                         double* grad, int32_t tex_id, double* grad_texels) {
+    return guard([&] {
+        nr::CamView cv = nr::make_camera(cam->pos, cam->euler, cam->fov, cam->aspect);
+        nr::adjoint_ptnee(*s->sc, cv, to_params(params), dL_dimg, grad, tex_id, grad_texels);
         return NR_OK;
     });
 }

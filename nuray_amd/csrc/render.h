@@ -51,7 +51,7 @@ struct RenderTarget {
     bool adj_valid = false;
     int adj_spp = 0, adj_spp0 = 0;
     uint32_t adj_seed = 0;
-    DevBuf adj_target, adj_dldi, adj_loss, adj_grad;
+    DevBuf adj_target, adj_dldi, adj_loss, adj_grad, adj_gtex;
     // radiance-cache rendering: spread state per queued ray, cache queries, training vertices
     DevBuf nstate[2], nrc_queries, nrc_recs, nrc_lb, nrc_tgt, nrc_batch, nrc_batch_tgt;
     ~RenderTarget();

@@ -70,9 +70,17 @@ int Scene::add_texture(int id, int w, int h, int fmt, const void* data) {
     if (id < 0 || w <= 0 || h <= 0) throw std::runtime_error("bad texture");
     if (int(textures.size()) <= id) textures.resize(id + 1);
     Texture& t = textures[id];
+    // same size and format on a built scene: rewrite the texels in place, no rebuild
+    const bool in_place = built && t.w == w && t.h == h && t.fmt == fmt && size_t(id) < tex_offsets.size();
     t.w = w; t.h = h; t.fmt = fmt;
     size_t bytes = size_t(w) * h * (fmt == 0 ? 4 : 12);
     t.bytes.assign(static_cast<const uint8_t*>(data), static_cast<const uint8_t*>(data) + bytes);
+    if (in_place) {
+        check_hip(hipMemcpyAsync(static_cast<uint8_t*>(d_texpool.ptr) + tex_offsets[id], t.bytes.data(), bytes,
+                                 hipMemcpyHostToDevice, stream), "texture update");
+        check_hip(hipStreamSynchronize(stream), "texture update sync");
+        return id;
+    }
     built = false;
     return id;
 }
@@ -164,6 +172,8 @@ void Scene::build(int nthreads) {
         td[j] = TexDesc{textures[j].w, textures[j].h, textures[j].fmt, 0, pool};
         pool += (textures[j].bytes.size() + 255) & ~size_t(255);
     }
+    tex_offsets.resize(textures.size());
+    for (size_t j = 0; j < textures.size(); ++j) tex_offsets[j] = td[j].offset;
     std::vector<uint8_t> poolh(pool);
     for (size_t j = 0; j < textures.size(); ++j)
         std::memcpy(poolh.data() + td[j].offset, textures[j].bytes.data(), textures[j].bytes.size());

@@ -38,6 +38,7 @@ public:
     std::vector<int32_t> mat;
     std::vector<float> mats;      // M*8
     std::vector<Texture> textures;
+    std::vector<size_t> tex_offsets;  // byte offset of each texture in the device pool (after build)
     int env_tex = -1;
     float env_gain = 1.0f;
     unsigned features = 0;        // material features present: 1 = GGX dielectric, 2 = albedo textures

@@ -379,3 +379,22 @@ def material_zoo(width=160, height=120):
                 mats=np.asarray(mats, np.float32), textures=[_checker_rgba(64, 64, 1), _checker_rgba(32, 32, 2)],
                 camera=dict(base["camera"], aspect=width / height), width=width, height=height,
                 envmap=dict(rgb=sky, gain=1.0), name="material_zoo")
+
+
+def textured_room(width=64, height=48, floor_tex=32, wall_tex=16, seed=4):
+    """The box room with linear float albedo textures for the inverse-texture problem: the floor is
+    Lambert with a floor_tex^2 texture (uv 0..2, wrapping), the back wall GGX-reflect (alpha 0.3)
+    with a wall_tex^2 F0 texture; other walls and both boxes Lambert; the ceiling light."""
+    sc = material_zoo(width, height)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    keep = np.isin(sc["mat"], [0, 1, 2, 3, 4, 5, 8])       # drop the dielectric spheres
+    sc["p"], sc["n"], sc["uv"], sc["mat"] = sc["p"][keep], sc["n"][keep], sc["uv"][keep], sc["mat"][keep]
+    sc["uv"][sc["mat"] == 0] *= 0.5
+    mats = sc["mats"].copy()
+    mats[8] = mat_row(LAMBERT, (0.7, 0.6, 0.5))
+    sc["mats"] = mats
+    sc["textures"] = [(0.2 + 0.6 * rng.random((floor_tex, floor_tex, 3))).astype(np.float32),
+                      (0.3 + 0.6 * rng.random((wall_tex, wall_tex, 3))).astype(np.float32)]
+    sc["envmap"] = None
+    sc["name"] = "textured_room"
+    return sc

@@ -33,12 +33,13 @@ struct MatD {  // f, pdf and their parameter derivatives at one direction
     double pdf = 0, dpdf_da = 0;
 };
 
-MatD mat_deriv(const Material& m, const V3d& wo, const V3d& n, const V3d& wi) {
+// kd: the material's albedo at the hit (Kd, or its texture's bilinear value).
+MatD mat_deriv(const Material& m, const V3d& kd, const V3d& wo, const V3d& n, const V3d& wi) {
     MatD r;
     if (m.type == MAT_LAMBERT) {
         const bool same = !(wi.dot(n) * wo.dot(n) < 0);
         const double inv = same ? 1.0 / 3.14159 : 0.0;
-        r.f = V3d(m.kd[0] * inv, m.kd[1] * inv, m.kd[2] * inv);
+        r.f = V3d(kd.x * inv, kd.y * inv, kd.z * inv);
         r.df_dkd = V3d(inv);
         r.pdf = 1.0 / 3.14159 * (std::fabs(wi.dot(n)) + 1e-4);
         return r;
@@ -74,7 +75,7 @@ MatD mat_deriv(const Material& m, const V3d& wo, const V3d& n, const V3d& wi) {
     const double den = 4 * (co + 1e-6) * (ci + 1e-6);
     const double s = Dv * Gl * Gv / den;
     const double ds = (dD * Gl * Gv + Dv * dGl * Gv + Dv * Gl * dGv) / den;
-    const V3d F(m.kd[0] + (1 - m.kd[0]) * fr, m.kd[1] + (1 - m.kd[1]) * fr, m.kd[2] + (1 - m.kd[2]) * fr);
+    const V3d F(kd.x + (1 - kd.x) * fr, kd.y + (1 - kd.y) * fr, kd.z + (1 - kd.z) * fr);
     r.f = F * s;
     r.df_dkd = V3d(s * (1 - fr));
     r.df_da = F * ds;
@@ -91,11 +92,48 @@ double ggx_score(double a, const V3d& wo, const V3d& n, const V3d& wi) {
     return 2 / a - 4 * a / (a * a + t * t);
 }
 
+// Bilinear corners of Texture::pixelUV / pixelBi (texture.cpp:54-72) at (u, v): texel indices
+// (-1 outside the image, which reads black) and weights, in the order of Texture::bilinear.
+void texel_corners(const Texture& t, float u, float v, int idx[4], double w[4]) {
+    const float fu = std::fmod(std::fmod(u, 1.0f) + 1.0f, 1.0f);
+    const float fv = std::fmod(std::fmod(1.0f - v, 1.0f) + 1.0f, 1.0f);
+    const float x = fu * t.w, y = fv * t.h;
+    const int x0 = int(std::floor(x)), y0 = int(std::floor(y)), x1 = x0 + 1, y1 = y0 + 1;
+    const int xs[4] = {x1, x1, x0, x0}, ys[4] = {y1, y0, y1, y0};
+    const float ws[4] = {(x - x0) * (y - y0), (x - x0) * (y1 - y), (x1 - x) * (y - y0), (x1 - x) * (y1 - y)};
+    for (int k = 0; k < 4; ++k) {
+        const bool in = xs[k] >= 0 && ys[k] >= 0 && xs[k] < t.w && ys[k] < t.h;
+        idx[k] = in ? ys[k] * t.w + xs[k] : -1;
+        w[k] = ws[k];
+    }
+}
+
 struct GradTracer {
     const Scene& sc;
     const BVH& bvh;
     RenderOpts opt;
     Tracer<double> tr;
+    int P;        // material parameters; texel keys start here
+    int tex_id;   // texture whose texels are differentiated (-1: none)
+
+    // albedo derivative v (RGB) of material mi at texture coordinates (u, v)
+    void add_albedo(SG& dL, const Material& M, int mi, float u, float vv, const V3d& v) const {
+        if (M.tex < 0) {
+            dL.add(4 * mi + 0, V3d(v.x, 0, 0));
+            dL.add(4 * mi + 1, V3d(0, v.y, 0));
+            dL.add(4 * mi + 2, V3d(0, 0, v.z));
+        } else if (M.tex == tex_id) {
+            int idx[4];
+            double w[4];
+            texel_corners(sc.textures[M.tex], u, vv, idx, w);
+            for (int k = 0; k < 4; ++k) {
+                if (idx[k] < 0 || w[k] == 0) continue;
+                dL.add(P + 3 * idx[k] + 0, V3d(v.x * w[k], 0, 0));
+                dL.add(P + 3 * idx[k] + 1, V3d(0, v.y * w[k], 0));
+                dL.add(P + 3 * idx[k] + 2, V3d(0, 0, v.z * w[k]));
+            }
+        }
+    }
 
     V3d trace(const V3d& o, const V3d& d, double fac, int prev_mat, double dfac, int depth, const Stream& rng,
               SG& dL) const {
@@ -114,6 +152,10 @@ struct GradTracer {
         if (n.dot(d) > 0 && !(M.type == MAT_GGX)) return V3d(0);
         const V3d hp = o + d * t;
         const int mi = tri.mat;
+        const V3d tc = tri.texc<double>(b1, b2);
+        const float tu = float(tc.x), tv = float(tc.y);
+        MatEval<double> me{&sc, &M};
+        const V3d kd = me.albedo(tc);
         V3d result(0);
         double fac_next = 1, dfac_next = 0;
         float u[4];
@@ -130,7 +172,7 @@ struct GradTracer {
             const V3d le(LM.kd[0], LM.kd[1], LM.kd[2]);
             const double lpdf = sc.light_p();
             const double lop = lpdf * lv.dot(lv) / std::max(1e-6, ln.dot(-wl));
-            const MatD md = mat_deriv(M, wo, n, wl);
+            const MatD md = mat_deriv(M, kd, wo, n, wl);
             double mis = lop / (lop + md.pdf);
             double dmis = -lop / ((lop + md.pdf) * (lop + md.pdf)) * md.dpdf_da;
             if (opt.mis == 0) mis = 0.5, dmis = 0;
@@ -144,11 +186,7 @@ struct GradTracer {
                 const double cn = std::max(0.0, n.dot((lp - hp).normalized()));
                 result += Ll * md.f * cn * mis;
                 const V3d k = Ll * cn;
-                if (M.tex < 0) {
-                    dL.add(4 * mi + 0, V3d(k.x * md.df_dkd.x * mis, 0, 0));
-                    dL.add(4 * mi + 1, V3d(0, k.y * md.df_dkd.y * mis, 0));
-                    dL.add(4 * mi + 2, V3d(0, 0, k.z * md.df_dkd.z * mis));
-                }
+                add_albedo(dL, M, mi, tu, tv, V3d(k.x * md.df_dkd.x * mis, k.y * md.df_dkd.y * mis, k.z * md.df_dkd.z * mis));
                 if (M.type == MAT_GGXREFL) dL.add(4 * mi + 3, k * (md.df_da * mis + md.f * dmis));
             }
         }
@@ -160,20 +198,17 @@ struct GradTracer {
         if (opt.maxdepth > 0 && depth + 1 >= opt.maxdepth) return result;
         float v[4];
         rng.draw(uint32_t(1 + depth), 1, v);
-        MatEval<double> me{&sc, &M};
         const V3d wi = me.sample(v, wo, n);
-        const MatD md = mat_deriv(M, wo, n, wi);
+        const MatD md = mat_deriv(M, kd, wo, n, wi);
         SG dLc;
         const V3d Lc = trace(hp + wi * 1e-3, wi, fac_next, mi, dfac_next, depth + 1, rng, dLc);
         const double cw = std::fabs(wi.dot(n));
         const V3d T = md.f * (cw / md.pdf / prr);
         result += Lc * T;
         dL.add_scaled(dLc, T);
-        if (M.tex < 0) {
+        {
             const V3d dT = md.df_dkd * (cw / md.pdf / prr);
-            dL.add(4 * mi + 0, V3d(dT.x * Lc.x, 0, 0));
-            dL.add(4 * mi + 1, V3d(0, dT.y * Lc.y, 0));
-            dL.add(4 * mi + 2, V3d(0, 0, dT.z * Lc.z));
+            add_albedo(dL, M, mi, tu, tv, V3d(dT.x * Lc.x, dT.y * Lc.y, dT.z * Lc.z));
         }
         if (M.type == MAT_GGXREFL) {
             const V3d dTa = (md.df_da * md.pdf - md.f * md.dpdf_da) * (cw / (md.pdf * md.pdf) / prr);
@@ -202,9 +237,11 @@ void orc_set_params(void* h, const float* theta, int n_mats) {
     }
 }
 
-// grad (n_mats x 4) of sum_p,c dldi[p,c] * mean_s max(0, L_s[p,c]), FP64.
+// grad (n_mats x 4) of sum_p,c dldi[p,c] * mean_s max(0, L_s[p,c]), FP64; with tex_id >= 0 also
+// grad_tex (w x h x 3) with respect to that texture's linear texel values.
 int orc_adjoint(void* h, const float* cam, int W, int H, int spp, int spp0, uint32_t seed, float prr, int mis,
-                int maxdepth, int rrdepth, const float* dldi, double* grad, int n_mats, int nthreads) {
+                int maxdepth, int rrdepth, const float* dldi, double* grad, int n_mats, int nthreads, int tex_id,
+                double* grad_tex) {
     OScene* o = static_cast<OScene*>(h);
     if (!o->built) return -1;
     Camera c;
@@ -215,12 +252,13 @@ int orc_adjoint(void* h, const float* cam, int W, int H, int spp, int spp0, uint
     RenderOpts opt;
     opt.prr = prr; opt.mis = mis; opt.maxdepth = maxdepth; opt.rrdepth = rrdepth; opt.seed = seed;
     const int P = 4 * n_mats;
+    const int NT = (tex_id >= 0 && grad_tex) ? 3 * o->sc.textures[tex_id].w * o->sc.textures[tex_id].h : 0;
     if (nthreads > 0) omp_set_num_threads(nthreads);
-    std::vector<double> acc(P, 0.0);
+    std::vector<double> acc(P + NT, 0.0);
 #pragma omp parallel
     {
-        std::vector<double> loc(P, 0.0);
-        GradTracer gt{o->sc, o->bvh, opt, Tracer<double>{o->sc, o->bvh, opt}};
+        std::vector<double> loc(P + NT, 0.0);
+        GradTracer gt{o->sc, o->bvh, opt, Tracer<double>{o->sc, o->bvh, opt}, P, NT ? tex_id : -1};
 #pragma omp for schedule(dynamic, 16)
         for (int px = 0; px < W * H; ++px) {
             const int x = px % W, y = px / W;
@@ -234,13 +272,14 @@ int orc_adjoint(void* h, const float* cam, int W, int H, int spp, int spp0, uint
                 const double g[3] = {L.x > 0 ? dldi[3 * px] / spp : 0.0, L.y > 0 ? dldi[3 * px + 1] / spp : 0.0,
                                      L.z > 0 ? dldi[3 * px + 2] / spp : 0.0};
                 for (const auto& e : dL.e)
-                    if (e.first < P) loc[e.first] += g[0] * e.second.x + g[1] * e.second.y + g[2] * e.second.z;
+                    if (e.first < P + NT) loc[e.first] += g[0] * e.second.x + g[1] * e.second.y + g[2] * e.second.z;
             }
         }
 #pragma omp critical
-        for (int k = 0; k < P; ++k) acc[k] += loc[k];
+        for (int k = 0; k < P + NT; ++k) acc[k] += loc[k];
     }
     for (int k = 0; k < P; ++k) grad[k] = acc[k];
+    for (int k = 0; k < NT; ++k) grad_tex[k] = acc[P + k];
     return 0;
 }
 

@@ -72,6 +72,15 @@ void orc_scene_set_texture_rgba8(void* h, int id, int w, int hh, const uint8_t* 
         for (int c = 0; c < 3; ++c) t.rgb[3 * i + c] = srgb8_decode(rgba[4 * i + c]);
 }
 
+// Linear RGB float texture (3 floats per texel, first row at v = 1).
+void orc_scene_set_texture_rgb32f(void* h, int id, int w, int hh, const float* rgb) {
+    auto& v = S(h)->sc.textures;
+    if (int(v.size()) <= id) v.resize(id + 1);
+    Texture& t = v[id];
+    t.w = w; t.h = hh;
+    t.rgb.assign(rgb, rgb + size_t(w) * hh * 3);
+}
+
 int orc_bvh_build(void* h) {
     S(h)->bvh.build(S(h)->sc);
     S(h)->built = true;

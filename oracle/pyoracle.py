@@ -58,7 +58,8 @@ def lib():
         L.orc_tri_test.argtypes = [f32p, f32p, f32p, f32p]
         L.orc_set_params.argtypes = [vp, f32p, C.c_int]
         L.orc_adjoint.argtypes = [vp, f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_float, C.c_int,
-                                  C.c_int, C.c_int, f32p, f64p, C.c_int, C.c_int]
+                                  C.c_int, C.c_int, f32p, f64p, C.c_int, C.c_int, C.c_int, vp]
+        L.orc_scene_set_texture_rgb32f.argtypes = [vp, C.c_int, C.c_int, C.c_int, f32p]
         L.orc_nrc_encode.argtypes = [f32p, C.c_int, f32p]
         L.orc_nrc_forward.argtypes = [f32p, f32p, C.c_int, C.c_int, f64p]
         L.orc_nrc_grad.argtypes = [f32p, f32p, f32p, C.c_int, C.c_int, f64p]
@@ -92,8 +93,12 @@ class Scene:
             rgb = _f32(env["rgb"])
             L.orc_scene_set_envmap(self.h, rgb.shape[1], rgb.shape[0], rgb, float(env.get("gain", 1.0)))
         for tid, tex in enumerate(sc.get("textures") or []):
-            t = np.ascontiguousarray(tex, np.uint8)
-            L.orc_scene_set_texture_rgba8(self.h, tid, t.shape[1], t.shape[0], t)
+            if np.asarray(tex).dtype == np.float32:   # linear RGB float texture
+                t = _f32(tex)
+                L.orc_scene_set_texture_rgb32f(self.h, tid, t.shape[1], t.shape[0], t)
+            else:
+                t = np.ascontiguousarray(tex, np.uint8)
+                L.orc_scene_set_texture_rgba8(self.h, tid, t.shape[1], t.shape[0], t)
         self.n_nodes = L.orc_bvh_build(self.h) if build_bvh else 0
 
     def __del__(self):
@@ -151,19 +156,26 @@ class Scene:
         lib().orc_material_eval(self.h, m, k, wo, n, wi_in, u_in, 1 if wi is None else 0, wi_out, f, pdf)
         return wi_out, f, pdf
 
+    def set_texture_rgb32f(self, tid, rgb):
+        rgb = _f32(rgb)
+        lib().orc_scene_set_texture_rgb32f(self.h, tid, rgb.shape[1], rgb.shape[0], rgb)
+
     def set_params(self, theta):
         theta = _f32(theta)
         lib().orc_set_params(self.h, theta.reshape(-1), len(theta))
 
     def adjoint(self, cam, W, H, spp, dldi, n_mats, spp0=0, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0,
-                nthreads=0):
-        """FP64 gradient (n_mats, 4) of sum(dldi * mean image) for the current parameters."""
+                nthreads=0, tex_id=-1, tex_shape=None):
+        """FP64 gradient (n_mats, 4) of sum(dldi * mean image) for the current parameters; with
+        tex_id >= 0 also the texel gradient (h, w, 3) of that texture."""
         g = np.zeros(n_mats * 4, np.float64)
+        gt = np.zeros((tex_shape[0], tex_shape[1], 3), np.float64) if tex_id >= 0 else None
         r = lib().orc_adjoint(self.h, _f32(cam), W, H, spp, spp0, seed, prr, mis, maxdepth, rrdepth,
-                              _f32(dldi).reshape(-1), g, n_mats, nthreads)
+                              _f32(dldi).reshape(-1), g, n_mats, nthreads, tex_id,
+                              None if gt is None else gt.ctypes.data)
         if r != 0:
             raise RuntimeError("oracle adjoint failed")
-        return g.reshape(n_mats, 4)
+        return g.reshape(n_mats, 4) if gt is None else (g.reshape(n_mats, 4), gt)
 
     def light_pick(self, u):
         return lib().orc_light_pick(self.h, float(u))

@@ -532,3 +532,37 @@ def test_nrc_oracle_gradient_central_difference(po):
         fd = (surrogate(wp) - surrogate(wm)) / (float(wp[i]) - float(wm[i]))
         assert abs(fd - g[i]) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(g).max()), (i, fd, g[i])
     assert g[5 * 4096 + 3 * 64 + 5] == 0.0 and g[45] == 0.0   # unused output row, zero input
+
+
+def test_adjoint_texel_gradient_equals_crn_central_difference(po, scenes):
+    """Texel values enter each sample through the bilinear albedo (texture.cpp:61-67), linearly in
+    each texel, so the central difference of the FP64 render with common random numbers equals
+    the texel gradient up to O(h^2) -- floor (Lambert) and back-wall (GGX-reflect F0) textures."""
+    sc = scenes.textured_room(width=12, height=10)
+    W, H, spp = 12, 10, 4
+    M = len(sc["mats"])
+    cam = scenes.camera_array(sc["camera"])
+    rng = np.random.Generator(np.random.PCG64(3))
+    dldi = rng.normal(size=(H, W, 3))
+    o = po.Scene(sc)
+    for tid in (0, 1):
+        tex = sc["textures"][tid].copy()
+        g, gt = o.adjoint(cam, W, H, spp, dldi, M, seed=13, tex_id=tid, tex_shape=tex.shape[:2])
+        assert np.abs(gt).max() > 0
+        # the textured material's Kd row gets no gradient
+        assert np.all(g[tid, :3] == 0)
+        flat = np.argsort(-np.abs(gt).reshape(-1))[:5]
+        for f in flat:
+            y, x, c = np.unravel_index(f, gt.shape)
+            h = 1e-3
+            tp, tm = tex.copy(), tex.copy()
+            tp[y, x, c] += h
+            tm[y, x, c] -= h
+            o.set_texture_rgb32f(tid, tp)
+            lp = (dldi.reshape(-1) * o.render(cam, W, H, spp, seed=13, fp64=True)[0] / spp).sum()
+            o.set_texture_rgb32f(tid, tm)
+            lm = (dldi.reshape(-1) * o.render(cam, W, H, spp, seed=13, fp64=True)[0] / spp).sum()
+            fd = (lp - lm) / (float(tp[y, x, c]) - float(tm[y, x, c]))
+            # the texture lookup is FP32 (texture.cpp), so the difference carries ~1e-7 / h rounding
+            assert abs(fd - gt[y, x, c]) <= 1e-4 * abs(fd) + 1e-9, (tid, y, x, c, fd, gt[y, x, c])
+        o.set_texture_rgb32f(tid, tex)
