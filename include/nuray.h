@@ -135,8 +135,9 @@ int nr_loss_grad_l2(nr_scene* s, const float* target, double* loss, float* dL_di
  * for the samples of cam / params (the last render's when they match).  When the last
  * nr_render_ptnee ran those samples as one wavefront batch its per-sample radiance is reused;
  * otherwise each sample is replayed forward first (recompute-only: no per-path storage, any
- * sample count).  Lambert and GGX-reflect materials, with or without albedo textures (a textured
- * material's Kd row gets no gradient). */
+ * sample count).  Gradients for Lambert and GGX-reflect materials, with or without albedo textures
+ * (a textured material's Kd row gets none); paths through translucent GGX dielectrics are
+ * replayed, but the dielectrics' own rows get no gradient. */
 int nr_adjoint_ptnee(nr_scene* s, const nr_camera* cam, const nr_params* params, const float* dL_dimg,
                      double* grad);
 /* The same, plus the gradient with respect to the linear texel values of albedo texture tex_id

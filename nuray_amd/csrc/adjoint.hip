@@ -92,6 +92,11 @@ __device__ __forceinline__ GgxParts mat_parts(const SceneView& sv, const Mat& m,
     GgxParts r;
     r.f = bsdf_eval<F>(sv, m, wo, n, wi, u, v);
     r.pdf = bsdf_pdf<F>(m, wo, n, wi);
+    if ((F & F_GGX) && m.type == MAT_GGX) {  // translucent dielectric: replayed, not differentiated
+        r.df_dkd = r.df_da = f3(0, 0, 0);
+        r.dpdf_da = 0.0f;
+        return r;
+    }
     const bool same = !(dot(wi, n) * dot(wo, n) < 0.0f);
     const float inv = same ? 1.0f / 3.14159f : 0.0f;  // Lambert: f = Kd / pi
     r.df_dkd = f3(inv, inv, inv);
@@ -298,7 +303,7 @@ __device__ __forceinline__ F3 replay(const AdjArgs& a, int slot, F3 Lt, F3 g, in
                 dal.z = g.z * Lrem.z * pi.df_dkd.z / pi.f.z;
                 va += g.z * Lrem.z * (pi.df_da.z / pi.f.z - dlp + score);
             }
-            add_albedo(m, mid, tu, tv, dal);
+            if (m.type != MAT_GGX) add_albedo(m, mid, tu, tv, dal);
             if (m.type == MAT_GGXREFL) gadd(gacc, a.grad, use_lds, 4 * mid + 3, va);
         }
         if (nb.x == 0.0f && nb.y == 0.0f && nb.z == 0.0f) break;
@@ -417,7 +422,6 @@ double loss_l2(Scene& sc, const float* target, float* dldi_host) {
 void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const float* dldi_host, double* grad_host,
                    int tex_id, double* gtex_host) {
     if (!sc.built) throw std::runtime_error("scene not built");
-    if (sc.features & F_GGX) throw std::runtime_error("the adjoint supports Lambert and GGX-reflect materials");
     if (p.W <= 0 || p.H <= 0 || p.spp <= 0) throw std::runtime_error("bad image size or spp");
     if (!sc.target) sc.target = new RenderTarget();
     RenderTarget& t = *sc.target;
@@ -459,10 +463,12 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
     const size_t shmem = size_t(sc.view.stack_cap) * ABLK * sizeof(int) +
                          (n_params <= LDS_PARAMS ? size_t(n_params) * sizeof(float) : 0);
     if (shmem > 160 * 1024) throw std::runtime_error("adjoint LDS budget exceeded");
-    const bool tex = (sc.features & F_TEX) != 0;
+    // kernel instance for the material features present (as the shade kernels)
+    const unsigned feat = sc.features & F_GGX ? F_ALL : (sc.features & F_TEX ? F_TEX : 0u);
     int per_cu = 0;
-    check_hip(tex ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adjoint<F_TEX>, ABLK, shmem)
-                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adjoint<0u>, ABLK, shmem),
+    check_hip(feat == F_ALL   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adjoint<F_ALL>, ABLK, shmem)
+              : feat == F_TEX ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adjoint<F_TEX>, ABLK, shmem)
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adjoint<0u>, ABLK, shmem),
               "occupancy");
     const int grid = std::max(1, per_cu) * device_cu_count();
     // launches of at most 16 M samples, so that no single dispatch runs for long
@@ -470,7 +476,8 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
     for (long long s0 = 0; s0 < P; s0 += chunk) {
         a.slot0 = int(s0);
         a.slot1 = int(std::min(P, s0 + chunk));
-        if (tex) hipLaunchKernelGGL(k_adjoint<F_TEX>, dim3(grid), dim3(ABLK), shmem, sc.stream, a);
+        if (feat == F_ALL) hipLaunchKernelGGL(k_adjoint<F_ALL>, dim3(grid), dim3(ABLK), shmem, sc.stream, a);
+        else if (feat == F_TEX) hipLaunchKernelGGL(k_adjoint<F_TEX>, dim3(grid), dim3(ABLK), shmem, sc.stream, a);
         else hipLaunchKernelGGL(k_adjoint<0u>, dim3(grid), dim3(ABLK), shmem, sc.stream, a);
         check_hip(hipGetLastError(), "k_adjoint launch");
     }

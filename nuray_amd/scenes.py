@@ -398,3 +398,88 @@ def textured_room(width=64, height=48, floor_tex=32, wall_tex=16, seed=4):
     sc["envmap"] = None
     sc["name"] = "textured_room"
     return sc
+
+
+def _tex_rgba8(size, k):
+    """Procedural RGBA8 albedo texture `k`: tinted 64-texel checker with a diagonal ramp."""
+    rng = np.random.Generator(np.random.PCG64(100 + k))
+    tint = (60 + 190 * rng.random((2, 3))).astype(np.float32)
+    idx = np.arange(size, dtype=np.int32)
+    chk = ((idx[:, None] >> 6) + (idx[None, :] >> 6)) & 1
+    ramp = ((idx[:, None] + idx[None, :]).astype(np.float32) * (0.5 / (2 * size)) + 0.5)
+    out = np.empty((size, size, 4), np.uint8)
+    for c in range(3):
+        v = np.where(chk == 1, tint[0, c], tint[1, c]) * ramp
+        out[..., c] = np.clip(v, 0, 255).astype(np.uint8)
+    out[..., 3] = 255
+    return out
+
+
+def large_textured(n_cells=2048, n_spheres=10000, tex_size=4096, n_tex=16, adj_tex=2048, width=3840, height=2160):
+    """The large textured scene: an n_cells^2 heightfield over x, z in [0, 2000] (amplitude 30,
+    value noise seed 1) split into 4 x 4 Lambert regions, each with its own tex_size^2 RGBA8 albedo
+    texture (uv = position / 250, wrapping) except region 5, which carries the adj_tex^2 linear
+    float texture that the adjoint optimises; n_spheres 80-triangle icospheres (r ~ U[2, 16],
+    centres in [0,2000] x [0,300] x [0,2000], seed 2), 10 % translucent GGX dielectric (ior 1.5,
+    alpha ~ U[0.05, 0.5]) and the rest GGX-reflect (alpha ~ U[0.05, 0.8], albedo ~ U[0.1, 0.9]^3,
+    seed 3); four 300 x 300 emissive quads at y = 600 and the 1024 x 512 sky."""
+    m = n_cells
+    g = np.arange(m + 1, dtype=np.float64) / m
+    X, Z = np.meshgrid(g, g, indexing="ij")
+    Y = 30.0 * _value_noise(X, Z, seed=1)
+    ext = 2000.0
+    P = np.stack([X * ext, Y, Z * ext], -1)
+    h = ext / m
+    dx = np.gradient(Y, h, axis=0)
+    dz = np.gradient(Y, h, axis=1)
+    Nn = np.stack([-dx, np.ones_like(Y), -dz], -1)
+    Nn /= np.sqrt((Nn * Nn).sum(-1, keepdims=True))
+    UV = np.stack([X * ext / 250.0, Z * ext / 250.0], -1)
+    a, b, c, d = (P[:-1, :-1], P[1:, :-1], P[1:, 1:], P[:-1, 1:])
+    na, nb, nc, nd = (Nn[:-1, :-1], Nn[1:, :-1], Nn[1:, 1:], Nn[:-1, 1:])
+    ua, ub, uc, ud = (UV[:-1, :-1], UV[1:, :-1], UV[1:, 1:], UV[:-1, 1:])
+    t1 = np.concatenate([a, d, c], -1).reshape(-1, 9)
+    t2 = np.concatenate([c, b, a], -1).reshape(-1, 9)
+    tp = np.stack([t1, t2], 1).reshape(-1, 9)
+    tn = np.stack([np.concatenate([na, nd, nc], -1).reshape(-1, 9),
+                   np.concatenate([nc, nb, na], -1).reshape(-1, 9)], 1).reshape(-1, 9)
+    tuv = np.stack([np.concatenate([ua, ud, uc], -1).reshape(-1, 6),
+                    np.concatenate([uc, ub, ua], -1).reshape(-1, 6)], 1).reshape(-1, 6)
+    ii, jj = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
+    region = ((ii * 4 // m) * 4 + (jj * 4 // m)).reshape(-1)
+    tmat = np.repeat(region, 2).astype(np.int32)
+    mats = [mat_row(LAMBERT, (1, 1, 1), tex=(n_tex if r == 5 else r)) for r in range(16)]
+    rng2 = np.random.Generator(np.random.PCG64(2))
+    rng3 = np.random.Generator(np.random.PCG64(3))
+    radius = 2.0 + 14.0 * rng2.random(n_spheres)
+    centre = rng2.random((n_spheres, 3)) * np.array([ext, 300.0, ext])
+    glass = rng3.random(n_spheres) < 0.1
+    rough = np.where(glass, 0.05 + 0.45 * rng3.random(n_spheres), 0.05 + 0.75 * rng3.random(n_spheres))
+    alb = np.where(glass[:, None], 0.8 + 0.2 * rng3.random((n_spheres, 3)), 0.1 + 0.8 * rng3.random((n_spheres, 3)))
+    base = len(mats)
+    for k in range(n_spheres):
+        mats.append(mat_row(GGX, alb[k], rough[k], ior=1.5) if glass[k] else mat_row(GGXREFL, alb[k], rough[k]))
+    sv, sf = _icosphere80()
+    sp_p = (sv[sf].reshape(1, 80, 9) * np.repeat(radius, 9).reshape(n_spheres, 1, 9)
+            + np.tile(centre, (1, 3)).reshape(n_spheres, 1, 9)).reshape(-1, 9)
+    sp_n = np.tile(sv[sf].reshape(80, 9), (n_spheres, 1))
+    sp_mat = (base + np.repeat(np.arange(n_spheres), 80)).astype(np.int32)
+    light_id = len(mats)
+    mats.append(mat_row(LIGHT, (8.0, 7.6, 7.0)))
+    lp = []
+    for cx, cz in ((500.0, 500.0), (1500.0, 500.0), (500.0, 1500.0), (1500.0, 1500.0)):
+        lp += _quad((cx - 150, 600, cz - 150), (cx + 150, 600, cz - 150), (cx + 150, 600, cz + 150),
+                    (cx - 150, 600, cz + 150), np.array([0, -1.0, 0]))
+    lp = np.asarray(lp)
+    ln = np.tile(np.array([0, -1.0, 0] * 3), (len(lp), 1))
+    p = np.concatenate([tp, sp_p, lp]).astype(np.float32)
+    n = np.concatenate([tn, sp_n, ln]).astype(np.float32)
+    uv = np.concatenate([tuv, np.zeros((len(sp_p) + len(lp), 6))]).astype(np.float32)
+    mat = np.concatenate([tmat, sp_mat, np.full(len(lp), light_id, np.int32)])
+    rng4 = np.random.Generator(np.random.PCG64(4))
+    textures = [_tex_rgba8(tex_size, k) for k in range(n_tex)]
+    textures.append((0.2 + 0.6 * rng4.random((adj_tex, adj_tex, 3))).astype(np.float32))
+    return dict(p=p, n=n, uv=uv, mat=mat, mats=np.asarray(mats, np.float32), textures=textures,
+                camera=dict(pos=(1000.0, 500.0, 2300.0), euler=(0.0, -25.0, 0.0), fov=45.0, aspect=width / height),
+                width=width, height=height, envmap=dict(rgb=_sky(), gain=1.0),
+                name=f"large{n_cells}_spheres{n_spheres}_tex{n_tex}x{tex_size}")

@@ -199,14 +199,20 @@ struct GradTracer {
         float v[4];
         rng.draw(uint32_t(1 + depth), 1, v);
         const V3d wi = me.sample(v, wo, n);
-        const MatD md = mat_deriv(M, kd, wo, n, wi);
+        MatD md;
+        if (M.type == MAT_GGX) {  // translucent dielectric: replayed, its parameters not differentiated
+            md.f = me.f(wo, n, wi, tc);
+            md.pdf = me.pdf(wo, n, wi);
+        } else {
+            md = mat_deriv(M, kd, wo, n, wi);
+        }
         SG dLc;
         const V3d Lc = trace(hp + wi * 1e-3, wi, fac_next, mi, dfac_next, depth + 1, rng, dLc);
         const double cw = std::fabs(wi.dot(n));
         const V3d T = md.f * (cw / md.pdf / prr);
         result += Lc * T;
         dL.add_scaled(dLc, T);
-        {
+        if (M.type != MAT_GGX) {
             const V3d dT = md.df_dkd * (cw / md.pdf / prr);
             add_albedo(dL, M, mi, tu, tv, V3d(dT.x * Lc.x, dT.y * Lc.y, dT.z * Lc.z));
         }

@@ -26,12 +26,22 @@ public:
     const Scene* sc = nullptr;
 
     // BVH::build(const std::vector<Triangle>&), bvh.cpp:91-97 (scene order in, recursion :60-70).
+    // Subtrees are independent, so they are built as OpenMP tasks (above 4096 primitives) and
+    // then flattened children-first, as the sequential recursion would store them.
     void build(const Scene& s) {
         sc = &s;
         nodes.clear();
         std::vector<int> all(s.tris.size());
         for (size_t i = 0; i < all.size(); ++i) all[i] = int(i);
-        root = all.empty() ? -1 : build_rec(all);
+        if (all.empty()) {
+            root = -1;
+            return;
+        }
+        TNode* t = nullptr;
+#pragma omp parallel
+#pragma omp single
+        t = build_tree(all);
+        root = flatten(t);
     }
 
     // BVH::_traversal, bvh.cpp:99-127 (recursive, left then right, right wins ties).
@@ -100,28 +110,55 @@ private:
                 std::vector<int>(prims.begin() + ans_mid, prims.end())};
     }
 
+    struct TNode {
+        V3f mn, mx;
+        TNode* l = nullptr;
+        TNode* r = nullptr;
+        std::vector<int> prims;
+    };
+
     // BVH::_build, bvh.cpp:60-70, with BVHNode constructors bvhnode.cpp:4-33.
-    int build_rec(const std::vector<int>& prims) {
+    TNode* build_tree(const std::vector<int>& prims) const {
+        TNode* nd = new TNode;
         if (prims.size() <= 5) {
-            Node nd;
-            nd.mn = V3f(1e18f); nd.mx = V3f(-1e18f);
+            nd->mn = V3f(1e18f); nd->mx = V3f(-1e18f);
             for (int t : prims)
                 for (int k = 0; k < 3; ++k) {
-                    nd.mn = vmin(nd.mn, sc->tris[t].p[k]);
-                    nd.mx = vmax(nd.mx, sc->tris[t].p[k]);
+                    nd->mn = vmin(nd->mn, sc->tris[t].p[k]);
+                    nd->mx = vmax(nd->mx, sc->tris[t].p[k]);
                 }
-            nd.prims = prims;
-            nodes.push_back(nd);
-            return int(nodes.size()) - 1;
+            nd->prims = prims;
+            return nd;
         }
         auto parts = divide(prims);
-        int l = build_rec(parts.first);
-        int r = build_rec(parts.second);
+        TNode* L = nullptr;
+        TNode* R = nullptr;
+        if (prims.size() >= 4096) {
+#pragma omp task shared(parts, L)
+            L = build_tree(parts.first);
+            R = build_tree(parts.second);
+#pragma omp taskwait
+        } else {
+            L = build_tree(parts.first);
+            R = build_tree(parts.second);
+        }
+        nd->l = L; nd->r = R;
+        nd->mn = vmin(L->mn, R->mn);
+        nd->mx = vmax(L->mx, R->mx);
+        return nd;
+    }
+
+    int flatten(TNode* t) {
         Node nd;
-        nd.l = l; nd.r = r;
-        nd.mn = vmin(nodes[l].mn, nodes[r].mn);
-        nd.mx = vmax(nodes[l].mx, nodes[r].mx);
-        nodes.push_back(nd);
+        nd.mn = t->mn; nd.mx = t->mx;
+        if (t->l) {
+            nd.l = flatten(t->l);
+            nd.r = flatten(t->r);
+        } else {
+            nd.prims = std::move(t->prims);
+        }
+        delete t;
+        nodes.push_back(std::move(nd));
         return int(nodes.size()) - 1;
     }
 

@@ -235,3 +235,25 @@ def test_texel_gradient_vs_central_fd_and_in_place_update(nr, scenes):
         print(f"texel ({y},{x},{c}): adjoint {gt[y, x, c]:.6e} fd {fd:.6e}")
         assert abs(fd - gt[y, x, c]) <= 5e-3 * abs(fd) + 1e-7
     dev.set_texture_rgb32f(0, tex)
+
+
+def test_adjoint_through_dielectrics_and_rgba8_texture(nr, po, scenes):
+    """Material zoo (GGX dielectric spheres on the paths, RGBA8 textures): material rows and the
+    floor texture's texel gradient against the FP64 oracle."""
+    sc = scenes.material_zoo(48, 36)
+    W, H, spp = 48, 36, 8
+    M = len(sc["mats"])
+    dldi = np.random.Generator(np.random.PCG64(6)).normal(size=(H, W, 3)).astype(np.float32) * 1e-2
+    dev = nr.Scene(sc)
+    dev.render(sc["camera"], W, H, spp, seed=23, fetch=False)
+    g, gt = dev.adjoint(sc["camera"], W, H, spp, M, seed=23, dl_dimg=dldi, tex_id=0, tex_shape=(64, 64))
+    ref = po.Scene(sc)
+    go, gto = ref.adjoint(scenes.camera_array(sc["camera"]), W, H, spp, dldi, M, seed=23, tex_id=0,
+                          tex_shape=(64, 64))
+    em = np.linalg.norm(g - go) / np.linalg.norm(go)
+    et = np.linalg.norm(gt - gto) / np.linalg.norm(gto)
+    print(f"zoo adjoint: material rows {em:.2e}, floor texels {et:.2e}")
+    # refraction makes FP32 and FP64 paths part more often than reflection does, and each texel
+    # sums few samples here: the texel comparison is looser than the material rows'
+    assert em < 5e-3 and et < 2e-2
+    assert not g[6:8].any()   # the dielectrics' rows
