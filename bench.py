@@ -195,6 +195,44 @@ def nrc_kernels(nr, n_query=1920 * 1080, reps=20):
             "train_tflops": tr_tf, "train_frac_bf16_peak": tr_tf / BF16_PEAK_TFLOPS}
 
 
+def large_scene(nr, scenes, spp=256):
+    """The large textured scene at 3840x2160 (9,188,616 triangles, 16 RGBA8 4096^2 textures, a
+    2048^2 float texture, 10 % GGX dielectric spheres), maxdepth 12, RR from vertex 3: one frame at
+    `spp` samples per pixel; the adjoint of the 2048^2 texture (12.6 M parameters, recompute-only)
+    over a 1920x1080 x 4 spp image; the radiance-cache MLP over a 3840x2160 frame of queries."""
+    t0 = time.perf_counter()
+    sc = scenes.large_textured()
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dev = nr.Scene(sc)
+    prep_s = time.perf_counter() - t0
+    W, H = sc["width"], sc["height"]
+    kw = dict(maxdepth=12, rrdepth=3)
+    dev.render(sc["camera"], W, H, 4, seed=1, fetch=False, **kw)
+    nr.synchronize()
+    t0 = time.perf_counter()
+    dev.render(sc["camera"], W, H, spp, seed=2, fetch=False, **kw)
+    nr.synchronize()
+    frame_s = time.perf_counter() - t0
+    aw, ah, aspp = 1920, 1080, 4
+    cam = dict(sc["camera"], aspect=aw / ah)
+    dldi = np.random.Generator(np.random.PCG64(1)).normal(size=(ah, aw, 3)).astype(np.float32)
+    M = len(sc["mats"])
+    dev.adjoint(cam, aw, ah, 1, M, seed=3, dl_dimg=dldi, tex_id=16, tex_shape=(2048, 2048), **kw)
+    nr.synchronize()
+    t0 = time.perf_counter()
+    dev.adjoint(cam, aw, ah, aspp, M, seed=3, dl_dimg=dldi, tex_id=16, tex_shape=(2048, 2048), **kw)
+    nr.synchronize()
+    adj_s = time.perf_counter() - t0
+    cache = nr.Nrc(seed=3)
+    infer_ms, _ = cache.bench(W * H, 10)
+    return {"triangles": int(len(sc["p"])), "generate_s": gen_s, "prepare_s": prep_s,
+            "bvh": dev.info(), "frame": f"{W}x{H}x{spp}spp", "frame_s": frame_s,
+            "msamples_per_s": W * H * spp / frame_s / 1e6,
+            "adjoint_texture_2048_ms": adj_s * 1e3, "adjoint_samples": aw * ah * aspp,
+            "adjoint_msamples_per_s": aw * ah * aspp / adj_s / 1e6, "nrc_infer_4k_ms": infer_ms}
+
+
 def measured_traffic():
     """Calibrated HBM bytes per closest-hit launch from the committed rocprofv3 counter summary."""
     path = os.path.join(ROOT, "profiles", "trace_closest_720p_traffic.txt")
@@ -214,6 +252,8 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-nrc", action="store_true", help="skip the time-to-RMSE measurement (about 40 s)")
+    ap.add_argument("--large", action="store_true",
+                    help="only measure the large textured scene (3840x2160 x 256 spp, texel adjoint) and exit")
     ap.add_argument("--mlp-only", action="store_true",
                     help="only time the radiance-cache MLP kernels (inference + training step) and exit")
     ap.add_argument("--copy-only", action="store_true",
@@ -234,6 +274,9 @@ def main():
         return
     if args.mlp_only:
         print(json.dumps(nrc_kernels(nr)), flush=True)
+        return
+    if args.large:
+        print(json.dumps(large_scene(nr, scenes)), flush=True)
         return
     sc = scenes.terrain_spheres()
     W, H, spp = sc["width"], sc["height"], args.spp

@@ -196,7 +196,7 @@ __device__ __forceinline__ F3 replay(const AdjArgs& a, int slot, F3 Lt, F3 g, in
         }
     };
     for (int depth = 0;; ++depth) {
-        const HitRec h = trace_ray<false>(sv, o, d, 0.0f, stk, ABLK, overflow);
+        const HitRec h = trace_ray<false, ABLK>(sv, o, d, 0.0f, stk, overflow);
         if (h.prim < 0) {  // environment: no parameter dependence
             if (!GRAD) {
                 const F3 e = mulv(beta, env_radiance(sv, d));
@@ -253,7 +253,7 @@ __device__ __forceinline__ F3 replay(const AdjArgs& a, int slot, F3 Lt, F3 g, in
             const F3 c = mul(mulv(mulv(beta, Ll), pl.f), cn * mis);
             if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
                 const F3 so = add(hp, mul(wl, 1e-3f));
-                const HitRec sh = trace_ray<true>(sv, so, wl, norm(lv), stk, ABLK, overflow);
+                const HitRec sh = trace_ray<true, ABLK>(sv, so, wl, norm(lv), stk, overflow);
                 if (sh.prim < 0) {  // visible: d(beta Ll f cos mis)
                     if (!GRAD) {
                         L = add(L, c);
@@ -323,7 +323,7 @@ template <unsigned F>
 __global__ __launch_bounds__(ABLK) void k_adjoint(AdjArgs a) {
     extern __shared__ char smem[];
     int* stk = reinterpret_cast<int*>(smem) + threadIdx.x;
-    float* gacc = reinterpret_cast<float*>(smem + size_t(a.sv.stack_cap) * ABLK * sizeof(int));
+    float* gacc = reinterpret_cast<float*>(smem + size_t(a.sv.stack_cap + 1) * ABLK * sizeof(int));
     const bool use_lds = a.n_params <= LDS_PARAMS;
     if (use_lds)
         for (int k = threadIdx.x; k < a.n_params; k += blockDim.x) gacc[k] = 0.0f;
@@ -460,7 +460,7 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
     AdjArgs a{sc.view, cam, p.W, p.H, p.W * p.H, int(P), p.seed, uint32_t(p.spp_offset), p.prr, p.mis, p.maxdepth,
               p.rrdepth, stored ? t.radiance.as<float4>() : nullptr, 0, int(P), t.adj_dldi.as<float>(),
               1.0f / float(p.spp), n_params, t.adj_grad.as<double>(), cnt + 15, tex_id, gtex};
-    const size_t shmem = size_t(sc.view.stack_cap) * ABLK * sizeof(int) +
+    const size_t shmem = size_t(sc.view.stack_cap + 1) * ABLK * sizeof(int) +
                          (n_params <= LDS_PARAMS ? size_t(n_params) * sizeof(float) : 0);
     if (shmem > 160 * 1024) throw std::runtime_error("adjoint LDS budget exceeded");
     // kernel instance for the material features present (as the shade kernels)

@@ -14,27 +14,30 @@ struct TravState {
 };
 
 // Start a ray: test the root box and seed the stack (stk points at this lane's slot 0; entries are
-// `stride` ints apart).
+// STRIDE ints apart).  Leaves and internal nodes alike go through the stack.
 __device__ __forceinline__ void trav_begin(const SceneView& sv, const SlabRay& sr, int* stk, TravState& s) {
     s = TravState{};
     if (slab_fast(sv.root_min[0], sv.root_min[1], sv.root_min[2], sv.root_max[0], sv.root_max[1], sv.root_max[2],
                   sr)) {
-        if (ref_is_leaf(sv.root_ref)) s.leaf = sv.root_ref, s.has_leaf = true;
-        else stk[0] = sv.root_ref, s.sp = 1;
+        stk[0] = sv.root_ref;
+        s.sp = 1;
     }
 }
 
-// Expand internal nodes until the lane holds a leaf or the stack is empty.
-template <bool STATS>
-__device__ __forceinline__ void trav_expand(const SceneView& sv, const SlabRay& sr, int* stk, int stride, int cap,
-                                            TravState& s, uint32_t& n_pf) {
-    while (s.sp > 0 && !s.has_leaf) {
+// Pop entries, expanding internal nodes, until the lane holds a leaf or the stack is empty.
+// Pushes are branch-free: both child slots are written every time (the entries past the new
+// top are dead), so the stack needs one spare slot beyond `cap` live entries.  Child order
+// does not matter for the result (the tie rule is keyed on leaf position, not visit order).
+template <bool STATS, int STRIDE>
+__device__ __forceinline__ void trav_expand(const SceneView& sv, const SlabRay& sr, int* stk, int cap, TravState& s,
+                                            uint32_t& n_pf) {
+    while (s.sp > 0) {
         --s.sp;
-        const int32_t e = stk[s.sp * stride];
+        const int32_t e = stk[s.sp * STRIDE];
         if (ref_is_leaf(e)) {
             s.leaf = e;
             s.has_leaf = true;
-            break;
+            return;
         }
         const NodePair* np = sv.nodes + e;
         const float4 a = np->a, b = np->b, c = np->c;
@@ -42,25 +45,14 @@ __device__ __forceinline__ void trav_expand(const SceneView& sv, const SlabRay& 
         if (STATS) ++n_pf;
         const bool hl = slab_fast(a.x, a.y, a.z, a.w, b.x, b.y, sr);
         const bool hr = slab_fast(b.z, b.w, c.x, c.y, c.z, c.w, sr);
-        int32_t push0 = 0, push1 = 0;
-        int npush = 0;
-        if (hl) {
-            if (ref_is_leaf(ch.x)) s.leaf = ch.x, s.has_leaf = true;
-            else push0 = ch.x, npush = 1;
-        }
-        if (hr) {
-            if (ref_is_leaf(ch.y) && !s.has_leaf) s.leaf = ch.y, s.has_leaf = true;
-            else if (npush == 0) push0 = ch.y, npush = 1;
-            else push1 = ch.y, npush = 2;
-        }
+        const int npush = int(hl) + int(hr);
         if (s.sp + npush > cap) {
             s.overflow = true;
             s.sp = 0;
-            s.has_leaf = false;
-            break;
+            return;
         }
-        if (npush >= 1) stk[s.sp * stride] = push0;
-        if (npush == 2) stk[(s.sp + 1) * stride] = push1;
+        stk[s.sp * STRIDE] = hr ? ch.y : ch.x;
+        stk[(s.sp + 1) * STRIDE] = ch.x;
         s.sp += npush;
     }
 }
@@ -90,10 +82,10 @@ __device__ __forceinline__ bool trav_leaf(const SceneView& sv, F3 o, F3 d, doubl
 }
 
 // Whole traversal of one ray by one lane (no refill): closest hit, or shadow test with segment
-// length len (returns hit.prim >= 0 iff blocked).  *overflow is set if the stack ran out.
-template <bool ANY>
-__device__ __forceinline__ HitRec trace_ray(const SceneView& sv, F3 o, F3 d, float len, int* stk, int stride,
-                                            bool& overflow) {
+// length len (returns hit.prim >= 0 iff blocked).  overflow is set if the stack ran out.  The
+// stack holds stack_cap + 1 slots STRIDE ints apart.
+template <bool ANY, int STRIDE>
+__device__ __forceinline__ HitRec trace_ray(const SceneView& sv, F3 o, F3 d, float len, int* stk, bool& overflow) {
     const SlabRay sr = slab_ray(o, d);
     TravState s;
     trav_begin(sv, sr, stk, s);
@@ -102,7 +94,7 @@ __device__ __forceinline__ HitRec trace_ray(const SceneView& sv, F3 o, F3 d, flo
     uint32_t npf = 0, ntt = 0;
     bool done = false;
     for (;;) {
-        trav_expand<false>(sv, sr, stk, stride, sv.stack_cap, s, npf);
+        trav_expand<false, STRIDE>(sv, sr, stk, sv.stack_cap, s, npf);
         if (!s.has_leaf) break;
         if (trav_leaf<ANY, false>(sv, o, d, double(len), s, bt, bb1, bb2, bprim, blf, ntt)) {
             done = true;

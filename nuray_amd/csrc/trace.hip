@@ -11,11 +11,12 @@
 //     independent of the order this kernel visits nodes in.
 //
 // Execution: persistent waves.  Every lane traverses one ray with a private stack kept in LDS at
-// stack[slot * blockDim + tid] (one 4-byte bank per lane, conflict-free).  Internal nodes are
+// stack[slot * BLOCK + tid] (one 4-byte bank per lane, conflict-free).  Internal nodes are
 // expanded until a lane holds a leaf, then leaves are tested together; a lane whose ray is done
 // goes idle, and once REFILL_MIN lanes are idle one atomic claims new rays for all of them
-// (ballot + prefix count), so long rays do not hold a whole wave hostage.  The stack depth is the BVH depth + 2, checked on
-// every push; an overflow is reported, never written past.
+// (ballot + prefix count), so long rays do not hold a whole wave hostage.  The stack holds BVH
+// depth + 2 live entries (checked on every push; an overflow is reported, never written past)
+// plus one spare slot for the branch-free push.
 #include "trace.h"
 #include "nr_traverse.h"
 #include "scene.h"
@@ -39,7 +40,6 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const Ray* __rest
                                                   int* __restrict__ err, TraceStats* __restrict__ stats) {
     extern __shared__ int stk_all[];
     const int tid = threadIdx.x;
-    const int bdim = blockDim.x;
     const int lane = tid & 63;
     int* const stk = stk_all + tid;
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const Ray* __rest
             continue;
         }
         if (i >= 0) {
-            trav_expand<STATS>(sv, sr, stk, bdim, cap, ts, n_pf);
+            trav_expand<STATS, BLOCK>(sv, sr, stk, cap, ts, n_pf);
             if (ts.has_leaf) done = trav_leaf<ANY, STATS>(sv, o, d, double(wr), ts, bt, bb1, bb2, bprim, blf, n_tt);
             // ---- finished: write the result, go idle
             if ((ts.sp == 0 && !ts.has_leaf) || done || ts.overflow) {
@@ -146,7 +146,7 @@ int grid_for(size_t shmem) {
 template <bool ANY>
 void launch(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, uint8_t* occ, const float4* contrib,
             float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s) {
-    const size_t shmem = size_t(sv.stack_cap) * BLOCK * sizeof(int);
+    const size_t shmem = size_t(sv.stack_cap + 1) * BLOCK * sizeof(int);  // + the spare push slot
     if (shmem > 160 * 1024) throw std::runtime_error("BVH too deep for the LDS traversal stack");
     if (stats) {
         hipLaunchKernelGGL((k_trace<ANY, true>), dim3(grid_for<ANY, true>(shmem)), dim3(BLOCK), shmem, s, sv, rays, n,
