@@ -28,7 +28,10 @@ namespace {
 constexpr int BLOCK = 256;
 
 // Lanes whose ray has finished are refilled when at least this many are idle.  A wave claims
-// rays from the global counter CLAIM at a time and hands them out from that private range.
+// rays from the global counter up to CLAIM at a time and hands them out from that private range;
+// for small batches the claim shrinks to the batch's share per wave, so that every wave starts
+// rays at once instead of a few waves working through long private ranges (the launch then
+// takes about one ray's latency, not several).
 constexpr int REFILL_MIN = 16;
 constexpr int CLAIM = 128;
 
@@ -45,6 +48,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const Ray* __rest
     const unsigned long long below = (1ull << lane) - 1ull;
     const int cap = sv.stack_cap;
     const int n = *n_ptr;
+    const int n_waves = int(gridDim.x) * (BLOCK / 64);
+    const int claim = max(1, min(CLAIM, (n + n_waves - 1) / n_waves));
 
     // per-lane ray state; i < 0 means the lane is idle
     int i = -1;
@@ -61,16 +66,16 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const Ray* __rest
     int rb = 0, re = 0;    // wave-uniform: claimed, not yet assigned ray indices [rb, re)
 
     for (;;) {
-        // ---- refill idle lanes from the wave's claimed range; claim CLAIM more when it is empty
+        // ---- refill idle lanes from the wave's claimed range; claim more when it is empty
         const unsigned long long idle_mask = __ballot(i < 0);
         const int n_idle = __popcll(idle_mask);
         if ((n_idle >= REFILL_MIN || n_idle == 64) && !(drained && rb == re)) {
             if (rb == re) {
                 int base = 0;
-                if (lane == 0) base = atomicAdd(work, CLAIM);
+                if (lane == 0) base = atomicAdd(work, claim);
                 base = __shfl(base, 0, 64);
                 if (base >= n) drained = true;
-                else rb = base, re = min(base + CLAIM, n);
+                else rb = base, re = min(base + claim, n);
             }
             const int take = min(n_idle, re - rb);
             if (i < 0) {
