@@ -252,6 +252,8 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-nrc", action="store_true", help="skip the time-to-RMSE measurement (about 40 s)")
+    ap.add_argument("--frames-only", action="store_true",
+                    help="only the ptnee frames (no adjoint, cache or CPU legs), e.g. under a profiler")
     ap.add_argument("--large", action="store_true",
                     help="only measure the large textured scene (3840x2160 x 256 spp, texel adjoint) and exit")
     ap.add_argument("--mlp-only", action="store_true",
@@ -322,13 +324,18 @@ def main():
                 "avg_launch_ms": closest_ms / max(launches, 1),
                 "rays_per_frame": cnt["rays"], "pair_fetch_per_ray": cnt["pair_fetch"] / max(cnt["rays"], 1),
                 "tri_tests_per_ray": cnt["tri_tests"] / max(cnt["rays"], 1)}
-    adj = adjoint_iteration(nr, scenes)
-    print(f"# adjoint iteration {adj['ms_per_iter']:.2f} ms (render {adj['render_ms']:.2f} ms, "
-          f"replay {adj['adjoint_ms']:.2f} ms)", file=sys.stderr, flush=True)
+    if args.frames_only:
+        args.no_cpu = args.no_nrc = True
+    adj = None if args.frames_only else adjoint_iteration(nr, scenes)
+    if adj:
+        print(f"# adjoint iteration {adj['ms_per_iter']:.2f} ms (render {adj['render_ms']:.2f} ms, "
+              f"replay {adj['adjoint_ms']:.2f} ms)", file=sys.stderr, flush=True)
     nrc = None if args.no_nrc else nrc_time_to_rmse(nr, scenes)
-    mlp = nrc_kernels(nr)
-    print(f"# cache MLP: inference {mlp['infer_ms']:.3f} ms / 1080p frame ({mlp['infer_tflops']:.0f} TFLOP/s), "
-          f"training step {mlp['train_step_ms']:.3f} ms ({mlp['train_tflops']:.0f} TFLOP/s)", file=sys.stderr, flush=True)
+    mlp = None if args.frames_only else nrc_kernels(nr)
+    if mlp:
+        print(f"# cache MLP: inference {mlp['infer_ms']:.3f} ms / 1080p frame ({mlp['infer_tflops']:.0f} TFLOP/s), "
+              f"training step {mlp['train_step_ms']:.3f} ms ({mlp['train_tflops']:.0f} TFLOP/s)", file=sys.stderr,
+              flush=True)
     if nrc:
         print(f"# time to RMSE 1e-2 at 1080p: PT {nrc['pt_s']:.1f} s, radiance cache "
               f"{nrc['nrc_s'] if nrc['nrc_s'] is None else round(nrc['nrc_s'], 1)} s", file=sys.stderr, flush=True)
