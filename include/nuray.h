@@ -28,12 +28,16 @@ typedef struct nr_scene nr_scene;
 
 /* Material rows passed to nr_scene_create: 8 floats {type, r, g, b, alpha, ior, texture, 0}.
  * type: NR_MAT_LAMBERT (MatLambert, Kd), NR_MAT_LIGHT (MatLight, Ke), NR_MAT_GGXREFL (MatGGXRefl,
- * F0 + roughness alpha), NR_MAT_GGX (MatGGX translucent dielectric, F0 + alpha + ior).
+ * F0 + roughness alpha), NR_MAT_GGX (MatGGX translucent dielectric, F0 + alpha + ior),
+ * NR_MAT_BLINNPHONG (MatBlinnPhong, Kd + exponent Ns in `alpha`; Ks via nr_scene_set_material_ext),
+ * NR_MAT_GLASS (MatGlass, F0 + ior).
  * texture: index of an albedo texture (nr_scene_set_texture_rgba8) or -1. */
 #define NR_MAT_LAMBERT 0
 #define NR_MAT_LIGHT 1
 #define NR_MAT_GGXREFL 2
 #define NR_MAT_GGX 3
+#define NR_MAT_BLINNPHONG 4
+#define NR_MAT_GLASS 5
 
 /* Camera(fov, asp, pos, yaw, pitch, roll), src/scene/camera.h:18 and camera.cpp:8-22,88-105. */
 typedef struct nr_camera {
@@ -57,6 +61,7 @@ typedef struct nr_params {
     int32_t rrdepth;       /* Russian roulette applies at vertex index >= rrdepth */
     int32_t batch_spp;     /* samples per pixel per wavefront batch (0 = automatic) */
     int32_t accumulate;    /* 1: add to the previous render's sums instead of clearing them */
+    int32_t integrator;    /* renderer: 0 = ptnee (RendererPTLS), 1 = pt (RendererPT, rendererpt.cpp:13-61) */
 } nr_params;
 
 typedef void (*nr_progress_fn)(float fraction, void* user);
@@ -75,6 +80,9 @@ nr_scene* nr_scene_create(const float* p, const float* n, const float* uv, const
 int nr_scene_destroy(nr_scene* s);
 /* Texture::load result (texture.cpp:5-31): RGBA8 texels, row-major, first row at v = 1. */
 int nr_scene_set_texture_rgba8(nr_scene* s, int32_t id, int32_t w, int32_t h, const uint8_t* rgba);
+/* MatBlinnPhong's specular colour (matblinnphong.h:16-17): ext = {Ks r, g, b, Ks texture or -1}
+ * for material row `id` (the loader's map_Ks).  Call before nr_bvh_build_sah. */
+int nr_scene_set_material_ext(nr_scene* s, int32_t id, const float* ext);
 /* Linear RGB float albedo texture (3 floats per texel, first row at v = 1): the form to optimise
  * with nr_adjoint_ptnee_tex.  Setting a texture of the same size and format on a built scene
  * rewrites it in place (no rebuild).  No reference counterpart (the reference's textures are 8-bit). */

@@ -224,9 +224,11 @@ struct Envmap {
 // One row of the C ABI's material table (nuray.h): {type, r, g, b, alpha, ior, texture, 0}.
 struct Material {
     int type = NR_MAT_LAMBERT;
-    vec3 kd;
-    float alpha = 0.5f, ior = 1.0f;
+    vec3 kd;                   // Kd / Ke / F0
+    float alpha = 0.5f, ior = 1.0f;  // alpha: GGX roughness, or the Blinn-Phong exponent Ns
     int tex = -1;
+    vec3 ks;                   // Blinn-Phong
+    int tex_ks = -1;
 };
 
 struct Triangle {
@@ -238,7 +240,7 @@ struct Triangle {
 // [-m=light r g b | -m=ggx r g b alpha ior | -m=ggxr r g b alpha | -m=glass r g b ior]",
 // OBJ v / vt / vn / f (triangles and quads split (0,1,2), (2,3,0), negative indices relative),
 // MTL newmtl / Kd / Ks / Ke / Ns / Ni / map_Kd / map_Ks; Ke > 0 -> light, Ks or map_Ks ->
-// Blinn-Phong, else Lambert (map_Kd textures).  Triangles keep the file order.
+// Blinn-Phong (Kd, Ks, Ns, map_Kd, map_Ks), else Lambert (map_Kd).  Triangles keep the file order.
 class Loader {
     std::vector<Triangle> tris_;
     std::vector<Material> mats_;
@@ -294,7 +296,8 @@ public:
                     m.type = NR_MAT_GGXREFL;
                     ls >> m.kd[0] >> m.kd[1] >> m.kd[2] >> m.alpha;
                 } else if (op == "-m=glass") {
-                    throw Error("MatGlass (-m=glass) is not available on the GPU path");
+                    m.type = NR_MAT_GLASS;
+                    ls >> m.kd[0] >> m.kd[1] >> m.kd[2] >> m.ior;
                 } else {
                     continue;
                 }
@@ -310,13 +313,19 @@ public:
         if (!is) throw Error("cannot open " + file);
         std::string name, line, map_kd, map_ks;
         vec3 kd, ks, ke;
+        float ns = 10.0f;
         auto commit = [&]() {
             Material m;
             if (ke.norm2() > 1e-6f) {
                 m.type = NR_MAT_LIGHT;
                 m.kd = ke;
             } else if (ks.norm2() > 1e-6f || !map_ks.empty()) {
-                throw Error("material " + name + " is Blinn-Phong (Ks / map_Ks), not available on the GPU path");
+                m.type = NR_MAT_BLINNPHONG;
+                m.kd = kd;
+                m.ks = ks;
+                m.alpha = ns;
+                if (!map_kd.empty()) m.tex = texture(map_kd);
+                if (!map_ks.empty()) m.tex_ks = texture(map_ks);
             } else {
                 m.type = NR_MAT_LAMBERT;
                 m.kd = kd;
@@ -325,6 +334,7 @@ public:
             mats_.push_back(m);
             dict_[file + ":" + name] = int(mats_.size()) - 1;
             kd = ks = ke = vec3(0.0f);
+            ns = 10.0f;
             map_kd.clear();
             map_ks.clear();
         };
@@ -344,6 +354,8 @@ public:
                 ks = vec3(num(b, 1), num(b, 2), num(b, 3));
             } else if (b[0] == "Ke") {
                 ke = vec3(num(b, 1), num(b, 2), num(b, 3));
+            } else if (b[0] == "Ns") {
+                ns = num(b, 1);
             } else if (b[0] == "map_Kd" && b.size() > 1) {
                 map_kd = dir_of(file) + "/" + b[1];
             } else if (b[0] == "map_Ks" && b.size() > 1) {
@@ -425,7 +437,7 @@ public:
 };
 
 // ------------------------------------------------------------------------------------- Renderer
-// Renderer (renderer.h:15-35) for "ptnee" and "nrc".  render() reads the Config keys imgw, imgh,
+// Renderer (renderer.h:15-35) for "ptnee", "pt" and "nrc".  render() reads the Config keys imgw, imgh,
 // spp, prr, mis (rendererptls.cpp), seed, maxdepth, rrdepth, and for "nrc" frames (frames of
 // nrc_spp samples each, training online), nrc_spread, nrc_train_paths.
 class Renderer {
@@ -437,7 +449,8 @@ class Renderer {
 
 public:
     explicit Renderer(const std::string& name = "ptnee") : name_(name) {
-        if (name_ != "ptnee" && name_ != "nrc") throw Error("renderer '" + name_ + "' is not available (ptnee, nrc)");
+        if (name_ != "ptnee" && name_ != "pt" && name_ != "nrc")
+            throw Error("renderer '" + name_ + "' is not available (ptnee, pt, nrc)");
         check(nr_init(0));
     }
     Renderer(const Renderer&) = delete;
@@ -482,6 +495,11 @@ public:
         if (!s_) throw Error(nr_last_error());
         for (size_t k = 0; k < texs.size(); ++k)
             check(nr_scene_set_texture_rgba8(s_, int32_t(k), texs[k].w, texs[k].h, texs[k].rgba.data()));
+        for (size_t j = 0; j < mats.size(); ++j)
+            if (mats[j].type == NR_MAT_BLINNPHONG) {
+                const float ext[4] = {mats[j].ks[0], mats[j].ks[1], mats[j].ks[2], float(mats[j].tex_ks)};
+                check(nr_scene_set_material_ext(s_, int32_t(j), ext));
+            }
         if (has_env_) check(nr_scene_set_envmap(s_, env_.w, env_.h, env_.rgb.data(), env_.gain));
         check(nr_bvh_build_sah(s_, 0));
     }
@@ -508,7 +526,8 @@ public:
         nr_params p = params_of(cfg);
         img.resize(p.width, p.height);
         const nr_camera c = cam.c();
-        if (name_ == "ptnee") {
+        if (name_ == "ptnee" || name_ == "pt") {
+            p.integrator = name_ == "pt" ? 1 : 0;
             struct Ctx { std::function<void(float)>* f; std::atomic<int>* flag; volatile int32_t* cancel; };
             volatile int32_t cancel = 0;
             Ctx ctx{&progress, con_flag, &cancel};

@@ -27,7 +27,7 @@ EXPORTS = [
     "nr_set_profiling", "nr_bench_copy", "nr_scene_set_params", "nr_loss_grad_l2", "nr_adjoint_ptnee",
     "nr_nrc_create", "nr_nrc_destroy", "nr_nrc_set_weights", "nr_nrc_get_weights", "nr_nrc_infer",
     "nr_nrc_train_step", "nr_nrc_gradient", "nr_render_nrc", "nr_nrc_bench", "nr_scene_set_texture_rgb32f",
-    "nr_adjoint_ptnee_tex",
+    "nr_adjoint_ptnee_tex", "nr_scene_set_material_ext",
 ]
 
 NRC_NPARAM = 5 * 4096 + 16 * 64
@@ -40,7 +40,8 @@ class nr_camera(C.Structure):
 class nr_params(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("spp_offset", C.c_int32),
                 ("seed", C.c_uint32), ("prr", C.c_float), ("mis", C.c_int32), ("maxdepth", C.c_int32),
-                ("rrdepth", C.c_int32), ("batch_spp", C.c_int32), ("accumulate", C.c_int32)]
+                ("rrdepth", C.c_int32), ("batch_spp", C.c_int32), ("accumulate", C.c_int32),
+                ("integrator", C.c_int32)]
 
 
 class nr_nrc_params(C.Structure):
@@ -88,6 +89,7 @@ def lib():
         L.nr_loss_grad_l2.argtypes = [vp, f32p, C.POINTER(C.c_double), vp]
         L.nr_adjoint_ptnee.argtypes = [vp, C.POINTER(nr_camera), C.POINTER(nr_params), vp, f64p]
         L.nr_scene_set_texture_rgb32f.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, f32p]
+        L.nr_scene_set_material_ext.argtypes = [vp, C.c_int32, f32p]
         L.nr_adjoint_ptnee_tex.argtypes = [vp, C.POINTER(nr_camera), C.POINTER(nr_params), vp, f64p, C.c_int32, vp]
         L.nr_nrc_create.restype = vp
         L.nr_nrc_create.argtypes = [C.c_uint32, C.c_float, C.c_int32]
@@ -165,6 +167,8 @@ class Scene:
                                    len(sc["mats"]))
         if not self.h:
             raise NuRayError(L.nr_last_error().decode())
+        for mid, ext in (sc.get("mats_ext") or {}).items():   # Blinn-Phong Ks rows
+            check(L.nr_scene_set_material_ext(self.h, int(mid), _f32(ext)))
         for tid, tex in enumerate(sc.get("textures") or []):
             if np.asarray(tex).dtype == np.float32:   # linear RGB float texture
                 t = _f32(tex)
@@ -215,8 +219,8 @@ class Scene:
         return tri, t, b1, b2
 
     def render(self, cam, W, H, spp, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0, spp_offset=0, batch_spp=0,
-               accumulate=False, fetch=True, progress=None, cancel=None):
-        p = nr_params(W, H, spp, spp_offset, seed, prr, mis, maxdepth, rrdepth, batch_spp, int(accumulate))
+               accumulate=False, fetch=True, progress=None, cancel=None, integrator=0):
+        p = nr_params(W, H, spp, spp_offset, seed, prr, mis, maxdepth, rrdepth, batch_spp, int(accumulate), integrator)
         c = make_camera(cam)
         cb = PROGRESS(lambda f, u: progress(f)) if progress else None
         rc = check(lib().nr_render_ptnee(self.h, C.byref(c), C.byref(p), None, None, None, None,
@@ -248,10 +252,10 @@ class Scene:
         return (loss.value, g) if want_grad else loss.value
 
     def adjoint(self, cam, W, H, spp, n_mats, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0, spp_offset=0,
-                dl_dimg=None, tex_id=-1, tex_shape=None):
+                dl_dimg=None, tex_id=-1, tex_shape=None, integrator=0):
         """Material gradient (n_mats, 4); with tex_id >= 0 also the texel gradient (h, w, 3) of that
         texture (tex_shape = (h, w))."""
-        p = nr_params(W, H, spp, spp_offset, seed, prr, mis, maxdepth, rrdepth, 0, 0)
+        p = nr_params(W, H, spp, spp_offset, seed, prr, mis, maxdepth, rrdepth, 0, 0, integrator)
         c = make_camera(cam)
         grad = np.zeros(n_mats * 4, np.float64)
         dl = None if dl_dimg is None else _f32(dl_dimg).reshape(-1)

@@ -92,7 +92,7 @@ __device__ __forceinline__ GgxParts mat_parts(const SceneView& sv, const Mat& m,
     GgxParts r;
     r.f = bsdf_eval<F>(sv, m, wo, n, wi, u, v);
     r.pdf = bsdf_pdf<F>(m, wo, n, wi);
-    if ((F & F_GGX) && m.type == MAT_GGX) {  // translucent dielectric: replayed, not differentiated
+    if (m.type == MAT_GGX || m.type == MAT_BLINNPHONG || m.type == MAT_GLASS) {  // replayed, not differentiated
         r.df_dkd = r.df_da = f3(0, 0, 0);
         r.dpdf_da = 0.0f;
         return r;
@@ -131,6 +131,7 @@ struct AdjArgs {
     int* err;
     int tex_id;              // texture whose texel gradient is wanted (-1: none)
     double* gtex;            // w * h * 3 texel gradient (linear texel values)
+    int nee;                 // 1: ptnee, 0: pt (no light sampling)
 };
 
 __device__ __forceinline__ void gadd(float* lds, double* glob, bool use_lds, int idx, float v) {
@@ -227,7 +228,7 @@ __device__ __forceinline__ F3 replay(const AdjArgs& a, int slot, F3 Lt, F3 g, in
         const F3 wo = neg(d);
         const float4 ua = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 0u);
         float fac_next = 1.0f, dfac_next = 0.0f;
-        if (mat_needs_nee(m) && sv.n_lights > 0) {
+        if (a.nee && mat_needs_nee(m) && sv.n_lights > 0) {
             const LightRec lr = sv.lights[pick_light(sv, ua.x)];
             float r1 = ua.y, r2 = ua.z;
             if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
@@ -303,7 +304,7 @@ __device__ __forceinline__ F3 replay(const AdjArgs& a, int slot, F3 Lt, F3 g, in
                 dal.z = g.z * Lrem.z * pi.df_dkd.z / pi.f.z;
                 va += g.z * Lrem.z * (pi.df_da.z / pi.f.z - dlp + score);
             }
-            if (m.type != MAT_GGX) add_albedo(m, mid, tu, tv, dal);
+            if (m.type == MAT_LAMBERT || m.type == MAT_GGXREFL) add_albedo(m, mid, tu, tv, dal);
             if (m.type == MAT_GGXREFL) gadd(gacc, a.grad, use_lds, 4 * mid + 3, va);
         }
         if (nb.x == 0.0f && nb.y == 0.0f && nb.z == 0.0f) break;
@@ -391,12 +392,7 @@ void set_material_params(Scene& sc, const float* theta, int n_mats) {
         m[4] = theta[4 * j + 3];
     }
     if (!sc.built) return;
-    std::vector<MatRec> mr(sc.mats.size() / 8);
-    for (size_t j = 0; j < mr.size(); ++j) {
-        const float* m = &sc.mats[8 * j];
-        mr[j].a = make_float4(m[0], m[1], m[2], m[3]);
-        mr[j].b = make_float4(m[4], m[5], m[6], 0);
-    }
+    const std::vector<MatRec> mr = sc.mat_records();
     sc.d_mats.upload(mr.data(), mr.size() * sizeof(MatRec), sc.stream);
     check_hip(hipStreamSynchronize(sc.stream), "params upload");
 }
@@ -459,12 +455,12 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
     if (P > 0x7FFFFFFFll) throw std::runtime_error("adjoint: more than 2^31 samples");
     AdjArgs a{sc.view, cam, p.W, p.H, p.W * p.H, int(P), p.seed, uint32_t(p.spp_offset), p.prr, p.mis, p.maxdepth,
               p.rrdepth, stored ? t.radiance.as<float4>() : nullptr, 0, int(P), t.adj_dldi.as<float>(),
-              1.0f / float(p.spp), n_params, t.adj_grad.as<double>(), cnt + 15, tex_id, gtex};
+              1.0f / float(p.spp), n_params, t.adj_grad.as<double>(), cnt + 15, tex_id, gtex, p.integrator == 1 ? 0 : 1};
     const size_t shmem = size_t(sc.view.stack_cap + 1) * ABLK * sizeof(int) +
                          (n_params <= LDS_PARAMS ? size_t(n_params) * sizeof(float) : 0);
     if (shmem > 160 * 1024) throw std::runtime_error("adjoint LDS budget exceeded");
     // kernel instance for the material features present (as the shade kernels)
-    const unsigned feat = sc.features & F_GGX ? F_ALL : (sc.features & F_TEX ? F_TEX : 0u);
+    const unsigned feat = sc.features & (F_GGX | F_EXT) ? F_ALL : (sc.features & F_TEX ? F_TEX : 0u);
     int per_cu = 0;
     check_hip(feat == F_ALL   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adjoint<F_ALL>, ABLK, shmem)
               : feat == F_TEX ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adjoint<F_TEX>, ABLK, shmem)

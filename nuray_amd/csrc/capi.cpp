@@ -83,6 +83,13 @@ int nr_scene_set_texture_rgba8(nr_scene* s, int32_t id, int32_t w, int32_t h, co
     });
 }
 
+int nr_scene_set_material_ext(nr_scene* s, int32_t id, const float* ext) {
+    return guard([&] {
+        s->sc->set_material_ext(id, ext);
+        return NR_OK;
+    });
+}
+
 int nr_scene_set_texture_rgb32f(nr_scene* s, int32_t id, int32_t w, int32_t h, const float* rgb) {
     return guard([&] {
         if (s->sc->env_tex >= 0 && id == s->sc->env_tex) throw std::runtime_error("texture id is the environment map");
@@ -204,6 +211,8 @@ int nr_render_ptnee(nr_scene* s, const nr_camera* cam, const nr_params* pp, floa
         p.W = pp->width; p.H = pp->height; p.spp = pp->spp; p.spp_offset = pp->spp_offset;
         p.seed = pp->seed; p.prr = pp->prr; p.mis = pp->mis; p.maxdepth = pp->maxdepth; p.rrdepth = pp->rrdepth;
         p.batch_spp = pp->batch_spp; p.accumulate = pp->accumulate;
+    p.integrator = pp->integrator;
+    if (p.integrator != 0 && p.integrator != 1) throw std::runtime_error("integrator must be 0 (ptnee) or 1 (pt)");
         nr::RenderStats rs;
         nr::TraceStats ts[2] = {};
         nr::KernelTimes kt;
@@ -260,6 +269,8 @@ static nr::RenderParams to_params(const nr_params* pp) {
     p.W = pp->width; p.H = pp->height; p.spp = pp->spp; p.spp_offset = pp->spp_offset;
     p.seed = pp->seed; p.prr = pp->prr; p.mis = pp->mis; p.maxdepth = pp->maxdepth; p.rrdepth = pp->rrdepth;
     p.batch_spp = pp->batch_spp; p.accumulate = pp->accumulate;
+    p.integrator = pp->integrator;
+    if (p.integrator != 0 && p.integrator != 1) throw std::runtime_error("integrator must be 0 (ptnee) or 1 (pt)");
     return p;
 }
 

@@ -6,7 +6,7 @@
 //
 // Tokens before --output form the scene description line (Loader::fromSceneDescription); the
 // params are Config items.  Camera(camfov, camasp, campos, cameuler) as cmd.cpp:98-100, envmap=<BMP
-// or PPM> with envmap_gain.  renderer = ptnee (default) or nrc.
+// or PPM> with envmap_gain.  renderer = ptnee (default), pt or nrc.
 #include <iostream>
 #include <string>
 #include <vector>

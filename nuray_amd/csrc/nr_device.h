@@ -4,7 +4,7 @@
 //   NodePair  64 B  one per internal BVH node, preorder; holds both children's boxes and refs
 //   TriGeom   48 B  one per triangle, in DFS leaf order: p0, e1 = p1-p0, e2 = p2-p0, loader id
 //   TriShade  64 B  same order: n0, n1, n2, uv0, uv1, uv2, material row
-//   MatRec    32 B  one per material: {type, r, g, b, alpha, ior, texture, 0}
+//   MatRec    48 B  one per material: {type, r, g, b, alpha, ior, texture, 0, Ks r, g, b, Ks texture}
 //   LightRec 112 B  one per emissive triangle, loader order, with the area CDF beside it
 //
 // Every float step that decides a hit follows src/hierarchy/bvhnode.cpp and
@@ -16,7 +16,7 @@
 
 namespace nr {
 
-enum MatType { MAT_LAMBERT = 0, MAT_LIGHT = 1, MAT_GGXREFL = 2, MAT_GGX = 3 };
+enum MatType { MAT_LAMBERT = 0, MAT_LIGHT = 1, MAT_GGXREFL = 2, MAT_GGX = 3, MAT_BLINNPHONG = 4, MAT_GLASS = 5 };
 
 // Leaf references inside NodePair::ref: bit 31 set, bits 27..30 = primitive count (1..5),
 // bits 0..26 = first primitive in TriGeom order.  Internal references are NodePair indices.
@@ -47,7 +47,8 @@ struct alignas(16) TriShade {
 };
 struct alignas(16) MatRec {
     float4 a;  // type, r, g, b
-    float4 b;  // alpha, ior, texture, 0
+    float4 b;  // alpha (Blinn-Phong: Ns), ior, texture, 0
+    float4 c;  // Blinn-Phong Ks r, g, b, Ks texture
 };
 struct alignas(16) LightRec {
     float4 p0, p1, p2;  // xyz used

@@ -4,6 +4,8 @@
 //   MatGGXRefl   src/material/matggxrefl.cpp:5-88   (Kd is F0; no diffuse lobe)
 //   MatGGX       src/material/matggx.cpp:5-173      (translucent; eta always 1/ior, matggx.cpp:43)
 //   MatLight     src/material/matlight.cpp:1-42
+//   MatBlinnPhong src/material/matblinnphong.cpp:5-85 (sampled wi not normalised, sin = sqrt(1 - cos))
+//   MatGlass     src/material/matglass.cpp:5-48
 //   Texture      src/scene/texture.cpp:54-72        (black outside [0,W)x[0,H), no wrap of x0+1)
 // `abs` is the floating-point absolute value throughout.
 #pragma once
@@ -12,21 +14,30 @@
 namespace nr {
 
 // Optional material features a kernel instance is compiled with (the host launches the instance
-// matching the scene): F_GGX = translucent GGX dielectric present, F_TEX = albedo textures present.
-constexpr unsigned F_GGX = 1u, F_TEX = 2u, F_ALL = 3u;
+// matching the scene): F_GGX = translucent GGX dielectric present, F_TEX = albedo textures present,
+// F_EXT = Blinn-Phong or glass present.
+constexpr unsigned F_GGX = 1u, F_TEX = 2u, F_EXT = 4u, F_ALL = 7u;
 
 struct Mat {
     int type;
     F3 kd;
     float alpha, ior;
     int tex;
+    F3 ks;       // Blinn-Phong
+    int tex_ks;
 };
 __device__ __forceinline__ Mat load_mat(const SceneView& sv, int id) {
     const MatRec m = sv.mats[id];
-    return Mat{int(m.a.x), f3(m.a.y, m.a.z, m.a.w), m.b.x, m.b.y, int(m.b.z)};
+    return Mat{int(m.a.x), f3(m.a.y, m.a.z, m.a.w), m.b.x, m.b.y, int(m.b.z), f3(m.c.x, m.c.y, m.c.z), int(m.c.w)};
 }
-__device__ __forceinline__ bool mat_needs_nee(const Mat& m) { return m.type == MAT_LAMBERT || m.type == MAT_GGXREFL; }
-__device__ __forceinline__ bool mat_transmissive(const Mat& m) { return m.type == MAT_LIGHT || m.type == MAT_GGX; }
+// requireLightSampling / isTransmission (matlambert.cpp:54-57, matggxrefl.cpp, matblinnphong.cpp:88-91,
+// matggx.cpp:155-169, matglass.cpp:55-69, matlight.cpp:25-28)
+__device__ __forceinline__ bool mat_needs_nee(const Mat& m) {
+    return m.type == MAT_LAMBERT || m.type == MAT_GGXREFL || m.type == MAT_BLINNPHONG;
+}
+__device__ __forceinline__ bool mat_transmissive(const Mat& m) {
+    return m.type == MAT_LIGHT || m.type == MAT_GGX || m.type == MAT_GLASS;
+}
 
 // ---------------------------------------------------------------- textures
 __device__ __forceinline__ F3 texel(const SceneView& sv, const TexDesc& t, int x, int y) {
@@ -65,6 +76,11 @@ template <unsigned F = F_ALL>
 __device__ __forceinline__ F3 mat_albedo(const SceneView& sv, const Mat& m, float u, float v) {
     if (F & F_TEX) return m.tex >= 0 ? tex_uv(sv, m.tex, u, v) : m.kd;
     return m.kd;
+}
+template <unsigned F = F_ALL>
+__device__ __forceinline__ F3 mat_specular(const SceneView& sv, const Mat& m, float u, float v) {
+    if (F & F_TEX) return m.tex_ks >= 0 ? tex_uv(sv, m.tex_ks, u, v) : m.ks;
+    return m.ks;
 }
 
 // ---------------------------------------------------------------- BSDFs
@@ -119,6 +135,36 @@ __device__ __forceinline__ F3 bsdf_sample(const Mat& m, float u0, float u1, floa
             wi = sub(mul(wo, -eta), mul(h, nh));
         }
         return normalized(wi);
+    }
+    if ((F & F_EXT) && m.type == MAT_BLINNPHONG) {  // matblinnphong.cpp:5-34, u0 picks the lobe
+        if (u0 < 0.5f) {
+            const float r2 = u1 * 0.99f;
+            const float phi = float(double(u2) * 3.14159 * 2);
+            const float r = sqrtf(r2), h = sqrtf(1.0f - r2);
+            float sp, cp;
+            sincosf(phi, &sp, &cp);
+            return add(add(mul(n, h), mul(ax1, r * cp)), mul(ax2, r * sp));
+        }
+        const float ct = powf(u1, 1.0f / (m.alpha + 2.0f));
+        const float st = sqrtf(1.0f - ct);  // the reference's sqrt(1 - cos)
+        const float phi = (2.0f * 3.14159f) * u2;
+        float sp, cp;
+        sincosf(phi, &sp, &cp);
+        const F3 h = add(add(mul(n, ct), mul(ax1, st * cp)), mul(ax2, st * sp));
+        return sub(mul(h, 2.0f * dot(h, wo)), wo);  // not normalised, as the reference
+    }
+    if ((F & F_EXT) && m.type == MAT_GLASS) {  // matglass.cpp:5-24, u0 picks refraction / reflection
+        const bool out = dot(wo, n) > 0.0f;
+        const float I = out ? 1.0f / m.ior : m.ior;
+        const F3 nn = out ? n : neg(n);
+        if (u0 < 0.5f) {
+            const F3 t = normalized(cross(cross(nn, wo), nn));
+            const float so = dot(wo, t);
+            const float si = fminf(so * I, 1.0f);
+            const float ci = sqrtf(1.0f - si * si);
+            return neg(add(mul(nn, ci), mul(t, si)));
+        }
+        return sub(mul(nn, 2.0f * dot(wo, nn)), wo);
     }
     return f3(0.0f, 0.0f, 0.0f);
 }
@@ -183,6 +229,22 @@ __device__ __forceinline__ F3 bsdf_eval(const SceneView& sv, const Mat& m, F3 wo
         const float s = A * (eta * eta) * g * dd / (c * c + 1e-8f);
         return f3((1.0f - Fr.x) * s, (1.0f - Fr.y) * s, (1.0f - Fr.z) * s);
     }
+    if ((F & F_EXT) && m.type == MAT_BLINNPHONG) {  // matblinnphong.cpp:36-49
+        if (dot(wi, n) * dot(wo, n) < 0.0f) return f3(0.0f, 0.0f, 0.0f);
+        if (double(dot(wi, n)) <= 1e-4 || dot(wo, n) <= 1e-4f) return f3(0.0f, 0.0f, 0.0f);
+        const F3 h = normalized(add(wo, wi));
+        const float sp = (m.alpha + 2.0f) / 3.14159f / 8.0f * powf(dot(n, h), m.alpha);
+        return add(divs(mat_albedo<F>(sv, m, u, v), 3.14159f), mul(mat_specular<F>(sv, m, u, v), sp));
+    }
+    if ((F & F_EXT) && m.type == MAT_GLASS) {  // matglass.cpp:26-44
+        if (fabsf(dot(wi, n)) < 1e-3f) return f3(0.0f, 0.0f, 0.0f);
+        const F3 f0 = mat_albedo<F>(sv, m, u, v);
+        const float fr = powf(1.0f - fabsf(dot(n, wo)), 5.0f);
+        const F3 Fr = f3(f0.x + (1.0f - f0.x) * fr, f0.y + (1.0f - f0.y) * fr, f0.z + (1.0f - f0.z) * fr);
+        if (dot(wi, n) * dot(wo, n) > 0.0f) return Fr;
+        const float I = dot(wo, n) > 0.0f ? 1.0f / m.ior : m.ior;
+        return mul(f3(1.0f - Fr.x, 1.0f - Fr.y, 1.0f - Fr.z), I * I);
+    }
     return f3(0.0f, 0.0f, 0.0f);
 }
 
@@ -209,6 +271,14 @@ __device__ __forceinline__ float bsdf_pdf(const Mat& m, F3 wo, F3 n, F3 wi) {
         if (d < 1e-6f) return 1e18f;
         return fmaxf(d * dot(n, wh) / 4.0f, 1.0f);
     }
+    if ((F & F_EXT) && m.type == MAT_BLINNPHONG) {  // matblinnphong.cpp:51-59
+        if (double(dot(wi, n)) <= 1e-4 || dot(wo, n) <= 1e-4f) return 1e-18f;
+        const float pd = float(1.0 / 3.14159 * double(fabsf(dot(wi, n)) + 1e-6f));
+        const F3 h = normalized(add(wo, wi));
+        const float ps = (m.alpha + 2.0f) / 3.14159f / 8.0f * powf(fabsf(dot(n, h)), m.alpha + 1.0f);
+        return 0.5f * pd + (1.0f - 0.5f) * ps;
+    }
+    if ((F & F_EXT) && m.type == MAT_GLASS) return 0.5f;  // matglass.cpp:46-49
     return float(1.0 / 3.14159);
 }
 

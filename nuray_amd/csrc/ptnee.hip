@@ -183,6 +183,7 @@ struct ShadeArgs {
     uint32_t seed, sample0;
     float prr;
     int mis, maxdepth, rrdepth;
+    int nee;  // 1: ptnee; 0: pt (rendererpt.cpp:13-61: no light sampling, so emitters keep fac = 1)
     const Ray* rin;
     const float4* bin;
     const HitRec* hits;
@@ -361,7 +362,7 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a, const NrcShade& n
                     }
                     const float4 ua = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 0u);
                     // NEE direction (rendererptls.cpp:50-69): light pick, point on it, wl
-                    const bool do_nee = !stop && mat_needs_nee(m) && sv.n_lights > 0;
+                    const bool do_nee = !stop && a.nee && mat_needs_nee(m) && sv.n_lights > 0;
                     F3 wl = f3(0, 0, 0), lv = f3(0, 0, 0), ke = f3(0, 0, 0);
                     float lcos = 0.0f, ncos = 0.0f;
                     if (do_nee) {
@@ -581,6 +582,7 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
                            cnt + C_ERR, ts, s);
             if (kt) mark(ev_c);
             ShadeArgs a{sc.view, npix, p.seed, uint32_t(p.spp_offset + b), p.prr, p.mis, p.maxdepth, p.rrdepth,
+                        p.integrator == 1 ? 0 : 1,
                         t.rays[cur].as<Ray>(), t.beta[cur].as<float4>(), t.hits.as<HitRec>(), cnt + C_RAY0 + cur,
                         t.radiance.as<float4>(), t.rays[nxt].as<Ray>(), t.beta[nxt].as<float4>(), cnt + C_RAY0 + nxt,
                         t.shadow.as<Ray>(), t.contrib.as<float4>(), cnt + C_SHADOW};
@@ -678,6 +680,7 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
         launch_closest(sc.view, t.rays[cur].as<Ray>(), cnt + C_RAY0 + cur, t.hits.as<HitRec>(), cnt + C_WORK0,
                        cnt + C_ERR, nullptr, s);
         ShadeArgs a{sc.view, npix, p.seed, uint32_t(p.spp_offset), p.prr, p.mis, p.maxdepth, p.rrdepth,
+                    p.integrator == 1 ? 0 : 1,
                     t.rays[cur].as<Ray>(), t.beta[cur].as<float4>(), t.hits.as<HitRec>(), cnt + C_RAY0 + cur,
                     t.radiance.as<float4>(), t.rays[nxt].as<Ray>(), t.beta[nxt].as<float4>(), cnt + C_RAY0 + nxt,
                     t.shadow.as<Ray>(), t.contrib.as<float4>(), cnt + C_SHADOW};

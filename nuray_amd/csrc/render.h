@@ -22,6 +22,7 @@ struct RenderParams {
     int rrdepth = 0;    // Russian roulette from this vertex index (reference: 0)
     int batch_spp = 0;  // samples per pixel per wavefront batch (0: auto)
     int accumulate = 0; // keep previous sums instead of clearing them
+    int integrator = 0; // 0: RendererPTLS (ptnee), 1: RendererPT (pt, no light sampling)
 };
 
 struct RenderStats {

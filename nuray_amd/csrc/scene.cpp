@@ -45,6 +45,8 @@ Scene::Scene(const float* P, const float* N, const float* UV, const int32_t* M, 
              uint32_t nmat)
     : p(P, P + 9 * size_t(ntri)), n(9 * size_t(ntri), 0.0f), uv(6 * size_t(ntri), 0.0f), mat(M, M + ntri),
       mats(MATS, MATS + 8 * size_t(nmat)) {
+    mats_ext.assign(4 * size_t(nmat), 0.0f);
+    for (uint32_t j = 0; j < nmat; ++j) mats_ext[4 * size_t(j) + 3] = -1.0f;
     if (N) std::memcpy(n.data(), N, sizeof(float) * 9 * size_t(ntri));
     if (UV) std::memcpy(uv.data(), UV, sizeof(float) * 6 * size_t(ntri));
     for (uint32_t i = 0; i < ntri; ++i) {
@@ -64,6 +66,24 @@ Scene::Scene(const float* P, const float* N, const float* UV, const int32_t* M, 
 Scene::~Scene() {
     delete target;
     if (stream) (void)hipStreamDestroy(stream);
+}
+
+std::vector<MatRec> Scene::mat_records() const {
+    std::vector<MatRec> mr(mats.size() / 8);
+    for (size_t j = 0; j < mr.size(); ++j) {
+        const float* m = &mats[8 * j];
+        const float* e = &mats_ext[4 * j];
+        mr[j].a = make_float4(m[0], m[1], m[2], m[3]);
+        mr[j].b = make_float4(m[4], m[5], m[6], 0);
+        mr[j].c = make_float4(e[0], e[1], e[2], e[3]);
+    }
+    return mr;
+}
+
+void Scene::set_material_ext(int id, const float* ext) {
+    if (id < 0 || size_t(id) >= mats.size() / 8) throw std::runtime_error("material id out of range");
+    for (int k = 0; k < 4; ++k) mats_ext[4 * size_t(id) + k] = ext[k];
+    built = false;
 }
 
 int Scene::add_texture(int id, int w, int h, int fmt, const void* data) {
@@ -93,13 +113,17 @@ void Scene::build(int nthreads) {
         if (tex >= 0 && (tex >= int(textures.size()) || textures[tex].w <= 0))
             throw std::runtime_error("material " + std::to_string(j) + " names missing texture " + std::to_string(tex));
         const int type = int(mats[8 * j]);
-        if (type < MAT_LAMBERT || type > MAT_GGX) throw std::runtime_error("unknown material type");
+        if (type < MAT_LAMBERT || type > MAT_GLASS) throw std::runtime_error("unknown material type");
+        const int tks = int(mats_ext[4 * j + 3]);
+        if (tks >= 0 && (tks >= int(textures.size()) || textures[tks].w <= 0))
+            throw std::runtime_error("material " + std::to_string(j) + " names missing Ks texture " + std::to_string(tks));
     }
     features = 0;
     for (int i = 0; i < N; ++i) {
         const float* m = &mats[8 * size_t(mat[i])];
         if (int(m[0]) == MAT_GGX) features |= 1u;
-        if (int(m[6]) >= 0) features |= 2u;
+        if (int(m[6]) >= 0 || int(mats_ext[4 * size_t(mat[i]) + 3]) >= 0) features |= 2u;
+        if (int(m[0]) == MAT_BLINNPHONG || int(m[0]) == MAT_GLASS) features |= 4u;
     }
     if (env_tex >= 0 && (env_tex >= int(textures.size()) || textures[env_tex].w <= 0))
         throw std::runtime_error("environment map missing");
@@ -159,12 +183,7 @@ void Scene::build(int nthreads) {
         s[k].s2 = make_float4(nn[8], t[0], t[1], t[2]);
         s[k].s3 = make_float4(t[3], t[4], t[5], mf);
     }
-    std::vector<MatRec> mr(mats.size() / 8);
-    for (size_t j = 0; j < mr.size(); ++j) {
-        const float* m = &mats[8 * j];
-        mr[j].a = make_float4(m[0], m[1], m[2], m[3]);
-        mr[j].b = make_float4(m[4], m[5], m[6], 0);
-    }
+    const std::vector<MatRec> mr = mat_records();
     // texture pool
     std::vector<TexDesc> td(textures.size());
     size_t pool = 0;

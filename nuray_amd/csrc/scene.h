@@ -37,6 +37,7 @@ public:
     std::vector<float> p, n, uv;  // N*9, N*9, N*6
     std::vector<int32_t> mat;
     std::vector<float> mats;      // M*8
+    std::vector<float> mats_ext;  // M*4: Blinn-Phong Ks rgb, Ks texture (-1 none)
     std::vector<Texture> textures;
     std::vector<size_t> tex_offsets;  // byte offset of each texture in the device pool (after build)
     int env_tex = -1;
@@ -62,6 +63,8 @@ public:
     ~Scene();
     int n_tris() const { return int(mat.size()); }
     int add_texture(int id, int w, int h, int fmt, const void* data);
+    void set_material_ext(int id, const float* ext);
+    std::vector<MatRec> mat_records() const;  // device material table
     // Host SAH build (reference rule), light sampler, upload of every device array.
     void build(int nthreads);
 };

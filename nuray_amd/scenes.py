@@ -483,3 +483,34 @@ def large_textured(n_cells=2048, n_spheres=10000, tex_size=4096, n_tex=16, adj_t
                 camera=dict(pos=(1000.0, 500.0, 2300.0), euler=(0.0, -25.0, 0.0), fov=45.0, aspect=width / height),
                 width=width, height=height, envmap=dict(rgb=_sky(), gain=1.0),
                 name=f"large{n_cells}_spheres{n_spheres}_tex{n_tex}x{tex_size}")
+
+
+BLINNPHONG, GLASS = 4, 5
+
+
+def material_zoo_ext(width=160, height=120):
+    """The box room with the remaining reference materials: a Blinn-Phong back wall (Kd from the
+    floor's texture, Ks 0.5, Ns 50), a glass sphere (F0 0.04, ior 1.5), a Blinn-Phong sphere with a
+    textured Ks and Ns 200, a GGX-reflect tall box, the textured Lambert floor, Lambert side walls,
+    the ceiling light and the sky."""
+    base = material_zoo(width, height)
+    keep = ~np.isin(base["mat"], [6, 7])                       # drop the GGX dielectric spheres
+    p, n, uv, mat = base["p"][keep], base["n"][keep], base["uv"][keep], base["mat"][keep].copy()
+    mats = base["mats"].copy()
+    mats[1] = mat_row(BLINNPHONG, (0.6, 0.5, 0.4), 50.0, tex=0)   # back wall: Kd texture, Ns in alpha
+    mats[8] = mat_row(GGXREFL, (0.8, 0.7, 0.5), 0.2)
+    mats = np.concatenate([mats, np.asarray([mat_row(GLASS, (0.04, 0.04, 0.04), ior=1.5),
+                                             mat_row(BLINNPHONG, (0.2, 0.3, 0.6), 200.0)], np.float32)])
+    sv, sf = _icosphere80()
+    extra_p, extra_n = [], []
+    for (c, r, m) in (((40.0, 30.0, 30.0), 30.0, 9), ((-50.0, 25.0, 60.0), 25.0, 10)):
+        for f in sf:
+            extra_p.append((sv[f] * r + np.asarray(c)).reshape(-1))
+            extra_n.append(sv[f].reshape(-1))
+        mat = np.concatenate([mat, np.full(len(sf), m, np.int32)])
+    p = np.concatenate([p, np.asarray(extra_p, np.float32)])
+    n = np.concatenate([n, np.asarray(extra_n, np.float32)])
+    uv = np.concatenate([uv, np.zeros((len(extra_p), 6), np.float32)])
+    return dict(base, p=p.astype(np.float32), n=n.astype(np.float32), uv=uv.astype(np.float32),
+                mat=mat.astype(np.int32), mats=mats.astype(np.float32),
+                mats_ext={1: [0.5, 0.5, 0.5, -1], 10: [0.9, 0.9, 0.9, 1]}, name="material_zoo_ext")

@@ -116,8 +116,21 @@ struct GradTracer {
     int P;        // material parameters; texel keys start here
     int tex_id;   // texture whose texels are differentiated (-1: none)
 
+    // f and pdf, with derivatives for the differentiated models (Lambert, GGX-reflect); the
+    // other models are replayed without derivatives
+    static bool differentiated(const Material& M) { return M.type == MAT_LAMBERT || M.type == MAT_GGXREFL; }
+    MatD parts(const Material& M, const MatEval<double>& me, const V3d& kd, const V3d& wo, const V3d& n,
+               const V3d& wi, const V3d& tc) const {
+        if (differentiated(M)) return mat_deriv(M, kd, wo, n, wi);
+        MatD md;
+        md.f = me.f(wo, n, wi, tc);
+        md.pdf = me.pdf(wo, n, wi);
+        return md;
+    }
+
     // albedo derivative v (RGB) of material mi at texture coordinates (u, v)
     void add_albedo(SG& dL, const Material& M, int mi, float u, float vv, const V3d& v) const {
+        if (!differentiated(M)) return;
         if (M.tex < 0) {
             dL.add(4 * mi + 0, V3d(v.x, 0, 0));
             dL.add(4 * mi + 1, V3d(0, v.y, 0));
@@ -149,18 +162,18 @@ struct GradTracer {
             if (prev_mat >= 0) dL.add(4 * prev_mat + 3, le * dfac);
             return le * fac;
         }
-        if (n.dot(d) > 0 && !(M.type == MAT_GGX)) return V3d(0);
+        MatEval<double> me{&sc, &M};
+        if (n.dot(d) > 0 && !me.transmissive()) return V3d(0);   // back faces of opaque models
         const V3d hp = o + d * t;
         const int mi = tri.mat;
         const V3d tc = tri.texc<double>(b1, b2);
         const float tu = float(tc.x), tv = float(tc.y);
-        MatEval<double> me{&sc, &M};
         const V3d kd = me.albedo(tc);
         V3d result(0);
         double fac_next = 1, dfac_next = 0;
         float u[4];
         rng.draw(uint32_t(1 + depth), 0, u);
-        if ((M.type == MAT_LAMBERT || M.type == MAT_GGXREFL) && !sc.light_tris.empty()) {
+        if (opt.nee && me.needs_nee() && !sc.light_tris.empty()) {
             const Tri& L = sc.tris[sc.pick_light(u[0])];
             double r1 = u[1], r2 = u[2];
             if (r1 + r2 > 1) { r1 = 1 - r1; r2 = 1 - r2; }
@@ -172,7 +185,7 @@ struct GradTracer {
             const V3d le(LM.kd[0], LM.kd[1], LM.kd[2]);
             const double lpdf = sc.light_p();
             const double lop = lpdf * lv.dot(lv) / std::max(1e-6, ln.dot(-wl));
-            const MatD md = mat_deriv(M, kd, wo, n, wl);
+            const MatD md = parts(M, me, kd, wo, n, wl, tc);
             double mis = lop / (lop + md.pdf);
             double dmis = -lop / ((lop + md.pdf) * (lop + md.pdf)) * md.dpdf_da;
             if (opt.mis == 0) mis = 0.5, dmis = 0;
@@ -199,20 +212,14 @@ struct GradTracer {
         float v[4];
         rng.draw(uint32_t(1 + depth), 1, v);
         const V3d wi = me.sample(v, wo, n);
-        MatD md;
-        if (M.type == MAT_GGX) {  // translucent dielectric: replayed, its parameters not differentiated
-            md.f = me.f(wo, n, wi, tc);
-            md.pdf = me.pdf(wo, n, wi);
-        } else {
-            md = mat_deriv(M, kd, wo, n, wi);
-        }
+        const MatD md = parts(M, me, kd, wo, n, wi, tc);
         SG dLc;
         const V3d Lc = trace(hp + wi * 1e-3, wi, fac_next, mi, dfac_next, depth + 1, rng, dLc);
         const double cw = std::fabs(wi.dot(n));
         const V3d T = md.f * (cw / md.pdf / prr);
         result += Lc * T;
         dL.add_scaled(dLc, T);
-        if (M.type != MAT_GGX) {
+        if (differentiated(M)) {
             const V3d dT = md.df_dkd * (cw / md.pdf / prr);
             add_albedo(dL, M, mi, tu, tv, V3d(dT.x * Lc.x, dT.y * Lc.y, dT.z * Lc.z));
         }
@@ -247,7 +254,7 @@ void orc_set_params(void* h, const float* theta, int n_mats) {
 // grad_tex (w x h x 3) with respect to that texture's linear texel values.
 int orc_adjoint(void* h, const float* cam, int W, int H, int spp, int spp0, uint32_t seed, float prr, int mis,
                 int maxdepth, int rrdepth, const float* dldi, double* grad, int n_mats, int nthreads, int tex_id,
-                double* grad_tex) {
+                double* grad_tex, int integrator) {
     OScene* o = static_cast<OScene*>(h);
     if (!o->built) return -1;
     Camera c;
@@ -257,6 +264,7 @@ int orc_adjoint(void* h, const float* cam, int W, int H, int spp, int spp0, uint
     c.aspect = cam[7];
     RenderOpts opt;
     opt.prr = prr; opt.mis = mis; opt.maxdepth = maxdepth; opt.rrdepth = rrdepth; opt.seed = seed;
+    opt.nee = integrator == 1 ? 0 : 1;
     const int P = 4 * n_mats;
     const int NT = (tex_id >= 0 && grad_tex) ? 3 * o->sc.textures[tex_id].w * o->sc.textures[tex_id].h : 0;
     if (nthreads > 0) omp_set_num_threads(nthreads);

@@ -53,6 +53,13 @@ void* orc_scene_create(const float* p, const float* n, const float* uv, const in
 
 void orc_scene_destroy(void* h) { delete S(h); }
 
+// Blinn-Phong's second colour: ext = {Ks r, g, b, Ks texture or -1}.
+void orc_scene_set_material_ext(void* h, int id, const float* ext) {
+    Material& m = S(h)->sc.mats[id];
+    m.ks[0] = ext[0]; m.ks[1] = ext[1]; m.ks[2] = ext[2];
+    m.tex_ks = int(ext[3]);
+}
+
 void orc_scene_set_envmap(void* h, int w, int hh, const float* rgb, float gain) {
     Texture& e = S(h)->sc.env;
     e.w = w; e.h = hh;
@@ -148,9 +155,10 @@ void orc_trav_stats(void* h, int n, const float* o, const float* d, uint64_t* ou
 
 // cam = {pos.xyz, yaw, pitch, roll, fov, aspect}.  Renders pixels [x0,x1)x[y0,y1), samples
 // [spp0, spp0+spp), accumulating per-pixel sums (W*H*3 doubles, full-image layout).
+// integrator: 0 = RendererPTLS (ptnee), 1 = RendererPT (pt, rendererpt.cpp:13-61).
 int orc_render(void* h, const float* cam, int W, int H, int x0, int y0, int x1, int y1, int spp0,
                int spp, uint32_t seed, float prr, int mis, int maxdepth, int rrdepth, int fp64,
-               int nthreads, double* sum, double* sum2) {
+               int nthreads, double* sum, double* sum2, int integrator) {
     OScene* o = S(h);
     if (!o->built) return -1;
     Camera c;
@@ -160,6 +168,7 @@ int orc_render(void* h, const float* cam, int W, int H, int x0, int y0, int x1, 
     c.aspect = cam[7];
     RenderOpts opt;
     opt.prr = prr; opt.mis = mis; opt.maxdepth = maxdepth; opt.rrdepth = rrdepth; opt.seed = seed;
+    opt.nee = integrator == 1 ? 0 : 1;
     const int B = 8;
     int tx = (x1 - x0 + B - 1) / B, ty = (y1 - y0 + B - 1) / B;
     if (nthreads > 0) omp_set_num_threads(nthreads);

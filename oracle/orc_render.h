@@ -45,6 +45,7 @@ struct RenderOpts {
     int maxdepth = 0;   // 0 = unbounded (reference); else vertices with index >= maxdepth are not created
     int rrdepth = 0;    // Russian roulette applies at vertices with index >= rrdepth (reference: 0)
     uint32_t seed = 0;
+    int nee = 1;        // 0: RendererPT (rendererpt.cpp:13-61): no light sampling, emitters at full weight
 };
 
 // Counters of the instrumented traversal: node-pair fetches and triangle tests.
@@ -85,7 +86,7 @@ struct Tracer {
         T fac_next = T(1);
         float u[4];
         rng.draw(uint32_t(1 + depth), 0, u);
-        if (mat.needs_nee()) {
+        if (opt.nee && mat.needs_nee()) {
             int li = sc.pick_light(u[0]);
             if (li >= 0) {
                 const Tri& L = sc.tris[li];

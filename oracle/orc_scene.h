@@ -6,15 +6,17 @@
 
 namespace orc {
 
-enum MatType { MAT_LAMBERT = 0, MAT_LIGHT = 1, MAT_GGXREFL = 2, MAT_GGX = 3 };
+enum MatType { MAT_LAMBERT = 0, MAT_LIGHT = 1, MAT_GGXREFL = 2, MAT_GGX = 3, MAT_BLINNPHONG = 4, MAT_GLASS = 5 };
 
 // One material row as passed through the C API: type, rgb (Kd / F0 / Ke), alpha, ior, texture.
 struct Material {
     int type = MAT_LAMBERT;
-    float kd[3] = {0, 0, 0};
-    float alpha = 0.5f;
+    float kd[3] = {0, 0, 0};   // Kd / Ke / F0
+    float alpha = 0.5f;        // GGX roughness; Blinn-Phong exponent Ns
     float ior = 1.0f;
-    int tex = -1;
+    int tex = -1;              // Kd / F0 texture
+    float ks[3] = {0, 0, 0};   // Blinn-Phong Ks
+    int tex_ks = -1;           // Blinn-Phong Ks texture
 };
 
 // src/scene/triangle.h:8-36, triangle.cpp:5-25.  A flat normal normalize((p1-p0)x(p2-p0))
@@ -141,8 +143,13 @@ struct MatEval {
         if (m->tex >= 0) return V3<T>(sc->textures[m->tex].uv(float(uv.x), float(uv.y)));
         return V3<T>(T(m->kd[0]), T(m->kd[1]), T(m->kd[2]));
     }
-    bool needs_nee() const { return m->type == MAT_LAMBERT || m->type == MAT_GGXREFL; }
-    bool transmissive() const { return m->type == MAT_LIGHT || m->type == MAT_GGX; }
+    // requireLightSampling / isTransmission (matblinnphong.cpp:88-91, matglass.cpp:55-69)
+    bool needs_nee() const { return m->type == MAT_LAMBERT || m->type == MAT_GGXREFL || m->type == MAT_BLINNPHONG; }
+    bool transmissive() const { return m->type == MAT_LIGHT || m->type == MAT_GGX || m->type == MAT_GLASS; }
+    V3<T> specular(const V3<T>& uv) const {
+        if (m->tex_ks >= 0) return V3<T>(sc->textures[m->tex_ks].uv(float(uv.x), float(uv.y)));
+        return V3<T>(T(m->ks[0]), T(m->ks[1]), T(m->ks[2]));
+    }
 
     // sampleBxdf: matlambert.cpp:5-18, matggxrefl.cpp:5-34, matggx.cpp:5-47.
     V3<T> sample(const float u[3], const V3<T>& wo, const V3<T>& n) const {
@@ -186,6 +193,35 @@ struct MatEval {
             }
             return wi.normalized();
         }
+        if (m->type == MAT_BLINNPHONG) {  // matblinnphong.cpp:5-34: lobe 1/2 each, wi not normalised
+            V3<T> ax1, ax2;
+            frame(n, ax1, ax2);
+            if (u[0] < 0.5f) {
+                T r2 = T(u[1]) * T(0.99f);
+                T phi = T(double(u[2]) * 3.14159 * 2);
+                T r = std::sqrt(r2), h = std::sqrt(T(1) - r2);
+                return h * n + (r * T(std::cos(phi))) * ax1 + (r * T(std::sin(phi))) * ax2;
+            }
+            const double r1 = u[1], r2 = u[2];
+            const double ct = std::pow(r1, double(1.0f / (m->alpha + 2.0f)));
+            const double st = std::sqrt(1 - ct);  // the reference's sqrt(1 - cos), not sqrt(1 - cos^2)
+            const double phi = double(2.0f * 3.14159f) * r2;
+            V3<T> h = T(float(ct)) * n + T(float(st * std::cos(phi))) * ax1 + T(float(st * std::sin(phi))) * ax2;
+            return (T(2) * h.dot(wo)) * h - wo;
+        }
+        if (m->type == MAT_GLASS) {  // matglass.cpp:5-24
+            const bool out = wo.dot(n) > 0;
+            const T I = out ? T(1) / T(m->ior) : T(m->ior);
+            const V3<T> nn = out ? n : -n;
+            if (u[0] < 0.5f) {
+                const V3<T> t = nn.cross(wo).cross(nn).normalized();
+                const T so = wo.dot(t);
+                const T si = std::min(so * I, T(1));
+                const T ci = std::sqrt(T(1) - si * si);
+                return -(ci * nn + si * t);
+            }
+            return (T(2) * wo.dot(nn)) * nn - wo;
+        }
         return V3<T>(0);
     }
 
@@ -223,6 +259,22 @@ struct MatEval {
             if (bi * bo >= 0) return ggx_fr(wi, wo, n, kd);
             return ggx_ft(wi, wo, n, kd);
         }
+        if (m->type == MAT_BLINNPHONG) {  // matblinnphong.cpp:36-49
+            if (wi.dot(n) * wo.dot(n) < 0) return V3<T>(0);
+            if (wi.dot(n) <= T(1e-4) || wo.dot(n) <= T(1e-4f)) return V3<T>(0);
+            const V3<T> h = (wo + wi).normalized();
+            const T ns = T(m->alpha);
+            return albedo(uv) / T(3.14159) +
+                   specular(uv) * ((ns + T(2)) / T(3.14159) / T(8) * T(std::pow(n.dot(h), ns)));
+        }
+        if (m->type == MAT_GLASS) {  // matglass.cpp:26-44
+            if (std::fabs(wi.dot(n)) < T(1e-3f)) return V3<T>(0);
+            const V3<T> f0 = albedo(uv);
+            const V3<T> F = f0 + (V3<T>(1) - f0) * T(std::pow(T(1) - std::fabs(n.dot(wo)), T(5)));
+            if (wi.dot(n) * wo.dot(n) > 0) return F;
+            const T I = wo.dot(n) > 0 ? T(1) / T(m->ior) : T(m->ior);
+            return (V3<T>(1) - F) * T(std::pow(I, T(2)));
+        }
         return V3<T>(0);
     }
 
@@ -249,6 +301,15 @@ struct MatEval {
             if (d < T(1e-6f)) return T(1e18f);
             return std::max(d * n.dot(wh) / T(4), T(1));
         }
+        if (m->type == MAT_BLINNPHONG) {  // matblinnphong.cpp:51-59
+            if (wi.dot(n) <= T(1e-4) || wo.dot(n) <= T(1e-4f)) return T(1e-18f);
+            const T pd = T(1.0 / 3.14159) * (std::fabs(wi.dot(n)) + T(1e-6f));
+            const V3<T> h = (wo + wi).normalized();
+            const T ns = T(m->alpha);
+            const T ps = (ns + T(2)) / T(3.14159) / T(8) * T(std::pow(std::fabs(n.dot(h)), ns + T(1)));
+            return T(0.5f) * pd + (T(1) - T(0.5f)) * ps;
+        }
+        if (m->type == MAT_GLASS) return T(0.5);  // matglass.cpp:46-49
         return T(1.0 / 3.14159);
     }
 

@@ -47,7 +47,7 @@ def lib():
         L.orc_trav_stats.argtypes = [vp, C.c_int, f32p, f32p, u64p]
         L.orc_render.argtypes = [vp, f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                  C.c_int, C.c_uint32, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                 f64p, f64p]
+                                 f64p, f64p, C.c_int]
         L.orc_philox.argtypes = [u32p, u32p, u32p]
         L.orc_srgb8_decode.argtypes = [C.c_int]
         L.orc_srgb8_decode.restype = C.c_float
@@ -58,7 +58,8 @@ def lib():
         L.orc_tri_test.argtypes = [f32p, f32p, f32p, f32p]
         L.orc_set_params.argtypes = [vp, f32p, C.c_int]
         L.orc_adjoint.argtypes = [vp, f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_float, C.c_int,
-                                  C.c_int, C.c_int, f32p, f64p, C.c_int, C.c_int, C.c_int, vp]
+                                  C.c_int, C.c_int, f32p, f64p, C.c_int, C.c_int, C.c_int, vp, C.c_int]
+        L.orc_scene_set_material_ext.argtypes = [vp, C.c_int, f32p]
         L.orc_scene_set_texture_rgb32f.argtypes = [vp, C.c_int, C.c_int, C.c_int, f32p]
         L.orc_nrc_encode.argtypes = [f32p, C.c_int, f32p]
         L.orc_nrc_forward.argtypes = [f32p, f32p, C.c_int, C.c_int, f64p]
@@ -92,6 +93,8 @@ class Scene:
         if env is not None:
             rgb = _f32(env["rgb"])
             L.orc_scene_set_envmap(self.h, rgb.shape[1], rgb.shape[0], rgb, float(env.get("gain", 1.0)))
+        for mid, ext in (sc.get("mats_ext") or {}).items():   # Blinn-Phong Ks rows
+            L.orc_scene_set_material_ext(self.h, int(mid), _f32(ext))
         for tid, tex in enumerate(sc.get("textures") or []):
             if np.asarray(tex).dtype == np.float32:   # linear RGB float texture
                 t = _f32(tex)
@@ -132,12 +135,12 @@ class Scene:
         return int(out[0]), int(out[1])
 
     def render(self, cam, W, H, spp, spp0=0, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0, fp64=False,
-               nthreads=0, rect=None, sums=None):
+               nthreads=0, rect=None, sums=None, integrator=0):
         x0, y0, x1, y1 = rect if rect is not None else (0, 0, W, H)
         if sums is None:
             sums = (np.zeros(W * H * 3), np.zeros(W * H * 3))
         r = lib().orc_render(self.h, _f32(cam), W, H, x0, y0, x1, y1, spp0, spp, seed, prr, mis, maxdepth,
-                             rrdepth, int(fp64), nthreads, sums[0], sums[1])
+                             rrdepth, int(fp64), nthreads, sums[0], sums[1], integrator)
         if r != 0:
             raise RuntimeError("oracle render failed")
         return sums
@@ -165,14 +168,14 @@ class Scene:
         lib().orc_set_params(self.h, theta.reshape(-1), len(theta))
 
     def adjoint(self, cam, W, H, spp, dldi, n_mats, spp0=0, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0,
-                nthreads=0, tex_id=-1, tex_shape=None):
+                nthreads=0, tex_id=-1, tex_shape=None, integrator=0):
         """FP64 gradient (n_mats, 4) of sum(dldi * mean image) for the current parameters; with
         tex_id >= 0 also the texel gradient (h, w, 3) of that texture."""
         g = np.zeros(n_mats * 4, np.float64)
         gt = np.zeros((tex_shape[0], tex_shape[1], 3), np.float64) if tex_id >= 0 else None
         r = lib().orc_adjoint(self.h, _f32(cam), W, H, spp, spp0, seed, prr, mis, maxdepth, rrdepth,
                               _f32(dldi).reshape(-1), g, n_mats, nthreads, tex_id,
-                              None if gt is None else gt.ctypes.data)
+                              None if gt is None else gt.ctypes.data, integrator)
         if r != 0:
             raise RuntimeError("oracle adjoint failed")
         return g.reshape(n_mats, 4) if gt is None else (g.reshape(n_mats, 4), gt)

@@ -59,9 +59,24 @@ int main(int argc, char** argv) {
     for (int i = 0; i < 15; ++i)
         for (int k = 0; k < 3; ++k) same = same && t.rgba[4 * i + k] == img.rgb[3 * i + k];
     EXPECT(same);
-    // unsupported things fail loudly
+    // forcing glass; unsupported things fail loudly
+    nuray::Loader l3;
+    l3.fromSceneDescription(obj + " -m=glass 0.9 0.9 0.9 1.5");
+    EXPECT(l3.materials().size() == 1 && l3.materials()[0].type == NR_MAT_GLASS && l3.materials()[0].ior == 1.5f);
+    // Ks > 0 in the MTL: Blinn-Phong with Kd, Ks and the exponent Ns (loader.cpp:126-140)
+    if (argc > 4) {
+        nuray::Loader l5;
+        l5.fromSceneDescription(std::string(argv[4]) + " -s 100");
+        int bp = 0;
+        for (const auto& m : l5.materials())
+            if (m.type == NR_MAT_BLINNPHONG) {
+                ++bp;
+                EXPECT(m.kd[0] == 0.01f && m.ks[0] == 0.95f && m.ks[2] == 0.95f && m.alpha == 100000.0f && m.tex_ks < 0);
+            }
+        EXPECT(bp == 1);
+    }
     bool threw = false;
-    try { nuray::Loader l3; l3.fromSceneDescription(obj + " -m=glass 1 1 1 1.5"); } catch (const nuray::Error&) { threw = true; }
+    try { nuray::Loader l4; l4.fromSceneDescription(obj + "_missing.obj"); } catch (const nuray::Error&) { threw = true; }
     EXPECT(threw);
     std::printf("%s %d triangles %d materials\n", fails ? "FAILED" : "OK", nt, nm);
     return fails ? 1 : 0;

@@ -28,9 +28,11 @@ def _parse_obj(path, scale, pos):
                     continue
                 if c[0] == "newmtl":
                     name = c[1]
-                    mats[name] = {"Kd": [0, 0, 0], "Ke": [0, 0, 0]}
-                elif c[0] in ("Kd", "Ke"):
+                    mats[name] = {"Kd": [0, 0, 0], "Ke": [0, 0, 0], "Ks": [0, 0, 0], "Ns": 10.0}
+                elif c[0] in ("Kd", "Ke", "Ks"):
                     mats[name][c[0]] = [float(x) for x in c[1:4]]
+                elif c[0] == "Ns":
+                    mats[name]["Ns"] = float(c[1])
         elif b[0] == "usemtl":
             cur = b[1]
             if cur not in order:
@@ -51,7 +53,8 @@ def test_facade_host_checks_and_loader(tmp_path):
     src = os.path.join(HERE, "cpp", "test_facade.cpp")
     subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-o", exe, src], check=True)
     dump = str(tmp_path / "dump.bin")
-    r = subprocess.run([exe, OBJ, dump, str(tmp_path / "t.bmp")], capture_output=True, text=True)
+    mirror = os.path.join(os.path.dirname(OBJ), "CornellBox-Mirror.obj")
+    r = subprocess.run([exe, OBJ, dump, str(tmp_path / "t.bmp"), mirror], capture_output=True, text=True)
     print(r.stdout, r.stderr)
     assert r.returncode == 0 and r.stdout.startswith("OK")
     raw = open(dump, "rb").read()

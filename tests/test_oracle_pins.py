@@ -249,10 +249,10 @@ def test_camera_closed_form(po):
 
 
 # ------------------------------------------------------------------ materials
-def _mat_scene(po, rows):
+def _mat_scene(po, rows, ext=None):
     p = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32)
-    return po.Scene(dict(p=p, n=None, uv=None, mat=np.zeros(1, np.int32), mats=np.array(rows, np.float32)),
-                    build_bvh=False)
+    return po.Scene(dict(p=p, n=None, uv=None, mat=np.zeros(1, np.int32), mats=np.array(rows, np.float32),
+                         mats_ext=ext), build_bvh=False)
 
 
 def test_lambert_f_pdf_and_sampling(po):
@@ -305,6 +305,76 @@ def test_ggx_ndf_normalised_and_sampling(po):
     t2 = np.sort((1 - hv[:, 2] ** 2) / hv[:, 2] ** 2 / a ** 2)
     x = t2 / (1 + t2)  # should be ~U (theta scaled by 0.9999)
     assert np.abs(x - (np.arange(k) + 0.5) / k).max() < 0.01
+
+
+def test_blinnphong_f_pdf_and_sampling(po):
+    """MatBlinnPhong (matblinnphong.cpp): f = Kd/pi + (Ns+2)/(8 pi) Ks (n.h)^Ns above the horizon;
+    pdf = (|cos|+1e-6)/(2 pi) + (Ns+2)/(16 pi) (n.h)^(Ns+1), 1e-18 below it; the diffuse lobe (u0 <
+    0.5) has cos^2 uniform on [0.01, 1]; the glossy lobe has cos_h = r1^(1/(Ns+2)) with the
+    reference's sin_h = sqrt(1 - cos_h), wi = 2 (h.wo) h - wo left unnormalised."""
+    ns = 20.0
+    kd, ks = np.array([0.5, 0.25, 0.125]), np.array([0.3, 0.6, 0.9])
+    s = _mat_scene(po, [[4, *kd, ns, 1, -1, 0]], ext={0: [*ks, -1]})
+    n = np.tile([0, 0, 1.0], (3, 1))
+    wo = np.tile([0, 0.6, 0.8], (3, 1))
+    wi = np.array([[0.6, 0, 0.8], [0, -0.6, 0.8], [0.8, 0, -0.6]])
+    _, f, pdf = s.material_eval(0, wo, n, wi=wi)
+    pi = 3.14159
+    for i in range(2):
+        h = (wo[i] + wi[i]) / np.linalg.norm(wo[i] + wi[i])
+        spec = (ns + 2) / pi / 8 * h[2] ** ns
+        assert np.allclose(f[i], kd / pi + ks * spec, rtol=1e-5)
+        want = 0.5 * (abs(wi[i, 2]) + 1e-6) / pi + 0.5 * (ns + 2) / pi / 8 * h[2] ** (ns + 1)
+        assert abs(pdf[i] - want) / want < 1e-5
+    assert not f[2].any() and pdf[2] == np.float32(1e-18)
+    rng = np.random.Generator(np.random.PCG64(3))
+    k = 200000
+    u = rng.random((k, 3)).astype(np.float32)
+    w, _, _ = s.material_eval(0, np.tile([0, 0, 1.0], (k, 1)), np.tile([0, 0, 1.0], (k, 1)), u3=u)
+    w = w.astype(np.float64)
+    diff = u[:, 0] < 0.5
+    c2 = np.sort(w[diff, 2] ** 2)
+    m = len(c2)
+    assert np.abs(c2 - (0.01 + 0.99 * (np.arange(m) + 0.5) / m)).max() < 0.01
+    assert np.allclose(np.linalg.norm(w[diff], axis=1), 1, atol=1e-5)
+    # glossy lobe with wo = n: wi.z = 2 c^2 - 1 and |wi_xy| = 2 c sqrt(1 - c), c = cos_h
+    g = w[~diff]
+    ct = u[~diff, 1].astype(np.float64) ** (1 / (ns + 2))
+    assert np.allclose(g[:, 2], 2 * ct * ct - 1, atol=1e-6)
+    assert np.allclose(np.hypot(g[:, 0], g[:, 1]), 2 * ct * np.sqrt(1 - ct), atol=1e-6)
+    x = np.sort(((g[:, 2] + 1) / 2) ** ((ns + 2) / 2))   # cos_h^(Ns+2) read back from wi: uniform
+    m = len(x)
+    assert np.abs(x - (np.arange(m) + 0.5) / m).max() < 0.01
+
+
+def test_glass_snell_fresnel_and_half_pdf(po):
+    """MatGlass (matglass.cpp): refraction with sin_i = sin_o / ior entering (ior outside-in
+    swapped when leaving, clamped at 1), mirror reflection, Schlick F = F0 + (1-F0)(1-|cos_o|)^5
+    for reflection, (1-F) eta^2 for transmission, pdf 1/2, f = 0 within 1e-3 of grazing."""
+    f0, ior = 0.04, 1.5
+    s = _mat_scene(po, [[5, f0, f0, f0, 0.5, ior, -1, 0]])
+    so = 0.6
+    wo = np.array([[so, 0, 0.8], [so, 0, 0.8], [so, 0, -0.8], [0.9, 0, -np.sqrt(1 - 0.81)]])
+    n = np.tile([0, 0, 1.0], (4, 1))
+    u = np.array([[0.2, 0, 0], [0.7, 0, 0], [0.2, 0, 0], [0.2, 0, 0]], np.float32)
+    w, _, pdf = s.material_eval(0, wo, n, u3=u)
+    si = so / ior
+    assert np.allclose(w[0], [-si, 0, -np.sqrt(1 - si * si)], atol=1e-6)     # refracted inwards
+    assert np.allclose(w[1], [-so, 0, 0.8], atol=1e-6)                     # mirror
+    sx = min(so * ior, 1.0)
+    assert np.allclose(w[2], [-sx, 0, np.sqrt(1 - sx * sx)], atol=1e-6)     # leaving the glass
+    assert np.allclose(w[3], [-1, 0, 0], atol=1e-6)                        # total internal: grazing
+    assert np.all(pdf == 0.5)
+    wi = np.array([[-si, 0, -np.sqrt(1 - si * si)], [-so, 0, 0.8], [-0.999, 0, np.sqrt(1 - 0.999 ** 2)]])
+    _, f, _ = s.material_eval(0, np.tile(wo[0], (3, 1)), n[:3], wi=wi)
+    F = f0 + (1 - f0) * (1 - 0.8) ** 5
+    assert np.allclose(f[0], (1 - F) / ior ** 2, rtol=1e-5)
+    assert np.allclose(f[1], F, rtol=1e-5)
+    assert np.allclose(f[2], F, rtol=1e-5)   # |cos_i| = 0.045 > 1e-3: still reflects
+    _, f, _ = s.material_eval(0, wo[2:3], n[:1], wi=np.array([[-0.6, 0, 0.8]]))   # leaving: eta = ior
+    assert np.allclose(f[0], (1 - F) * ior ** 2, rtol=1e-5)
+    _, f, _ = s.material_eval(0, wo[:1], n[:1], wi=np.array([[-1, 0, 5e-4]]))
+    assert not f.any()
 
 
 def test_light_sampler_area_weights_and_2x_pdf(po, scenes):
@@ -367,11 +437,10 @@ def test_lambert_plane_under_constant_sky(po):
     assert abs(z) < 5, (mean, expect)
 
 
-def test_nee_twice_weight_with_fixed_mis(po):
-    """mis = 0 fixes the MIS weight to 0.5 (rendererptls.cpp:65-68).  With LightSampler::p() being
-    half the true area pdf (triangle.cpp:57-60), the NEE term estimates twice the direct light,
-    the BSDF-sampled emitter hit the direct light itself: the total is 1.5 x E_direct."""
-    rho, Le, a, h = 0.5, 10.0, 2.0, 10.0
+def light_over_plane(rho=0.5, Le=10.0, a=2.0, h=10.0):
+    """A Lambert plane (albedo rho, +-1e4) under an a x a emitter of radiance Le facing down at
+    height h; returns the scene, a 1-pixel camera looking straight down at the centre, and the
+    irradiance Ed at the centre (numerical quadrature of the square's form factor)."""
     R = 1e4
     p = np.array([[-R, 0, R, R, 0, R, R, 0, -R], [R, 0, -R, -R, 0, -R, -R, 0, R],
                   [-a / 2, h, -a / 2, a / 2, h, -a / 2, a / 2, h, a / 2],
@@ -383,20 +452,43 @@ def test_nee_twice_weight_with_fixed_mis(po):
         p[3] = np.concatenate([p[3, :3], p[3, 6:], p[3, 3:6]])
     sc = dict(p=p, n=None, uv=None, mat=np.array([0, 0, 1, 1], np.int32),
               mats=np.array([[0, rho, rho, rho, 0.5, 1, -1, 0], [1, Le, Le, Le, 0.5, 1, -1, 0]], np.float32))
-    o = po.Scene(sc)
-    cam = np.array([0, 5, 0, 0, -90, 0, 0.01, 1], np.float32)   # looks straight down at the centre
-    spp = 100000
-    s, s2 = o.render(cam, 1, 1, spp, seed=4, mis=0, maxdepth=2)
-    m = s[0] / spp
-    # irradiance at the centre from a square a x a at height h (radiance Le): numerical quadrature
+    cam = np.array([0, h / 2, 0, 0, -90, 0, 0.01, 1], np.float32)
     x = np.linspace(-a / 2, a / 2, 1201)
     X, Y = np.meshgrid(x, x)
     Ed = Le * np.trapezoid(np.trapezoid(h * h / (X ** 2 + Y ** 2 + h * h) ** 2, x), x)
+    return sc, cam, Ed
+
+
+def test_nee_twice_weight_with_fixed_mis(po):
+    """mis = 0 fixes the MIS weight to 0.5 (rendererptls.cpp:65-68).  With LightSampler::p() being
+    half the true area pdf (triangle.cpp:57-60), the NEE term estimates twice the direct light,
+    the BSDF-sampled emitter hit the direct light itself: the total is 1.5 x E_direct."""
+    rho = 0.5
+    sc, cam, Ed = light_over_plane(rho)
+    o = po.Scene(sc)
+    spp = 100000
+    s, s2 = o.render(cam, 1, 1, spp, seed=4, mis=0, maxdepth=2)
+    m = s[0] / spp
     # BSDF term: Lambert samples cos >= 0.1 with density cos/(0.99 pi) but reports (cos + 1e-4)/pi,
     # so its expectation carries a factor 1/0.99 (the light subtends cos ~ 1: 1e-4 is negligible)
     expect = (0.5 * 2.0 + 0.5 / 0.99) * rho / 3.14159 * Ed
     se = math.sqrt(s2[0] / spp - m * m) / math.sqrt(spp)
     assert abs(m - expect) < 5 * se + 1e-3 * expect, (m, expect, se)
+
+
+def test_pt_direct_light_closed_form(po):
+    """The pt integrator (rendererpt.cpp:13-57) has no light sampling: the plane's radiance is the
+    BSDF-sampled emitter hit alone, rho/(0.99 pi) E_direct (the Lambert pdf's 1/0.99 as above), with
+    Russian roulette from the first hit unbiased."""
+    rho = 0.5
+    sc, cam, Ed = light_over_plane(rho, a=10.0, h=5.0)   # subtends cos >= 0.57: inside the sampled cone
+    spp = 200000
+    s, s2 = po.Scene(sc).render(cam, 1, 1, spp, seed=5, integrator=1)
+    m = s[0] / spp
+    expect = rho / (0.99 * 3.14159) * Ed
+    se = math.sqrt(s2[0] / spp - m * m) / math.sqrt(spp)
+    assert abs(m - expect) < 5 * se + 1e-3 * expect, (m, expect, se)
+    assert 5 * se < 0.1 * expect   # the test has the power to see a dropped 1/prr or a doubled term
 
 
 # ------------------------------------------------------------------ adjoint (FP64 gradient)
@@ -425,6 +517,34 @@ def test_adjoint_albedo_equals_crn_central_difference(po, scenes):
         tm[mi, ch] -= h
         fd = (loss(tp) - loss(tm)) / (float(tp[mi, ch]) - float(tm[mi, ch]))   # float32 step taken
         assert abs(g[mi, ch] - fd) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(g).max()), (mi, ch, g[mi, ch], fd)
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_adjoint_through_glass_and_blinnphong_equals_crn_difference(po, scenes, integrator):
+    """The same check on paths that cross glass (refraction inside the sphere, back-face hits of a
+    transmissive model) and Blinn-Phong (replayed, not differentiated), for ptnee and pt: the
+    Lambert and GGX-reflect rows must equal the central difference of the FP64 render."""
+    sc = scenes.material_zoo_ext(16, 12)
+    W, H, spp = 16, 12, 6
+    M = len(sc["mats"])
+    theta0 = sc["mats"][:, 1:5].astype(np.float32).copy()
+    cam = scenes.camera_array(sc["camera"])
+    dldi = np.random.Generator(np.random.PCG64(12)).normal(size=(H, W, 3))
+    o = po.Scene(sc)
+    g = o.adjoint(cam, W, H, spp, dldi, M, seed=17, integrator=integrator)
+
+    def loss(theta):
+        o.set_params(theta)
+        s, _ = o.render(cam, W, H, spp, seed=17, fp64=True, integrator=integrator)
+        return (dldi.reshape(-1) * s / spp).sum()
+    h = 1e-4
+    for mi, ch in [(2, 0), (3, 1), (4, 2), (8, 0), (8, 2)]:
+        tp, tm = theta0.copy(), theta0.copy()
+        tp[mi, ch] += h
+        tm[mi, ch] -= h
+        fd = (loss(tp) - loss(tm)) / (float(tp[mi, ch]) - float(tm[mi, ch]))
+        assert abs(g[mi, ch] - fd) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(g).max()), (mi, ch, g[mi, ch], fd)
+    o.set_params(theta0)
 
 
 def test_adjoint_roughness_expectation_on_ggx_plane(po):
