@@ -62,6 +62,10 @@ typedef struct nr_params {
     int32_t batch_spp;     /* samples per pixel per wavefront batch (0 = automatic) */
     int32_t accumulate;    /* 1: add to the previous render's sums instead of clearing them */
     int32_t integrator;    /* renderer: 0 = ptnee (RendererPTLS), 1 = pt (RendererPT, rendererpt.cpp:13-61) */
+    int32_t env_is;        /* 1: also sample the environment map as a light at Lambert and GGX-reflect
+                              vertices, weighted against BSDF sampling by the balance heuristic so that
+                              the expected image stays the reference's (a variance reduction the
+                              reference does not have; ptnee only, not with the adjoint or the cache) */
 } nr_params;
 
 typedef void (*nr_progress_fn)(float fraction, void* user);

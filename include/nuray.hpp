@@ -515,6 +515,7 @@ public:
         p.seed = uint32_t(cfg.getValueInt("seed", 0));
         p.maxdepth = cfg.getValueInt("maxdepth", 0);
         p.rrdepth = cfg.getValueInt("rrdepth", 0);
+        p.env_is = cfg.getValueInt("envmap_is", 0);
         return p;
     }
 

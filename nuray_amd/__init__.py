@@ -41,7 +41,7 @@ class nr_params(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("spp_offset", C.c_int32),
                 ("seed", C.c_uint32), ("prr", C.c_float), ("mis", C.c_int32), ("maxdepth", C.c_int32),
                 ("rrdepth", C.c_int32), ("batch_spp", C.c_int32), ("accumulate", C.c_int32),
-                ("integrator", C.c_int32)]
+                ("integrator", C.c_int32), ("env_is", C.c_int32)]
 
 
 class nr_nrc_params(C.Structure):
@@ -219,8 +219,9 @@ class Scene:
         return tri, t, b1, b2
 
     def render(self, cam, W, H, spp, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0, spp_offset=0, batch_spp=0,
-               accumulate=False, fetch=True, progress=None, cancel=None, integrator=0):
-        p = nr_params(W, H, spp, spp_offset, seed, prr, mis, maxdepth, rrdepth, batch_spp, int(accumulate), integrator)
+               accumulate=False, fetch=True, progress=None, cancel=None, integrator=0, env_is=False):
+        p = nr_params(W, H, spp, spp_offset, seed, prr, mis, maxdepth, rrdepth, batch_spp, int(accumulate), integrator,
+                      int(env_is))
         c = make_camera(cam)
         cb = PROGRESS(lambda f, u: progress(f)) if progress else None
         rc = check(lib().nr_render_ptnee(self.h, C.byref(c), C.byref(p), None, None, None, None,
@@ -252,10 +253,10 @@ class Scene:
         return (loss.value, g) if want_grad else loss.value
 
     def adjoint(self, cam, W, H, spp, n_mats, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0, spp_offset=0,
-                dl_dimg=None, tex_id=-1, tex_shape=None, integrator=0):
+                dl_dimg=None, tex_id=-1, tex_shape=None, integrator=0, env_is=False):
         """Material gradient (n_mats, 4); with tex_id >= 0 also the texel gradient (h, w, 3) of that
         texture (tex_shape = (h, w))."""
-        p = nr_params(W, H, spp, spp_offset, seed, prr, mis, maxdepth, rrdepth, 0, 0, integrator)
+        p = nr_params(W, H, spp, spp_offset, seed, prr, mis, maxdepth, rrdepth, 0, 0, integrator, int(env_is))
         c = make_camera(cam)
         grad = np.zeros(n_mats * 4, np.float64)
         dl = None if dl_dimg is None else _f32(dl_dimg).reshape(-1)

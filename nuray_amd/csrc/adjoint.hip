@@ -419,11 +419,11 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
                    int tex_id, double* gtex_host) {
     if (!sc.built) throw std::runtime_error("scene not built");
     if (p.W <= 0 || p.H <= 0 || p.spp <= 0) throw std::runtime_error("bad image size or spp");
+    if (p.env_is) throw std::runtime_error("the adjoint does not replay environment sampling");
     if (!sc.target) sc.target = new RenderTarget();
     RenderTarget& t = *sc.target;
     // the stored per-sample radiance serves when it belongs to exactly these samples
-    const bool stored = t.adj_valid && t.W == p.W && t.H == p.H && t.adj_spp == p.spp && t.adj_seed == p.seed &&
-                        t.adj_spp0 == p.spp_offset;
+    const bool stored = t.adj_matches(cam, p);
     const size_t n3 = size_t(p.W) * p.H * 3;
     if (dldi_host) {
         t.adj_dldi.ensure(n3 * sizeof(float));

@@ -203,16 +203,13 @@ int nr_intersect(nr_scene* s, const float* o, const float* d, uint32_t n, int32_
     });
 }
 
+static nr::RenderParams to_params(const nr_params* pp);
+
 int nr_render_ptnee(nr_scene* s, const nr_camera* cam, const nr_params* pp, float* rgb_linear, double* sum,
                     double* sum2, uint8_t* rgb8, nr_progress_fn progress, void* user, volatile int32_t* cancel) {
     return guard([&] {
         nr::CamView cv = nr::make_camera(cam->pos, cam->euler, cam->fov, cam->aspect);
-        nr::RenderParams p;
-        p.W = pp->width; p.H = pp->height; p.spp = pp->spp; p.spp_offset = pp->spp_offset;
-        p.seed = pp->seed; p.prr = pp->prr; p.mis = pp->mis; p.maxdepth = pp->maxdepth; p.rrdepth = pp->rrdepth;
-        p.batch_spp = pp->batch_spp; p.accumulate = pp->accumulate;
-    p.integrator = pp->integrator;
-    if (p.integrator != 0 && p.integrator != 1) throw std::runtime_error("integrator must be 0 (ptnee) or 1 (pt)");
+        const nr::RenderParams p = to_params(pp);
         nr::RenderStats rs;
         nr::TraceStats ts[2] = {};
         nr::KernelTimes kt;
@@ -271,6 +268,8 @@ static nr::RenderParams to_params(const nr_params* pp) {
     p.batch_spp = pp->batch_spp; p.accumulate = pp->accumulate;
     p.integrator = pp->integrator;
     if (p.integrator != 0 && p.integrator != 1) throw std::runtime_error("integrator must be 0 (ptnee) or 1 (pt)");
+    p.env_is = pp->env_is;
+    if (p.env_is != 0 && p.env_is != 1) throw std::runtime_error("env_is must be 0 or 1");
     return p;
 }
 

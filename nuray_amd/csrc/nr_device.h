@@ -78,6 +78,12 @@ struct SceneView {
     int32_t env_tex;   // texture index of the environment map or -1
     float env_gain;
     int32_t stack_cap; // traversal stack entries per lane
+    // environment sampler: one cell per texel square of the map (env_w x env_h cells, 0 if none),
+    // {alias probability, alias cell (int bits), pdf of the cell, pdf of the alias cell}; the
+    // solid-angle density of a direction in cell k is pdf_k * env_norm / sin(theta)
+    const float4* env_tab;
+    int32_t env_w, env_h;
+    float env_norm;
 };
 
 // Queued ray: origin + tmax / shadow length, direction + payload.

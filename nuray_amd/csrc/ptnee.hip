@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstring>
 #include <vector>
+#include "nr_envis.h"
 #include "nr_material.h"
 #include "nrc.h"
 #include "render.h"
@@ -116,7 +117,7 @@ __device__ __forceinline__ F3 camera_dir(const CamView& cam, int W, int H, int p
 }
 
 __global__ void k_raygen(CamView cam, int W, int H, int npix, int P, uint32_t seed, uint32_t sample0, Ray* rays,
-                         float4* beta, float4* radiance) {
+                         float4* beta, float4* radiance, float4* radiance_env) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= P) return;
     const int pixel = slot % npix;
@@ -125,6 +126,7 @@ __global__ void k_raygen(CamView cam, int W, int H, int npix, int P, uint32_t se
     rays[slot].d = make_float4(d.x, d.y, d.z, __int_as_float(slot));
     beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
     radiance[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (radiance_env) radiance_env[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // Sample index of training path i of a frame: its own Philox counters, disjoint from the render
@@ -195,6 +197,10 @@ struct ShadeArgs {
     Ray* shadow;
     float4* contrib;
     int* n_shadow;
+    // sky sampling (ENV kernels): BSDF-escape weight per path slot, and the slot offset of the
+    // radiance region that sky shadow rays add to (one writer per slot per shadow launch)
+    float* skyw;
+    int env_base;
 };
 
 constexpr int NWAVE = BLK / 64;
@@ -273,7 +279,7 @@ __device__ __forceinline__ int pick_light_lds(const float* cdf, int n, float u) 
     return lo < n ? lo : n - 1;
 }
 
-template <unsigned F, bool NRC>
+template <unsigned F, bool NRC, bool ENV = false>
 __device__ __forceinline__ void shade_body(const ShadeArgs& a, const NrcShade& nrc) {
     __shared__ float s_cdf[LDS_LIGHTS];
     const SceneView& sv = a.sv;
@@ -289,9 +295,9 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a, const NrcShade& n
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride, parity ^= 1) {
         const int j = base + int(threadIdx.x);
         const bool active = j < n;
-        bool emit_sh = false, emit_next = false, emit_q = false, emit_rec = false;
-        Ray sh_ray, next_ray;
-        float4 sh_c = make_float4(0, 0, 0, 0), next_b = make_float4(0, 0, 0, 0);
+        bool emit_sh = false, emit_next = false, emit_q = false, emit_rec = false, emit_env = false;
+        Ray sh_ray, next_ray, env_ray;
+        float4 sh_c = make_float4(0, 0, 0, 0), next_b = make_float4(0, 0, 0, 0), env_c = make_float4(0, 0, 0, 0);
         float4 ns = make_float4(0, 0, 0, 0), rec_l = make_float4(0, 0, 0, 0);
         NrcQuery qrec;
         if (active) {
@@ -307,6 +313,7 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a, const NrcShade& n
             F3 Le = f3(0.0f, 0.0f, 0.0f);  // radiance this vertex adds directly
             if (h.prim < 0) {
                 Le = mulv(beta, env_radiance(sv, d));
+                if (ENV && depth > 0) Le = mul(Le, a.skyw[slot]);   // BSDF-escape MIS weight
             } else {
                 const TriShade ts = sv.shade[h.prim];
                 const float b1 = h.b1, b2 = h.b2, w0 = 1.0f - b1 - b2;
@@ -383,30 +390,55 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a, const NrcShade& n
                         ke = f3(lr.ke.x, lr.ke.y, lr.ke.z);
                     }
                     // Russian roulette and the continuation direction (rendererptls.cpp:80-88)
-                    bool do_next = !stop;
+                    bool can_next = !stop;  // the BSDF strategy exists at this vertex (before roulette)
+                    if (a.maxdepth > 0 && depth + 1 >= a.maxdepth) can_next = false;
+                    bool do_next = can_next;
                     float prr = 1.0f;
                     if (depth >= a.rrdepth) {
                         do_next = do_next && !(ua.w > a.prr);
                         prr = a.prr;
                     }
-                    if (a.maxdepth > 0 && depth + 1 >= a.maxdepth) do_next = false;
                     F3 wi = f3(0, 0, 0);
                     if (do_next) {
                         const float4 ub = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 1u);
                         wi = bsdf_sample<F>(m, ub.x, ub.y, ub.z, wo, nrm);
                     }
-                    // f and pdf for both directions through one copy of the material code
-                    F3 f_l = f3(0, 0, 0), f_i = f3(0, 0, 0);
-                    float p_l = 1.0f, p_i = 1.0f;
+                    // sky sample (its own random dimension, so renders without it draw the same numbers)
+                    const bool env_vertex = ENV && can_next && sv.env_w > 0 && env_sampling_vertex(m);
+                    bool do_env = false;
+                    F3 we = f3(0, 0, 0);
+                    float pe = 0.0f;
+                    if (env_vertex) {
+                        const float4 ue = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 2u);
+                        do_env = env_sample(sv, ue, we, pe);
+                    }
+                    // f and pdf for every direction through one copy of the material code
+                    F3 f_l = f3(0, 0, 0), f_i = f3(0, 0, 0), f_e = f3(0, 0, 0);
+                    float p_l = 1.0f, p_i = 1.0f, p_e = 1.0f;
 #pragma unroll 1
-                    for (int k = 0; k < 2; ++k) {
-                        const bool act = k == 0 ? do_nee : do_next;
+                    for (int k = 0; k < (ENV ? 3 : 2); ++k) {
+                        const bool act = k == 0 ? do_nee : (k == 1 ? do_next : do_env);
                         if (!act) continue;
-                        const F3 dir = k == 0 ? wl : wi;
+                        const F3 dir = k == 0 ? wl : (k == 1 ? wi : we);
                         const F3 fr = bsdf_eval<F>(sv, m, wo, nrm, dir, tu, tv);
                         const float pr = bsdf_pdf<F>(m, wo, nrm, dir);
                         if (k == 0) f_l = fr, p_l = pr;
-                        else f_i = fr, p_i = pr;
+                        else if (k == 1) f_i = fr, p_i = pr;
+                        else f_e = fr, p_e = pr;
+                    }
+                    if (ENV && do_env) {  // g(we) L / (p_env + q), g = q / p_rep * f |cos|
+                        const float qe = bsdf_sample_density(m, wo, nrm, we);
+                        if (qe > 0.0f) {
+                            const float w = qe / p_e * fabsf(dot(we, nrm)) / (pe + qe);
+                            const F3 c = mul(mulv(mulv(beta, f_e), env_radiance(sv, we)), w);
+                            if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
+                                emit_env = true;
+                                const F3 eo = add(hp, mul(we, 1e-3f));
+                                env_ray.o = make_float4(eo.x, eo.y, eo.z, 1e18f);   // blocked by any hit
+                                env_ray.d = make_float4(we.x, we.y, we.z, __int_as_float(a.env_base + slot));
+                                env_c = make_float4(c.x, c.y, c.z, 0.0f);
+                            }
+                        }
                     }
                     float fac_next = 1.0f;
                     if (do_nee) {  // rendererptls.cpp:61-77
@@ -436,6 +468,15 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a, const NrcShade& n
                             next_ray.d = make_float4(wi.x, wi.y, wi.z, __int_as_float(slot));
                             next_b = make_float4(nb.x, nb.y, nb.z, fac_next);
                             if (NRC) ns.z = p_i;
+                            if (ENV) {
+                                float sw = 1.0f;
+                                if (env_vertex) {
+                                    const float qi = bsdf_sample_density(m, wo, nrm, wi);
+                                    const float pi_env = env_pdf(sv, wi);
+                                    sw = qi + pi_env > 0.0f ? qi / (qi + pi_env) : 1.0f;
+                                }
+                                a.skyw[slot] = sw;
+                            }
                         }
                     }
                 }
@@ -467,6 +508,23 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a, const NrcShade& n
                 nrc.recs[pos[3]] = qrec;
                 nrc.rec_lb[pos[3]] = rec_l;
             }
+        } else if (ENV) {  // light and sky shadow rays share the shadow queue
+            const bool e[4] = {emit_sh, emit_next, emit_env, false};
+            int* const c[4] = {a.n_shadow, a.n_out, a.n_shadow, a.n_shadow};
+            int pos[4];
+            block_append4(s_app4, parity, e, c, pos);
+            if (emit_sh) {
+                a.shadow[pos[0]] = sh_ray;
+                a.contrib[pos[0]] = sh_c;
+            }
+            if (emit_next) {
+                a.rout[pos[1]] = next_ray;
+                a.bout[pos[1]] = next_b;
+            }
+            if (emit_env) {
+                a.shadow[pos[2]] = env_ray;
+                a.contrib[pos[2]] = env_c;
+            }
         } else {
             int ps, pn;
             block_append2(s_app, parity, emit_sh, emit_next, a.n_shadow, a.n_out, ps, pn);
@@ -487,19 +545,28 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     shade_body<0u, false>(a, NrcShade{});
 }
 __global__ __launch_bounds__(BLK) void k_shade_full(ShadeArgs a) { shade_body<F_ALL, false>(a, NrcShade{}); }
+// With sky sampling.
+__global__ __launch_bounds__(BLK) void k_shade_basic_env(ShadeArgs a) { shade_body<0u, false, true>(a, NrcShade{}); }
+__global__ __launch_bounds__(BLK) void k_shade_full_env(ShadeArgs a) { shade_body<F_ALL, false, true>(a, NrcShade{}); }
 // Radiance-cache rendering (spread state, cache queries, training records).
 __global__ __launch_bounds__(BLK) void k_shade_nrc_basic(ShadeArgs a, NrcShade n) { shade_body<0u, true>(a, n); }
 __global__ __launch_bounds__(BLK) void k_shade_nrc_full(ShadeArgs a, NrcShade n) { shade_body<F_ALL, true>(a, n); }
 
 // Per-pixel reduction of the batch's kb samples: sums of max(0, L) and its square in FP64
 // (rendererptls.cpp:173-175: max(0, trace(...)) per component, which also maps NaN to 0).
-__global__ void k_finalize(const float4* __restrict__ radiance, int npix, int kb, double* __restrict__ sum,
-                           double* __restrict__ sum2) {
+__global__ void k_finalize(const float4* __restrict__ radiance, const float4* __restrict__ radiance_env, int npix,
+                           int kb, double* __restrict__ sum, double* __restrict__ sum2) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npix) return;
     double s[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
     for (int k = 0; k < kb; ++k) {
-        const float4 L = radiance[size_t(k) * npix + p];
+        float4 L = radiance[size_t(k) * npix + p];
+        if (radiance_env) {  // sky shadow-ray contributions kept apart during the wavefront loop
+            const float4 E = radiance_env[size_t(k) * npix + p];
+            L.x += E.x;
+            L.y += E.y;
+            L.z += E.z;
+        }
         const float v[3] = {0.0f < L.x ? L.x : 0.0f, 0.0f < L.y ? L.y : 0.0f, 0.0f < L.z ? L.z : 0.0f};
         for (int c = 0; c < 3; ++c) {
             s[c] += double(v[c]);
@@ -541,10 +608,20 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
     if (!sc.target) sc.target = new RenderTarget();
     RenderTarget& t = *sc.target;
     t.ensure(p.W, p.H, size_t(kb) * npix);
-    t.adj_valid = kb >= p.spp && !p.accumulate;
-    t.adj_spp = p.spp;
-    t.adj_spp0 = p.spp_offset;
-    t.adj_seed = p.seed;
+    // sky sampling: a second radiance region and shadow-queue half for the sky rays, the
+    // BSDF-escape weights; without an environment sampler (no map, or a black one) it is a no-op
+    const bool env = p.env_is && sc.view.env_w > 0;
+    if (p.env_is && p.integrator != 0) throw std::runtime_error("environment sampling needs the ptnee integrator");
+    if (env) {
+        const size_t c = size_t(kb) * npix;
+        t.radiance.ensure(sizeof(float4) * 2 * c);
+        t.shadow.ensure(sizeof(Ray) * 2 * c);
+        t.contrib.ensure(sizeof(float4) * 2 * c);
+        t.skyw.ensure(sizeof(float) * c);
+    }
+    t.adj_valid = kb >= p.spp && !p.accumulate && !p.env_is;
+    t.adj_p = p;
+    t.adj_cam = cam;
     hipStream_t s = sc.stream;
     int* cnt = t.counters.as<int>();
     TraceStats* ts = trace_stats ? t.stats.as<TraceStats>() : nullptr;
@@ -571,9 +648,10 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
         const int k = std::min(kb, p.spp - b);
         const int P = k * npix;
         hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, int(C_RAY0), P, -1);
+        float4* const rad_env = env ? t.radiance.as<float4>() + P : nullptr;
         hipLaunchKernelGGL(k_raygen, dim3((P + BLK - 1) / BLK), dim3(BLK), 0, s, cam, p.W, p.H, npix, P, p.seed,
                            uint32_t(p.spp_offset + b), t.rays[0].as<Ray>(), t.beta[0].as<float4>(),
-                           t.radiance.as<float4>());
+                           t.radiance.as<float4>(), rad_env);
         for (int it = 0;; ++it) {
             const int cur = it & 1, nxt = cur ^ 1;
             hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, -1, 0, int(C_RAY0 + nxt));
@@ -585,10 +663,17 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
                         p.integrator == 1 ? 0 : 1,
                         t.rays[cur].as<Ray>(), t.beta[cur].as<float4>(), t.hits.as<HitRec>(), cnt + C_RAY0 + cur,
                         t.radiance.as<float4>(), t.rays[nxt].as<Ray>(), t.beta[nxt].as<float4>(), cnt + C_RAY0 + nxt,
-                        t.shadow.as<Ray>(), t.contrib.as<float4>(), cnt + C_SHADOW};
+                        t.shadow.as<Ray>(), t.contrib.as<float4>(), cnt + C_SHADOW,
+                        env ? t.skyw.as<float>() : nullptr, env ? P : 0};
             if (kt) mark(ev_h);
-            if (sc.features == 0) hipLaunchKernelGGL(k_shade_basic, dim3(grid_shade), dim3(BLK), 0, s, a);
-            else hipLaunchKernelGGL(k_shade_full, dim3(grid_shade), dim3(BLK), 0, s, a);
+            if (env) {
+                if (sc.features == 0) hipLaunchKernelGGL(k_shade_basic_env, dim3(grid_shade), dim3(BLK), 0, s, a);
+                else hipLaunchKernelGGL(k_shade_full_env, dim3(grid_shade), dim3(BLK), 0, s, a);
+            } else if (sc.features == 0) {
+                hipLaunchKernelGGL(k_shade_basic, dim3(grid_shade), dim3(BLK), 0, s, a);
+            } else {
+                hipLaunchKernelGGL(k_shade_full, dim3(grid_shade), dim3(BLK), 0, s, a);
+            }
             if (kt) mark(ev_h);
             if (kt) mark(ev_s);
             launch_shadow(sc.view, t.shadow.as<Ray>(), cnt + C_SHADOW, nullptr, nullptr, t.contrib.as<float4>(),
@@ -605,8 +690,8 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
             }
         }
         if (status) break;
-        hipLaunchKernelGGL(k_finalize, dim3((npix + BLK - 1) / BLK), dim3(BLK), 0, s, t.radiance.as<float4>(), npix,
-                           k, t.sum.as<double>(), t.sum2.as<double>());
+        hipLaunchKernelGGL(k_finalize, dim3((npix + BLK - 1) / BLK), dim3(BLK), 0, s, t.radiance.as<float4>(), rad_env,
+                           npix, k, t.sum.as<double>(), t.sum2.as<double>());
         check_hip(hipGetLastError(), "ptnee launch");
         t.spp_total += k;
         rs.samples += (long long)P;
@@ -646,6 +731,7 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
     if (!sc.built) throw std::runtime_error("scene not built (call nr_bvh_build_sah)");
     if (p.W <= 0 || p.H <= 0 || p.spp <= 0) throw std::runtime_error("bad image size or spp");
     if (np.train_paths < 0 || np.train_maxlen < 1) throw std::runtime_error("bad radiance-cache parameters");
+    if (p.env_is) throw std::runtime_error("environment sampling is not available with the radiance cache");
     const int npix = p.W * p.H;
     const size_t P = size_t(p.spp) * npix, T = size_t(np.train_paths);
     if (P + T > (size_t(64) << 20)) throw std::runtime_error("radiance-cache frame larger than one wavefront batch");
@@ -683,7 +769,7 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
                     p.integrator == 1 ? 0 : 1,
                     t.rays[cur].as<Ray>(), t.beta[cur].as<float4>(), t.hits.as<HitRec>(), cnt + C_RAY0 + cur,
                     t.radiance.as<float4>(), t.rays[nxt].as<Ray>(), t.beta[nxt].as<float4>(), cnt + C_RAY0 + nxt,
-                    t.shadow.as<Ray>(), t.contrib.as<float4>(), cnt + C_SHADOW};
+                    t.shadow.as<Ray>(), t.contrib.as<float4>(), cnt + C_SHADOW, nullptr, 0};
         NrcShade ns{t.nstate[cur].as<float4>(), t.nstate[nxt].as<float4>(), t.nrc_queries.as<NrcQuery>(),
                     t.nrc_recs.as<NrcQuery>(), t.nrc_lb.as<float4>(), cnt + C_QUERY, cnt + C_REC, int(cap),
                     int(rec_cap), int(P), int(T), np.c_spread, np.train_maxlen, np.use_cache,
@@ -703,7 +789,7 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
     }
     // the cache answers every stopped path (render paths and training-path tails)
     nrc.infer_accumulate(t.nrc_queries.as<NrcQuery>(), cnt + C_QUERY, int(cap), t.radiance.as<float4>(), s);
-    hipLaunchKernelGGL(k_finalize, dim3((npix + BLK - 1) / BLK), dim3(BLK), 0, s, t.radiance.as<float4>(), npix,
+    hipLaunchKernelGGL(k_finalize, dim3((npix + BLK - 1) / BLK), dim3(BLK), 0, s, t.radiance.as<float4>(), nullptr, npix,
                        p.spp, t.sum.as<double>(), t.sum2.as<double>());
     check_hip(hipGetLastError(), "nrc frame launch");
     t.spp_total += p.spp;

@@ -1,5 +1,6 @@
 // Wavefront ptnee renderer: buffers and host orchestration.
 #pragma once
+#include <cstring>
 #include "scene.h"
 #include "trace.h"
 
@@ -23,6 +24,7 @@ struct RenderParams {
     int batch_spp = 0;  // samples per pixel per wavefront batch (0: auto)
     int accumulate = 0; // keep previous sums instead of clearing them
     int integrator = 0; // 0: RendererPTLS (ptnee), 1: RendererPT (pt, no light sampling)
+    int env_is = 0;     // 1: sample the sky at Lambert / GGX-reflect vertices (same expected image)
 };
 
 struct RenderStats {
@@ -43,20 +45,29 @@ struct RenderTarget {
     DevBuf sum, sum2;                  // double, W*H*3
     DevBuf rays[2], beta[2];           // extension-ray queues and their (throughput, MIS factor)
     DevBuf hits, shadow, contrib;      // closest hits; shadow rays and their NEE contributions
-    DevBuf radiance;                   // float4 per path slot
+    DevBuf radiance;                   // float4 per path slot (x2 with sky sampling: sky rays' region)
+    DevBuf skyw;                       // sky sampling: BSDF-escape MIS weight per path slot
     DevBuf counters;                   // int[16]
     DevBuf stats;                      // TraceStats[2]
     int* pinned = nullptr;             // host-visible counter readback
     long long spp_total = 0;
     // adjoint: the per-sample radiance of the last render is replayable when it ran as one batch
     bool adj_valid = false;
-    int adj_spp = 0, adj_spp0 = 0;
-    uint32_t adj_seed = 0;
+    RenderParams adj_p;   // the render those samples came from
+    CamView adj_cam{};
     DevBuf adj_target, adj_dldi, adj_loss, adj_grad, adj_gtex;
     // radiance-cache rendering: spread state per queued ray, cache queries, training vertices
     DevBuf nstate[2], nrc_queries, nrc_recs, nrc_lb, nrc_tgt, nrc_batch, nrc_batch_tgt;
     ~RenderTarget();
     void ensure(int W, int H, size_t cap);
+    // the stored per-sample radiance belongs to exactly the samples of (cam, p)
+    bool adj_matches(const CamView& cam, const RenderParams& p) const {
+        const RenderParams& q = adj_p;
+        return adj_valid && W == p.W && H == p.H && q.W == p.W && q.H == p.H && q.spp == p.spp &&
+               q.spp_offset == p.spp_offset && q.seed == p.seed && q.prr == p.prr && q.mis == p.mis &&
+               q.maxdepth == p.maxdepth && q.rrdepth == p.rrdepth && q.integrator == p.integrator &&
+               std::memcmp(&adj_cam, &cam, sizeof(CamView)) == 0;
+    }
 };
 
 // RendererNRC::render parameters (renderernrc.cpp:185-281), with the spread rule of the

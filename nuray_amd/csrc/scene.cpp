@@ -1,5 +1,6 @@
 // Host scene assembly and upload.
 #include "scene.h"
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -99,6 +100,7 @@ int Scene::add_texture(int id, int w, int h, int fmt, const void* data) {
         check_hip(hipMemcpyAsync(static_cast<uint8_t*>(d_texpool.ptr) + tex_offsets[id], t.bytes.data(), bytes,
                                  hipMemcpyHostToDevice, stream), "texture update");
         check_hip(hipStreamSynchronize(stream), "texture update sync");
+        if (id == env_tex) build_env_sampler();
         return id;
     }
     built = false;
@@ -228,7 +230,88 @@ void Scene::build(int nthreads) {
     view.env_tex = env_tex;
     view.env_gain = env_gain;
     view.stack_cap = bvh.depth + 2;
+    build_env_sampler();
     built = true;
+}
+
+// Environment sampler.  The lookup (envmap.cpp:17-24, texture.cpp:54-72) maps a direction to the
+// continuous texel coordinate x = u W, y = (1 - v) H with u = atan2(z, x) / (2 pi') + 1/2 and
+// 1 - v = acos(y) / pi' (pi' = 3.14159), and blends texels floor(x), floor(x) + 1 (a texel past
+// the last one reads black).  Cell (i, j) = [j, j+1) x [i, i+1) therefore sees at most the largest
+// of its four corner texels; its weight is that bound (largest channel, times |gain|) times
+// sin(theta) at the cell centre, so that the density in solid angle follows the radiance.  A cell
+// is drawn by Vose's alias method (built in double, in the order below), then a point uniformly
+// inside it; the direction's solid-angle density is pdf_cell * W H / (2 pi'^2 sin theta).
+void Scene::build_env_sampler() {
+    env_tab.clear();
+    view.env_tab = nullptr;
+    view.env_w = view.env_h = 0;
+    view.env_norm = 0.0f;
+    if (env_tex < 0 || env_tex >= int(textures.size())) return;
+    const Texture& t = textures[env_tex];
+    const int W = t.w, H = t.h;
+    if (W <= 0 || H <= 0) return;
+    float lut[256];
+    for (int c = 0; c < 256; ++c) lut[c] = float(std::pow(double(c) / 255.0, double(2.2f)));
+    // largest channel magnitude of every texel (0 outside the map)
+    std::vector<float> mx(size_t(W) * H);
+    for (size_t k = 0; k < mx.size(); ++k) {
+        float m = 0.0f;
+        for (int c = 0; c < 3; ++c) {
+            float v;
+            if (t.fmt == 0) v = lut[t.bytes[4 * k + c]];
+            else std::memcpy(&v, &t.bytes[12 * k + 4 * c], 4);
+            m = std::max(m, std::fabs(v));
+        }
+        mx[k] = m;
+    }
+    auto at = [&](int x, int y) { return (x < W && y < H) ? mx[size_t(y) * W + x] : 0.0f; };
+    const size_t N = size_t(W) * H;
+    std::vector<double> w(N);
+    double sum = 0.0;
+    const double gain = std::fabs(double(env_gain));
+    for (int i = 0; i < H; ++i) {
+        const double st = std::sin((i + 0.5) / H * double(3.14159f));
+        for (int j = 0; j < W; ++j) {
+            const float c = std::max(std::max(at(j, i), at(j + 1, i)), std::max(at(j, i + 1), at(j + 1, i + 1)));
+            const double x = double(c) * gain * st;
+            w[size_t(i) * W + j] = x;
+            sum += x;
+        }
+    }
+    if (!(sum > 0.0) || !std::isfinite(sum)) return;   // black map: the sky is never sampled
+    std::vector<double> sc(N);
+    std::vector<float> prob(N, 1.0f);
+    std::vector<int32_t> alias(N);
+    std::vector<int32_t> small, large;
+    for (size_t k = 0; k < N; ++k) {
+        sc[k] = w[k] / sum * double(N);
+        alias[k] = int32_t(k);
+        (sc[k] < 1.0 ? small : large).push_back(int32_t(k));
+    }
+    while (!small.empty() && !large.empty()) {
+        const int32_t l = small.back();
+        small.pop_back();
+        const int32_t g = large.back();
+        large.pop_back();
+        prob[l] = float(sc[l]);
+        alias[l] = g;
+        sc[g] = (sc[g] + sc[l]) - 1.0;
+        (sc[g] < 1.0 ? small : large).push_back(g);
+    }
+    env_tab.resize(N);
+    for (size_t k = 0; k < N; ++k) {
+        int32_t a = alias[k];
+        float af;
+        std::memcpy(&af, &a, 4);
+        env_tab[k] = make_float4(prob[k], af, float(w[k] / sum), float(w[size_t(a)] / sum));
+    }
+    d_env.upload(env_tab.data(), env_tab.size() * sizeof(float4), stream);
+    check_hip(hipStreamSynchronize(stream), "env sampler upload");
+    view.env_tab = d_env.as<float4>();
+    view.env_w = W;
+    view.env_h = H;
+    view.env_norm = float(double(W) * H / (2.0 * double(3.14159f) * double(3.14159f)));
 }
 
 }  // namespace nr

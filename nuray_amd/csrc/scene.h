@@ -52,7 +52,8 @@ public:
     float light_sum = 0.0f;
 
     // device mirror
-    DevBuf d_nodes, d_tris, d_shade, d_mats, d_lights, d_cdf, d_tex, d_texpool, d_lut;
+    DevBuf d_nodes, d_tris, d_shade, d_mats, d_lights, d_cdf, d_tex, d_texpool, d_lut, d_env;
+    std::vector<float4> env_tab;  // environment sampler cells (SceneView::env_tab)
     SceneView view{};
     hipStream_t stream = nullptr;
     int device = 0;
@@ -67,6 +68,9 @@ public:
     std::vector<MatRec> mat_records() const;  // device material table
     // Host SAH build (reference rule), light sampler, upload of every device array.
     void build(int nthreads);
+    // Alias table over the environment map's cells (empty without a map or with a black one);
+    // uploads it and points view.env_tab at it.
+    void build_env_sampler();
 };
 
 }  // namespace nr

@@ -66,6 +66,7 @@ void orc_scene_set_envmap(void* h, int w, int hh, const float* rgb, float gain) 
     e.rgb.assign(rgb, rgb + size_t(w) * hh * 3);
     S(h)->sc.env_gain = gain;
     S(h)->sc.has_env = true;
+    S(h)->sc.env_sampler.build(e, gain);
 }
 
 // RGBA8 texture, decoded like Texture::load (texture.cpp:5-31).
@@ -158,7 +159,7 @@ void orc_trav_stats(void* h, int n, const float* o, const float* d, uint64_t* ou
 // integrator: 0 = RendererPTLS (ptnee), 1 = RendererPT (pt, rendererpt.cpp:13-61).
 int orc_render(void* h, const float* cam, int W, int H, int x0, int y0, int x1, int y1, int spp0,
                int spp, uint32_t seed, float prr, int mis, int maxdepth, int rrdepth, int fp64,
-               int nthreads, double* sum, double* sum2, int integrator) {
+               int nthreads, double* sum, double* sum2, int integrator, int env_is) {
     OScene* o = S(h);
     if (!o->built) return -1;
     Camera c;
@@ -169,6 +170,7 @@ int orc_render(void* h, const float* cam, int W, int H, int x0, int y0, int x1, 
     RenderOpts opt;
     opt.prr = prr; opt.mis = mis; opt.maxdepth = maxdepth; opt.rrdepth = rrdepth; opt.seed = seed;
     opt.nee = integrator == 1 ? 0 : 1;
+    opt.env_is = env_is;
     const int B = 8;
     int tx = (x1 - x0 + B - 1) / B, ty = (y1 - y0 + B - 1) / B;
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -249,6 +251,32 @@ void orc_material_eval(void* h, int m, int n, const float* wo, const float* nn, 
         f_out[3 * i] = f.x; f_out[3 * i + 1] = f.y; f_out[3 * i + 2] = f.z;
         pdf_out[i] = me.pdf(WO, N, WI);
     }
+}
+
+// Sky sampler of the optional environment sampling: n directions from n x 4 uniforms (FP32 mode),
+// ok[i] = 0 for a rejected (polar) sample; and the density at n given directions.
+void orc_env_sample(void* h, int n, const float* u4, float* d_out, float* pdf_out, int* ok) {
+    const EnvSampler& es = S(h)->sc.env_sampler;
+    for (int i = 0; i < n; ++i) {
+        V3f d(0.0f);
+        float p = 0.0f;
+        ok[i] = es.ready() && es.sample(u4 + 4 * i, d, p) ? 1 : 0;
+        d_out[3 * i] = d.x; d_out[3 * i + 1] = d.y; d_out[3 * i + 2] = d.z;
+        pdf_out[i] = p;
+    }
+}
+void orc_env_density(void* h, int n, const double* d, double* out) {
+    const EnvSampler& es = S(h)->sc.env_sampler;
+    for (int i = 0; i < n; ++i) out[i] = es.ready() ? es.density(V3d(d[3 * i], d[3 * i + 1], d[3 * i + 2])) : 0.0;
+}
+// True density of material m's sampler at wi (FP64).
+void orc_material_sample_density(void* h, int m, int n, const double* wo, const double* nn, const double* wi,
+                                 double* out) {
+    OScene* o = S(h);
+    MatEval<double> me{&o->sc, &o->sc.mats[m]};
+    for (int i = 0; i < n; ++i)
+        out[i] = me.sample_density(V3d(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), V3d(nn[3 * i], nn[3 * i + 1], nn[3 * i + 2]),
+                                   V3d(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]));
 }
 
 int orc_light_pick(void* h, float u) { return S(h)->sc.pick_light(u); }

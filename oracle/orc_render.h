@@ -46,6 +46,7 @@ struct RenderOpts {
     int rrdepth = 0;    // Russian roulette applies at vertices with index >= rrdepth (reference: 0)
     uint32_t seed = 0;
     int nee = 1;        // 0: RendererPT (rendererpt.cpp:13-61): no light sampling, emitters at full weight
+    int env_is = 0;     // 1: sky sampling at Lambert / GGX-reflect vertices (same expected image)
 };
 
 // Counters of the instrumented traversal: node-pair fetches and triangle tests.
@@ -68,11 +69,13 @@ struct Tracer {
         return V3<T>(sc.env.uv(float(u), float(v))) * T(sc.env_gain);
     }
 
-    // RendererPTLS::trace, rendererptls.cpp:13-93 (recursive, one vertex per call).
-    V3<T> trace(const V3<T>& o, const V3<T>& d, T fac, int depth, const Stream& rng) const {
+    // RendererPTLS::trace, rendererptls.cpp:13-93 (recursive, one vertex per call).  skyw is the
+    // balance-heuristic weight of a BSDF-sampled escape when the previous vertex also sampled the
+    // sky (1 otherwise).
+    V3<T> trace(const V3<T>& o, const V3<T>& d, T fac, int depth, const Stream& rng, T skyw = T(1)) const {
         T t, b1, b2;
         int ti;
-        if (!hit(o, d, t, b1, b2, ti)) return env(d);
+        if (!hit(o, d, t, b1, b2, ti)) return env(d) * skyw;
         const Tri& tri = sc.tris[ti];
         const Material& M = sc.mats[tri.mat];
         MatEval<T> mat{&sc, &M};
@@ -114,18 +117,47 @@ struct Tracer {
                 }
             }
         }
+        // Sky sampling: the reference's sky term at this vertex is the integral of g L V with
+        // g = q / p_rep * f |cos| (q the sampler's true density, p_rep the reported pdf); the sky
+        // sample adds g L V / (p_env + q), the BSDF escape is weighted by q / (q + p_env).
+        const bool can_next = !(opt.maxdepth > 0 && depth + 1 >= opt.maxdepth);
+        const bool env_vertex = opt.env_is && opt.nee && sc.env_sampler.ready() && can_next &&
+                                (M.type == MAT_LAMBERT || M.type == MAT_GGXREFL);
+        if (env_vertex) {
+            float ue[4];
+            rng.draw(uint32_t(1 + depth), 2, ue);
+            V3<T> we;
+            T pe = 0;
+            if (sc.env_sampler.sample(ue, we, pe)) {
+                const T qe = mat.sample_density(wo, n, we);
+                if (qe > 0) {
+                    T st, sb1, sb2;
+                    int sti;
+                    if (!hit(hp + we * T(1e-3), we, st, sb1, sb2, sti)) {
+                        const V3<T> f = mat.f(wo, n, we, uv);
+                        const T pr = mat.pdf(wo, n, we);
+                        result += f * env(we) * (qe / pr * std::fabs(we.dot(n)) / (pe + qe));
+                    }
+                }
+            }
+        }
         T prr = T(1);
         if (depth >= opt.rrdepth) {
             if (u[3] > opt.prr) return result;
             prr = T(opt.prr);
         }
-        if (opt.maxdepth > 0 && depth + 1 >= opt.maxdepth) return result;
+        if (!can_next) return result;
         float v[4];
         rng.draw(uint32_t(1 + depth), 1, v);
         V3<T> wi = mat.sample(v, wo, n);
         T pdf = mat.pdf(wo, n, wi);
         V3<T> f = mat.f(wo, n, wi, uv);
-        V3<T> Li = trace(hp + wi * T(1e-3), wi, fac_next, depth + 1, rng);
+        T sw = T(1);
+        if (env_vertex) {
+            const T qi = mat.sample_density(wo, n, wi), pi_env = sc.env_sampler.density(wi);
+            sw = qi + pi_env > 0 ? qi / (qi + pi_env) : T(1);
+        }
+        V3<T> Li = trace(hp + wi * T(1e-3), wi, fac_next, depth + 1, rng, sw);
         result += Li * std::fabs(wi.dot(n)) * f / pdf / prr;
         return result;
     }

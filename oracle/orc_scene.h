@@ -74,6 +74,97 @@ struct Light {  // one emissive triangle (lightsampler.cpp:13-22)
     int tri;
 };
 
+// Sky sampler of the optional environment sampling (not in the reference: a variance reduction of
+// its estimator that keeps its expected image).  The lookup reads the continuous texel coordinate
+// x = u W, y = (1 - v) H, u = atan2(z, x) / (2 pi') + 1/2, 1 - v = acos(y) / pi' (pi' = 3.14159),
+// blending texels floor(x) and floor(x) + 1 (black past the last one).  Cell (i, j) =
+// [j, j+1) x [i, i+1) is bounded by its four corner texels; its weight is the largest corner
+// channel times |gain| times sin(theta) at the cell centre.  Cells are drawn with Vose's alias
+// method built in double (small and large lists as stacks in index order), then a point uniform in
+// the cell; the solid-angle density is pdf_cell W H / (2 pi'^2 sin theta).
+struct EnvSampler {
+    int w = 0, h = 0;
+    std::vector<float> prob, pdf;
+    std::vector<int> alias;
+    float norm = 0.0f;
+
+    void build(const Texture& t, float gain) {
+        w = h = 0;
+        prob.clear(); pdf.clear(); alias.clear();
+        if (t.w <= 0 || t.h <= 0) return;
+        const size_t N = size_t(t.w) * t.h;
+        auto corner = [&](int x, int y) -> float {
+            if (x >= t.w || y >= t.h) return 0.0f;
+            const float* q = &t.rgb[3 * (size_t(y) * t.w + x)];
+            return std::max(std::fabs(q[0]), std::max(std::fabs(q[1]), std::fabs(q[2])));
+        };
+        std::vector<double> wt(N);
+        double sum = 0;
+        for (int i = 0; i < t.h; ++i) {
+            const double st = std::sin((i + 0.5) / t.h * double(3.14159f));
+            for (int j = 0; j < t.w; ++j) {
+                const float c = std::max(std::max(corner(j, i), corner(j + 1, i)),
+                                         std::max(corner(j, i + 1), corner(j + 1, i + 1)));
+                wt[size_t(i) * t.w + j] = double(c) * std::fabs(double(gain)) * st;
+                sum += wt[size_t(i) * t.w + j];
+            }
+        }
+        if (!(sum > 0) || !std::isfinite(sum)) return;
+        std::vector<double> scaled(N);
+        prob.assign(N, 1.0f);
+        alias.resize(N);
+        pdf.resize(N);
+        std::vector<int> lo, hi;
+        for (size_t k = 0; k < N; ++k) {
+            scaled[k] = wt[k] / sum * double(N);
+            alias[k] = int(k);
+            pdf[k] = float(wt[k] / sum);
+            if (scaled[k] < 1.0) lo.push_back(int(k));
+            else hi.push_back(int(k));
+        }
+        while (!lo.empty() && !hi.empty()) {
+            const int a = lo.back(), b = hi.back();
+            lo.pop_back();
+            hi.pop_back();
+            prob[a] = float(scaled[a]);
+            alias[a] = b;
+            scaled[b] = (scaled[b] + scaled[a]) - 1.0;
+            if (scaled[b] < 1.0) lo.push_back(b);
+            else hi.push_back(b);
+        }
+        w = t.w;
+        h = t.h;
+        norm = float(double(w) * h / (2.0 * double(3.14159f) * double(3.14159f)));
+    }
+    bool ready() const { return w > 0; }
+
+    // Direction and its solid-angle density from four uniforms (cell, alias choice, point in cell).
+    template <class T>
+    bool sample(const float u[4], V3<T>& d, T& density) const {
+        const int N = w * h;
+        int k = std::min(int(u[0] * float(N)), N - 1);
+        if (!(u[1] < prob[k])) k = alias[k];
+        const int i = k / w, j = k - i * w;
+        const T th = (T(i) + T(u[3])) / T(h) * T(3.14159f);
+        const T ph = ((T(j) + T(u[2])) / T(w) - T(0.5f)) * (T(2.0f) * T(3.14159f));
+        const T st = std::sin(th);
+        d = V3<T>(st * std::cos(ph), std::cos(th), st * std::sin(ph));
+        if (!(st > T(1e-7f))) return false;
+        density = T(pdf[k]) * T(norm) / st;
+        return true;
+    }
+    // Density of `sample` at a unit direction (sin theta from x and z: precise near the poles).
+    template <class T>
+    T density(const V3<T>& d) const {
+        const T st = std::sqrt(d.x * d.x + d.z * d.z);
+        if (!(st > T(1e-7f))) return T(0);
+        const T th = std::atan2(st, d.y), ph = std::atan2(d.z, d.x);
+        const int j = std::min(std::max(int(std::floor((ph / (T(2.0f) * T(3.14159f)) + T(0.5f)) * T(w))), 0), w - 1);
+        const int i = std::min(std::max(int(std::floor(th / T(3.14159f) * T(h))), 0), h - 1);
+        return T(pdf[size_t(i) * w + j]) * T(norm) / st;
+    }
+};
+
 struct Scene {
     std::vector<Tri> tris;
     std::vector<Material> mats;
@@ -81,6 +172,7 @@ struct Scene {
     Texture env;
     float env_gain = 1.0f;
     bool has_env = false;
+    EnvSampler env_sampler;   // built with the map (orc_scene_set_envmap)
     // light sampler state (lightsampler.cpp:4-39)
     std::vector<int> light_tris;
     std::vector<float> light_cdf;
@@ -149,6 +241,39 @@ struct MatEval {
     V3<T> specular(const V3<T>& uv) const {
         if (m->tex_ks >= 0) return V3<T>(sc->textures[m->tex_ks].uv(float(uv.x), float(uv.y)));
         return V3<T>(T(m->ks[0]), T(m->ks[1]), T(m->ks[2]));
+    }
+
+    // True solid-angle density of sample() at wi, for the models that take part in the optional
+    // sky sampling (0 otherwise).  Lambert (matlambert.cpp:5-18): r^2 = 0.99f U, so cos^2 is
+    // uniform on [1 - 0.99f, 1]: q = cos / (0.99f pi) for cos^2 > 1 - 0.99f.  GGX-reflect
+    // (matggxrefl.cpp:5-34): theta_h = s atan(alpha sqrt(U/(1-U))), s = 0.9999, phi uniform on
+    // [0, 2 * 3.14159f): theta0 = theta_h / s has density
+    // p0 = 2 a^2 tan(theta0) (1 + tan^2 theta0) / (a^2 + tan^2 theta0)^2, the half vector
+    // q_h = p0 / (s 2 (3.14159f) sin theta_h), the reflection q = q_h / (4 wo.h).
+    T sample_density(const V3<T>& wo, const V3<T>& n, const V3<T>& wi) const {
+        const T pi = T(3.14159265358979);
+        if (m->type == MAT_LAMBERT) {
+            const T c = n.dot(wi);
+            return c > T(0.1f) ? c / (T(0.99f) * pi) : T(0);
+        }
+        if (m->type != MAT_GGXREFL) return T(0);
+        const V3<T> hv = wo + wi;
+        const T hl = std::sqrt(hv.dot(hv));
+        if (!(hl > T(1e-7f))) return T(0);
+        const V3<T> h = hv * (T(1) / hl);
+        const T ch = n.dot(h), woh = wo.dot(h);
+        if (!(ch > 0) || !(woh > 0)) return T(0);
+        const V3<T> cr = n.cross(h);
+        const T sh = std::sqrt(cr.dot(cr));
+        const T s = T(0.9999);
+        const T th0 = std::atan2(sh, ch) / s;
+        if (!(th0 < T(1.5707963f))) return T(0);
+        const T t = std::tan(th0);
+        const T a2 = T(m->alpha) * T(m->alpha);
+        const T k = a2 + t * t;
+        const T tan_over_sin = sh > T(1e-6f) ? t / sh : T(1) / s;
+        const T qh = T(2) * a2 * tan_over_sin * (T(1) + t * t) / (k * k) / (s * T(2.0f) * T(3.14159f));
+        return qh / (T(4) * woh);
     }
 
     // sampleBxdf: matlambert.cpp:5-18, matggxrefl.cpp:5-34, matggx.cpp:5-47.

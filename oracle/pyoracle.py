@@ -47,7 +47,10 @@ def lib():
         L.orc_trav_stats.argtypes = [vp, C.c_int, f32p, f32p, u64p]
         L.orc_render.argtypes = [vp, f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                  C.c_int, C.c_uint32, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                 f64p, f64p, C.c_int]
+                                 f64p, f64p, C.c_int, C.c_int]
+        L.orc_env_sample.argtypes = [vp, C.c_int, f32p, f32p, f32p, C.POINTER(C.c_int)]
+        L.orc_env_density.argtypes = [vp, C.c_int, f64p, f64p]
+        L.orc_material_sample_density.argtypes = [vp, C.c_int, C.c_int, f64p, f64p, f64p, f64p]
         L.orc_philox.argtypes = [u32p, u32p, u32p]
         L.orc_srgb8_decode.argtypes = [C.c_int]
         L.orc_srgb8_decode.restype = C.c_float
@@ -135,12 +138,12 @@ class Scene:
         return int(out[0]), int(out[1])
 
     def render(self, cam, W, H, spp, spp0=0, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0, fp64=False,
-               nthreads=0, rect=None, sums=None, integrator=0):
+               nthreads=0, rect=None, sums=None, integrator=0, env_is=False):
         x0, y0, x1, y1 = rect if rect is not None else (0, 0, W, H)
         if sums is None:
             sums = (np.zeros(W * H * 3), np.zeros(W * H * 3))
         r = lib().orc_render(self.h, _f32(cam), W, H, x0, y0, x1, y1, spp0, spp, seed, prr, mis, maxdepth,
-                             rrdepth, int(fp64), nthreads, sums[0], sums[1], integrator)
+                             rrdepth, int(fp64), nthreads, sums[0], sums[1], integrator, int(env_is))
         if r != 0:
             raise RuntimeError("oracle render failed")
         return sums
@@ -158,6 +161,27 @@ class Scene:
         wi_out = np.zeros((k, 3), np.float32); f = np.zeros((k, 3), np.float32); pdf = np.zeros(k, np.float32)
         lib().orc_material_eval(self.h, m, k, wo, n, wi_in, u_in, 1 if wi is None else 0, wi_out, f, pdf)
         return wi_out, f, pdf
+
+    def env_sample(self, u4):
+        """Sky sampler of the optional environment sampling: directions, densities, accepted flags."""
+        u4 = _f32(u4)
+        n = len(u4)
+        d = np.zeros((n, 3), np.float32); p = np.zeros(n, np.float32); ok = np.zeros(n, np.int32)
+        lib().orc_env_sample(self.h, n, u4, d, p, ok.ctypes.data_as(C.POINTER(C.c_int)))
+        return d, p, ok.astype(bool)
+
+    def env_density(self, d):
+        d = np.ascontiguousarray(d, np.float64).reshape(-1, 3)
+        out = np.zeros(len(d))
+        lib().orc_env_density(self.h, len(d), d, out)
+        return out
+
+    def sample_density(self, m, wo, n, wi):
+        """True density of material m's BSDF sampler at wi (Lambert and GGX-reflect, FP64)."""
+        a = [np.ascontiguousarray(x, np.float64).reshape(-1, 3) for x in (wo, n, wi)]
+        out = np.zeros(len(a[2]))
+        lib().orc_material_sample_density(self.h, m, len(a[2]), a[0], a[1], a[2], out)
+        return out
 
     def set_texture_rgb32f(self, tid, rgb):
         rgb = _f32(rgb)
