@@ -8,8 +8,9 @@ triangles).  The scene is built and resident in HBM before the timed region; a s
 nr_render_ptnee call (all wavefront batches of the frame, FP64 accumulation included).
 
 The inverse-rendering iteration (render + loss + path-replay adjoint + Adam on the 256x256
-64-material box room) is timed after the frames and reported as "adjoint"; the time to RMSE 1e-2
-of plain ptnee and of the radiance cache at 1920x1080, 1 spp per frame, as "nrc".
+64-material box room) is timed after the frames and reported as "adjoint"; the same frame with
+sky importance sampling as "env_sampling" (time, variance, equal-error gain); the time to RMSE
+1e-2 of plain ptnee and of the radiance cache at 1920x1080, 1 spp per frame, as "nrc".
 
 Printed JSON line: value = Msamples/s over the timed steps, the traversal roofline (HIP events
 around every closest-hit launch; algorithmic bytes from the kernel's own node / triangle
@@ -233,6 +234,30 @@ def large_scene(nr, scenes, spp=256):
             "adjoint_msamples_per_s": aw * ah * aspp / adj_s / 1e6, "nrc_infer_4k_ms": infer_ms}
 
 
+def env_sampling(nr, dev, sc, spp):
+    """The same 720p frame with and without sky sampling (nr_params.env_is): time per frame, mean
+    per-pixel variance of one sample, and the time-to-equal-error gain var_off t_off / (var_on t_on)."""
+    W, H = sc["width"], sc["height"]
+    kw = dict(spp=spp, seed=11, maxdepth=10, rrdepth=3, prr=0.8)
+    res = {}
+    for env in (False, True):
+        dev.render(sc["camera"], W, H, fetch=False, env_is=env, **kw)   # buffers sized for this mode
+        nr.synchronize()
+        t0 = time.perf_counter()
+        dev.render(sc["camera"], W, H, fetch=False, env_is=env, **kw)
+        nr.synchronize()
+        dt = time.perf_counter() - t0
+        r = dev.fetch(W, H)
+        m = r["sum"] / spp
+        var = float(np.mean(np.maximum(r["sum2"] / spp - m * m, 0) * spp / (spp - 1)))
+        res[env] = (dt, var, float(m.mean()))
+    (t0_, v0, m0), (t1, v1, m1) = res[False], res[True]
+    return {"frame": f"{W}x{H}x{spp}spp", "off_ms": t0_ * 1e3, "on_ms": t1 * 1e3,
+            "off_msamples_per_s": W * H * spp / t0_ / 1e6, "on_msamples_per_s": W * H * spp / t1 / 1e6,
+            "var_off": v0, "var_on": v1, "mean_off": m0, "mean_on": m1,
+            "variance_ratio": v0 / v1, "equal_error_speedup": v0 * t0_ / (v1 * t1)}
+
+
 def measured_traffic():
     """Calibrated HBM bytes per closest-hit launch from the committed rocprofv3 counter summary."""
     path = os.path.join(ROOT, "profiles", "trace_closest_720p_traffic.txt")
@@ -330,6 +355,11 @@ def main():
     if adj:
         print(f"# adjoint iteration {adj['ms_per_iter']:.2f} ms (render {adj['render_ms']:.2f} ms, "
               f"replay {adj['adjoint_ms']:.2f} ms)", file=sys.stderr, flush=True)
+    env = None if args.frames_only else env_sampling(nr, dev, sc, spp)
+    if env:
+        print(f"# sky sampling: {env['on_ms']:.1f} ms vs {env['off_ms']:.1f} ms per frame, variance "
+              f"{env['variance_ratio']:.2f}x lower, {env['equal_error_speedup']:.2f}x faster to equal error",
+              file=sys.stderr, flush=True)
     nrc = None if args.no_nrc else nrc_time_to_rmse(nr, scenes)
     mlp = None if args.frames_only else nrc_kernels(nr)
     if mlp:
@@ -348,7 +378,7 @@ def main():
                                f"({len(sc['p'])} triangles) + 1024x512 sky", "global_batch": W * H * spp,
                    "seq_len": None, "parallelism": "single-gpu", "bvh_build_ms": info["build_ms"]},
         "roofline": roofline, "cpu_baseline": cpu, "hbm_copy_gbs": copy_gbs,
-        "adjoint": adj, "nrc": nrc, "nrc_mlp": mlp,
+        "adjoint": adj, "nrc": nrc, "nrc_mlp": mlp, "env_sampling": env,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
