@@ -57,6 +57,51 @@ __device__ __forceinline__ void trav_expand(const SceneView& sv, const SlabRay& 
     }
 }
 
+// Wave-cooperative form of trav_expand for the persistent kernels: every lane keeps expanding
+// internal nodes as long as some lane of the wave is still looking for a leaf.  A lane that already
+// holds a leaf goes on expanding internal nodes (its leaf is postponed); when it pops a second leaf
+// it puts that back and parks until the wave's leaf round.  The loop ends when every lane holds a
+// leaf or has an empty stack, so the leaf tests that follow run together and the expansion loop
+// keeps its lanes busy.  The set of nodes and leaves visited is the same as in the sequential DFS
+// (only their interleaving changes), and the result does not depend on the order.
+template <bool STATS, int STRIDE>
+__device__ __forceinline__ void trav_expand_wave(const SceneView& sv, const SlabRay& sr, int* stk, int cap,
+                                                 TravState& s, uint32_t& n_pf) {
+    bool parked = false;
+    for (;;) {
+        if (!__any(!s.has_leaf && s.sp > 0)) return;
+        if (s.sp > 0 && !parked) {
+            --s.sp;
+            const int32_t e = stk[s.sp * STRIDE];
+            if (ref_is_leaf(e)) {
+                if (!s.has_leaf) {
+                    s.leaf = e;
+                    s.has_leaf = true;
+                } else {
+                    ++s.sp;  // the entry is still in its slot
+                    parked = true;
+                }
+            } else {
+                const NodePair* np = sv.nodes + e;
+                const float4 a = np->a, b = np->b, c = np->c;
+                const int4 ch = np->d;
+                if (STATS) ++n_pf;
+                const bool hl = slab_fast(a.x, a.y, a.z, a.w, b.x, b.y, sr);
+                const bool hr = slab_fast(b.z, b.w, c.x, c.y, c.z, c.w, sr);
+                const int npush = int(hl) + int(hr);
+                if (s.sp + npush > cap) {
+                    s.overflow = true;
+                    s.sp = 0;
+                } else {
+                    stk[s.sp * STRIDE] = hr ? ch.y : ch.x;
+                    stk[(s.sp + 1) * STRIDE] = ch.x;
+                    s.sp += npush;
+                }
+            }
+        }
+    }
+}
+
 // Leaf loop, bvhnode.cpp:50-66, merged into the running winner with the DFS tie rule (closest
 // hit), or the shadow rule (double)t + 2e-3 <= len (any hit: returns true when blocked).
 template <bool ANY, bool STATS>

@@ -103,7 +103,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const Ray* __rest
             continue;
         }
         if (i >= 0) {
-            trav_expand<STATS, BLOCK>(sv, sr, stk, cap, ts, n_pf);
+            trav_expand_wave<STATS, BLOCK>(sv, sr, stk, cap, ts, n_pf);
             if (ts.has_leaf) done = trav_leaf<ANY, STATS>(sv, o, d, double(wr), ts, bt, bb1, bb2, bprim, blf, n_tt);
             // ---- finished: write the result, go idle
             if ((ts.sp == 0 && !ts.has_leaf) || done || ts.overflow) {
