@@ -348,7 +348,8 @@ def main():
                 "kernel": "k_trace<closest>", "bytes_per_launch": bytes_frame / max(cnt["closest_launches"], 1),
                 "avg_launch_ms": closest_ms / max(launches, 1),
                 "rays_per_frame": cnt["rays"], "pair_fetch_per_ray": cnt["pair_fetch"] / max(cnt["rays"], 1),
-                "tri_tests_per_ray": cnt["tri_tests"] / max(cnt["rays"], 1)}
+                "tri_tests_per_ray": cnt["tri_tests"] / max(cnt["rays"], 1),
+                "rays_retried_with_full_stack": cnt["retried"]}
     if args.frames_only:
         args.no_cpu = args.no_nrc = True
     adj = None if args.frames_only else adjoint_iteration(nr, scenes)

@@ -27,7 +27,7 @@ EXPORTS = [
     "nr_set_profiling", "nr_bench_copy", "nr_scene_set_params", "nr_loss_grad_l2", "nr_adjoint_ptnee",
     "nr_nrc_create", "nr_nrc_destroy", "nr_nrc_set_weights", "nr_nrc_get_weights", "nr_nrc_infer",
     "nr_nrc_train_step", "nr_nrc_gradient", "nr_render_nrc", "nr_nrc_bench", "nr_scene_set_texture_rgb32f",
-    "nr_adjoint_ptnee_tex", "nr_scene_set_material_ext",
+    "nr_adjoint_ptnee_tex", "nr_scene_set_material_ext", "nr_scene_set_short_stack",
 ]
 
 NRC_NPARAM = 5 * 4096 + 16 * 64
@@ -84,6 +84,7 @@ def lib():
         L.nr_render_fetch.argtypes = [vp, vp, vp, vp, vp]
         L.nr_render_stats.argtypes = [vp, f64p]
         L.nr_set_profiling.argtypes = [vp, C.c_int32]
+        L.nr_scene_set_short_stack.argtypes = [vp, C.c_int32]
         L.nr_bench_copy.argtypes = [C.c_uint64, C.c_int32, C.POINTER(C.c_double)]
         L.nr_scene_set_params.argtypes = [vp, f32p, C.c_uint32]
         L.nr_loss_grad_l2.argtypes = [vp, f32p, C.POINTER(C.c_double), vp]
@@ -291,12 +292,16 @@ class Scene:
     def set_profiling(self, mode):
         check(lib().nr_set_profiling(self.h, mode))
 
+    def set_short_stack(self, entries):
+        """LDS stack entries of the first trace launch (rays needing more are traced again)."""
+        check(lib().nr_scene_set_short_stack(self.h, entries))
+
     def stats(self):
         out = np.zeros(16, np.float64)
         check(lib().nr_render_stats(self.h, out))
         keys = ["ms", "samples", "iterations", "closest_ms", "closest_launches", "rays", "pair_fetch", "tri_tests",
                 "shadow_ms", "shadow_rays", "shadow_pair_fetch", "shadow_tri_tests", "shade_ms", "shade_launches",
-                "shadow_launches", "reserved"]
+                "shadow_launches", "retried"]
         return dict(zip(keys, out.tolist()))
 
 

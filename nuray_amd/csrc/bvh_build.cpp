@@ -18,6 +18,7 @@
 #include "bvh_build.h"
 #include <algorithm>
 #include <atomic>
+#include <stdexcept>
 #include <chrono>
 #include <cmath>
 #include <omp.h>
@@ -40,6 +41,10 @@ struct Builder {
     std::vector<int32_t> idx;
     std::vector<Tmp> nodes;
     std::atomic<int> n_nodes{0};
+    // set when no split had a cost below the initial 1e38 (surface areas that overflow FP32, or
+    // NaN coordinates): the reference would recurse on an empty part forever; this build stops
+    // and reports it
+    std::atomic<bool> degenerate{false};
 
     int alloc(int lo, int hi, int l, int r) {
         int id = n_nodes.fetch_add(1);
@@ -88,6 +93,10 @@ struct Builder {
         if (hi - lo <= 5) return alloc(lo, hi, -1, -1);
         thread_local std::vector<float> suffix;
         int mid = lo + divide(lo, hi, suffix);
+        if (mid == lo || mid == hi) {
+            degenerate = true;
+            return alloc(lo, hi, -1, -1);
+        }
         int l, r;
         if (hi - lo > 4096) {
 #pragma omp task shared(l) firstprivate(lo, mid)
@@ -132,12 +141,15 @@ BvhBuild build_sah(const float* p, int n, int nthreads) {
     }
     b.nodes.resize(2 * size_t(n));
     int root;
-    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+    [[maybe_unused]] const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
 #pragma omp parallel num_threads(nt)
     {
 #pragma omp single
         root = b.rec(0, n);
     }
+    if (b.degenerate)
+        throw std::runtime_error("SAH build: no split has a finite cost (non-finite coordinates, or a scene "
+                                 "extent whose FP32 surface areas overflow)");
     const int nn = b.n_nodes.load();
 
     // Preorder numbering over all nodes, preorder numbering of internal nodes for NodePairs,

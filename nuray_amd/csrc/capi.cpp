@@ -176,23 +176,26 @@ int nr_intersect(nr_scene* s, const float* o, const float* d, uint32_t n, int32_
             rays[i].o = make_float4(o[3 * i], o[3 * i + 1], o[3 * i + 2], any_hit ? tmax[i] : 0.0f);
             rays[i].d = make_float4(d[3 * i], d[3 * i + 1], d[3 * i + 2], 0.0f);
         }
-        nr::DevBuf dr, dh, dc;
+        nr::DevBuf dr, dh, dc, dretry;
         dr.upload(rays.data(), n * sizeof(nr::Ray), sc.stream);
         dh.ensure(n * sizeof(nr::HitRec));
-        dc.ensure(4 * sizeof(int));
-        int hc[4] = {int(n), 0, 0, 0};
+        dretry.ensure(n * sizeof(int));
+        // c[0] ray count, c[1..3] the launch's work counters, c[4] overflow flag
+        int hc[8] = {int(n), 0, 0, 0, 0, 0, 0, 0};
+        dc.ensure(sizeof(hc));
         nr::check_hip(hipMemcpyAsync(dc.ptr, hc, sizeof(hc), hipMemcpyHostToDevice, sc.stream), "H2D");
         int* c = dc.as<int>();
         if (any_hit)
             nr::launch_shadow(sc.view, dr.as<nr::Ray>(), c, nullptr, dh.as<nr::HitRec>(), nullptr, nullptr, c + 1,
-                              c + 2, nullptr, sc.stream);
+                              c + 4, nullptr, sc.stream, dretry.as<int>());
         else
-            nr::launch_closest(sc.view, dr.as<nr::Ray>(), c, dh.as<nr::HitRec>(), c + 1, c + 2, nullptr, sc.stream);
+            nr::launch_closest(sc.view, dr.as<nr::Ray>(), c, dh.as<nr::HitRec>(), c + 1, c + 4, nullptr, sc.stream,
+                               dretry.as<int>());
         std::vector<nr::HitRec> h(n);
         nr::check_hip(hipMemcpyAsync(h.data(), dh.ptr, n * sizeof(nr::HitRec), hipMemcpyDeviceToHost, sc.stream), "D2H");
         nr::check_hip(hipMemcpyAsync(hc, dc.ptr, sizeof(hc), hipMemcpyDeviceToHost, sc.stream), "D2H");
         nr::check_hip(hipStreamSynchronize(sc.stream), "intersect sync");
-        if (hc[2]) throw std::runtime_error("traversal stack overflow");
+        if (hc[4]) throw std::runtime_error("traversal stack overflow");
         for (uint32_t i = 0; i < n; ++i) {
             tri[i] = h[i].prim >= 0 ? sc.bvh.order[h[i].prim] : -1;
             t[i] = h[i].t;
@@ -220,7 +223,8 @@ int nr_render_ptnee(nr_scene* s, const nr_camera* cam, const nr_params* pp, floa
         o[3] = kt.closest_ms; o[4] = kt.closest_launches;
         o[5] = double(ts[0].rays); o[6] = double(ts[0].pair_fetch); o[7] = double(ts[0].tri_tests);
         o[8] = kt.shadow_ms; o[9] = double(ts[1].rays); o[10] = double(ts[1].pair_fetch); o[11] = double(ts[1].tri_tests);
-        o[12] = kt.shade_ms; o[13] = kt.shade_launches; o[14] = kt.shadow_launches; o[15] = 0;
+        o[12] = kt.shade_ms; o[13] = kt.shade_launches; o[14] = kt.shadow_launches;
+        o[15] = double(ts[0].retried + ts[1].retried);
         if (rgb_linear || sum || sum2 || rgb8) nr::render_fetch(*s->sc, rgb_linear, sum, sum2, rgb8);
         return st ? NR_CANCELLED : NR_OK;
     });
@@ -243,6 +247,15 @@ int nr_render_stats(nr_scene* s, double* out16) {
 int nr_set_profiling(nr_scene* s, int32_t mode) {
     return guard([&] {
         s->profiling = mode;
+        return NR_OK;
+    });
+}
+
+int nr_scene_set_short_stack(nr_scene* s, int32_t entries) {
+    return guard([&] {
+        if (entries < 1) throw std::runtime_error("short stack needs at least one entry");
+        s->sc->short_stack = entries;
+        s->sc->view.short_stack = entries;
         return NR_OK;
     });
 }

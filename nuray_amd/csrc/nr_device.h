@@ -77,7 +77,8 @@ struct SceneView {
     float light_p;
     int32_t env_tex;   // texture index of the environment map or -1
     float env_gain;
-    int32_t stack_cap; // traversal stack entries per lane
+    int32_t stack_cap; // traversal stack entries per lane (full depth)
+    int32_t short_stack; // stack entries of the first persistent trace launch (see trace.hip)
     // environment sampler: one cell per texel square of the map (env_w x env_h cells, 0 if none),
     // {alias probability, alias cell (int bits), pdf of the cell, pdf of the alias cell}; the
     // solid-angle density of a direction in cell k is pdf_k * env_norm / sin(theta)

@@ -66,6 +66,7 @@ void RenderTarget::ensure(int w, int h, size_t c) {
         shadow.ensure(sizeof(Ray) * c);
         contrib.ensure(sizeof(float4) * c);
         radiance.ensure(sizeof(float4) * c);
+        retry.ensure(sizeof(int) * c);
         cap = c;
     }
     counters.ensure(sizeof(int) * 16);
@@ -79,7 +80,8 @@ constexpr int BLK = 256;
 constexpr int LDS_LIGHTS = 2048;  // light CDFs up to this size are searched in LDS
 
 // Counter slots
-enum { C_RAY0 = 0, C_RAY1 = 1, C_SHADOW = 2, C_WORK0 = 3, C_WORK1 = 4, C_ERR = 5, C_QUERY = 6, C_REC = 7 };
+// (C_WORK0 and C_WORK1 are three ints each: ray claims, retry count, retry claims of a trace launch)
+enum { C_RAY0 = 0, C_RAY1 = 1, C_SHADOW = 2, C_ERR = 5, C_QUERY = 6, C_REC = 7, C_WORK0 = 8, C_WORK1 = 11 };
 
 __global__ void k_zero_doubles(double* a, double* b, size_t n) {
     for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
@@ -97,8 +99,10 @@ __global__ void k_set_counters(int* c, int ray_slot, int ray_value, int clear_sl
         if (ray_slot >= 0) c[ray_slot] = ray_value;
         if (clear_slot >= 0) c[clear_slot] = 0;
         c[C_SHADOW] = 0;
-        c[C_WORK0] = 0;
-        c[C_WORK1] = 0;
+        for (int k = 0; k < 3; ++k) {
+            c[C_WORK0 + k] = 0;
+            c[C_WORK1 + k] = 0;
+        }
     }
 }
 
@@ -618,6 +622,7 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
         t.shadow.ensure(sizeof(Ray) * 2 * c);
         t.contrib.ensure(sizeof(float4) * 2 * c);
         t.skyw.ensure(sizeof(float) * c);
+        t.retry.ensure(sizeof(int) * 2 * c);
     }
     t.adj_valid = kb >= p.spp && !p.accumulate && !p.env_is;
     t.adj_p = p;
@@ -657,7 +662,7 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
             hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, -1, 0, int(C_RAY0 + nxt));
             if (kt) mark(ev_c);
             launch_closest(sc.view, t.rays[cur].as<Ray>(), cnt + C_RAY0 + cur, t.hits.as<HitRec>(), cnt + C_WORK0,
-                           cnt + C_ERR, ts, s);
+                           cnt + C_ERR, ts, s, t.retry.as<int>());
             if (kt) mark(ev_c);
             ShadeArgs a{sc.view, npix, p.seed, uint32_t(p.spp_offset + b), p.prr, p.mis, p.maxdepth, p.rrdepth,
                         p.integrator == 1 ? 0 : 1,
@@ -677,7 +682,8 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
             if (kt) mark(ev_h);
             if (kt) mark(ev_s);
             launch_shadow(sc.view, t.shadow.as<Ray>(), cnt + C_SHADOW, nullptr, nullptr, t.contrib.as<float4>(),
-                          t.radiance.as<float4>(), cnt + C_WORK1, cnt + C_ERR, ts ? ts + 1 : nullptr, s);
+                          t.radiance.as<float4>(), cnt + C_WORK1, cnt + C_ERR, ts ? ts + 1 : nullptr, s,
+                          t.retry.as<int>());
             if (kt) mark(ev_s);
             rs.iterations++;
             if (p.maxdepth > 0) {
@@ -764,7 +770,7 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
         const int cur = it & 1, nxt = cur ^ 1;
         hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, -1, 0, int(C_RAY0 + nxt));
         launch_closest(sc.view, t.rays[cur].as<Ray>(), cnt + C_RAY0 + cur, t.hits.as<HitRec>(), cnt + C_WORK0,
-                       cnt + C_ERR, nullptr, s);
+                       cnt + C_ERR, nullptr, s, t.retry.as<int>());
         ShadeArgs a{sc.view, npix, p.seed, uint32_t(p.spp_offset), p.prr, p.mis, p.maxdepth, p.rrdepth,
                     p.integrator == 1 ? 0 : 1,
                     t.rays[cur].as<Ray>(), t.beta[cur].as<float4>(), t.hits.as<HitRec>(), cnt + C_RAY0 + cur,
@@ -777,7 +783,7 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
         if (sc.features == 0) hipLaunchKernelGGL(k_shade_nrc_basic, dim3(grid_shade), dim3(BLK), 0, s, a, ns);
         else hipLaunchKernelGGL(k_shade_nrc_full, dim3(grid_shade), dim3(BLK), 0, s, a, ns);
         launch_shadow(sc.view, t.shadow.as<Ray>(), cnt + C_SHADOW, nullptr, nullptr, t.contrib.as<float4>(),
-                      t.radiance.as<float4>(), cnt + C_WORK1, cnt + C_ERR, nullptr, s);
+                      t.radiance.as<float4>(), cnt + C_WORK1, cnt + C_ERR, nullptr, s, t.retry.as<int>());
         rs.iterations++;
         if (p.maxdepth > 0) {
             if (it + 1 >= p.maxdepth) break;

@@ -47,6 +47,7 @@ struct RenderTarget {
     DevBuf hits, shadow, contrib;      // closest hits; shadow rays and their NEE contributions
     DevBuf radiance;                   // float4 per path slot (x2 with sky sampling: sky rays' region)
     DevBuf skyw;                       // sky sampling: BSDF-escape MIS weight per path slot
+    DevBuf retry;                      // rays a trace launch hands to its full-stack retry launch
     DevBuf counters;                   // int[16]
     DevBuf stats;                      // TraceStats[2]
     int* pinned = nullptr;             // host-visible counter readback

@@ -230,6 +230,7 @@ void Scene::build(int nthreads) {
     view.env_tex = env_tex;
     view.env_gain = env_gain;
     view.stack_cap = bvh.depth + 2;
+    view.short_stack = short_stack;
     build_env_sampler();
     built = true;
 }

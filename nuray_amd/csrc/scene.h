@@ -43,6 +43,7 @@ public:
     int env_tex = -1;
     float env_gain = 1.0f;
     unsigned features = 0;        // material features present: 1 = GGX dielectric, 2 = albedo textures
+    int short_stack = 18;         // LDS stack entries of the first trace launch
 
     // derived host data
     BvhBuild bvh;

@@ -9,18 +9,25 @@ namespace nr {
 // position.  Only the radiance accumulator is indexed by path slot (float4, xyz used).
 
 struct TraceStats {
-    unsigned long long rays, pair_fetch, tri_tests, pad;
+    unsigned long long rays, pair_fetch, tri_tests, retried;
 };
 
-// Closest hit for rays[0 .. *n): writes hits[i].  `work` must be zero at launch.
+// `work` points at three ints that must be zero at launch (ray claims, retry count, retry claims).
+// With `retry` (room for *n ints) the persistent launch keeps a short LDS stack (more waves per
+// CU); a ray whose stack would overflow it is abandoned, listed in `retry`, and traced again from
+// the start by a second launch with the full-depth stack.  Without `retry` every ray gets the
+// full-depth stack at once.  A ray that overflows the full-depth stack sets *err.
+//
+// Closest hit for rays[0 .. *n): writes hits[i].
 void launch_closest(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, int* work, int* err,
-                    TraceStats* stats, hipStream_t s);
+                    TraceStats* stats, hipStream_t s, int* retry = nullptr);
 // Shadow test for rays[0 .. *n) with segment length rays[i].o.w: occluded iff some accepted
 // triangle has (double)t + 2e-3 <= (double)len (rendererptls.cpp:70-71).  Writes occ[i] (0/1)
 // and the first occluder found (prim / t) when `hits` is non-null; when `contrib` is non-null
 // a visible ray adds contrib[i].xyz to radiance[slot], slot = bits of rays[i].d.w.
 void launch_shadow(const SceneView& sv, const Ray* rays, const int* n, uint8_t* occ, HitRec* hits,
-                   const float4* contrib, float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s);
+                   const float4* contrib, float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s,
+                   int* retry = nullptr);
 
 int device_cu_count();
 

@@ -12,11 +12,14 @@
 //
 // Execution: persistent waves.  Every lane traverses one ray with a private stack kept in LDS at
 // stack[slot * BLOCK + tid] (one 4-byte bank per lane, conflict-free).  Internal nodes are
-// expanded until a lane holds a leaf, then leaves are tested together; a lane whose ray is done
-// goes idle, and once REFILL_MIN lanes are idle one atomic claims new rays for all of them
-// (ballot + prefix count), so long rays do not hold a whole wave hostage.  The stack holds BVH
-// depth + 2 live entries (checked on every push; an overflow is reported, never written past)
-// plus one spare slot for the branch-free push.
+// expanded wave-cooperatively (trav_expand_wave) until every lane holds a leaf, then leaves are
+// tested together; a lane whose ray is done goes idle, and once REFILL_MIN lanes are idle one
+// atomic claims new rays for all of them (ballot + prefix count), so long rays do not hold a whole
+// wave hostage.  The first launch gives each lane SHORT_STACK live entries (plus one spare slot
+// for the branch-free push): the LDS then allows 8 waves per SIMD instead of 6.  The rare ray that
+// needs more is abandoned, listed, and traced again from its start by a second launch whose
+// stack holds BVH depth + 2 entries (checked on every push; an overflow there is reported, never
+// written past).
 #include "trace.h"
 #include "nr_traverse.h"
 #include "scene.h"
@@ -34,19 +37,23 @@ constexpr int BLOCK = 256;
 // takes about one ray's latency, not several).
 constexpr int REFILL_MIN = 16;
 constexpr int CLAIM = 128;
+// Live stack entries of the first launch default to SceneView::short_stack = 18:
+// (18 + 1) x 256 x 4 B = 19 KB of LDS per block, 8 blocks (32 waves) per CU.
 
+// rays are rays[idx[i]] for i < *n_ptr when idx is given (the retry launch), else rays[i].
 template <bool ANY, bool STATS>
-__global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const Ray* __restrict__ rays,
-                                                  const int* __restrict__ n_ptr, HitRec* __restrict__ hits,
-                                                  uint8_t* __restrict__ occ, const float4* __restrict__ contrib,
-                                                  float4* __restrict__ radiance, int* __restrict__ work,
-                                                  int* __restrict__ err, TraceStats* __restrict__ stats) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_trace(SceneView sv, const Ray* __restrict__ rays,
+                                                  const int* __restrict__ n_ptr, const int* __restrict__ idx,
+                                                  HitRec* __restrict__ hits, uint8_t* __restrict__ occ,
+                                                  const float4* __restrict__ contrib, float4* __restrict__ radiance,
+                                                  int* __restrict__ work, int* __restrict__ err,
+                                                  int* __restrict__ retry, int* __restrict__ n_retry, int cap,
+                                                  TraceStats* __restrict__ stats) {
     extern __shared__ int stk_all[];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     int* const stk = stk_all + tid;
     const unsigned long long below = (1ull << lane) - 1ull;
-    const int cap = sv.stack_cap;
     const int n = *n_ptr;
     const int n_waves = int(gridDim.x) * (BLOCK / 64);
     const int claim = max(1, min(CLAIM, (n + n_waves - 1) / n_waves));
@@ -82,7 +89,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const Ray* __rest
                 const int rank = __popcll(idle_mask & below);
                 if (rank < take) {
                     i = rb + rank;
-                    const Ray r = rays[i];
+                    const Ray r = rays[idx ? idx[i] : i];
                     o = f3(r.o.x, r.o.y, r.o.z);
                     d = f3(r.d.x, r.d.y, r.d.z);
                     sr = slab_ray(o, d);
@@ -107,7 +114,14 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const Ray* __rest
             if (ts.has_leaf) done = trav_leaf<ANY, STATS>(sv, o, d, double(wr), ts, bt, bb1, bb2, bprim, blf, n_tt);
             // ---- finished: write the result, go idle
             if ((ts.sp == 0 && !ts.has_leaf) || done || ts.overflow) {
+                if (ts.overflow && retry) {  // traced again from the start with the full stack
+                    retry[atomicAdd(n_retry, 1)] = i;
+                    if (STATS) atomicAdd(&stats->retried, 1ull);
+                    i = -1;
+                    continue;
+                }
                 if (ts.overflow) atomicOr(err, 1);
+                if (idx) i = idx[i];
                 if (STATS) {
                     atomicAdd(&stats->rays, 1ull);
                     atomicAdd(&stats->pair_fetch, (unsigned long long)n_pf);
@@ -150,17 +164,32 @@ int grid_for(size_t shmem) {
 
 template <bool ANY>
 void launch(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, uint8_t* occ, const float4* contrib,
-            float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s) {
-    const size_t shmem = size_t(sv.stack_cap + 1) * BLOCK * sizeof(int);  // + the spare push slot
-    if (shmem > 160 * 1024) throw std::runtime_error("BVH too deep for the LDS traversal stack");
+            float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s, int* retry) {
+    const int full = sv.stack_cap;
+    const size_t shmem_full = size_t(full + 1) * BLOCK * sizeof(int);  // + the spare push slot
+    if (shmem_full > 160 * 1024) throw std::runtime_error("BVH too deep for the LDS traversal stack");
+    const int cap = retry ? std::min(full, std::max(1, sv.short_stack)) : full;
+    int* const rl = cap < full ? retry : nullptr;
+    const size_t shmem = size_t(cap + 1) * BLOCK * sizeof(int);
     if (stats) {
         hipLaunchKernelGGL((k_trace<ANY, true>), dim3(grid_for<ANY, true>(shmem)), dim3(BLOCK), shmem, s, sv, rays, n,
-                           hits, occ, contrib, radiance, work, err, stats);
+                           nullptr, hits, occ, contrib, radiance, work, err, rl, work + 1, cap, stats);
     } else {
         hipLaunchKernelGGL((k_trace<ANY, false>), dim3(grid_for<ANY, false>(shmem)), dim3(BLOCK), shmem, s, sv, rays,
-                           n, hits, occ, contrib, radiance, work, err, stats);
+                           n, nullptr, hits, occ, contrib, radiance, work, err, rl, work + 1, cap, stats);
     }
     check_hip(hipGetLastError(), "k_trace launch");
+    if (!rl) return;
+    // the abandoned rays (count in work[1], claims in work[2]): one block per CU is plenty
+    const dim3 g(device_cu_count());
+    if (stats) {
+        hipLaunchKernelGGL((k_trace<ANY, true>), g, dim3(BLOCK), shmem_full, s, sv, rays, work + 1, rl, hits, occ,
+                           contrib, radiance, work + 2, err, nullptr, nullptr, full, stats);
+    } else {
+        hipLaunchKernelGGL((k_trace<ANY, false>), g, dim3(BLOCK), shmem_full, s, sv, rays, work + 1, rl, hits, occ,
+                           contrib, radiance, work + 2, err, nullptr, nullptr, full, stats);
+    }
+    check_hip(hipGetLastError(), "k_trace retry launch");
 }
 
 }  // namespace
@@ -176,13 +205,14 @@ int device_cu_count() {
 }
 
 void launch_closest(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, int* work, int* err,
-                    TraceStats* stats, hipStream_t s) {
-    launch<false>(sv, rays, n, hits, nullptr, nullptr, nullptr, work, err, stats, s);
+                    TraceStats* stats, hipStream_t s, int* retry) {
+    launch<false>(sv, rays, n, hits, nullptr, nullptr, nullptr, work, err, stats, s, retry);
 }
 
 void launch_shadow(const SceneView& sv, const Ray* rays, const int* n, uint8_t* occ, HitRec* hits,
-                   const float4* contrib, float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s) {
-    launch<true>(sv, rays, n, hits, occ, contrib, radiance, work, err, stats, s);
+                   const float4* contrib, float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s,
+                   int* retry) {
+    launch<true>(sv, rays, n, hits, occ, contrib, radiance, work, err, stats, s, retry);
 }
 
 }  // namespace nr

@@ -4,6 +4,7 @@
 // sort sequence (std::sort, unstable), same leaf rule, same recursive closest-hit traversal
 // with no distance culling.
 #pragma once
+#include <atomic>
 #include "orc_scene.h"
 #include <tuple>
 
@@ -24,12 +25,14 @@ public:
     std::vector<Node> nodes;
     int root = -1;
     const Scene* sc = nullptr;
+    mutable std::atomic<bool> degenerate{false};   // a split with no finite cost was met
 
     // BVH::build(const std::vector<Triangle>&), bvh.cpp:91-97 (scene order in, recursion :60-70).
     // Subtrees are independent, so they are built as OpenMP tasks (above 4096 primitives) and
     // then flattened children-first, as the sequential recursion would store them.
     void build(const Scene& s) {
         sc = &s;
+        degenerate = false;
         nodes.clear();
         std::vector<int> all(s.tris.size());
         for (size_t i = 0; i < all.size(); ++i) all[i] = int(i);
@@ -131,6 +134,12 @@ private:
             return nd;
         }
         auto parts = divide(prims);
+        if (parts.first.empty() || parts.second.empty()) {  // no finite-cost split: the reference
+            degenerate = true;                              // would recurse forever; stop, report
+            nd->mn = V3f(1e18f); nd->mx = V3f(-1e18f);
+            nd->prims = prims;
+            return nd;
+        }
         TNode* L = nullptr;
         TNode* R = nullptr;
         if (prims.size() >= 4096) {

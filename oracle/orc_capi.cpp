@@ -91,6 +91,7 @@ void orc_scene_set_texture_rgb32f(void* h, int id, int w, int hh, const float* r
 
 int orc_bvh_build(void* h) {
     S(h)->bvh.build(S(h)->sc);
+    if (S(h)->bvh.degenerate) return -1;
     S(h)->built = true;
     return int(S(h)->bvh.nodes.size());
 }

@@ -106,6 +106,8 @@ class Scene:
                 t = np.ascontiguousarray(tex, np.uint8)
                 L.orc_scene_set_texture_rgba8(self.h, tid, t.shape[1], t.shape[0], t)
         self.n_nodes = L.orc_bvh_build(self.h) if build_bvh else 0
+        if self.n_nodes < 0:
+            raise RuntimeError("oracle SAH build: no split has a finite cost")
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -49,3 +49,23 @@ def test_degenerate_inputs(po):
     p = np.zeros((9, 9), np.float32)
     sc = dict(p=p, n=None, uv=None, mat=np.zeros(9, np.int32), mats=np.array([[0, 1, 1, 1, .5, 1, -1, 0]], np.float32))
     _same(po.Scene(sc).export(), nr.bvh_build_host(p))
+
+
+def test_degenerate_split_is_reported_not_a_crash(po):
+    """Nested squares growing by 1.15x per step reach 1.5e18: their FP32 surface areas overflow,
+    no split has a finite SAH cost, and the reference's recursion would never end.  Both the
+    library's builder and the oracle's report it as an error."""
+    tris = []
+    for k in range(300):
+        s, z = 1.15 ** k, -float(k)
+        a, b, c, d = [-s, -s, z], [s, -s, z], [s, s, z], [-s, s, z]
+        tris += [a + b + c, a + c + d]
+    p = np.array(tris, np.float32)
+    import nuray_amd as nr
+
+    with pytest.raises(nr.NuRayError, match="finite cost"):
+        nr.bvh_build_host(p)
+    sc = dict(p=p, n=None, uv=None, mat=np.zeros(len(p), np.int32),
+              mats=np.array([[0, .5, .5, .5, .5, 1, -1, 0]], np.float32))
+    with pytest.raises(RuntimeError, match="finite cost"):
+        po.Scene(sc)

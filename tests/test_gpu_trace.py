@@ -76,3 +76,25 @@ def test_empty_and_tiny_batches(nr, po, scenes):
     _check_equal(ref.intersect(o, d), dev.intersect(o, d), "single ray")
     tri, t, _, _ = dev.intersect(o, d)
     assert abs(t[0] - 100.0) < 1e-3  # the floor at y = 0
+
+
+@pytest.mark.parametrize("entries", [1, 2, 4])
+def test_short_stack_retry_is_bit_exact(entries, nr, po, scenes):
+    """With a first-launch stack of only `entries` LDS entries most rays overflow it and are traced
+    again with the full-depth stack: hits stay bit-identical to the oracle, and a whole render
+    (closest-hit and shadow launches) equals the default render bit for bit."""
+    sc = SCENES["terrain64"](scenes)
+    ref = po.Scene(sc)
+    dev = nr.Scene(sc)
+    dev.set_short_stack(entries)
+    o, d = make_batch(sc, 1 << 17, seed=13, oracle=ref)
+    _check_equal(ref.intersect(o, d), dev.intersect(o, d), f"short stack {entries}")
+    dev.set_profiling(2)
+    a = dev.render(sc["camera"], 96, 54, 4, seed=5, maxdepth=6, rrdepth=2)
+    retried = dev.stats()["retried"]
+    dev.set_short_stack(18)
+    b = dev.render(sc["camera"], 96, 54, 4, seed=5, maxdepth=6, rrdepth=2)
+    dev.close()
+    print(f"short stack {entries}: {retried:.0f} rays retried")
+    assert retried > 0
+    assert np.array_equal(a["sum"], b["sum"]) and np.array_equal(a["sum2"], b["sum2"])
