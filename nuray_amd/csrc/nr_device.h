@@ -167,38 +167,48 @@ __device__ __forceinline__ bool tri_valid(float t, float b1, float b2) {
 // one; inside it the exact division path runs.  Hit / miss outcomes and hit values are
 // therefore bit-identical to the unfiltered functions.
 //
-// Slab: per ray r = 1 / (d + 1e-9f) (correctly rounded).  For every bound,
-// t' = fl(fl(b - o) * r) differs from t = fl(fl(b - o) / dd) by at most ~2.5 ulp relative, so
-// t' has t's sign and |t' - t| <= 2^-22 |t|.  With finite quotients (dd != 0: `rays_ok`) the
-// reference's swap / std::max / std::min chain reduces to t_in = max(-1e18, min pairs) and
-// t_out = min(1e18, max pairs), each also within 2^-22 relative of its exact value.  Then
-//   t_out >= 0  <=>  t_out' >= 0                                   (sign preserved), and
-//   t_in <= t_out is decided by t_out' - t_in' against a band of 2^-20 (|t_in'| + |t_out'|).
-// Values below 1e-30 in magnitude (denormal range) always take the exact path.
+// Slab: per ray r = fl(1 / dd) with dd = d + 1e-9f (correctly rounded) and c = fl(o * r).  For
+// every bound b the filter uses t' = fl(b * r - c) (one fused multiply-add) in place of the
+// reference's t = fl(fl(b - o) / dd).  Writing r = (1 + d2) / dd, c = o r (1 + d3),
+// t' = (b r - c)(1 + d1) with |d1|, |d2|, |d3| <= u = 2^-24:
+//   |t' - t| <= 4.03 u |t| + 1.02 u |c|,
+// i.e. a relative error plus an absolute one bounded per ray by A = max_k |c_k|.  With finite
+// quotients (dd != 0: `finite`) the reference's swap / std::max / std::min chain reduces to
+// t_in = max(-1e18, min pairs), t_out = min(1e18, max pairs); max and min move by at most the
+// largest error of the elements that attain them, so |t_in' - t_in| <= 4.2 u |t_in'| + 1.2 u A,
+// and likewise for t_out.  Hence, with a20 = max(2^-20 A, 1e-30):
+//   |t_out'| > a20                                      => t_out >= 0  <=>  t_out' >= 0, and
+//   |t_out' - t_in'| > 2^-20 (|t_in'| + |t_out'|) + 2 a20 => t_in <= t_out <=> t_out' > t_in'
+// (2^-20 = 16 u covers 4.2 u per side plus the rounding of the difference; the 1e-30 floor keeps
+// every value the bounds rely on in the normal range).  Everything else takes the exact path.
 struct SlabRay {
-    F3 o, dd, r;
+    F3 o, d, r, c;
+    float a20;
     bool finite;  // every dd component non-zero
 };
 __device__ __forceinline__ SlabRay slab_ray(F3 o, F3 d) {
     SlabRay s;
     s.o = o;
-    s.dd = f3(d.x + 1e-9f, d.y + 1e-9f, d.z + 1e-9f);
-    s.r = f3(1.0f / s.dd.x, 1.0f / s.dd.y, 1.0f / s.dd.z);
-    s.finite = s.dd.x != 0.0f && s.dd.y != 0.0f && s.dd.z != 0.0f;
+    s.d = d;
+    const F3 dd = f3(d.x + 1e-9f, d.y + 1e-9f, d.z + 1e-9f);
+    s.r = f3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
+    s.c = f3(o.x * s.r.x, o.y * s.r.y, o.z * s.r.z);
+    s.a20 = fmaxf(fmaxf(fmaxf(fabsf(s.c.x), fabsf(s.c.y)), fabsf(s.c.z)) * 9.5367431640625e-07f, 1e-30f);
+    s.finite = dd.x != 0.0f && dd.y != 0.0f && dd.z != 0.0f;
     return s;
 }
 __device__ __forceinline__ bool slab_fast(float mnx, float mny, float mnz, float mxx, float mxy, float mxz,
                                           const SlabRay& s) {
-    const float ax = (mnx - s.o.x) * s.r.x, bx = (mxx - s.o.x) * s.r.x;
-    const float ay = (mny - s.o.y) * s.r.y, by = (mxy - s.o.y) * s.r.y;
-    const float az = (mnz - s.o.z) * s.r.z, bz = (mxz - s.o.z) * s.r.z;
+    const float ax = fmaf(mnx, s.r.x, -s.c.x), bx = fmaf(mxx, s.r.x, -s.c.x);
+    const float ay = fmaf(mny, s.r.y, -s.c.y), by = fmaf(mxy, s.r.y, -s.c.y);
+    const float az = fmaf(mnz, s.r.z, -s.c.z), bz = fmaf(mxz, s.r.z, -s.c.z);
     const float t_in = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), -1e18f));
     const float t_out = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), 1e18f));
-    const float band = 9.5367431640625e-07f * (fabsf(t_in) + fabsf(t_out));  // 2^-20
     const float diff = t_out - t_in;
-    const bool clear = s.finite && fabsf(t_out) >= 1e-30f && fabsf(t_in) >= 1e-30f && fabsf(diff) > band;
+    const float band = fmaf(9.5367431640625e-07f, fabsf(t_in) + fabsf(t_out), 2.0f * s.a20);  // 2^-20
+    const bool clear = s.finite && fabsf(t_out) > s.a20 && fabsf(diff) > band;
     if (__builtin_expect(clear, 1)) return t_out >= 0.0f && diff > 0.0f;
-    return slab(mnx, mny, mnz, mxx, mxy, mxz, s.o, s.dd);
+    return slab(mnx, mny, mnz, mxx, mxy, mxz, s.o, f3(s.d.x + 1e-9f, s.d.y + 1e-9f, s.d.z + 1e-9f));
 }
 
 // Triangle: q is formed exactly as in tri_test (so |q| >= ~2.5e-15: a float plus 1e-6 in double
