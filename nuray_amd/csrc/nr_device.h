@@ -197,6 +197,20 @@ __device__ __forceinline__ SlabRay slab_ray(F3 o, F3 d) {
     s.finite = dd.x != 0.0f && dd.y != 0.0f && dd.z != 0.0f;
     return s;
 }
+// The filtered test without its exact fallback: returns the fast decision and sets `clear` when
+// the decision is proven; otherwise the caller runs slab().
+__device__ __forceinline__ bool slab_filter(float mnx, float mny, float mnz, float mxx, float mxy, float mxz,
+                                            const SlabRay& s, bool& clear) {
+    const float ax = fmaf(mnx, s.r.x, -s.c.x), bx = fmaf(mxx, s.r.x, -s.c.x);
+    const float ay = fmaf(mny, s.r.y, -s.c.y), by = fmaf(mxy, s.r.y, -s.c.y);
+    const float az = fmaf(mnz, s.r.z, -s.c.z), bz = fmaf(mxz, s.r.z, -s.c.z);
+    const float t_in = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), -1e18f));
+    const float t_out = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), 1e18f));
+    const float diff = t_out - t_in;
+    const float band = fmaf(9.5367431640625e-07f, fabsf(t_in) + fabsf(t_out), 2.0f * s.a20);  // 2^-20
+    clear = s.finite && fabsf(t_out) > s.a20 && fabsf(diff) > band;
+    return t_out >= 0.0f && diff > 0.0f;
+}
 __device__ __forceinline__ bool slab_fast(float mnx, float mny, float mnz, float mxx, float mxy, float mxz,
                                           const SlabRay& s) {
     const float ax = fmaf(mnx, s.r.x, -s.c.x), bx = fmaf(mxx, s.r.x, -s.c.x);

@@ -12,7 +12,7 @@
 //
 // Execution: persistent waves.  Every lane traverses one ray with a private stack kept in LDS at
 // stack[slot * BLOCK + tid] (one 4-byte bank per lane, conflict-free).  Internal nodes are
-// expanded wave-cooperatively (trav_expand_wave) until every lane holds a leaf, then leaves are
+// expanded wave-cooperatively (trav_expand_wave) until (nearly) every lane holds a leaf, then leaves are
 // tested together; a lane whose ray is done goes idle, and once REFILL_MIN lanes are idle one
 // atomic claims new rays for all of them (ballot + prefix count), so long rays do not hold a whole
 // wave hostage.  The first launch gives each lane SHORT_STACK live entries (plus one spare slot
