@@ -343,10 +343,17 @@ def main():
 
     bytes_frame = B_PAIR * cnt["pair_fetch"] + B_TRI * cnt["tri_tests"] + B_RAY * cnt["rays"]
     achieved = bytes_frame * args.steps / (closest_ms / 1e3) / 1e9 if closest_ms > 0 else None
+    traffic = measured_traffic()
+    avg_ms = closest_ms / max(launches, 1)
+    # achieved counts algorithmic bytes; the L2s and the Infinity Cache serve most of them, so it can
+    # exceed the HBM peak.  hbm_gbs is the measured HBM traffic (rocprofv3 counters) over the same
+    # launch time: the kernel's own share of HBM bandwidth.
+    hbm_gbs = traffic / (avg_ms * 1e-3) / 1e9 if traffic and avg_ms > 0 else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": measured_traffic(),
+                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                "hbm_gbs": hbm_gbs, "hbm_frac": hbm_gbs / HBM_PEAK_GBS if hbm_gbs else None,
                 "kernel": "k_trace<closest>", "bytes_per_launch": bytes_frame / max(cnt["closest_launches"], 1),
-                "avg_launch_ms": closest_ms / max(launches, 1),
+                "avg_launch_ms": avg_ms,
                 "rays_per_frame": cnt["rays"], "pair_fetch_per_ray": cnt["pair_fetch"] / max(cnt["rays"], 1),
                 "tri_tests_per_ray": cnt["tri_tests"] / max(cnt["rays"], 1),
                 "rays_retried_with_full_stack": cnt["retried"]}

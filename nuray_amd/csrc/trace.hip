@@ -40,8 +40,9 @@ constexpr int CLAIM = 128;
 // Live stack entries of the first launch default to SceneView::short_stack = 18:
 // (18 + 1) x 256 x 4 B = 19 KB of LDS per block, 8 blocks (32 waves) per CU.
 
-// rays are rays[idx[i]] for i < *n_ptr when idx is given (the retry launch), else rays[i].
-template <bool ANY, bool STATS>
+// rays are rays[idx[i]] for i < *n_ptr when idx is given (the retry launch, RETRY = true: a
+// separate kernel symbol so that profiles keep the two launches apart), else rays[i].
+template <bool ANY, bool STATS, bool RETRY>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_trace(SceneView sv, const Ray* __restrict__ rays,
                                                   const int* __restrict__ n_ptr, const int* __restrict__ idx,
                                                   HitRec* __restrict__ hits, uint8_t* __restrict__ occ,
@@ -153,7 +154,7 @@ int grid_for(size_t shmem) {
     static int cached_shmem = -1, cached_blocks = 0;
     if (cached_shmem != int(shmem)) {
         int per_cu = 0;
-        check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<ANY, STATS>, BLOCK, shmem),
+        check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<ANY, STATS, false>, BLOCK, shmem),
                   "occupancy");
         if (per_cu < 1) per_cu = 1;
         cached_blocks = per_cu * device_cu_count();
@@ -172,10 +173,10 @@ void launch(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, ui
     int* const rl = cap < full ? retry : nullptr;
     const size_t shmem = size_t(cap + 1) * BLOCK * sizeof(int);
     if (stats) {
-        hipLaunchKernelGGL((k_trace<ANY, true>), dim3(grid_for<ANY, true>(shmem)), dim3(BLOCK), shmem, s, sv, rays, n,
+        hipLaunchKernelGGL((k_trace<ANY, true, false>), dim3(grid_for<ANY, true>(shmem)), dim3(BLOCK), shmem, s, sv, rays, n,
                            nullptr, hits, occ, contrib, radiance, work, err, rl, work + 1, cap, stats);
     } else {
-        hipLaunchKernelGGL((k_trace<ANY, false>), dim3(grid_for<ANY, false>(shmem)), dim3(BLOCK), shmem, s, sv, rays,
+        hipLaunchKernelGGL((k_trace<ANY, false, false>), dim3(grid_for<ANY, false>(shmem)), dim3(BLOCK), shmem, s, sv, rays,
                            n, nullptr, hits, occ, contrib, radiance, work, err, rl, work + 1, cap, stats);
     }
     check_hip(hipGetLastError(), "k_trace launch");
@@ -183,10 +184,10 @@ void launch(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, ui
     // the abandoned rays (count in work[1], claims in work[2]): one block per CU is plenty
     const dim3 g(device_cu_count());
     if (stats) {
-        hipLaunchKernelGGL((k_trace<ANY, true>), g, dim3(BLOCK), shmem_full, s, sv, rays, work + 1, rl, hits, occ,
+        hipLaunchKernelGGL((k_trace<ANY, true, true>), g, dim3(BLOCK), shmem_full, s, sv, rays, work + 1, rl, hits, occ,
                            contrib, radiance, work + 2, err, nullptr, nullptr, full, stats);
     } else {
-        hipLaunchKernelGGL((k_trace<ANY, false>), g, dim3(BLOCK), shmem_full, s, sv, rays, work + 1, rl, hits, occ,
+        hipLaunchKernelGGL((k_trace<ANY, false, true>), g, dim3(BLOCK), shmem_full, s, sv, rays, work + 1, rl, hits, occ,
                            contrib, radiance, work + 2, err, nullptr, nullptr, full, stats);
     }
     check_hip(hipGetLastError(), "k_trace retry launch");
