@@ -134,47 +134,56 @@ def nrc_time_to_rmse(nr, scenes, eps=1e-2, ref_spp=4096, warm=256, k_small=16, k
     def mse():
         return float(((dev.fetch(W, H)["sum"] / dev_spp[0] - ref) ** 2).mean()) - var_ref
     dev_spp = [1]
-    # plain ptnee
-    K = 256
-    nr.synchronize()
-    t0 = time.perf_counter()
-    for k in range(K):
-        dev.render(cam, W, H, 1, seed=5, spp_offset=k, accumulate=k > 0, fetch=False, **kw)
-    nr.synchronize()
-    pt_ms = (time.perf_counter() - t0) * 1e3 / K
-    dev_spp[0] = K
-    pt_s2 = mse() * K
-    pt_time = pt_ms * 1e-3 * pt_s2 / eps ** 2
-    # radiance cache
-    cache = nr.Nrc(seed=1)
-    nr.synchronize()
-    t0 = time.perf_counter()
-    for k in range(warm):
-        dev.render_nrc(cache, cam, W, H, spp=1, seed=3, spp_offset=k, **kw)
-    nr.synchronize()
-    warm_s = time.perf_counter() - t0
-    fit = {}
-    frame = 0
-    for kk in (k_small, k_large):
+
+    def legs(env_is):
+        # plain ptnee
+        K = 256
         nr.synchronize()
         t0 = time.perf_counter()
-        for k in range(kk):
-            st = dev.render_nrc(cache, cam, W, H, spp=1, seed=11, spp_offset=warm + frame, accumulate=k > 0, **kw)
-            frame += 1
+        for k in range(K):
+            dev.render(cam, W, H, 1, seed=5, spp_offset=k, accumulate=k > 0, fetch=False, env_is=env_is, **kw)
         nr.synchronize()
-        fit[kk] = ((time.perf_counter() - t0) * 1e3 / kk, None)
-        dev_spp[0] = kk
-        fit[kk] = (fit[kk][0], mse())
-    s2 = (fit[k_small][1] - fit[k_large][1]) / (1.0 / k_small - 1.0 / k_large)
-    b2 = fit[k_large][1] - s2 / k_large
-    nrc_ms = fit[k_large][0]
-    nrc_time = warm_s + nrc_ms * 1e-3 * s2 / (eps ** 2 - b2) if b2 < eps ** 2 else None
-    return {"eps": eps, "pt_s": pt_time, "nrc_s": nrc_time, "speedup": (pt_time / nrc_time) if nrc_time else None,
-            "pt_frame_ms": pt_ms, "pt_var_per_frame": pt_s2, "nrc_frame_ms": nrc_ms, "nrc_var_per_frame": s2,
-            "nrc_bias2": b2, "nrc_warmup_s": warm_s, "nrc_queries_per_frame": st["queries"],
-            "nrc_trained_per_frame": st["trained"], "reference_spp": ref_spp, "reference_s": ref_s,
-            "workload": "1920x1080, 1 spp per frame, heightfield + spheres scene, maxdepth 10; time to RMSE "
-                        "extrapolated from measured per-frame variance and cache bias"}
+        pt_ms = (time.perf_counter() - t0) * 1e3 / K
+        dev_spp[0] = K
+        pt_s2 = mse() * K
+        pt_time = pt_ms * 1e-3 * pt_s2 / eps ** 2
+        # radiance cache
+        cache = nr.Nrc(seed=1)
+        nr.synchronize()
+        t0 = time.perf_counter()
+        for k in range(warm):
+            dev.render_nrc(cache, cam, W, H, spp=1, seed=3, spp_offset=k, env_is=env_is, **kw)
+        nr.synchronize()
+        warm_s = time.perf_counter() - t0
+        fit = {}
+        frame = 0
+        for kk in (k_small, k_large):
+            nr.synchronize()
+            t0 = time.perf_counter()
+            for k in range(kk):
+                st = dev.render_nrc(cache, cam, W, H, spp=1, seed=11, spp_offset=warm + frame, accumulate=k > 0,
+                                    env_is=env_is, **kw)
+                frame += 1
+            nr.synchronize()
+            fit[kk] = ((time.perf_counter() - t0) * 1e3 / kk, None)
+            dev_spp[0] = kk
+            fit[kk] = (fit[kk][0], mse())
+        s2 = (fit[k_small][1] - fit[k_large][1]) / (1.0 / k_small - 1.0 / k_large)
+        b2 = fit[k_large][1] - s2 / k_large
+        nrc_ms = fit[k_large][0]
+        nrc_time = warm_s + nrc_ms * 1e-3 * s2 / (eps ** 2 - b2) if b2 < eps ** 2 else None
+        return {"pt_s": pt_time, "nrc_s": nrc_time, "speedup": (pt_time / nrc_time) if nrc_time else None,
+                "pt_frame_ms": pt_ms, "pt_var_per_frame": pt_s2, "nrc_frame_ms": nrc_ms, "nrc_var_per_frame": s2,
+                "nrc_bias2": b2, "nrc_warmup_s": warm_s, "nrc_queries_per_frame": st["queries"],
+                "nrc_trained_per_frame": st["trained"]}
+
+    out = {"eps": eps, **legs(False)}
+    out["with_sky_sampling"] = legs(True)
+    out.update({"reference_spp": ref_spp, "reference_s": ref_s,
+                "workload": "1920x1080, 1 spp per frame, heightfield + spheres scene, maxdepth 10; time to RMSE "
+                            "extrapolated from measured per-frame variance and cache bias; with_sky_sampling: "
+                            "both legs with nr_params.env_is (same expected image)"})
+    return out
 
 
 BF16_PEAK_TFLOPS = 2500.0
@@ -375,8 +384,9 @@ def main():
               f"training step {mlp['train_step_ms']:.3f} ms ({mlp['train_tflops']:.0f} TFLOP/s)", file=sys.stderr,
               flush=True)
     if nrc:
-        print(f"# time to RMSE 1e-2 at 1080p: PT {nrc['pt_s']:.1f} s, radiance cache "
-              f"{nrc['nrc_s'] if nrc['nrc_s'] is None else round(nrc['nrc_s'], 1)} s", file=sys.stderr, flush=True)
+        for lab, r in (("", nrc), (" with sky sampling", nrc["with_sky_sampling"])):
+            print(f"# time to RMSE 1e-2 at 1080p{lab}: PT {r['pt_s']:.1f} s, radiance cache "
+                  f"{r['nrc_s'] if r['nrc_s'] is None else round(r['nrc_s'], 1)} s", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(sc, spp)
     out = {
         "metric": "Msamples/s at 720p", "value": value, "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,

@@ -65,7 +65,7 @@ typedef struct nr_params {
     int32_t env_is;        /* 1: also sample the environment map as a light at Lambert and GGX-reflect
                               vertices, weighted against BSDF sampling by the balance heuristic so that
                               the expected image stays the reference's (a variance reduction the
-                              reference does not have; ptnee only, not with the adjoint or the cache) */
+                              reference does not have; ptnee and the radiance cache, not the adjoint) */
 } nr_params;
 
 typedef void (*nr_progress_fn)(float fraction, void* user);

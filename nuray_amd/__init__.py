@@ -276,10 +276,10 @@ class Scene:
 
     def render_nrc(self, cache, cam, W, H, spp=1, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0, spp_offset=0,
                    accumulate=False, train_paths=65536, train_maxlen=5, spread=0.01, use_cache=True, train=True,
-                   train_steps=4, min_depth=1, fetch=False):
+                   train_steps=4, min_depth=1, fetch=False, env_is=False):
         """One radiance-cache frame (nr_render_nrc).  Returns the frame statistics (and the fetched
         image when fetch=True)."""
-        p = nr_params(W, H, spp, spp_offset, seed, prr, mis, maxdepth, rrdepth, 0, int(accumulate))
+        p = nr_params(W, H, spp, spp_offset, seed, prr, mis, maxdepth, rrdepth, 0, int(accumulate), 0, int(env_is))
         c = make_camera(cam)
         npar = nr_nrc_params(train_paths, train_maxlen, spread, int(use_cache), int(train), train_steps, min_depth)
         st = np.zeros(4, np.float64)

@@ -143,7 +143,7 @@ __device__ __forceinline__ uint32_t train_sample(uint32_t frame_sample0, int T, 
 // [P, P + T) training paths from pixels drawn uniformly; every path starts with an empty
 // spread state.
 __global__ void k_raygen_nrc(CamView cam, int W, int H, int npix, int P, int T, uint32_t seed, uint32_t sample0,
-                             Ray* rays, float4* beta, float4* radiance, float4* state) {
+                             Ray* rays, float4* beta, float4* radiance, float4* state, float4* radiance_env) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= P + T) return;
     int pixel;
@@ -161,6 +161,7 @@ __global__ void k_raygen_nrc(CamView cam, int W, int H, int npix, int P, int T, 
     rays[slot].d = make_float4(d.x, d.y, d.z, __int_as_float(slot));
     beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
     radiance[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (radiance_env) radiance_env[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     state[slot] = make_float4(0.0f, 0.0f, 1.0f, __int_as_float(pixel));
 }
 
@@ -365,6 +366,12 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a, const NrcShade& n
                         if (train && !query) {  // the tail vertex answers from the cache: no target
                             emit_rec = true;
                             rec_l = a.radiance[slot];
+                            if (ENV) {  // sky shadow rays of earlier vertices landed in the second region
+                                const float4 E = a.radiance[a.env_base + slot];
+                                rec_l.x += E.x;
+                                rec_l.y += E.y;
+                                rec_l.z += E.z;
+                            }
                         }
                         if (query) {
                             emit_q = true;
@@ -512,6 +519,14 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a, const NrcShade& n
                 nrc.recs[pos[3]] = qrec;
                 nrc.rec_lb[pos[3]] = rec_l;
             }
+            if (ENV) {  // sky shadow rays share the shadow queue
+                int pe, unused;
+                block_append2(s_app, parity, emit_env, false, a.n_shadow, a.n_shadow, pe, unused);
+                if (emit_env) {
+                    a.shadow[pe] = env_ray;
+                    a.contrib[pe] = env_c;
+                }
+            }
         } else if (ENV) {  // light and sky shadow rays share the shadow queue
             const bool e[4] = {emit_sh, emit_next, emit_env, false};
             int* const c[4] = {a.n_shadow, a.n_out, a.n_shadow, a.n_shadow};
@@ -555,6 +570,20 @@ __global__ __launch_bounds__(BLK) void k_shade_full_env(ShadeArgs a) { shade_bod
 // Radiance-cache rendering (spread state, cache queries, training records).
 __global__ __launch_bounds__(BLK) void k_shade_nrc_basic(ShadeArgs a, NrcShade n) { shade_body<0u, true>(a, n); }
 __global__ __launch_bounds__(BLK) void k_shade_nrc_full(ShadeArgs a, NrcShade n) { shade_body<F_ALL, true>(a, n); }
+__global__ __launch_bounds__(BLK) void k_shade_nrc_basic_env(ShadeArgs a, NrcShade n) { shade_body<0u, true, true>(a, n); }
+__global__ __launch_bounds__(BLK) void k_shade_nrc_full_env(ShadeArgs a, NrcShade n) { shade_body<F_ALL, true, true>(a, n); }
+
+// Adds the sky shadow rays' region into the main per-slot radiance.
+__global__ void k_fold_env(float4* __restrict__ radiance, const float4* __restrict__ radiance_env, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 E = radiance_env[i];
+    float4 L = radiance[i];
+    L.x += E.x;
+    L.y += E.y;
+    L.z += E.z;
+    radiance[i] = L;
+}
 
 // Per-pixel reduction of the batch's kb samples: sums of max(0, L) and its square in FP64
 // (rendererptls.cpp:173-175: max(0, trace(...)) per component, which also maps NaN to 0).
@@ -737,7 +766,8 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
     if (!sc.built) throw std::runtime_error("scene not built (call nr_bvh_build_sah)");
     if (p.W <= 0 || p.H <= 0 || p.spp <= 0) throw std::runtime_error("bad image size or spp");
     if (np.train_paths < 0 || np.train_maxlen < 1) throw std::runtime_error("bad radiance-cache parameters");
-    if (p.env_is) throw std::runtime_error("environment sampling is not available with the radiance cache");
+    if (p.env_is && p.integrator != 0) throw std::runtime_error("environment sampling needs the ptnee integrator");
+    const bool env = p.env_is && sc.view.env_w > 0;
     const int npix = p.W * p.H;
     const size_t P = size_t(p.spp) * npix, T = size_t(np.train_paths);
     if (P + T > (size_t(64) << 20)) throw std::runtime_error("radiance-cache frame larger than one wavefront batch");
@@ -750,6 +780,14 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
     t.nrc_recs.ensure(sizeof(NrcQuery) * rec_cap);
     t.nrc_lb.ensure(sizeof(float4) * rec_cap);
     t.nrc_tgt.ensure(sizeof(float4) * rec_cap);
+    if (env) {  // sky shadow rays: second radiance region and shadow-queue half, BSDF-escape weights
+        t.radiance.ensure(sizeof(float4) * 2 * cap);
+        t.shadow.ensure(sizeof(Ray) * 2 * cap);
+        t.contrib.ensure(sizeof(float4) * 2 * cap);
+        t.skyw.ensure(sizeof(float) * cap);
+        t.retry.ensure(sizeof(int) * 2 * cap);
+    }
+    float4* const rad_env = env ? t.radiance.as<float4>() + cap : nullptr;
     t.adj_valid = false;
     hipStream_t s = sc.stream;
     int* cnt = t.counters.as<int>();
@@ -765,7 +803,7 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
     hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, int(C_RAY0), int(cap), -1);
     hipLaunchKernelGGL(k_raygen_nrc, dim3((cap + BLK - 1) / BLK), dim3(BLK), 0, s, cam, p.W, p.H, npix, int(P), int(T),
                        p.seed, uint32_t(p.spp_offset), t.rays[0].as<Ray>(), t.beta[0].as<float4>(),
-                       t.radiance.as<float4>(), t.nstate[0].as<float4>());
+                       t.radiance.as<float4>(), t.nstate[0].as<float4>(), rad_env);
     for (int it = 0;; ++it) {
         const int cur = it & 1, nxt = cur ^ 1;
         hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, -1, 0, int(C_RAY0 + nxt));
@@ -775,13 +813,20 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
                     p.integrator == 1 ? 0 : 1,
                     t.rays[cur].as<Ray>(), t.beta[cur].as<float4>(), t.hits.as<HitRec>(), cnt + C_RAY0 + cur,
                     t.radiance.as<float4>(), t.rays[nxt].as<Ray>(), t.beta[nxt].as<float4>(), cnt + C_RAY0 + nxt,
-                    t.shadow.as<Ray>(), t.contrib.as<float4>(), cnt + C_SHADOW, nullptr, 0};
+                    t.shadow.as<Ray>(), t.contrib.as<float4>(), cnt + C_SHADOW, env ? t.skyw.as<float>() : nullptr,
+                    env ? int(cap) : 0};
         NrcShade ns{t.nstate[cur].as<float4>(), t.nstate[nxt].as<float4>(), t.nrc_queries.as<NrcQuery>(),
                     t.nrc_recs.as<NrcQuery>(), t.nrc_lb.as<float4>(), cnt + C_QUERY, cnt + C_REC, int(cap),
                     int(rec_cap), int(P), int(T), np.c_spread, np.train_maxlen, np.use_cache,
                     std::max(1, np.min_depth)};
-        if (sc.features == 0) hipLaunchKernelGGL(k_shade_nrc_basic, dim3(grid_shade), dim3(BLK), 0, s, a, ns);
-        else hipLaunchKernelGGL(k_shade_nrc_full, dim3(grid_shade), dim3(BLK), 0, s, a, ns);
+        if (env) {
+            if (sc.features == 0) hipLaunchKernelGGL(k_shade_nrc_basic_env, dim3(grid_shade), dim3(BLK), 0, s, a, ns);
+            else hipLaunchKernelGGL(k_shade_nrc_full_env, dim3(grid_shade), dim3(BLK), 0, s, a, ns);
+        } else if (sc.features == 0) {
+            hipLaunchKernelGGL(k_shade_nrc_basic, dim3(grid_shade), dim3(BLK), 0, s, a, ns);
+        } else {
+            hipLaunchKernelGGL(k_shade_nrc_full, dim3(grid_shade), dim3(BLK), 0, s, a, ns);
+        }
         launch_shadow(sc.view, t.shadow.as<Ray>(), cnt + C_SHADOW, nullptr, nullptr, t.contrib.as<float4>(),
                       t.radiance.as<float4>(), cnt + C_WORK1, cnt + C_ERR, nullptr, s, t.retry.as<int>());
         rs.iterations++;
@@ -793,6 +838,9 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
             if (*t.pinned == 0) break;
         }
     }
+    if (env)
+        hipLaunchKernelGGL(k_fold_env, dim3((cap + BLK - 1) / BLK), dim3(BLK), 0, s, t.radiance.as<float4>(), rad_env,
+                           int(cap));
     // the cache answers every stopped path (render paths and training-path tails)
     nrc.infer_accumulate(t.nrc_queries.as<NrcQuery>(), cnt + C_QUERY, int(cap), t.radiance.as<float4>(), s);
     hipLaunchKernelGGL(k_finalize, dim3((npix + BLK - 1) / BLK), dim3(BLK), 0, s, t.radiance.as<float4>(), nullptr, npix,
