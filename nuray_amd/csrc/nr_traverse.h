@@ -66,8 +66,7 @@ __device__ __forceinline__ void trav_expand(const SceneView& sv, const SlabRay& 
 // Every call makes progress: while no lane holds a leaf the loop pops an entry per searching lane.
 // The set of nodes and leaves visited is the same as in the sequential DFS (only their interleaving
 // changes), and the result does not depend on the order.
-constexpr int LEAF_ROUND_SEARCHERS = 16;
-template <bool STATS, int STRIDE>
+template <bool STATS, int STRIDE, int LEAF_ROUND_SEARCHERS = 16>
 __device__ __forceinline__ void trav_expand_wave(const SceneView& sv, const SlabRay& sr, int* stk, int cap,
                                                  TravState& s, uint32_t& n_pf) {
     bool parked = false;

@@ -36,6 +36,9 @@ constexpr int BLOCK = 256;
 // rays at once instead of a few waves working through long private ranges (the launch then
 // takes about one ray's latency, not several).
 constexpr int REFILL_MIN = 16;
+// Leaf-round threshold of the wave-cooperative expansion for shadow rays (closest hit: 16; a sweep
+// of 8 / 16 / 24 / 32 put both optimum near these values).
+constexpr int SHADOW_SEARCHERS = 8;
 constexpr int CLAIM = 128;
 // Live stack entries of the first launch default to SceneView::short_stack = 18:
 // (18 + 1) x 256 x 4 B = 19 KB of LDS per block, 8 blocks (32 waves) per CU.
@@ -111,7 +114,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
             continue;
         }
         if (i >= 0) {
-            trav_expand_wave<STATS, BLOCK>(sv, sr, stk, cap, ts, n_pf);
+            trav_expand_wave<STATS, BLOCK, ANY ? SHADOW_SEARCHERS : 16>(sv, sr, stk, cap, ts, n_pf);
             if (ts.has_leaf) done = trav_leaf<ANY, STATS>(sv, o, d, double(wr), ts, bt, bb1, bb2, bprim, blf, n_tt);
             // ---- finished: write the result, go idle
             if ((ts.sp == 0 && !ts.has_leaf) || done || ts.overflow) {
