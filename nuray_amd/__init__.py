@@ -274,6 +274,17 @@ class Scene:
         rgb = _f32(rgb)
         check(lib().nr_scene_set_texture_rgb32f(self.h, tex_id, rgb.shape[1], rgb.shape[0], rgb))
 
+    def set_texture_rgba8(self, tex_id, rgba):
+        t = np.ascontiguousarray(rgba, np.uint8)
+        check(lib().nr_scene_set_texture_rgba8(self.h, tex_id, t.shape[1], t.shape[0], t))
+
+    def set_envmap(self, rgb, gain=1.0):
+        rgb = _f32(rgb)
+        check(lib().nr_scene_set_envmap(self.h, rgb.shape[1], rgb.shape[0], rgb, float(gain)))
+
+    def set_material_ext(self, mat_id, ext):
+        check(lib().nr_scene_set_material_ext(self.h, int(mat_id), _f32(ext)))
+
     def render_nrc(self, cache, cam, W, H, spp=1, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0, spp_offset=0,
                    accumulate=False, train_paths=65536, train_maxlen=5, spread=0.01, use_cache=True, train=True,
                    train_steps=4, min_depth=1, fetch=False, env_is=False):

@@ -100,12 +100,14 @@ int nr_scene_set_texture_rgb32f(nr_scene* s, int32_t id, int32_t w, int32_t h, c
 
 int nr_scene_set_envmap(nr_scene* s, int32_t w, int32_t h, const float* rgb, float gain) {
     return guard([&] {
-        // the environment map lives in the texture pool after the albedo textures
-        int id = int(s->sc->textures.size());
-        if (s->sc->env_tex >= 0) id = s->sc->env_tex;
-        s->sc->add_texture(id, w, h, 1, rgb);
-        s->sc->env_tex = id;
-        s->sc->env_gain = gain;
+        // the environment map lives in the texture pool after the albedo textures; on a built
+        // scene the texels (and the sky sampler) are replaced without rebuilding the BVH
+        nr::Scene& sc = *s->sc;
+        const int id = sc.env_tex >= 0 ? sc.env_tex : int(sc.textures.size());
+        sc.env_tex = id;
+        sc.env_gain = gain;
+        sc.view.env_gain = gain;
+        sc.add_texture(id, w, h, 1, rgb);
         return NR_OK;
     });
 }

@@ -84,7 +84,7 @@ std::vector<MatRec> Scene::mat_records() const {
 void Scene::set_material_ext(int id, const float* ext) {
     if (id < 0 || size_t(id) >= mats.size() / 8) throw std::runtime_error("material id out of range");
     for (int k = 0; k < 4; ++k) mats_ext[4 * size_t(id) + k] = ext[k];
-    built = false;
+    if (built) upload_materials();
 }
 
 int Scene::add_texture(int id, int w, int h, int fmt, const void* data) {
@@ -103,13 +103,14 @@ int Scene::add_texture(int id, int w, int h, int fmt, const void* data) {
         if (id == env_tex) build_env_sampler();
         return id;
     }
-    built = false;
+    if (built) upload_textures();  // a new or resized texture: new pool, the BVH stays
     return id;
 }
 
-void Scene::build(int nthreads) {
+// Every texture a material or the environment names must exist (kernels index without checks);
+// the material feature bits select the shade kernels.
+void Scene::check_materials() {
     const int N = n_tris();
-    // every texture a material or the environment names must exist (kernels index without checks)
     for (size_t j = 0; j < mats.size() / 8; ++j) {
         const int tex = int(mats[8 * j + 6]);
         if (tex >= 0 && (tex >= int(textures.size()) || textures[tex].w <= 0))
@@ -129,6 +130,49 @@ void Scene::build(int nthreads) {
     }
     if (env_tex >= 0 && (env_tex >= int(textures.size()) || textures[env_tex].w <= 0))
         throw std::runtime_error("environment map missing");
+}
+
+void Scene::upload_materials(bool check) {
+    if (check) check_materials();
+    const std::vector<MatRec> mr = mat_records();
+    d_mats.upload(mr.data(), mr.size() * sizeof(MatRec), stream);
+    check_hip(hipStreamSynchronize(stream), "material upload");
+    view.mats = d_mats.as<MatRec>();
+}
+
+// Texture pool (every texture 256-byte aligned), descriptors, the RGBA8 decode table and the sky
+// sampler; points the view at them.
+void Scene::upload_textures(bool check) {
+    if (check) check_materials();
+    std::vector<TexDesc> td(textures.size());
+    size_t pool = 0;
+    for (size_t j = 0; j < textures.size(); ++j) {
+        td[j] = TexDesc{textures[j].w, textures[j].h, textures[j].fmt, 0, pool};
+        pool += (textures[j].bytes.size() + 255) & ~size_t(255);
+    }
+    tex_offsets.resize(textures.size());
+    for (size_t j = 0; j < textures.size(); ++j) tex_offsets[j] = td[j].offset;
+    std::vector<uint8_t> poolh(pool);
+    for (size_t j = 0; j < textures.size(); ++j)
+        std::memcpy(poolh.data() + td[j].offset, textures[j].bytes.data(), textures[j].bytes.size());
+    // texture.cpp:18-20 decode: pow(c / 255, 2.2f) in double, stored as float
+    float lut[256];
+    for (int c = 0; c < 256; ++c) lut[c] = float(std::pow(double(c) / 255.0, double(2.2f)));
+    d_tex.upload(td.data(), td.size() * sizeof(TexDesc), stream);
+    d_texpool.upload(poolh.data(), poolh.size(), stream);
+    d_lut.upload(lut, sizeof(lut), stream);
+    check_hip(hipStreamSynchronize(stream), "texture upload");
+    view.tex = d_tex.as<TexDesc>();
+    view.texpool = d_texpool.as<uint8_t>();
+    view.lut = d_lut.as<float>();
+    view.env_tex = env_tex;
+    view.env_gain = env_gain;
+    build_env_sampler();
+}
+
+void Scene::build(int nthreads) {
+    const int N = n_tris();
+    check_materials();
     bvh = build_sah(p.data(), N, nthreads);
 
     // LightSampler::setPrimitives (lightsampler.cpp:4-39): emissive triangles in loader order,
@@ -185,53 +229,27 @@ void Scene::build(int nthreads) {
         s[k].s2 = make_float4(nn[8], t[0], t[1], t[2]);
         s[k].s3 = make_float4(t[3], t[4], t[5], mf);
     }
-    const std::vector<MatRec> mr = mat_records();
-    // texture pool
-    std::vector<TexDesc> td(textures.size());
-    size_t pool = 0;
-    for (size_t j = 0; j < textures.size(); ++j) {
-        td[j] = TexDesc{textures[j].w, textures[j].h, textures[j].fmt, 0, pool};
-        pool += (textures[j].bytes.size() + 255) & ~size_t(255);
-    }
-    tex_offsets.resize(textures.size());
-    for (size_t j = 0; j < textures.size(); ++j) tex_offsets[j] = td[j].offset;
-    std::vector<uint8_t> poolh(pool);
-    for (size_t j = 0; j < textures.size(); ++j)
-        std::memcpy(poolh.data() + td[j].offset, textures[j].bytes.data(), textures[j].bytes.size());
-    // texture.cpp:18-20 decode: pow(c / 255, 2.2f) in double, stored as float
-    float lut[256];
-    for (int c = 0; c < 256; ++c) lut[c] = float(std::pow(double(c) / 255.0, double(2.2f)));
-
     d_nodes.upload(bvh.pairs.data(), bvh.pairs.size() * sizeof(NodePair), stream);
     d_tris.upload(g.data(), g.size() * sizeof(TriGeom), stream);
     d_shade.upload(s.data(), s.size() * sizeof(TriShade), stream);
-    d_mats.upload(mr.data(), mr.size() * sizeof(MatRec), stream);
     d_lights.upload(lights.data(), lights.size() * sizeof(LightRec), stream);
     d_cdf.upload(light_cdf.data(), light_cdf.size() * sizeof(float), stream);
-    d_tex.upload(td.data(), td.size() * sizeof(TexDesc), stream);
-    d_texpool.upload(poolh.data(), poolh.size(), stream);
-    d_lut.upload(lut, sizeof(lut), stream);
     check_hip(hipStreamSynchronize(stream), "upload sync");
 
     view = SceneView{};
     view.nodes = d_nodes.as<NodePair>();
     view.tris = d_tris.as<TriGeom>();
     view.shade = d_shade.as<TriShade>();
-    view.mats = d_mats.as<MatRec>();
     view.lights = d_lights.as<LightRec>();
     view.light_cdf = d_cdf.as<float>();
-    view.tex = d_tex.as<TexDesc>();
-    view.texpool = d_texpool.as<uint8_t>();
-    view.lut = d_lut.as<float>();
     for (int j = 0; j < 3; ++j) { view.root_min[j] = bvh.root_min[j]; view.root_max[j] = bvh.root_max[j]; }
     view.root_ref = N ? bvh.root_ref : make_leaf(0, 0);
     view.n_lights = int(lights.size());
     view.light_p = float(1.0 / (double(light_sum) + 1e-8));  // LightSampler::p, lightsampler.cpp:100-103
-    view.env_tex = env_tex;
-    view.env_gain = env_gain;
     view.stack_cap = bvh.depth + 2;
     view.short_stack = short_stack;
-    build_env_sampler();
+    upload_materials(false);  // checked above
+    upload_textures(false);
     built = true;
 }
 

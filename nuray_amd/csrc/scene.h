@@ -69,6 +69,11 @@ public:
     std::vector<MatRec> mat_records() const;  // device material table
     // Host SAH build (reference rule), light sampler, upload of every device array.
     void build(int nthreads);
+    // Material table / texture pool (and sky sampler) uploads; on a built scene a texture or
+    // material change re-uploads only these, the BVH stays.
+    void check_materials();
+    void upload_materials(bool check = true);
+    void upload_textures(bool check = true);
     // Alias table over the environment map's cells (empty without a map or with a black one);
     // uploads it and points view.env_tab at it.
     void build_env_sampler();
