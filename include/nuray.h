@@ -82,17 +82,20 @@ int nr_synchronize(void);
 nr_scene* nr_scene_create(const float* p, const float* n, const float* uv, const int32_t* mat_id,
                           uint32_t n_tris, const float* mats, uint32_t n_mats);
 int nr_scene_destroy(nr_scene* s);
-/* Texture::load result (texture.cpp:5-31): RGBA8 texels, row-major, first row at v = 1. */
+/* Texture::load result (texture.cpp:5-31): RGBA8 texels, row-major, first row at v = 1.
+ * Textures, material rows and the environment map may change after nr_bvh_build_sah: the texture
+ * pool and material table are re-uploaded, the BVH stays. */
 int nr_scene_set_texture_rgba8(nr_scene* s, int32_t id, int32_t w, int32_t h, const uint8_t* rgba);
 /* MatBlinnPhong's specular colour (matblinnphong.h:16-17): ext = {Ks r, g, b, Ks texture or -1}
- * for material row `id` (the loader's map_Ks).  Call before nr_bvh_build_sah. */
+ * for material row `id` (the loader's map_Ks). */
 int nr_scene_set_material_ext(nr_scene* s, int32_t id, const float* ext);
 /* Linear RGB float albedo texture (3 floats per texel, first row at v = 1): the form to optimise
  * with nr_adjoint_ptnee_tex.  Setting a texture of the same size and format on a built scene
  * rewrites it in place (no rebuild).  No reference counterpart (the reference's textures are 8-bit). */
 int nr_scene_set_texture_rgb32f(nr_scene* s, int32_t id, int32_t w, int32_t h, const float* rgb);
 /* Renderer::setEnvmap / Envmap(texture, gain) (renderer.h:30, envmap.cpp:17-24): linear RGB float
- * texels, row-major, looked up with the reference's bilinear rule. */
+ * texels, row-major, looked up with the reference's bilinear rule.  Allowed before or after
+ * nr_bvh_build_sah (replacing the map rebuilds the sky sampler of nr_params.env_is). */
 int nr_scene_set_envmap(nr_scene* s, int32_t w, int32_t h, const float* rgb, float gain);
 
 /* Renderer::prepare (renderer.cpp:30-39): LightSampler::setPrimitives + BVH::build with the
