@@ -460,6 +460,7 @@ public:
         if (cache_) nr_nrc_destroy(cache_);
     }
     nr_scene* handle() const { return s_; }
+    const std::string& name() const { return name_; }
 
     void setEnvmap(const Envmap& e) {
         env_ = e;
@@ -581,9 +582,11 @@ struct RenderFunction {
     static std::vector<float> forward(RenderContext& ctx, Renderer& r, const Camera& cam, const Config& cfg,
                                       const std::vector<float>& theta) {
         if (!r.handle()) throw Error("forward before prepare");
+        if (r.name() == "nrc") throw Error("the radiance-cache renderer has no adjoint (use ptnee or pt)");
         ctx.scene = r.handle();
         ctx.cam = cam.c();
         ctx.params = Renderer::params_of(cfg);
+        ctx.params.integrator = r.name() == "pt" ? 1 : 0;
         ctx.params.batch_spp = 0;
         ctx.params.accumulate = 0;
         ctx.n_mats = uint32_t(theta.size() / 4);
