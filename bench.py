@@ -60,8 +60,10 @@ def cpu_baseline(sc, spp_frame, target_s=12.0):
 
 def adjoint_iteration(nr, scenes, iters=20, warmup=5):
     """The inverse-rendering iteration on the 64-material box room: upload theta, 256x256 render
-    at 32 spp, L2 loss against a 1024-spp target, path-replay adjoint to the 256 parameters, Adam
-    step on the host.  Returns ms per iteration and its parts."""
+    at 32 spp (one lane per path: the same image as the wavefront, bit for bit, without a launch
+    per bounce), L2 loss against a 1024-spp target, path-replay adjoint to the 256 parameters,
+    Adam step on the host.  Returns ms per iteration and its parts (the wavefront render of the
+    same frame timed beside it)."""
     sc, theta_star, theta0 = scenes.adjoint_problem(width=256, height=256)
     W, H, M, cam = 256, 256, len(theta0), sc["camera"]
     dev = nr.Scene(sc)
@@ -75,7 +77,7 @@ def adjoint_iteration(nr, scenes, iters=20, warmup=5):
     def step(it):
         nonlocal theta, m, v
         dev.set_params(theta.astype(np.float32))
-        dev.render(cam, W, H, 32, seed=it, fetch=False)
+        dev.render(cam, W, H, 32, seed=it, fetch=False, path_kernel=True)
         dev.loss_l2(tgt)
         g = dev.adjoint(cam, W, H, 32, M, seed=it)
         g[emit] = 0
@@ -95,16 +97,21 @@ def adjoint_iteration(nr, scenes, iters=20, warmup=5):
     nr.synchronize()
     t0 = time.perf_counter()
     for it in range(iters):
-        dev.render(cam, W, H, 32, seed=it, fetch=False)
+        dev.render(cam, W, H, 32, seed=it, fetch=False, path_kernel=True)
     nr.synchronize()
     render_ms = (time.perf_counter() - t0) * 1e3 / iters
+    t0 = time.perf_counter()
+    for it in range(iters):
+        dev.render(cam, W, H, 32, seed=it, fetch=False)
+    nr.synchronize()
+    wavefront_ms = (time.perf_counter() - t0) * 1e3 / iters
     dev.loss_l2(tgt)
     t0 = time.perf_counter()
     for it in range(iters):
         dev.adjoint(cam, W, H, 32, M, seed=iters - 1)
     nr.synchronize()
     adjoint_ms = (time.perf_counter() - t0) * 1e3 / iters
-    return {"ms_per_iter": ms, "render_ms": render_ms, "adjoint_ms": adjoint_ms,
+    return {"ms_per_iter": ms, "render_ms": render_ms, "adjoint_ms": adjoint_ms, "wavefront_render_ms": wavefront_ms,
             "workload": "box room, 64 GGX-reflect materials (256 parameters), 256x256 at 32 spp per "
                         "iteration, L2 loss, path-replay adjoint, Adam lr 1e-2"}
 
@@ -371,7 +378,8 @@ def main():
     adj = None if args.frames_only else adjoint_iteration(nr, scenes)
     if adj:
         print(f"# adjoint iteration {adj['ms_per_iter']:.2f} ms (render {adj['render_ms']:.2f} ms, "
-              f"replay {adj['adjoint_ms']:.2f} ms)", file=sys.stderr, flush=True)
+              f"replay {adj['adjoint_ms']:.2f} ms; wavefront render {adj['wavefront_render_ms']:.2f} ms)",
+              file=sys.stderr, flush=True)
     env = None if args.frames_only else env_sampling(nr, dev, sc, spp)
     if env:
         print(f"# sky sampling: {env['on_ms']:.1f} ms vs {env['off_ms']:.1f} ms per frame, variance "

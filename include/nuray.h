@@ -66,6 +66,10 @@ typedef struct nr_params {
                               vertices, weighted against BSDF sampling by the balance heuristic so that
                               the expected image stays the reference's (a variance reduction the
                               reference does not have; ptnee and the radiance cache, not the adjoint) */
+    int32_t path_kernel;   /* 1: trace each sample's whole path in one lane (one launch per batch) instead of
+                              the wavefront (one launch per bounce); the per-sample radiance and the image
+                              are bit-identical.  Pays off for small frames of long paths, e.g. inverse
+                              rendering of a closed room; not with env_is or the radiance cache */
 } nr_params;
 
 typedef void (*nr_progress_fn)(float fraction, void* user);

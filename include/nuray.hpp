@@ -517,6 +517,7 @@ public:
         p.maxdepth = cfg.getValueInt("maxdepth", 0);
         p.rrdepth = cfg.getValueInt("rrdepth", 0);
         p.env_is = cfg.getValueInt("envmap_is", 0);
+        p.path_kernel = cfg.getValueInt("path_kernel", 0);
         return p;
     }
 

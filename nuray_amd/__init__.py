@@ -41,7 +41,7 @@ class nr_params(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("spp_offset", C.c_int32),
                 ("seed", C.c_uint32), ("prr", C.c_float), ("mis", C.c_int32), ("maxdepth", C.c_int32),
                 ("rrdepth", C.c_int32), ("batch_spp", C.c_int32), ("accumulate", C.c_int32),
-                ("integrator", C.c_int32), ("env_is", C.c_int32)]
+                ("integrator", C.c_int32), ("env_is", C.c_int32), ("path_kernel", C.c_int32)]
 
 
 class nr_nrc_params(C.Structure):
@@ -220,9 +220,10 @@ class Scene:
         return tri, t, b1, b2
 
     def render(self, cam, W, H, spp, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0, spp_offset=0, batch_spp=0,
-               accumulate=False, fetch=True, progress=None, cancel=None, integrator=0, env_is=False):
+               accumulate=False, fetch=True, progress=None, cancel=None, integrator=0, env_is=False,
+               path_kernel=False):
         p = nr_params(W, H, spp, spp_offset, seed, prr, mis, maxdepth, rrdepth, batch_spp, int(accumulate), integrator,
-                      int(env_is))
+                      int(env_is), int(path_kernel))
         c = make_camera(cam)
         cb = PROGRESS(lambda f, u: progress(f)) if progress else None
         rc = check(lib().nr_render_ptnee(self.h, C.byref(c), C.byref(p), None, None, None, None,

@@ -14,5 +14,9 @@ double loss_l2(Scene& sc, const float* target, float* dldi_host);
 // (w * h * 3 doubles), by replaying the samples described by cam / p.
 void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const float* dldi_host, double* grad,
                    int tex_id = -1, double* gtex = nullptr);
+// Forward-only render of the P samples from sample0 on, one lane per path (RenderParams::path_kernel):
+// radiance[slot] for slot = sample * W * H + pixel, exactly the wavefront's per-sample radiance.
+void render_paths(Scene& sc, const CamView& cam, const RenderParams& p, uint32_t sample0, int P, float4* radiance,
+                  int* err);
 
 }  // namespace nr

@@ -105,6 +105,21 @@ __device__ __forceinline__ GgxParts mat_parts(const SceneView& sv, const Mat& m,
     return r;
 }
 
+// f and pdf at one vertex: with derivatives for the gradient pass; for the forward pass exactly
+// the wavefront shade kernel's bsdf_eval / bsdf_pdf, so that a forward replay reproduces the
+// wavefront's per-sample radiance bit for bit.
+template <unsigned F, bool GRAD>
+__device__ __forceinline__ GgxParts vertex_parts(const SceneView& sv, const Mat& m, F3 wo, F3 n, F3 wi, float u,
+                                                 float v) {
+    if (GRAD) return mat_parts<F>(sv, m, wo, n, wi, u, v);
+    GgxParts r;
+    r.f = bsdf_eval<F>(sv, m, wo, n, wi, u, v);
+    r.pdf = bsdf_pdf<F>(m, wo, n, wi);
+    r.df_dkd = r.df_da = f3(0.0f, 0.0f, 0.0f);
+    r.dpdf_da = 0.0f;
+    return r;
+}
+
 // d log p(wi) / d alpha of the reference GGX sampler at a fixed direction.
 __device__ __forceinline__ float ggx_score(float a, F3 wo, F3 n, F3 wi) {
     const F3 h = normalized(add(wo, wi));
@@ -241,7 +256,7 @@ __device__ __forceinline__ F3 replay(const AdjArgs& a, int slot, F3 Lt, F3 g, in
             const F3 wl = normalized(lv);
             const float lcos = dot(ln, neg(wl));
             const float ncos = dot(nrm, normalized(sub(lp, hp)));
-            const GgxParts pl = mat_parts<F>(sv, m, wo, nrm, wl, tu, tv);
+            const GgxParts pl = vertex_parts<F, GRAD>(sv, m, wo, nrm, wl, tu, tv);
             const float lpdf = sv.light_p;
             const float lop = lpdf * dot(lv, lv) / ((1e-6f < lcos) ? lcos : 1e-6f);
             float mis = lop / (lop + pl.pdf);
@@ -282,7 +297,7 @@ __device__ __forceinline__ F3 replay(const AdjArgs& a, int slot, F3 Lt, F3 g, in
         if (!cont) break;
         const float4 ub = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 1u);
         const F3 wi = bsdf_sample<F>(m, ub.x, ub.y, ub.z, wo, nrm);
-        const GgxParts pi = mat_parts<F>(sv, m, wo, nrm, wi, tu, tv);
+        const GgxParts pi = vertex_parts<F, GRAD>(sv, m, wo, nrm, wi, tu, tv);
         const float cw = fabsf(dot(wi, nrm));
         const F3 nb = f3(beta.x * cw * pi.f.x / pi.pdf / prr, beta.y * cw * pi.f.y / pi.pdf / prr,
                          beta.z * cw * pi.f.z / pi.pdf / prr);
@@ -353,6 +368,23 @@ __global__ __launch_bounds__(ABLK) void k_adjoint(AdjArgs a) {
             if (gacc[k] != 0.0f) atomicAdd(a.grad + k, double(gacc[k]));
 }
 
+// Forward-only render, one lane per path: each lane replays its samples' paths to the end (the
+// radiance is the wavefront's, bit for bit: same Philox numbers, same traversal, same BSDF code,
+// same order of additions) and writes it to radiance[slot].  For small frames of long paths
+// (inverse rendering on a closed room) this avoids the wavefront's per-bounce launches and tails.
+template <unsigned F>
+__global__ __launch_bounds__(ABLK) void k_render_paths(AdjArgs a, float4* __restrict__ radiance) {
+    extern __shared__ char smem[];
+    int* stk = reinterpret_cast<int*>(smem) + threadIdx.x;
+    bool overflow = false;
+    for (int slot = a.slot0 + blockIdx.x * blockDim.x + threadIdx.x; slot < a.slot1; slot += gridDim.x * blockDim.x) {
+        const F3 L = replay<F, false>(a, slot, f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), stk, nullptr, false,
+                                      overflow);
+        radiance[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+    }
+    if (overflow) atomicOr(a.err, 1);
+}
+
 // L2 loss (mean over pixels and channels) of the rendered mean against the target, and its
 // gradient with respect to the mean image.
 __global__ void k_loss_l2(const double* __restrict__ sum, double inv_spp, const float* __restrict__ target, size_t n3,
@@ -413,6 +445,31 @@ double loss_l2(Scene& sc, const float* target, float* dldi_host) {
         check_hip(hipMemcpyAsync(dldi_host, t.adj_dldi.ptr, n3 * sizeof(float), hipMemcpyDeviceToHost, sc.stream), "D2H");
     check_hip(hipStreamSynchronize(sc.stream), "loss sync");
     return loss;
+}
+
+void render_paths(Scene& sc, const CamView& cam, const RenderParams& p, uint32_t sample0, int P, float4* radiance,
+                  int* err) {
+    if (p.env_is) throw std::runtime_error("the path kernel does not sample the sky (env_is)");
+    AdjArgs a{sc.view, cam, p.W, p.H, p.W * p.H, P, p.seed, sample0, p.prr, p.mis, p.maxdepth, p.rrdepth, nullptr,
+              0, P, nullptr, 0.0f, 0, nullptr, err, -1, nullptr, p.integrator == 1 ? 0 : 1};
+    const size_t shmem = size_t(sc.view.stack_cap + 1) * ABLK * sizeof(int);
+    if (shmem > 160 * 1024) throw std::runtime_error("BVH too deep for the LDS traversal stack");
+    const unsigned feat = sc.features & (F_GGX | F_EXT) ? F_ALL : (sc.features & F_TEX ? F_TEX : 0u);
+    int per_cu = 0;
+    check_hip(feat == F_ALL   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_paths<F_ALL>, ABLK, shmem)
+              : feat == F_TEX ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_paths<F_TEX>, ABLK, shmem)
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_paths<0u>, ABLK, shmem),
+              "occupancy");
+    const int grid = std::max(1, per_cu) * device_cu_count();
+    const long long chunk = 16ll << 20;  // launches of at most 16 M samples
+    for (long long s0 = 0; s0 < P; s0 += chunk) {
+        a.slot0 = int(s0);
+        a.slot1 = int(std::min<long long>(P, s0 + chunk));
+        if (feat == F_ALL) hipLaunchKernelGGL(k_render_paths<F_ALL>, dim3(grid), dim3(ABLK), shmem, sc.stream, a, radiance);
+        else if (feat == F_TEX) hipLaunchKernelGGL(k_render_paths<F_TEX>, dim3(grid), dim3(ABLK), shmem, sc.stream, a, radiance);
+        else hipLaunchKernelGGL(k_render_paths<0u>, dim3(grid), dim3(ABLK), shmem, sc.stream, a, radiance);
+        check_hip(hipGetLastError(), "k_render_paths launch");
+    }
 }
 
 void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const float* dldi_host, double* grad_host,

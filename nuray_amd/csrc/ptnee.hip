@@ -21,6 +21,7 @@
 #include "nr_envis.h"
 #include "nr_material.h"
 #include "nrc.h"
+#include "adjoint.h"
 #include "render.h"
 
 namespace nr {
@@ -645,6 +646,7 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
     // BSDF-escape weights; without an environment sampler (no map, or a black one) it is a no-op
     const bool env = p.env_is && sc.view.env_w > 0;
     if (p.env_is && p.integrator != 0) throw std::runtime_error("environment sampling needs the ptnee integrator");
+    if (p.path_kernel && p.env_is) throw std::runtime_error("the path kernel does not sample the sky (env_is)");
     if (env) {
         const size_t c = size_t(kb) * npix;
         t.radiance.ensure(sizeof(float4) * 2 * c);
@@ -681,47 +683,54 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
         if (cancel && *cancel) { status = 1; break; }
         const int k = std::min(kb, p.spp - b);
         const int P = k * npix;
-        hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, int(C_RAY0), P, -1);
         float4* const rad_env = env ? t.radiance.as<float4>() + P : nullptr;
-        hipLaunchKernelGGL(k_raygen, dim3((P + BLK - 1) / BLK), dim3(BLK), 0, s, cam, p.W, p.H, npix, P, p.seed,
-                           uint32_t(p.spp_offset + b), t.rays[0].as<Ray>(), t.beta[0].as<float4>(),
-                           t.radiance.as<float4>(), rad_env);
-        for (int it = 0;; ++it) {
-            const int cur = it & 1, nxt = cur ^ 1;
-            hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, -1, 0, int(C_RAY0 + nxt));
+        if (p.path_kernel) {  // one lane per path: the same per-sample radiance without per-bounce launches
             if (kt) mark(ev_c);
-            launch_closest(sc.view, t.rays[cur].as<Ray>(), cnt + C_RAY0 + cur, t.hits.as<HitRec>(), cnt + C_WORK0,
-                           cnt + C_ERR, ts, s, t.retry.as<int>());
+            render_paths(sc, cam, p, uint32_t(p.spp_offset + b), P, t.radiance.as<float4>(), cnt + C_ERR);
             if (kt) mark(ev_c);
-            ShadeArgs a{sc.view, npix, p.seed, uint32_t(p.spp_offset + b), p.prr, p.mis, p.maxdepth, p.rrdepth,
-                        p.integrator == 1 ? 0 : 1,
-                        t.rays[cur].as<Ray>(), t.beta[cur].as<float4>(), t.hits.as<HitRec>(), cnt + C_RAY0 + cur,
-                        t.radiance.as<float4>(), t.rays[nxt].as<Ray>(), t.beta[nxt].as<float4>(), cnt + C_RAY0 + nxt,
-                        t.shadow.as<Ray>(), t.contrib.as<float4>(), cnt + C_SHADOW,
-                        env ? t.skyw.as<float>() : nullptr, env ? P : 0};
-            if (kt) mark(ev_h);
-            if (env) {
-                if (sc.features == 0) hipLaunchKernelGGL(k_shade_basic_env, dim3(grid_shade), dim3(BLK), 0, s, a);
-                else hipLaunchKernelGGL(k_shade_full_env, dim3(grid_shade), dim3(BLK), 0, s, a);
-            } else if (sc.features == 0) {
-                hipLaunchKernelGGL(k_shade_basic, dim3(grid_shade), dim3(BLK), 0, s, a);
-            } else {
-                hipLaunchKernelGGL(k_shade_full, dim3(grid_shade), dim3(BLK), 0, s, a);
-            }
-            if (kt) mark(ev_h);
-            if (kt) mark(ev_s);
-            launch_shadow(sc.view, t.shadow.as<Ray>(), cnt + C_SHADOW, nullptr, nullptr, t.contrib.as<float4>(),
-                          t.radiance.as<float4>(), cnt + C_WORK1, cnt + C_ERR, ts ? ts + 1 : nullptr, s,
-                          t.retry.as<int>());
-            if (kt) mark(ev_s);
             rs.iterations++;
-            if (p.maxdepth > 0) {
-                if (it + 1 >= p.maxdepth) break;
-            } else {
-                check_hip(hipMemcpyAsync(t.pinned, cnt + C_RAY0 + nxt, sizeof(int), hipMemcpyDeviceToHost, s), "cnt");
-                check_hip(hipStreamSynchronize(s), "sync");
-                if (*t.pinned == 0) break;
-                if (cancel && *cancel) { status = 1; break; }
+        } else {
+            hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, int(C_RAY0), P, -1);
+            hipLaunchKernelGGL(k_raygen, dim3((P + BLK - 1) / BLK), dim3(BLK), 0, s, cam, p.W, p.H, npix, P, p.seed,
+                               uint32_t(p.spp_offset + b), t.rays[0].as<Ray>(), t.beta[0].as<float4>(),
+                               t.radiance.as<float4>(), rad_env);
+            for (int it = 0;; ++it) {
+                const int cur = it & 1, nxt = cur ^ 1;
+                hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, -1, 0, int(C_RAY0 + nxt));
+                if (kt) mark(ev_c);
+                launch_closest(sc.view, t.rays[cur].as<Ray>(), cnt + C_RAY0 + cur, t.hits.as<HitRec>(), cnt + C_WORK0,
+                               cnt + C_ERR, ts, s, t.retry.as<int>());
+                if (kt) mark(ev_c);
+                ShadeArgs a{sc.view, npix, p.seed, uint32_t(p.spp_offset + b), p.prr, p.mis, p.maxdepth, p.rrdepth,
+                            p.integrator == 1 ? 0 : 1,
+                            t.rays[cur].as<Ray>(), t.beta[cur].as<float4>(), t.hits.as<HitRec>(), cnt + C_RAY0 + cur,
+                            t.radiance.as<float4>(), t.rays[nxt].as<Ray>(), t.beta[nxt].as<float4>(), cnt + C_RAY0 + nxt,
+                            t.shadow.as<Ray>(), t.contrib.as<float4>(), cnt + C_SHADOW,
+                            env ? t.skyw.as<float>() : nullptr, env ? P : 0};
+                if (kt) mark(ev_h);
+                if (env) {
+                    if (sc.features == 0) hipLaunchKernelGGL(k_shade_basic_env, dim3(grid_shade), dim3(BLK), 0, s, a);
+                    else hipLaunchKernelGGL(k_shade_full_env, dim3(grid_shade), dim3(BLK), 0, s, a);
+                } else if (sc.features == 0) {
+                    hipLaunchKernelGGL(k_shade_basic, dim3(grid_shade), dim3(BLK), 0, s, a);
+                } else {
+                    hipLaunchKernelGGL(k_shade_full, dim3(grid_shade), dim3(BLK), 0, s, a);
+                }
+                if (kt) mark(ev_h);
+                if (kt) mark(ev_s);
+                launch_shadow(sc.view, t.shadow.as<Ray>(), cnt + C_SHADOW, nullptr, nullptr, t.contrib.as<float4>(),
+                              t.radiance.as<float4>(), cnt + C_WORK1, cnt + C_ERR, ts ? ts + 1 : nullptr, s,
+                              t.retry.as<int>());
+                if (kt) mark(ev_s);
+                rs.iterations++;
+                if (p.maxdepth > 0) {
+                    if (it + 1 >= p.maxdepth) break;
+                } else {
+                    check_hip(hipMemcpyAsync(t.pinned, cnt + C_RAY0 + nxt, sizeof(int), hipMemcpyDeviceToHost, s), "cnt");
+                    check_hip(hipStreamSynchronize(s), "sync");
+                    if (*t.pinned == 0) break;
+                    if (cancel && *cancel) { status = 1; break; }
+                }
             }
         }
         if (status) break;
@@ -766,6 +775,7 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
     if (!sc.built) throw std::runtime_error("scene not built (call nr_bvh_build_sah)");
     if (p.W <= 0 || p.H <= 0 || p.spp <= 0) throw std::runtime_error("bad image size or spp");
     if (np.train_paths < 0 || np.train_maxlen < 1) throw std::runtime_error("bad radiance-cache parameters");
+    if (p.path_kernel) throw std::runtime_error("the radiance cache renders with the wavefront (path_kernel = 0)");
     if (p.env_is && p.integrator != 0) throw std::runtime_error("environment sampling needs the ptnee integrator");
     const bool env = p.env_is && sc.view.env_w > 0;
     const int npix = p.W * p.H;
