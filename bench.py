@@ -54,8 +54,28 @@ def cpu_baseline(sc, spp_frame, target_s=12.0):
     o.render(cam, W, H, k, spp0=1, **kw)
     dt = time.perf_counter() - t0
     samples = W * H * k
+    # one host thread on every 8th row of the frame (a bounded sample spread over the whole image)
+    one = 0
+    t1 = time.perf_counter()
+    for y in range(0, H, 8):
+        o.render(cam, W, H, 2, spp0=1 + k, rect=(0, y, W, y + 1), **dict(kw, nthreads=1))
+        one += W * 2
+    dt1 = time.perf_counter() - t1
     return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
-            "sample": f"the same 1280x720 frame at {k} spp (samples 1..{k}; {samples} camera samples in {dt:.1f} s)"}
+            "sample": f"the same 1280x720 frame at {k} spp (samples 1..{k}; {samples} camera samples in {dt:.1f} s)",
+            "cpu_model": _cpu_model(), "one_thread_value": one / dt1 / 1e6,
+            "one_thread_sample": f"every 8th row of the frame at 2 spp ({one} camera samples in {dt1:.1f} s)"}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def adjoint_iteration(nr, scenes, iters=20, warmup=5):
