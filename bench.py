@@ -29,9 +29,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes of one closest-hit ray: 64 B per node-pair fetch, 48 B per triangle test,
-# 32 B ray in, 16 B hit out
-B_PAIR, B_TRI, B_RAY = 64, 48, 48
+# algorithmic bytes of one closest-hit ray: 128 B per 4-wide node fetch (four child boxes and
+# references: the bytes of two binary node pairs), 48 B per triangle test, 32 B ray in, 16 B hit out
+B_NODE, B_TRI, B_RAY = 128, 48, 48
 
 
 def cpu_baseline(sc, spp_frame, target_s=12.0):
@@ -353,7 +353,7 @@ def main():
 
     for _ in range(args.warmup):
         dev.render(sc["camera"], W, H, **rp)
-    # counters pass (untimed): node-pair fetches / triangle tests / rays of the closest-hit kernel
+    # counters pass (untimed): node fetches / triangle tests / rays of the closest-hit kernel
     dev.set_profiling(2)
     dev.render(sc["camera"], W, H, **rp)
     cnt = dev.stats()
@@ -377,7 +377,7 @@ def main():
           f"shade {shade_ms / args.steps:.1f} ms, shadow {shadow_ms / args.steps:.1f} ms; "
           f"{st['iterations']:.0f} iterations", file=sys.stderr, flush=True)
 
-    bytes_frame = B_PAIR * cnt["pair_fetch"] + B_TRI * cnt["tri_tests"] + B_RAY * cnt["rays"]
+    bytes_frame = B_NODE * cnt["node_fetch"] + B_TRI * cnt["tri_tests"] + B_RAY * cnt["rays"]
     achieved = bytes_frame * args.steps / (closest_ms / 1e3) / 1e9 if closest_ms > 0 else None
     traffic = measured_traffic()
     avg_ms = closest_ms / max(launches, 1)
@@ -390,7 +390,7 @@ def main():
                 "hbm_gbs": hbm_gbs, "hbm_frac": hbm_gbs / HBM_PEAK_GBS if hbm_gbs else None,
                 "kernel": "k_trace<closest>", "bytes_per_launch": bytes_frame / max(cnt["closest_launches"], 1),
                 "avg_launch_ms": avg_ms,
-                "rays_per_frame": cnt["rays"], "pair_fetch_per_ray": cnt["pair_fetch"] / max(cnt["rays"], 1),
+                "rays_per_frame": cnt["rays"], "node_fetch_per_ray": cnt["node_fetch"] / max(cnt["rays"], 1),
                 "tri_tests_per_ray": cnt["tri_tests"] / max(cnt["rays"], 1),
                 "rays_retried_with_full_stack": cnt["retried"]}
     if args.frames_only:

@@ -134,16 +134,16 @@ int nr_render_ptnee(nr_scene* s, const nr_camera* cam, const nr_params* params, 
 int nr_render_fetch(nr_scene* s, float* rgb_linear, double* sum, double* sum2, uint8_t* rgb8);
 /* Statistics of the last render: [0] wall ms, [1] samples, [2] wavefront iterations,
  * [3] closest-hit kernel ms, [4] closest-hit launches, [5] rays traced (closest),
- * [6] node-pair fetches (closest), [7] triangle tests (closest), [8] shadow kernel ms,
- * [9] shadow rays, [10] shadow node-pair fetches, [11] shadow triangle tests, [12] shade kernel ms,
+ * [6] node fetches (closest; 4-wide nodes of 128 B), [7] triangle tests (closest), [8] shadow kernel ms,
+ * [9] shadow rays, [10] shadow node fetches, [11] shadow triangle tests, [12] shade kernel ms,
  * [13] shade launches, [14] shadow launches, [15] rays traced again with the full-depth stack
  * (profiling level 2).
  * Kernel times are filled when the render ran with timing on (nr_set_profiling). */
 int nr_render_stats(nr_scene* s, double* out16);
-/* 1: time the traversal launches with HIP events; 2: also count node-pair fetches / triangle tests. */
+/* 1: time the traversal launches with HIP events; 2: also count node fetches / triangle tests. */
 int nr_set_profiling(nr_scene* s, int32_t mode);
 /* Traversal tuning (no counterpart in the reference, whose traversal recurses on the CPU stack):
- * live LDS stack entries per lane of the first persistent trace launch (default 18, at least 1).
+ * live LDS stack entries per lane of the first persistent trace launch (default 16, at least 1).
  * A ray that needs more is traced again from its start by a launch with the full-depth stack, so
  * results do not depend on it; smaller values raise occupancy, larger ones retry fewer rays. */
 int nr_scene_set_short_stack(nr_scene* s, int32_t entries);

@@ -311,8 +311,8 @@ class Scene:
     def stats(self):
         out = np.zeros(16, np.float64)
         check(lib().nr_render_stats(self.h, out))
-        keys = ["ms", "samples", "iterations", "closest_ms", "closest_launches", "rays", "pair_fetch", "tri_tests",
-                "shadow_ms", "shadow_rays", "shadow_pair_fetch", "shadow_tri_tests", "shade_ms", "shade_launches",
+        keys = ["ms", "samples", "iterations", "closest_ms", "closest_launches", "rays", "node_fetch", "tri_tests",
+                "shadow_ms", "shadow_rays", "shadow_node_fetch", "shadow_tri_tests", "shade_ms", "shade_launches",
                 "shadow_launches", "retried"]
         return dict(zip(keys, out.tolist()))
 

@@ -115,6 +115,66 @@ struct Box {
     float mn[3], mx[3];
 };
 
+inline void set4(float4& v, int k, float x) { (k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w) = x; }
+inline void set4(int4& v, int k, int32_t x) { (k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w) = x; }
+
+// The 4-wide form of the tree for the persistent trace kernels.  Each Node4 starts from one binary
+// node's two children and opens the internal child with the largest surface area until it has four
+// (or only leaves are left).  Leaves stay the binary tree's leaves, with the same references, and
+// every box is the binary node's own box, so a leaf is reached iff its exact box passes the slab
+// test (its ancestors' boxes contain it) and the result of the walk is the reference's.  Node4s are
+// numbered in preorder so that a subtree sits together in memory.
+template <class RefOf>
+void collapse_wide(const std::vector<Tmp>& nodes, const std::vector<Box>& bx, int root, RefOf ref_of, BvhBuild& out) {
+    out.wide.clear();
+    out.wide_depth = 0;
+    if (nodes[root].l < 0) {
+        out.wide_root = ref_of(root);
+        return;
+    }
+    auto area = [&](int v) {
+        const Box& B = bx[v];
+        const double dx = double(B.mx[0]) - B.mn[0], dy = double(B.mx[1]) - B.mn[1], dz = double(B.mx[2]) - B.mn[2];
+        return dx * dy + dy * dz + dz * dx;
+    };
+    struct Frame {
+        int v, depth, parent_slot;  // parent_slot = parent Node4 * 4 + slot, -1 for the root
+    };
+    std::vector<Frame> st{{root, 1, -1}};
+    while (!st.empty()) {
+        const Frame f = st.back();
+        st.pop_back();
+        const int me = int(out.wide.size());
+        out.wide.emplace_back();
+        if (f.parent_slot >= 0) set4(out.wide[f.parent_slot / 4].ref, f.parent_slot % 4, me);
+        out.wide_depth = std::max(out.wide_depth, f.depth);
+        int c[4] = {nodes[f.v].l, nodes[f.v].r, -1, -1};
+        int k = 2;
+        while (k < 4) {
+            int best = -1;
+            double ba = -1.0;
+            for (int j = 0; j < k; ++j)
+                if (nodes[c[j]].l >= 0 && area(c[j]) > ba) { ba = area(c[j]); best = j; }
+            if (best < 0) break;
+            const int x = c[best];
+            c[best] = nodes[x].l;
+            c[k++] = nodes[x].r;
+        }
+        Node4& nd = out.wide[me];
+        nd.pad = make_int4(k, 0, 0, 0);
+        for (int j = 0; j < 4; ++j) {
+            const bool used = j < k;
+            const Box B = used ? bx[c[j]] : Box{{1e30f, 1e30f, 1e30f}, {1e30f, 1e30f, 1e30f}};
+            set4(nd.lox, j, B.mn[0]); set4(nd.loy, j, B.mn[1]); set4(nd.loz, j, B.mn[2]);
+            set4(nd.hix, j, B.mx[0]); set4(nd.hiy, j, B.mx[1]); set4(nd.hiz, j, B.mx[2]);
+            set4(nd.ref, j, used && nodes[c[j]].l < 0 ? ref_of(c[j]) : make_leaf(0, 0));
+        }
+        for (int j = k - 1; j >= 0; --j)  // internal children, first slot on top of the stack
+            if (nodes[c[j]].l >= 0) st.push_back({c[j], f.depth + 1, me * 4 + j});
+    }
+    out.wide_root = 0;
+}
+
 }  // namespace
 
 BvhBuild build_sah(const float* p, int n, int nthreads) {
@@ -213,6 +273,7 @@ BvhBuild build_sah(const float* p, int n, int nthreads) {
     for (int j = 0; j < 3; ++j) { out.root_min[j] = bx[root].mn[j]; out.root_max[j] = bx[root].mx[j]; }
     out.root_ref = ref_of(root);
     out.order = b.idx;
+    collapse_wide(b.nodes, bx, root, ref_of, out);
 
     out.left.resize(nn); out.right.resize(nn);
     out.leaf_first.resize(nn); out.leaf_count.resize(nn);

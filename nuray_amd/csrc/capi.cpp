@@ -223,8 +223,8 @@ int nr_render_ptnee(nr_scene* s, const nr_camera* cam, const nr_params* pp, floa
         double* o = s->stats;
         o[0] = rs.ms; o[1] = double(rs.samples); o[2] = rs.iterations;
         o[3] = kt.closest_ms; o[4] = kt.closest_launches;
-        o[5] = double(ts[0].rays); o[6] = double(ts[0].pair_fetch); o[7] = double(ts[0].tri_tests);
-        o[8] = kt.shadow_ms; o[9] = double(ts[1].rays); o[10] = double(ts[1].pair_fetch); o[11] = double(ts[1].tri_tests);
+        o[5] = double(ts[0].rays); o[6] = double(ts[0].node_fetch); o[7] = double(ts[0].tri_tests);
+        o[8] = kt.shadow_ms; o[9] = double(ts[1].rays); o[10] = double(ts[1].node_fetch); o[11] = double(ts[1].tri_tests);
         o[12] = kt.shade_ms; o[13] = kt.shade_launches; o[14] = kt.shadow_launches;
         o[15] = double(ts[0].retried + ts[1].retried);
         if (rgb_linear || sum || sum2 || rgb8) nr::render_fetch(*s->sc, rgb_linear, sum, sum2, rgb8);

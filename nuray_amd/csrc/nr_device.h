@@ -2,6 +2,7 @@
 //
 // Layout in HBM (all arrays 64-byte aligned by hipMalloc):
 //   NodePair  64 B  one per internal BVH node, preorder; holds both children's boxes and refs
+//   Node4    128 B  the same tree collapsed 4 wide (boxes of up to four descendants, refs)
 //   TriGeom   48 B  one per triangle, in DFS leaf order: p0, e1 = p1-p0, e2 = p2-p0, loader id
 //   TriShade  64 B  same order: n0, n1, n2, uv0, uv1, uv2, material row
 //   MatRec    48 B  one per material: {type, r, g, b, alpha, ior, texture, 0, Ks r, g, b, Ks texture}
@@ -34,6 +35,18 @@ struct alignas(64) NodePair {
     float4 c;  // rmin.z rmax.x rmax.y rmax.z
     int4 d;    // lref rref 0 0
 };
+// The persistent trace kernels walk a 4-wide form of the same tree: each node holds up to four
+// descendants of a binary node (opened largest surface area first) with their exact boxes, as
+// per-axis float4s (child k in component k).  The leaves are the binary tree's own leaves, so every
+// leaf box is tested with its exact bounds, and every other box contains the leaf boxes below it.
+// Internal references are Node4 indices.  An unused slot holds a point box at 1e30 and an empty leaf
+// (no triangles), so even a ray that reaches it changes nothing.
+struct alignas(128) Node4 {
+    float4 lox, loy, loz;  // child k: min x, y, z in component k
+    float4 hix, hiy, hiz;  // child k: max x, y, z
+    int4 ref;              // child references
+    int4 pad;
+};
 struct alignas(16) TriGeom {
     float4 g0;  // p0.xyz e1.x
     float4 g1;  // e1.yz e2.xy
@@ -63,6 +76,7 @@ struct TexDesc {
 // Everything a kernel needs to read the scene.
 struct SceneView {
     const NodePair* nodes;
+    const Node4* wide;  // the same tree, 4 wide (persistent trace kernels)
     const TriGeom* tris;
     const TriShade* shade;
     const MatRec* mats;
@@ -79,6 +93,8 @@ struct SceneView {
     float env_gain;
     int32_t stack_cap; // traversal stack entries per lane (full depth)
     int32_t short_stack; // stack entries of the first persistent trace launch (see trace.hip)
+    int32_t wide_root;       // root as a Node4 index, or a leaf reference
+    int32_t wide_stack_cap;  // full-depth stack entries of the 4-wide walk: 3 per level + 1
     // environment sampler: one cell per texel square of the map (env_w x env_h cells, 0 if none),
     // {alias probability, alias cell (int bits), pdf of the cell, pdf of the alias cell}; the
     // solid-angle density of a direction in cell k is pdf_k * env_norm / sin(theta)
@@ -181,10 +197,12 @@ __device__ __forceinline__ bool tri_valid(float t, float b1, float b2) {
 //   |t_out' - t_in'| > 2^-20 (|t_in'| + |t_out'|) + 2 a20 => t_in <= t_out <=> t_out' > t_in'
 // (2^-20 = 16 u covers 4.2 u per side plus the rounding of the difference; the 1e-30 floor keeps
 // every value the bounds rely on in the normal range).  Everything else takes the exact path.
+// Both comparisons use a2 = 2 a20 (the first one is then stricter than it needs to be), and
+// a2 = +inf when some dd component is zero, which sends every box of such a ray to the exact
+// path: one per-ray threshold and no branches in the filter.
 struct SlabRay {
     F3 o, d, r, c;
-    float a20;
-    bool finite;  // every dd component non-zero
+    float a2;  // 2 a20, or +inf when some component of dd is zero
 };
 __device__ __forceinline__ SlabRay slab_ray(F3 o, F3 d) {
     SlabRay s;
@@ -193,8 +211,9 @@ __device__ __forceinline__ SlabRay slab_ray(F3 o, F3 d) {
     const F3 dd = f3(d.x + 1e-9f, d.y + 1e-9f, d.z + 1e-9f);
     s.r = f3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
     s.c = f3(o.x * s.r.x, o.y * s.r.y, o.z * s.r.z);
-    s.a20 = fmaxf(fmaxf(fmaxf(fabsf(s.c.x), fabsf(s.c.y)), fabsf(s.c.z)) * 9.5367431640625e-07f, 1e-30f);
-    s.finite = dd.x != 0.0f && dd.y != 0.0f && dd.z != 0.0f;
+    const float a20 = fmaxf(fmaxf(fmaxf(fabsf(s.c.x), fabsf(s.c.y)), fabsf(s.c.z)) * 9.5367431640625e-07f, 1e-30f);
+    const bool finite = dd.x != 0.0f && dd.y != 0.0f && dd.z != 0.0f;
+    s.a2 = finite ? 2.0f * a20 : __builtin_inff();
     return s;
 }
 // The filtered test without its exact fallback: returns the fast decision and sets `clear` when
@@ -207,8 +226,8 @@ __device__ __forceinline__ bool slab_filter(float mnx, float mny, float mnz, flo
     const float t_in = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), -1e18f));
     const float t_out = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), 1e18f));
     const float diff = t_out - t_in;
-    const float band = fmaf(9.5367431640625e-07f, fabsf(t_in) + fabsf(t_out), 2.0f * s.a20);  // 2^-20
-    clear = s.finite && fabsf(t_out) > s.a20 && fabsf(diff) > band;
+    const float band = fmaf(9.5367431640625e-07f, fabsf(t_in) + fabsf(t_out), s.a2);  // 2^-20
+    clear = (fabsf(t_out) > s.a2) & (fabsf(diff) > band);
     return t_out >= 0.0f && diff > 0.0f;
 }
 __device__ __forceinline__ bool slab_fast(float mnx, float mny, float mnz, float mxx, float mxy, float mxz,
@@ -219,8 +238,8 @@ __device__ __forceinline__ bool slab_fast(float mnx, float mny, float mnz, float
     const float t_in = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), -1e18f));
     const float t_out = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), 1e18f));
     const float diff = t_out - t_in;
-    const float band = fmaf(9.5367431640625e-07f, fabsf(t_in) + fabsf(t_out), 2.0f * s.a20);  // 2^-20
-    const bool clear = s.finite && fabsf(t_out) > s.a20 && fabsf(diff) > band;
+    const float band = fmaf(9.5367431640625e-07f, fabsf(t_in) + fabsf(t_out), s.a2);  // 2^-20
+    const bool clear = (fabsf(t_out) > s.a2) & (fabsf(diff) > band);
     if (__builtin_expect(clear, 1)) return t_out >= 0.0f && diff > 0.0f;
     return slab(mnx, mny, mnz, mxx, mxy, mxz, s.o, f3(s.d.x + 1e-9f, s.d.y + 1e-9f, s.d.z + 1e-9f));
 }

@@ -108,6 +108,72 @@ __device__ __forceinline__ void trav_expand_wave(const SceneView& sv, const Slab
     }
 }
 
+// The 4-wide walk of the persistent trace kernels (Node4, see nr_device.h): the same visiting rule
+// on the same leaves as the binary walk, with half the node steps.  Pushes are branch-free: the four
+// child references are written at their compacted positions in slot order (a missed child's entry is
+// overwritten by the next hit one or left dead above the new top), so the stack needs three spare
+// slots beyond `cap` live entries.
+__device__ __forceinline__ void trav_begin_wide(const SceneView& sv, const SlabRay& sr, int* stk, TravState& s) {
+    s = TravState{};
+    if (slab_fast(sv.root_min[0], sv.root_min[1], sv.root_min[2], sv.root_max[0], sv.root_max[1], sv.root_max[2],
+                  sr)) {
+        stk[0] = sv.wide_root;
+        s.sp = 1;
+    }
+}
+
+template <bool STATS, int STRIDE, int LEAF_ROUND_SEARCHERS = 16>
+__device__ __forceinline__ void trav_expand_wave4(const SceneView& sv, const SlabRay& sr, int* stk, int cap,
+                                                  TravState& s, uint32_t& n_nf) {
+    bool parked = false;
+    for (;;) {
+        const int searching = __popcll(__ballot(!s.has_leaf && s.sp > 0));
+        if (searching == 0 || (searching <= LEAF_ROUND_SEARCHERS && __any(s.has_leaf))) return;
+        if (s.sp > 0 && !parked) {
+            const int32_t e = stk[(s.sp - 1) * STRIDE];
+            if (ref_is_leaf(e)) {
+                parked = s.has_leaf;
+                s.leaf = s.has_leaf ? s.leaf : e;
+                s.sp -= s.has_leaf ? 0 : 1;
+                s.has_leaf = true;
+            } else {
+                --s.sp;
+                const Node4* nd = sv.wide + e;
+                const float4 lx = nd->lox, ly = nd->loy, lz = nd->loz;
+                const float4 hx = nd->hix, hy = nd->hiy, hz = nd->hiz;
+                const int4 ch = nd->ref;
+                if (STATS) ++n_nf;
+                bool c0, c1, c2, c3;
+                bool h0 = slab_filter(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, sr, c0);
+                bool h1 = slab_filter(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, sr, c1);
+                bool h2 = slab_filter(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, sr, c2);
+                bool h3 = slab_filter(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, sr, c3);
+                if (__builtin_expect(!(c0 & c1 & c2 & c3), 0)) {  // inside an error band: the exact test
+                    const F3 dd = f3(sr.d.x + 1e-9f, sr.d.y + 1e-9f, sr.d.z + 1e-9f);
+                    if (!c0) h0 = slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, sr.o, dd);
+                    if (!c1) h1 = slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, sr.o, dd);
+                    if (!c2) h2 = slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, sr.o, dd);
+                    if (!c3) h3 = slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, sr.o, dd);
+                }
+                // sp <= cap - 1 here, so the writes (at most sp + 3) stay inside cap + 3 slots even
+                // when the push overflows (which then ends the ray with the overflow flag)
+                int p = s.sp;
+                stk[p * STRIDE] = ch.x;
+                p += int(h0);
+                stk[p * STRIDE] = ch.y;
+                p += int(h1);
+                stk[p * STRIDE] = ch.z;
+                p += int(h2);
+                stk[p * STRIDE] = ch.w;
+                p += int(h3);
+                const bool ovf = p > cap;
+                s.overflow = s.overflow || ovf;
+                s.sp = ovf ? 0 : p;
+            }
+        }
+    }
+}
+
 // Leaf loop, bvhnode.cpp:50-66, merged into the running winner with the DFS tie rule (closest
 // hit), or the shadow rule (double)t + 2e-3 <= len (any hit: returns true when blocked).
 template <bool ANY, bool STATS>

@@ -230,6 +230,7 @@ void Scene::build(int nthreads) {
         s[k].s3 = make_float4(t[3], t[4], t[5], mf);
     }
     d_nodes.upload(bvh.pairs.data(), bvh.pairs.size() * sizeof(NodePair), stream);
+    d_wide.upload(bvh.wide.data(), bvh.wide.size() * sizeof(Node4), stream);
     d_tris.upload(g.data(), g.size() * sizeof(TriGeom), stream);
     d_shade.upload(s.data(), s.size() * sizeof(TriShade), stream);
     d_lights.upload(lights.data(), lights.size() * sizeof(LightRec), stream);
@@ -238,6 +239,7 @@ void Scene::build(int nthreads) {
 
     view = SceneView{};
     view.nodes = d_nodes.as<NodePair>();
+    view.wide = d_wide.as<Node4>();
     view.tris = d_tris.as<TriGeom>();
     view.shade = d_shade.as<TriShade>();
     view.lights = d_lights.as<LightRec>();
@@ -248,6 +250,8 @@ void Scene::build(int nthreads) {
     view.light_p = float(1.0 / (double(light_sum) + 1e-8));  // LightSampler::p, lightsampler.cpp:100-103
     view.stack_cap = bvh.depth + 2;
     view.short_stack = short_stack;
+    view.wide_root = N ? bvh.wide_root : make_leaf(0, 0);
+    view.wide_stack_cap = 3 * bvh.wide_depth + 2;
     upload_materials(false);  // checked above
     upload_textures(false);
     built = true;

@@ -43,7 +43,7 @@ public:
     int env_tex = -1;
     float env_gain = 1.0f;
     unsigned features = 0;        // material features present: 1 = GGX dielectric, 2 = albedo textures
-    int short_stack = 18;         // LDS stack entries of the first trace launch
+    int short_stack = 16;         // LDS stack entries of the first trace launch
 
     // derived host data
     BvhBuild bvh;
@@ -53,7 +53,7 @@ public:
     float light_sum = 0.0f;
 
     // device mirror
-    DevBuf d_nodes, d_tris, d_shade, d_mats, d_lights, d_cdf, d_tex, d_texpool, d_lut, d_env;
+    DevBuf d_nodes, d_wide, d_tris, d_shade, d_mats, d_lights, d_cdf, d_tex, d_texpool, d_lut, d_env;
     std::vector<float4> env_tab;  // environment sampler cells (SceneView::env_tab)
     SceneView view{};
     hipStream_t stream = nullptr;

@@ -9,7 +9,7 @@ namespace nr {
 // position.  Only the radiance accumulator is indexed by path slot (float4, xyz used).
 
 struct TraceStats {
-    unsigned long long rays, pair_fetch, tri_tests, retried;
+    unsigned long long rays, node_fetch, tri_tests, retried;
 };
 
 // `work` points at three ints that must be zero at launch (ray claims, retry count, retry claims).

@@ -10,16 +10,17 @@
 //     "later leaf" is the one with the larger first-primitive index, which makes the winner
 //     independent of the order this kernel visits nodes in.
 //
-// Execution: persistent waves.  Every lane traverses one ray with a private stack kept in LDS at
-// stack[slot * BLOCK + tid] (one 4-byte bank per lane, conflict-free).  Internal nodes are
-// expanded wave-cooperatively (trav_expand_wave) until (nearly) every lane holds a leaf, then leaves are
-// tested together; a lane whose ray is done goes idle, and once REFILL_MIN lanes are idle one
-// atomic claims new rays for all of them (ballot + prefix count), so long rays do not hold a whole
-// wave hostage.  The first launch gives each lane SHORT_STACK live entries (plus one spare slot
-// for the branch-free push): the LDS then allows 8 waves per SIMD instead of 6.  The rare ray that
-// needs more is abandoned, listed, and traced again from its start by a second launch whose
-// stack holds BVH depth + 2 entries (checked on every push; an overflow there is reported, never
-// written past).
+// Execution: persistent waves over the 4-wide form of the tree (Node4: the binary tree's own
+// leaves and boxes, half the node steps).  Every lane traverses one ray with a private stack kept
+// in LDS at stack[slot * BLOCK + tid] (one 4-byte bank per lane, conflict-free).  Internal nodes
+// are expanded wave-cooperatively (trav_expand_wave4) until (nearly) every lane holds a leaf, then
+// leaves are tested together; a lane whose ray is done goes idle, and once REFILL_MIN lanes are
+// idle one atomic claims new rays for all of them (ballot + prefix count), so long rays do not hold
+// a whole wave hostage.  The first launch gives each lane SceneView::short_stack live entries (plus
+// three spare slots for the branch-free push).  The rare ray
+// that needs more is abandoned, listed, and traced again from its start by a second launch whose
+// stack holds 3 entries per Node4 level + 2 (checked on every push; an overflow there is reported,
+// never written past).
 #include "trace.h"
 #include "nr_traverse.h"
 #include "scene.h"
@@ -35,18 +36,21 @@ constexpr int BLOCK = 256;
 // for small batches the claim shrinks to the batch's share per wave, so that every wave starts
 // rays at once instead of a few waves working through long private ranges (the launch then
 // takes about one ray's latency, not several).
-constexpr int REFILL_MIN = 16;
-// Leaf-round threshold of the wave-cooperative expansion for shadow rays (closest hit: 16; a sweep
-// of 8 / 16 / 24 / 32 put both optimum near these values).
+constexpr int REFILL_MIN = 24;
+// Leaf-round thresholds of the wave-cooperative expansion (sweeps of 4 to 32 on the 720p terrain
+// frame put the optimum near these values; within a few per cent of each other).
+constexpr int CLOSEST_SEARCHERS = 24;
 constexpr int SHADOW_SEARCHERS = 8;
 constexpr int CLAIM = 128;
-// Live stack entries of the first launch default to SceneView::short_stack = 18:
-// (18 + 1) x 256 x 4 B = 19 KB of LDS per block, 8 blocks (32 waves) per CU.
+// Live stack entries of the first launch default to SceneView::short_stack = 16:
+// (16 + 3) x 256 x 4 B = 19 KB of LDS per block.  The kernel is compiled for 6 waves per SIMD
+// (6 blocks per CU, 114 KB of LDS): the 4-wide node step needs 76-79 VGPRs, and capping it at 64 for
+// 8 waves spills registers to scratch memory, which costs more than the extra waves bring.
 
 // rays are rays[idx[i]] for i < *n_ptr when idx is given (the retry launch, RETRY = true: a
 // separate kernel symbol so that profiles keep the two launches apart), else rays[i].
 template <bool ANY, bool STATS, bool RETRY>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_trace(SceneView sv, const Ray* __restrict__ rays,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_trace(SceneView sv, const Ray* __restrict__ rays,
                                                   const int* __restrict__ n_ptr, const int* __restrict__ idx,
                                                   HitRec* __restrict__ hits, uint8_t* __restrict__ occ,
                                                   const float4* __restrict__ contrib, float4* __restrict__ radiance,
@@ -104,7 +108,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
                     bprim = blf = -1;
                     done = false;
                     n_pf = n_tt = 0;
-                    trav_begin(sv, sr, stk, ts);
+                    trav_begin_wide(sv, sr, stk, ts);
                 }
             }
             rb += take;
@@ -114,7 +118,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
             continue;
         }
         if (i >= 0) {
-            trav_expand_wave<STATS, BLOCK, ANY ? SHADOW_SEARCHERS : 16>(sv, sr, stk, cap, ts, n_pf);
+            trav_expand_wave4<STATS, BLOCK, ANY ? SHADOW_SEARCHERS : CLOSEST_SEARCHERS>(sv, sr, stk, cap, ts, n_pf);
             if (ts.has_leaf) done = trav_leaf<ANY, STATS>(sv, o, d, double(wr), ts, bt, bb1, bb2, bprim, blf, n_tt);
             // ---- finished: write the result, go idle
             if ((ts.sp == 0 && !ts.has_leaf) || done || ts.overflow) {
@@ -128,7 +132,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
                 if (idx) i = idx[i];
                 if (STATS) {
                     atomicAdd(&stats->rays, 1ull);
-                    atomicAdd(&stats->pair_fetch, (unsigned long long)n_pf);
+                    atomicAdd(&stats->node_fetch, (unsigned long long)n_pf);
                     atomicAdd(&stats->tri_tests, (unsigned long long)n_tt);
                 }
                 if (!ANY) {
@@ -169,12 +173,12 @@ int grid_for(size_t shmem) {
 template <bool ANY>
 void launch(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, uint8_t* occ, const float4* contrib,
             float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s, int* retry) {
-    const int full = sv.stack_cap;
-    const size_t shmem_full = size_t(full + 1) * BLOCK * sizeof(int);  // + the spare push slot
+    const int full = sv.wide_stack_cap;
+    const size_t shmem_full = size_t(full + 3) * BLOCK * sizeof(int);  // + the spare push slots
     if (shmem_full > 160 * 1024) throw std::runtime_error("BVH too deep for the LDS traversal stack");
     const int cap = retry ? std::min(full, std::max(1, sv.short_stack)) : full;
     int* const rl = cap < full ? retry : nullptr;
-    const size_t shmem = size_t(cap + 1) * BLOCK * sizeof(int);
+    const size_t shmem = size_t(cap + 3) * BLOCK * sizeof(int);
     if (stats) {
         hipLaunchKernelGGL((k_trace<ANY, true, false>), dim3(grid_for<ANY, true>(shmem)), dim3(BLOCK), shmem, s, sv, rays, n,
                            nullptr, hits, occ, contrib, radiance, work, err, rl, work + 1, cap, stats);

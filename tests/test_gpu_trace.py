@@ -92,7 +92,7 @@ def test_short_stack_retry_is_bit_exact(entries, nr, po, scenes):
     dev.set_profiling(2)
     a = dev.render(sc["camera"], 96, 54, 4, seed=5, maxdepth=6, rrdepth=2)
     retried = dev.stats()["retried"]
-    dev.set_short_stack(18)
+    dev.set_short_stack(16)
     b = dev.render(sc["camera"], 96, 54, 4, seed=5, maxdepth=6, rrdepth=2)
     dev.close()
     print(f"short stack {entries}: {retried:.0f} rays retried")
