@@ -199,7 +199,12 @@ __device__ __forceinline__ bool tri_valid(float t, float b1, float b2) {
 // every value the bounds rely on in the normal range).  Everything else takes the exact path.
 // Both comparisons use a2 = 2 a20 (the first one is then stricter than it needs to be), and
 // a2 = +inf when some dd component is zero, which sends every box of such a ray to the exact
-// path: one per-ray threshold and no branches in the filter.
+// path: one per-ray threshold and no branches in the filter.  The reference's -1e18 / 1e18 start
+// values are applied as med3(az, bz, -+1e18) on the z pair (one instruction each instead of a
+// min / max and a clamp).  That equals max(min(az, bz), -1e18) and min(max(az, bz), 1e18) unless
+// both z values lie beyond the same bound; then t_out' < -1e18 (both below) or t_in' >= t_out'
+// (both above), so the filter reports a miss, as the reference does, or hands the box to the
+// exact test.
 struct SlabRay {
     F3 o, d, r, c;
     float a2;  // 2 a20, or +inf when some component of dd is zero
@@ -223,8 +228,8 @@ __device__ __forceinline__ bool slab_filter(float mnx, float mny, float mnz, flo
     const float ax = fmaf(mnx, s.r.x, -s.c.x), bx = fmaf(mxx, s.r.x, -s.c.x);
     const float ay = fmaf(mny, s.r.y, -s.c.y), by = fmaf(mxy, s.r.y, -s.c.y);
     const float az = fmaf(mnz, s.r.z, -s.c.z), bz = fmaf(mxz, s.r.z, -s.c.z);
-    const float t_in = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), -1e18f));
-    const float t_out = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), 1e18f));
+    const float t_in = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), __builtin_amdgcn_fmed3f(az, bz, -1e18f));
+    const float t_out = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), __builtin_amdgcn_fmed3f(az, bz, 1e18f));
     const float diff = t_out - t_in;
     const float band = fmaf(9.5367431640625e-07f, fabsf(t_in) + fabsf(t_out), s.a2);  // 2^-20
     clear = (fabsf(t_out) > s.a2) & (fabsf(diff) > band);
@@ -235,8 +240,8 @@ __device__ __forceinline__ bool slab_fast(float mnx, float mny, float mnz, float
     const float ax = fmaf(mnx, s.r.x, -s.c.x), bx = fmaf(mxx, s.r.x, -s.c.x);
     const float ay = fmaf(mny, s.r.y, -s.c.y), by = fmaf(mxy, s.r.y, -s.c.y);
     const float az = fmaf(mnz, s.r.z, -s.c.z), bz = fmaf(mxz, s.r.z, -s.c.z);
-    const float t_in = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), -1e18f));
-    const float t_out = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), 1e18f));
+    const float t_in = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), __builtin_amdgcn_fmed3f(az, bz, -1e18f));
+    const float t_out = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), __builtin_amdgcn_fmed3f(az, bz, 1e18f));
     const float diff = t_out - t_in;
     const float band = fmaf(9.5367431640625e-07f, fabsf(t_in) + fabsf(t_out), s.a2);  // 2^-20
     const bool clear = (fabsf(t_out) > s.a2) & (fabsf(diff) > band);
