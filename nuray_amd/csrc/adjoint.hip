@@ -453,7 +453,7 @@ void render_paths(Scene& sc, const CamView& cam, const RenderParams& p, uint32_t
     AdjArgs a{sc.view, cam, p.W, p.H, p.W * p.H, P, p.seed, sample0, p.prr, p.mis, p.maxdepth, p.rrdepth, nullptr,
               0, P, nullptr, 0.0f, 0, nullptr, err, -1, nullptr, p.integrator == 1 ? 0 : 1};
     const size_t shmem = size_t(sc.view.stack_cap + 1) * ABLK * sizeof(int);
-    if (shmem > 160 * 1024) throw std::runtime_error("BVH too deep for the LDS traversal stack");
+    if (shmem > device_max_lds()) throw std::runtime_error("BVH too deep for the LDS traversal stack");
     const unsigned feat = sc.features & (F_GGX | F_EXT) ? F_ALL : (sc.features & F_TEX ? F_TEX : 0u);
     int per_cu = 0;
     check_hip(feat == F_ALL   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_paths<F_ALL>, ABLK, shmem)
@@ -515,7 +515,7 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
               1.0f / float(p.spp), n_params, t.adj_grad.as<double>(), cnt + 15, tex_id, gtex, p.integrator == 1 ? 0 : 1};
     const size_t shmem = size_t(sc.view.stack_cap + 1) * ABLK * sizeof(int) +
                          (n_params <= LDS_PARAMS ? size_t(n_params) * sizeof(float) : 0);
-    if (shmem > 160 * 1024) throw std::runtime_error("adjoint LDS budget exceeded");
+    if (shmem > device_max_lds()) throw std::runtime_error("adjoint LDS budget exceeded (BVH too deep)");
     // kernel instance for the material features present (as the shade kernels)
     const unsigned feat = sc.features & (F_GGX | F_EXT) ? F_ALL : (sc.features & F_TEX ? F_TEX : 0u);
     int per_cu = 0;

@@ -30,5 +30,7 @@ void launch_shadow(const SceneView& sv, const Ray* rays, const int* n, uint8_t* 
                    int* retry = nullptr);
 
 int device_cu_count();
+// Largest dynamic LDS allocation one workgroup may request on the current device, in bytes.
+size_t device_max_lds();
 
 }  // namespace nr

@@ -175,7 +175,7 @@ void launch(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, ui
             float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s, int* retry) {
     const int full = sv.wide_stack_cap;
     const size_t shmem_full = size_t(full + 3) * BLOCK * sizeof(int);  // + the spare push slots
-    if (shmem_full > 160 * 1024) throw std::runtime_error("BVH too deep for the LDS traversal stack");
+    if (shmem_full > device_max_lds()) throw std::runtime_error("BVH too deep for the LDS traversal stack");
     const int cap = retry ? std::min(full, std::max(1, sv.short_stack)) : full;
     int* const rl = cap < full ? retry : nullptr;
     const size_t shmem = size_t(cap + 3) * BLOCK * sizeof(int);
@@ -201,6 +201,16 @@ void launch(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, ui
 }
 
 }  // namespace
+
+size_t device_max_lds() {
+    static int lds = 0;
+    if (!lds) {
+        int dev = 0;
+        check_hip(hipGetDevice(&dev), "hipGetDevice");
+        check_hip(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "LDS per block");
+    }
+    return size_t(lds);
+}
 
 int device_cu_count() {
     static int cus = 0;

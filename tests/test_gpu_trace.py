@@ -98,3 +98,41 @@ def test_short_stack_retry_is_bit_exact(entries, nr, po, scenes):
     print(f"short stack {entries}: {retried:.0f} rays retried")
     assert retried > 0
     assert np.array_equal(a["sum"], b["sum"]) and np.array_equal(a["sum2"], b["sum2"])
+
+
+def test_deep_chain_tree(nr, po, scenes):
+    """100 coincident triangles stacked in the room make a 100-level chain in the reference's SAH
+    tree (all splits cost the same, the first one wins).  Rays that reach the stack overflow the
+    short LDS stack and are traced again with the full-depth stack of the 4-wide walk (3 entries
+    per level); hits stay bit-identical to the oracle, including the tie among the coincident
+    triangles (the later leaf in DFS order wins).  The per-lane binary walk of the path kernel
+    renders the same image as the wavefront."""
+    sc = scenes.box_room()
+    k = 100
+    tri = np.array([[-20, 60, -10, 20, 60, -10, 0, 60, 30]], np.float32)
+    sc["p"] = np.concatenate([sc["p"], np.tile(tri, (k, 1))])
+    sc["mat"] = np.concatenate([sc["mat"], np.zeros(k, sc["mat"].dtype)])
+    ref = po.Scene(sc)
+    dev = nr.Scene(sc)
+    assert dev.info()["depth"] >= 100
+    rng = np.random.Generator(np.random.PCG64(21))
+    o, d = make_batch(sc, 1 << 16, seed=17, oracle=ref)
+    n = 1 << 14
+    down_o = np.stack([rng.uniform(-25, 25, n), np.full(n, 150.0), rng.uniform(-15, 35, n)], 1)
+    down_d = np.tile([[0.0, -1.0, 0.0]], (n, 1))
+    graze_o = np.stack([rng.uniform(-25, 25, n), np.full(n, 60.0), np.full(n, -90.0)], 1)
+    graze_d = np.tile([[0.0, 0.0, 1.0]], (n, 1))
+    o = np.concatenate([o, down_o, graze_o]).astype(np.float32)
+    d = np.concatenate([d, down_d, graze_d]).astype(np.float32)
+    got = dev.intersect(o, d)
+    _check_equal(ref.intersect(o, d), got, "deep chain")
+    assert (got[0][len(o) - 2 * n:len(o) - n] >= 36).mean() > 0.2  # the stack covers ~32 % of their square
+    dev.set_profiling(2)
+    a = dev.render(sc["camera"], 64, 48, 4, seed=3)
+    retried = dev.stats()["retried"]
+    dev.set_profiling(0)
+    b = dev.render(sc["camera"], 64, 48, 4, seed=3, path_kernel=True)
+    dev.close()
+    print(f"deep chain: {retried:.0f} rays retried")
+    assert retried > 0
+    assert np.array_equal(a["sum"], b["sum"]) and np.array_equal(a["sum2"], b["sum2"])
