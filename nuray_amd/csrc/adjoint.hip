@@ -147,6 +147,7 @@ struct AdjArgs {
     int tex_id;              // texture whose texel gradient is wanted (-1: none)
     double* gtex;            // w * h * 3 texel gradient (linear texel values)
     int nee;                 // 1: ptnee, 0: pt (no light sampling)
+    int rad_off;             // radiance[] holds samples from this index on
 };
 
 __device__ __forceinline__ void gadd(float* lds, double* glob, bool use_lds, int idx, float v) {
@@ -178,30 +179,47 @@ __device__ __forceinline__ void scatter_texels(const AdjArgs& a, const SceneView
     }
 }
 
-// One sample's path, replayed from its Philox counters.  GRAD = false: the sample's radiance
-// (the forward estimator, accumulated in vertex order as the wavefront does).  GRAD = true: the
-// parameter gradient of g . max(0, L), with Lt the sample's radiance.
-template <unsigned F, bool GRAD>
-__device__ __forceinline__ F3 replay(const AdjArgs& a, int slot, F3 Lt, F3 g, int* stk, float* gacc, bool use_lds,
-                                     bool& overflow) {
-    const SceneView& sv = a.sv;
-    const int pixel = slot % a.npix;
-    const uint32_t sample = a.sample0 + uint32_t(slot / a.npix);
-    // camera ray, identical to k_raygen
-    const int x = pixel % a.W, y = pixel / a.W;
-    const float4 u0 = rng4(a.seed, uint32_t(pixel), sample, 0u, 0u);
+// One sample's path, replayed from its Philox counters one vertex at a time.  GRAD = false: the
+// sample's radiance (the forward estimator, accumulated in vertex order as the wavefront does).
+// GRAD = true: the parameter gradient of g . max(0, L), with Lt the sample's radiance.  The kernels
+// drive path_step from a flat loop in which a lane whose path has ended starts its next sample at
+// once (path regeneration), so a wave does not wait at each sample for its longest path.
+struct PathState {
+    F3 o, d, beta;
+    F3 L;         // forward: radiance so far; gradient: radiance not yet accounted for
+    F3 g;         // gradient: dLoss / d(sample radiance) per channel (zero in the forward replay)
+    float fac, dfac;  // MIS factor for the next emitter hit and d/d alpha of it
+    int prev_mat, depth, pixel;
+    uint32_t sample;
+};
+
+// Camera ray of sample `slot` (identical to k_raygen) and the path's starting state.
+__device__ __forceinline__ void path_begin(const AdjArgs& a, int slot, F3 Lt, F3 g, PathState& p) {
+    p.pixel = slot % a.npix;
+    p.sample = a.sample0 + uint32_t(slot / a.npix);
+    const int x = p.pixel % a.W, y = p.pixel / a.W;
+    const float4 u0 = rng4(a.seed, uint32_t(p.pixel), p.sample, 0u, 0u);
     const float cx = (float(x) + u0.x) / float(a.W) * 2.0f - 1.0f;
     const float cy = (float(y) + u0.y) / float(a.H) * 2.0f - 1.0f;
     const CamView& cam = a.cam;
-    F3 d = normalized(sub(add(f3(cam.gaze[0], cam.gaze[1], cam.gaze[2]),
-                              mul(mul(mul(f3(cam.hand[0], cam.hand[1], cam.hand[2]), cam.fw), cx), 0.5f)),
-                          mul(mul(mul(f3(cam.up[0], cam.up[1], cam.up[2]), cam.fh), cy), 0.5f)));
-    F3 o = f3(cam.pos[0], cam.pos[1], cam.pos[2]);
-    F3 beta = f3(1.0f, 1.0f, 1.0f);
-    F3 L = f3(0.0f, 0.0f, 0.0f);     // forward: radiance so far
-    F3 Lrem = Lt;                    // gradient: radiance of this sample not yet accounted for
-    float fac = 1.0f, dfac = 0.0f;   // MIS factor for the next emitter hit and d/d alpha of it
-    int prev_mat = -1;
+    p.d = normalized(sub(add(f3(cam.gaze[0], cam.gaze[1], cam.gaze[2]),
+                             mul(mul(mul(f3(cam.hand[0], cam.hand[1], cam.hand[2]), cam.fw), cx), 0.5f)),
+                         mul(mul(mul(f3(cam.up[0], cam.up[1], cam.up[2]), cam.fh), cy), 0.5f)));
+    p.o = f3(cam.pos[0], cam.pos[1], cam.pos[2]);
+    p.beta = f3(1.0f, 1.0f, 1.0f);
+    p.L = Lt;
+    p.g = g;
+    p.fac = 1.0f;
+    p.dfac = 0.0f;
+    p.prev_mat = -1;
+    p.depth = 0;
+}
+
+// One vertex of the path; returns false when the path has ended.
+template <unsigned F, bool GRAD>
+__device__ __forceinline__ bool path_step(const AdjArgs& a, PathState& p, int* stk, float* gacc, bool use_lds,
+                                          bool& overflow) {
+    const SceneView& sv = a.sv;
     auto add_albedo = [&](const Mat& m, int mid, float tu, float tv, F3 v) {
         if (m.tex < 0) {
             gadd(gacc, a.grad, use_lds, 4 * mid + 0, v.x);
@@ -211,130 +229,133 @@ __device__ __forceinline__ F3 replay(const AdjArgs& a, int slot, F3 Lt, F3 g, in
             scatter_texels(a, sv, m.tex, tu, tv, v);
         }
     };
-    for (int depth = 0;; ++depth) {
-        const HitRec h = trace_ray<false, ABLK>(sv, o, d, 0.0f, stk, overflow);
-        if (h.prim < 0) {  // environment: no parameter dependence
-            if (!GRAD) {
-                const F3 e = mulv(beta, env_radiance(sv, d));
-                L = add(L, e);
-            }
-            break;
+    const HitRec h = trace_ray<false, ABLK>(sv, p.o, p.d, 0.0f, stk, overflow);
+    if (h.prim < 0) {  // environment: no parameter dependence
+        if (!GRAD) {
+            const F3 e = mulv(p.beta, env_radiance(sv, p.d));
+            p.L = add(p.L, e);
         }
-        const TriShade ts = sv.shade[h.prim];
-        const float b1 = h.b1, b2 = h.b2, w0 = 1.0f - b1 - b2;
-        const F3 nrm = normalized(add(add(mul(f3(ts.s0.x, ts.s0.y, ts.s0.z), w0), mul(f3(ts.s0.w, ts.s1.x, ts.s1.y), b1)),
-                                      mul(f3(ts.s1.z, ts.s1.w, ts.s2.x), b2)));
-        const int mid = __float_as_int(ts.s3.w);
-        const Mat m = load_mat(sv, mid);
-        if (m.type == MAT_LIGHT) {
-            if (!GRAD) {
-                L = add(L, mulv(beta, mul(m.kd, fac)));
-            } else if (prev_mat >= 0 && dfac != 0.0f) {
-                // Le * fac: fac depends on the previous vertex's roughness through its NEE MIS weight
-                const float v = g.x * beta.x * m.kd.x * dfac + g.y * beta.y * m.kd.y * dfac + g.z * beta.z * m.kd.z * dfac;
-                gadd(gacc, a.grad, use_lds, 4 * prev_mat + 3, v);
-            }
-            break;
+        return false;
+    }
+    const TriShade ts = sv.shade[h.prim];
+    const float b1 = h.b1, b2 = h.b2, w0 = 1.0f - b1 - b2;
+    const F3 nrm = normalized(add(add(mul(f3(ts.s0.x, ts.s0.y, ts.s0.z), w0), mul(f3(ts.s0.w, ts.s1.x, ts.s1.y), b1)),
+                                  mul(f3(ts.s1.z, ts.s1.w, ts.s2.x), b2)));
+    const int mid = __float_as_int(ts.s3.w);
+    const Mat m = load_mat(sv, mid);
+    if (m.type == MAT_LIGHT) {
+        if (!GRAD) {
+            p.L = add(p.L, mulv(p.beta, mul(m.kd, p.fac)));
+        } else if (p.prev_mat >= 0 && p.dfac != 0.0f) {
+            // Le * fac: fac depends on the previous vertex's roughness through its NEE MIS weight
+            const F3 g = p.g, beta = p.beta;
+            const float v = g.x * beta.x * m.kd.x * p.dfac + g.y * beta.y * m.kd.y * p.dfac + g.z * beta.z * m.kd.z * p.dfac;
+            gadd(gacc, a.grad, use_lds, 4 * p.prev_mat + 3, v);
         }
-        if (dot(nrm, d) > 0.0f && !mat_transmissive(m)) break;
-        const float tu = w0 * ts.s2.y + b1 * ts.s2.w + b2 * ts.s3.y;
-        const float tv = w0 * ts.s2.z + b1 * ts.s3.x + b2 * ts.s3.z;
-        const F3 hp = add(o, mul(d, h.t));
-        const F3 wo = neg(d);
-        const float4 ua = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 0u);
-        float fac_next = 1.0f, dfac_next = 0.0f;
-        if (a.nee && mat_needs_nee(m) && sv.n_lights > 0) {
-            const LightRec lr = sv.lights[pick_light(sv, ua.x)];
-            float r1 = ua.y, r2 = ua.z;
-            if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
-            const float lw = 1.0f - r1 - r2;
-            const F3 lp = add(add(mul(f3(lr.p0.x, lr.p0.y, lr.p0.z), lw), mul(f3(lr.p1.x, lr.p1.y, lr.p1.z), r1)),
-                              mul(f3(lr.p2.x, lr.p2.y, lr.p2.z), r2));
-            const F3 ln = normalized(add(add(mul(f3(lr.n0.x, lr.n0.y, lr.n0.z), lw), mul(f3(lr.n1.x, lr.n1.y, lr.n1.z), r1)),
-                                         mul(f3(lr.n2.x, lr.n2.y, lr.n2.z), r2)));
-            const F3 lv = sub(lp, hp);
-            const F3 wl = normalized(lv);
-            const float lcos = dot(ln, neg(wl));
-            const float ncos = dot(nrm, normalized(sub(lp, hp)));
-            const GgxParts pl = vertex_parts<F, GRAD>(sv, m, wo, nrm, wl, tu, tv);
-            const float lpdf = sv.light_p;
-            const float lop = lpdf * dot(lv, lv) / ((1e-6f < lcos) ? lcos : 1e-6f);
-            float mis = lop / (lop + pl.pdf);
-            float dmis = -lop / ((lop + pl.pdf) * (lop + pl.pdf)) * pl.dpdf_da;
-            if (a.mis == 0) mis = 0.5f, dmis = 0.0f;
-            fac_next = 1.0f - mis;
-            dfac_next = -dmis;
-            const F3 Ll = mul(divs(f3(lr.ke.x, lr.ke.y, lr.ke.z), dot(lv, lv)), (0.0f < lcos ? lcos : 0.0f) / lpdf);
-            const float cn = (0.0f < ncos ? ncos : 0.0f);
-            const F3 c = mul(mulv(mulv(beta, Ll), pl.f), cn * mis);
-            if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
-                const F3 so = add(hp, mul(wl, 1e-3f));
-                const HitRec sh = trace_ray<true, ABLK>(sv, so, wl, norm(lv), stk, overflow);
-                if (sh.prim < 0) {  // visible: d(beta Ll f cos mis)
-                    if (!GRAD) {
-                        L = add(L, c);
-                    } else {
-                        const F3 k = mul(mulv(beta, Ll), cn);
-                        add_albedo(m, mid, tu, tv,
-                                   f3(g.x * k.x * pl.df_dkd.x * mis, g.y * k.y * pl.df_dkd.y * mis,
-                                      g.z * k.z * pl.df_dkd.z * mis));
-                        const float va = g.x * k.x * (pl.df_da.x * mis + pl.f.x * dmis) +
-                                         g.y * k.y * (pl.df_da.y * mis + pl.f.y * dmis) +
-                                         g.z * k.z * (pl.df_da.z * mis + pl.f.z * dmis);
-                        gadd(gacc, a.grad, use_lds, 4 * mid + 3, va);
-                        Lrem = sub(Lrem, c);
-                    }
+        return false;
+    }
+    if (dot(nrm, p.d) > 0.0f && !mat_transmissive(m)) return false;
+    const float tu = w0 * ts.s2.y + b1 * ts.s2.w + b2 * ts.s3.y;
+    const float tv = w0 * ts.s2.z + b1 * ts.s3.x + b2 * ts.s3.z;
+    const F3 hp = add(p.o, mul(p.d, h.t));
+    const F3 wo = neg(p.d);
+    const F3 g = p.g, beta = p.beta;
+    const float4 ua = rng4(a.seed, uint32_t(p.pixel), p.sample, uint32_t(1 + p.depth), 0u);
+    float fac_next = 1.0f, dfac_next = 0.0f;
+    if (a.nee && mat_needs_nee(m) && sv.n_lights > 0) {
+        const LightRec lr = sv.lights[pick_light(sv, ua.x)];
+        float r1 = ua.y, r2 = ua.z;
+        if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
+        const float lw = 1.0f - r1 - r2;
+        const F3 lp = add(add(mul(f3(lr.p0.x, lr.p0.y, lr.p0.z), lw), mul(f3(lr.p1.x, lr.p1.y, lr.p1.z), r1)),
+                          mul(f3(lr.p2.x, lr.p2.y, lr.p2.z), r2));
+        const F3 ln = normalized(add(add(mul(f3(lr.n0.x, lr.n0.y, lr.n0.z), lw), mul(f3(lr.n1.x, lr.n1.y, lr.n1.z), r1)),
+                                     mul(f3(lr.n2.x, lr.n2.y, lr.n2.z), r2)));
+        const F3 lv = sub(lp, hp);
+        const F3 wl = normalized(lv);
+        const float lcos = dot(ln, neg(wl));
+        const float ncos = dot(nrm, normalized(sub(lp, hp)));
+        const GgxParts pl = vertex_parts<F, GRAD>(sv, m, wo, nrm, wl, tu, tv);
+        const float lpdf = sv.light_p;
+        const float lop = lpdf * dot(lv, lv) / ((1e-6f < lcos) ? lcos : 1e-6f);
+        float mis = lop / (lop + pl.pdf);
+        float dmis = -lop / ((lop + pl.pdf) * (lop + pl.pdf)) * pl.dpdf_da;
+        if (a.mis == 0) mis = 0.5f, dmis = 0.0f;
+        fac_next = 1.0f - mis;
+        dfac_next = -dmis;
+        const F3 Ll = mul(divs(f3(lr.ke.x, lr.ke.y, lr.ke.z), dot(lv, lv)), (0.0f < lcos ? lcos : 0.0f) / lpdf);
+        const float cn = (0.0f < ncos ? ncos : 0.0f);
+        const F3 c = mul(mulv(mulv(beta, Ll), pl.f), cn * mis);
+        if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
+            const F3 so = add(hp, mul(wl, 1e-3f));
+            const HitRec sh = trace_ray<true, ABLK>(sv, so, wl, norm(lv), stk, overflow);
+            if (sh.prim < 0) {  // visible: d(beta Ll f cos mis)
+                if (!GRAD) {
+                    p.L = add(p.L, c);
+                } else {
+                    const F3 k = mul(mulv(beta, Ll), cn);
+                    add_albedo(m, mid, tu, tv,
+                               f3(g.x * k.x * pl.df_dkd.x * mis, g.y * k.y * pl.df_dkd.y * mis,
+                                  g.z * k.z * pl.df_dkd.z * mis));
+                    const float va = g.x * k.x * (pl.df_da.x * mis + pl.f.x * dmis) +
+                                     g.y * k.y * (pl.df_da.y * mis + pl.f.y * dmis) +
+                                     g.z * k.z * (pl.df_da.z * mis + pl.f.z * dmis);
+                    gadd(gacc, a.grad, use_lds, 4 * mid + 3, va);
+                    p.L = sub(p.L, c);
                 }
             }
         }
-        bool cont = true;
-        float prr = 1.0f;
-        if (depth >= a.rrdepth) {
-            cont = !(ua.w > a.prr);
-            prr = a.prr;
-        }
-        if (a.maxdepth > 0 && depth + 1 >= a.maxdepth) cont = false;
-        if (!cont) break;
-        const float4 ub = rng4(a.seed, uint32_t(pixel), sample, uint32_t(1 + depth), 1u);
-        const F3 wi = bsdf_sample<F>(m, ub.x, ub.y, ub.z, wo, nrm);
-        const GgxParts pi = vertex_parts<F, GRAD>(sv, m, wo, nrm, wi, tu, tv);
-        const float cw = fabsf(dot(wi, nrm));
-        const F3 nb = f3(beta.x * cw * pi.f.x / pi.pdf / prr, beta.y * cw * pi.f.y / pi.pdf / prr,
-                         beta.z * cw * pi.f.z / pi.pdf / prr);
-        if (GRAD) {
-            // everything still in Lrem arrives through this bounce: d log T = d log f - d log pdf + score
-            const float score = m.type == MAT_GGXREFL ? ggx_score(m.alpha, wo, nrm, wi) : 0.0f;
-            const float dlp = pi.pdf > 0.0f ? pi.dpdf_da / pi.pdf : 0.0f;
-            float va = 0.0f;
-            F3 dal = f3(0.0f, 0.0f, 0.0f);
-            if (pi.f.x != 0.0f) {
-                dal.x = g.x * Lrem.x * pi.df_dkd.x / pi.f.x;
-                va += g.x * Lrem.x * (pi.df_da.x / pi.f.x - dlp + score);
-            }
-            if (pi.f.y != 0.0f) {
-                dal.y = g.y * Lrem.y * pi.df_dkd.y / pi.f.y;
-                va += g.y * Lrem.y * (pi.df_da.y / pi.f.y - dlp + score);
-            }
-            if (pi.f.z != 0.0f) {
-                dal.z = g.z * Lrem.z * pi.df_dkd.z / pi.f.z;
-                va += g.z * Lrem.z * (pi.df_da.z / pi.f.z - dlp + score);
-            }
-            if (m.type == MAT_LAMBERT || m.type == MAT_GGXREFL) add_albedo(m, mid, tu, tv, dal);
-            if (m.type == MAT_GGXREFL) gadd(gacc, a.grad, use_lds, 4 * mid + 3, va);
-        }
-        if (nb.x == 0.0f && nb.y == 0.0f && nb.z == 0.0f) break;
-        beta = nb;
-        fac = fac_next;
-        dfac = dfac_next;
-        prev_mat = mid;
-        o = add(hp, mul(wi, 1e-3f));
-        d = wi;
     }
-    return L;
+    bool cont = true;
+    float prr = 1.0f;
+    if (p.depth >= a.rrdepth) {
+        cont = !(ua.w > a.prr);
+        prr = a.prr;
+    }
+    if (a.maxdepth > 0 && p.depth + 1 >= a.maxdepth) cont = false;
+    if (!cont) return false;
+    const float4 ub = rng4(a.seed, uint32_t(p.pixel), p.sample, uint32_t(1 + p.depth), 1u);
+    const F3 wi = bsdf_sample<F>(m, ub.x, ub.y, ub.z, wo, nrm);
+    const GgxParts pi = vertex_parts<F, GRAD>(sv, m, wo, nrm, wi, tu, tv);
+    const float cw = fabsf(dot(wi, nrm));
+    const F3 nb = f3(beta.x * cw * pi.f.x / pi.pdf / prr, beta.y * cw * pi.f.y / pi.pdf / prr,
+                     beta.z * cw * pi.f.z / pi.pdf / prr);
+    if (GRAD) {
+        // everything still in p.L (the radiance not yet accounted for) arrives through this bounce:
+        // d log T = d log f - d log pdf + score
+        const F3 Lrem = p.L;
+        const float score = m.type == MAT_GGXREFL ? ggx_score(m.alpha, wo, nrm, wi) : 0.0f;
+        const float dlp = pi.pdf > 0.0f ? pi.dpdf_da / pi.pdf : 0.0f;
+        float va = 0.0f;
+        F3 dal = f3(0.0f, 0.0f, 0.0f);
+        if (pi.f.x != 0.0f) {
+            dal.x = g.x * Lrem.x * pi.df_dkd.x / pi.f.x;
+            va += g.x * Lrem.x * (pi.df_da.x / pi.f.x - dlp + score);
+        }
+        if (pi.f.y != 0.0f) {
+            dal.y = g.y * Lrem.y * pi.df_dkd.y / pi.f.y;
+            va += g.y * Lrem.y * (pi.df_da.y / pi.f.y - dlp + score);
+        }
+        if (pi.f.z != 0.0f) {
+            dal.z = g.z * Lrem.z * pi.df_dkd.z / pi.f.z;
+            va += g.z * Lrem.z * (pi.df_da.z / pi.f.z - dlp + score);
+        }
+        if (m.type == MAT_LAMBERT || m.type == MAT_GGXREFL) add_albedo(m, mid, tu, tv, dal);
+        if (m.type == MAT_GGXREFL) gadd(gacc, a.grad, use_lds, 4 * mid + 3, va);
+    }
+    if (nb.x == 0.0f && nb.y == 0.0f && nb.z == 0.0f) return false;
+    p.beta = nb;
+    p.fac = fac_next;
+    p.dfac = dfac_next;
+    p.prev_mat = mid;
+    p.o = add(hp, mul(wi, 1e-3f));
+    p.d = wi;
+    ++p.depth;
+    return true;
 }
 
-// Persistent grid over samples [slot0, slot1).  Stored radiance (the last single-batch render)
-// or, without it, a forward replay per sample first (recompute-only: nothing per path is kept).
+// Persistent grid over samples [slot0, slot1) with their radiance in a.radiance[slot - a.rad_off]
+// (the last single-batch render, or a forward pass of k_render_paths just before).
 template <unsigned F>
 __global__ __launch_bounds__(ABLK) void k_adjoint(AdjArgs a) {
     extern __shared__ char smem[];
@@ -345,21 +366,31 @@ __global__ __launch_bounds__(ABLK) void k_adjoint(AdjArgs a) {
         for (int k = threadIdx.x; k < a.n_params; k += blockDim.x) gacc[k] = 0.0f;
     __syncthreads();
     bool overflow = false;
-    for (int slot = a.slot0 + blockIdx.x * blockDim.x + threadIdx.x; slot < a.slot1; slot += gridDim.x * blockDim.x) {
-        const int pixel = slot % a.npix;
-        F3 Lt;
-        if (a.radiance) {
-            const float4 r = a.radiance[slot];
-            Lt = f3(r.x, r.y, r.z);
-        } else {
-            Lt = replay<F, false>(a, slot, f3(0, 0, 0), f3(0, 0, 0), stk, gacc, use_lds, overflow);
+    const int stride = int(gridDim.x * blockDim.x);
+    int slot = a.slot0 + int(blockIdx.x * blockDim.x + threadIdx.x);
+    PathState p;
+    // the lane's next sample with a non-zero weight: max(0, L) per channel at the camera
+    // (rendererptls.cpp:173) has zero slope where L <= 0 or NaN
+    auto next = [&]() -> bool {
+        for (; slot < a.slot1; slot += stride) {
+            const int pixel = slot % a.npix;
+            const float4 r = a.radiance[slot - a.rad_off];
+            const F3 g = f3(r.x > 0.0f ? a.dldi[3 * pixel] * a.inv_spp : 0.0f,
+                            r.y > 0.0f ? a.dldi[3 * pixel + 1] * a.inv_spp : 0.0f,
+                            r.z > 0.0f ? a.dldi[3 * pixel + 2] * a.inv_spp : 0.0f);
+            if (g.x != 0.0f || g.y != 0.0f || g.z != 0.0f) {
+                path_begin(a, slot, f3(r.x, r.y, r.z), g, p);
+                return true;
+            }
         }
-        // max(0, L) per channel at the camera (rendererptls.cpp:173): zero slope where L <= 0 or NaN
-        const F3 g = f3(Lt.x > 0.0f ? a.dldi[3 * pixel] * a.inv_spp : 0.0f,
-                        Lt.y > 0.0f ? a.dldi[3 * pixel + 1] * a.inv_spp : 0.0f,
-                        Lt.z > 0.0f ? a.dldi[3 * pixel + 2] * a.inv_spp : 0.0f);
-        if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) continue;
-        replay<F, true>(a, slot, Lt, g, stk, gacc, use_lds, overflow);
+        return false;
+    };
+    bool active = next();
+    while (active) {
+        if (!path_step<F, true>(a, p, stk, gacc, use_lds, overflow)) {
+            slot += stride;
+            active = next();
+        }
     }
     if (overflow) atomicOr(a.err, 1);
     __syncthreads();
@@ -370,17 +401,24 @@ __global__ __launch_bounds__(ABLK) void k_adjoint(AdjArgs a) {
 
 // Forward-only render, one lane per path: each lane replays its samples' paths to the end (the
 // radiance is the wavefront's, bit for bit: same Philox numbers, same traversal, same BSDF code,
-// same order of additions) and writes it to radiance[slot].  For small frames of long paths
-// (inverse rendering on a closed room) this avoids the wavefront's per-bounce launches and tails.
+// same order of additions) and writes it to radiance[slot - a.rad_off].  For small frames of long
+// paths (inverse rendering on a closed room) this avoids the wavefront's per-bounce launches and
+// tails.
 template <unsigned F>
-__global__ __launch_bounds__(ABLK) void k_render_paths(AdjArgs a, float4* __restrict__ radiance) {
+__global__ __launch_bounds__(ABLK) __attribute__((amdgpu_waves_per_eu(4))) void k_render_paths(AdjArgs a, float4* __restrict__ radiance) {
     extern __shared__ char smem[];
     int* stk = reinterpret_cast<int*>(smem) + threadIdx.x;
     bool overflow = false;
-    for (int slot = a.slot0 + blockIdx.x * blockDim.x + threadIdx.x; slot < a.slot1; slot += gridDim.x * blockDim.x) {
-        const F3 L = replay<F, false>(a, slot, f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), stk, nullptr, false,
-                                      overflow);
-        radiance[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+    const int stride = int(gridDim.x * blockDim.x);
+    int slot = a.slot0 + int(blockIdx.x * blockDim.x + threadIdx.x);
+    PathState p;
+    if (slot < a.slot1) path_begin(a, slot, f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), p);
+    while (slot < a.slot1) {
+        if (!path_step<F, false>(a, p, stk, nullptr, false, overflow)) {
+            radiance[slot - a.rad_off] = make_float4(p.L.x, p.L.y, p.L.z, 0.0f);
+            slot += stride;
+            if (slot < a.slot1) path_begin(a, slot, f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), p);
+        }
     }
     if (overflow) atomicOr(a.err, 1);
 }
@@ -447,28 +485,49 @@ double loss_l2(Scene& sc, const float* target, float* dldi_host) {
     return loss;
 }
 
+namespace {
+
+// Launch shape of k_render_paths for the scene's material features (as the shade kernels).
+struct PathsShape {
+    unsigned feat;
+    int grid;
+    size_t shmem;
+};
+PathsShape paths_shape(const Scene& sc) {
+    PathsShape L;
+    L.shmem = size_t(sc.view.stack_cap + 1) * ABLK * sizeof(int);
+    if (L.shmem > device_max_lds()) throw std::runtime_error("BVH too deep for the LDS traversal stack");
+    L.feat = sc.features & (F_GGX | F_EXT) ? F_ALL : (sc.features & F_TEX ? F_TEX : 0u);
+    int per_cu = 0;
+    check_hip(L.feat == F_ALL   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_paths<F_ALL>, ABLK, L.shmem)
+              : L.feat == F_TEX ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_paths<F_TEX>, ABLK, L.shmem)
+                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_paths<0u>, ABLK, L.shmem),
+              "occupancy");
+    L.grid = std::max(1, per_cu) * device_cu_count();
+    return L;
+}
+// Samples [a.slot0, a.slot1) into out[slot - a.rad_off].
+void launch_paths(const Scene& sc, const PathsShape& L, const AdjArgs& a, float4* out) {
+    if (L.feat == F_ALL) hipLaunchKernelGGL(k_render_paths<F_ALL>, dim3(L.grid), dim3(ABLK), L.shmem, sc.stream, a, out);
+    else if (L.feat == F_TEX) hipLaunchKernelGGL(k_render_paths<F_TEX>, dim3(L.grid), dim3(ABLK), L.shmem, sc.stream, a, out);
+    else hipLaunchKernelGGL(k_render_paths<0u>, dim3(L.grid), dim3(ABLK), L.shmem, sc.stream, a, out);
+    check_hip(hipGetLastError(), "k_render_paths launch");
+}
+
+constexpr long long PATH_CHUNK = 16ll << 20;  // launches of at most 16 M samples
+
+}  // namespace
+
 void render_paths(Scene& sc, const CamView& cam, const RenderParams& p, uint32_t sample0, int P, float4* radiance,
                   int* err) {
     if (p.env_is) throw std::runtime_error("the path kernel does not sample the sky (env_is)");
     AdjArgs a{sc.view, cam, p.W, p.H, p.W * p.H, P, p.seed, sample0, p.prr, p.mis, p.maxdepth, p.rrdepth, nullptr,
-              0, P, nullptr, 0.0f, 0, nullptr, err, -1, nullptr, p.integrator == 1 ? 0 : 1};
-    const size_t shmem = size_t(sc.view.stack_cap + 1) * ABLK * sizeof(int);
-    if (shmem > device_max_lds()) throw std::runtime_error("BVH too deep for the LDS traversal stack");
-    const unsigned feat = sc.features & (F_GGX | F_EXT) ? F_ALL : (sc.features & F_TEX ? F_TEX : 0u);
-    int per_cu = 0;
-    check_hip(feat == F_ALL   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_paths<F_ALL>, ABLK, shmem)
-              : feat == F_TEX ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_paths<F_TEX>, ABLK, shmem)
-                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_paths<0u>, ABLK, shmem),
-              "occupancy");
-    const int grid = std::max(1, per_cu) * device_cu_count();
-    const long long chunk = 16ll << 20;  // launches of at most 16 M samples
-    for (long long s0 = 0; s0 < P; s0 += chunk) {
+              0, P, nullptr, 0.0f, 0, nullptr, err, -1, nullptr, p.integrator == 1 ? 0 : 1, 0};
+    const PathsShape L = paths_shape(sc);
+    for (long long s0 = 0; s0 < P; s0 += PATH_CHUNK) {
         a.slot0 = int(s0);
-        a.slot1 = int(std::min<long long>(P, s0 + chunk));
-        if (feat == F_ALL) hipLaunchKernelGGL(k_render_paths<F_ALL>, dim3(grid), dim3(ABLK), shmem, sc.stream, a, radiance);
-        else if (feat == F_TEX) hipLaunchKernelGGL(k_render_paths<F_TEX>, dim3(grid), dim3(ABLK), shmem, sc.stream, a, radiance);
-        else hipLaunchKernelGGL(k_render_paths<0u>, dim3(grid), dim3(ABLK), shmem, sc.stream, a, radiance);
-        check_hip(hipGetLastError(), "k_render_paths launch");
+        a.slot1 = int(std::min<long long>(P, s0 + PATH_CHUNK));
+        launch_paths(sc, L, a, radiance);
     }
 }
 
@@ -512,7 +571,15 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
     if (P > 0x7FFFFFFFll) throw std::runtime_error("adjoint: more than 2^31 samples");
     AdjArgs a{sc.view, cam, p.W, p.H, p.W * p.H, int(P), p.seed, uint32_t(p.spp_offset), p.prr, p.mis, p.maxdepth,
               p.rrdepth, stored ? t.radiance.as<float4>() : nullptr, 0, int(P), t.adj_dldi.as<float>(),
-              1.0f / float(p.spp), n_params, t.adj_grad.as<double>(), cnt + 15, tex_id, gtex, p.integrator == 1 ? 0 : 1};
+              1.0f / float(p.spp), n_params, t.adj_grad.as<double>(), cnt + 15, tex_id, gtex, p.integrator == 1 ? 0 : 1,
+              0};
+    // without the render's own per-sample radiance, each chunk is first rendered forward (one lane
+    // per path) into a chunk-sized buffer: at most 16 M x 16 B, whatever the sample count
+    PathsShape fwd{};
+    if (!stored) {
+        fwd = paths_shape(sc);
+        t.adj_rad.ensure(sizeof(float4) * size_t(std::min<long long>(P, PATH_CHUNK)));
+    }
     const size_t shmem = size_t(sc.view.stack_cap + 1) * ABLK * sizeof(int) +
                          (n_params <= LDS_PARAMS ? size_t(n_params) * sizeof(float) : 0);
     if (shmem > device_max_lds()) throw std::runtime_error("adjoint LDS budget exceeded (BVH too deep)");
@@ -525,10 +592,14 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
               "occupancy");
     const int grid = std::max(1, per_cu) * device_cu_count();
     // launches of at most 16 M samples, so that no single dispatch runs for long
-    const long long chunk = 16ll << 20;
-    for (long long s0 = 0; s0 < P; s0 += chunk) {
+    for (long long s0 = 0; s0 < P; s0 += PATH_CHUNK) {
         a.slot0 = int(s0);
-        a.slot1 = int(std::min(P, s0 + chunk));
+        a.slot1 = int(std::min(P, s0 + PATH_CHUNK));
+        if (!stored) {
+            a.radiance = t.adj_rad.as<float4>();
+            a.rad_off = a.slot0;
+            launch_paths(sc, fwd, a, t.adj_rad.as<float4>());
+        }
         if (feat == F_ALL) hipLaunchKernelGGL(k_adjoint<F_ALL>, dim3(grid), dim3(ABLK), shmem, sc.stream, a);
         else if (feat == F_TEX) hipLaunchKernelGGL(k_adjoint<F_TEX>, dim3(grid), dim3(ABLK), shmem, sc.stream, a);
         else hipLaunchKernelGGL(k_adjoint<0u>, dim3(grid), dim3(ABLK), shmem, sc.stream, a);

@@ -20,6 +20,7 @@
 #include <atomic>
 #include <stdexcept>
 #include <chrono>
+#include <cstring>
 #include <cmath>
 #include <omp.h>
 
@@ -274,6 +275,20 @@ BvhBuild build_sah(const float* p, int n, int nthreads) {
     out.root_ref = ref_of(root);
     out.order = b.idx;
     collapse_wide(b.nodes, bx, root, ref_of, out);
+    // small trees: the leaves' own boxes in DFS order for the per-lane walks (a leaf is reached by
+    // the tree walk iff its box passes the slab test, so testing every leaf box gives the same hits)
+    out.flat.clear();
+    if ((nn + 1) / 2 <= FLAT_LEAVES) {
+        for (int v : order_nodes) {
+            const Tmp& t = b.nodes[v];
+            if (t.l >= 0) continue;
+            const int32_t r = ref_of(v);
+            float rf;
+            std::memcpy(&rf, &r, 4);
+            out.flat.push_back(make_float4(bx[v].mn[0], bx[v].mn[1], bx[v].mn[2], rf));
+            out.flat.push_back(make_float4(bx[v].mx[0], bx[v].mx[1], bx[v].mx[2], 0.0f));
+        }
+    }
 
     out.left.resize(nn); out.right.resize(nn);
     out.leaf_first.resize(nn); out.leaf_count.resize(nn);

@@ -16,6 +16,7 @@ struct BvhBuild {
     std::vector<Node4> wide;       // the tree collapsed 4 wide (Node4 index 0 = root)
     int32_t wide_root = 0;         // Node4 index 0 or a leaf reference
     int wide_depth = 0;            // Node4s on the longest root-to-leaf path
+    std::vector<float4> flat;      // small trees: 2 float4 per leaf (see SceneView::flat), else empty
     // canonical preorder export over all nodes (same format as the test oracle's export)
     std::vector<int32_t> left, right, leaf_first, leaf_count;
     std::vector<float> box;        // 6 per node: min xyz, max xyz

@@ -22,6 +22,7 @@ enum MatType { MAT_LAMBERT = 0, MAT_LIGHT = 1, MAT_GGXREFL = 2, MAT_GGX = 3, MAT
 // Leaf references inside NodePair::ref: bit 31 set, bits 27..30 = primitive count (1..5),
 // bits 0..26 = first primitive in TriGeom order.  Internal references are NodePair indices.
 constexpr uint32_t LEAF_BIT = 0x80000000u;
+constexpr int FLAT_LEAVES = 64;  // scenes with at most this many leaves also get a flat leaf list
 __host__ __device__ inline bool ref_is_leaf(int32_t r) { return (uint32_t(r) & LEAF_BIT) != 0; }
 __host__ __device__ inline int ref_first(int32_t r) { return int(uint32_t(r) & 0x07FFFFFFu); }
 __host__ __device__ inline int ref_count(int32_t r) { return int((uint32_t(r) >> 27) & 0xFu); }
@@ -95,6 +96,10 @@ struct SceneView {
     int32_t short_stack; // stack entries of the first persistent trace launch (see trace.hip)
     int32_t wide_root;       // root as a Node4 index, or a leaf reference
     int32_t wide_stack_cap;  // full-depth stack entries of the 4-wide walk: 3 per level + 1
+    // small scenes (n_flat > 0 leaves, at most FLAT_LEAVES): the leaves' exact boxes in DFS order,
+    // two float4 per leaf {min xyz, leaf reference bits}, {max xyz, 0}, for the per-lane walks
+    const float4* flat;
+    int32_t n_flat;
     // environment sampler: one cell per texel square of the map (env_w x env_h cells, 0 if none),
     // {alias probability, alias cell (int bits), pdf of the cell, pdf of the alias cell}; the
     // solid-angle density of a direction in cell k is pdf_k * env_norm / sin(theta)

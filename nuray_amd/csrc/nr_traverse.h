@@ -198,11 +198,35 @@ __device__ __forceinline__ bool trav_leaf(const SceneView& sv, F3 o, F3 d, doubl
     return false;
 }
 
+// Small scenes (SceneView::n_flat > 0): every lane of a wave tests the same leaf boxes in the same
+// order -- a wave-uniform loop whose box and triangle loads have one address per wave -- instead
+// of walking the tree, where the lanes' different paths leave most of a wave idle.  Exact: the tree
+// walk reaches a leaf iff the leaf's own box passes the slab test (every ancestor box contains it),
+// and the winner does not depend on the order in which leaves are tested.
+template <bool ANY>
+__device__ __forceinline__ HitRec trace_flat(const SceneView& sv, F3 o, F3 d, float len) {
+    const SlabRay sr = slab_ray(o, d);
+    TravState s;
+    float bt = ANY ? fminf(len, 1e18f) : 1e18f, bb1 = 0.0f, bb2 = 0.0f;
+    int bprim = -1, blf = -1;
+    uint32_t ntt = 0;
+    bool done = false;
+    for (int k = 0; k < sv.n_flat; ++k) {
+        const float4 lo = sv.flat[2 * k], hi = sv.flat[2 * k + 1];
+        if (done || !slab_fast(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, sr)) continue;
+        s.leaf = __float_as_int(lo.w);
+        done = trav_leaf<ANY, false>(sv, o, d, double(len), s, bt, bb1, bb2, bprim, blf, ntt);
+    }
+    if (ANY) return HitRec{done ? bt : 1e18f, bb1, bb2, done ? bprim : -1};
+    return HitRec{bt, bb1, bb2, bprim};
+}
+
 // Whole traversal of one ray by one lane (no refill): closest hit, or shadow test with segment
 // length len (returns hit.prim >= 0 iff blocked).  overflow is set if the stack ran out.  The
 // stack holds stack_cap + 1 slots STRIDE ints apart.
 template <bool ANY, int STRIDE>
 __device__ __forceinline__ HitRec trace_ray(const SceneView& sv, F3 o, F3 d, float len, int* stk, bool& overflow) {
+    if (sv.n_flat > 0) return trace_flat<ANY>(sv, o, d, len);
     const SlabRay sr = slab_ray(o, d);
     TravState s;
     trav_begin(sv, sr, stk, s);

@@ -58,6 +58,7 @@ struct RenderTarget {
     RenderParams adj_p;   // the render those samples came from
     CamView adj_cam{};
     DevBuf adj_target, adj_dldi, adj_loss, adj_grad, adj_gtex;
+    DevBuf adj_rad;                    // float4 per sample of one adjoint chunk (recomputed radiance)
     // radiance-cache rendering: spread state per queued ray, cache queries, training vertices
     DevBuf nstate[2], nrc_queries, nrc_recs, nrc_lb, nrc_tgt, nrc_batch, nrc_batch_tgt;
     ~RenderTarget();

@@ -231,6 +231,7 @@ void Scene::build(int nthreads) {
     }
     d_nodes.upload(bvh.pairs.data(), bvh.pairs.size() * sizeof(NodePair), stream);
     d_wide.upload(bvh.wide.data(), bvh.wide.size() * sizeof(Node4), stream);
+    d_flat.upload(bvh.flat.data(), bvh.flat.size() * sizeof(float4), stream);
     d_tris.upload(g.data(), g.size() * sizeof(TriGeom), stream);
     d_shade.upload(s.data(), s.size() * sizeof(TriShade), stream);
     d_lights.upload(lights.data(), lights.size() * sizeof(LightRec), stream);
@@ -240,6 +241,8 @@ void Scene::build(int nthreads) {
     view = SceneView{};
     view.nodes = d_nodes.as<NodePair>();
     view.wide = d_wide.as<Node4>();
+    view.flat = d_flat.as<float4>();
+    view.n_flat = int(bvh.flat.size() / 2);
     view.tris = d_tris.as<TriGeom>();
     view.shade = d_shade.as<TriShade>();
     view.lights = d_lights.as<LightRec>();
