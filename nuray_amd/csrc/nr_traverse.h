@@ -199,22 +199,28 @@ __device__ __forceinline__ bool trav_leaf(const SceneView& sv, F3 o, F3 d, doubl
 }
 
 // Small scenes (SceneView::n_flat > 0): every lane of a wave tests the same leaf boxes in the same
-// order -- a wave-uniform loop whose box and triangle loads have one address per wave -- instead
-// of walking the tree, where the lanes' different paths leave most of a wave idle.  Exact: the tree
-// walk reaches a leaf iff the leaf's own box passes the slab test (every ancestor box contains it),
-// and the winner does not depend on the order in which leaves are tested.
+// order -- a wave-uniform loop whose box loads have one address per wave -- and records the leaves
+// its ray passes in a 64-bit mask; then each lane tests the triangles of its own leaves only.  The
+// tree walk instead leaves most of a wave idle on scenes this small (the lanes' walks differ in
+// length).  Exact: the tree walk reaches a leaf iff the leaf's own box passes the slab test (every
+// ancestor box contains it), and the winner does not depend on the order in which leaves are tested.
 template <bool ANY>
 __device__ __forceinline__ HitRec trace_flat(const SceneView& sv, F3 o, F3 d, float len) {
     const SlabRay sr = slab_ray(o, d);
+    uint64_t pass = 0;
+    for (int k = 0; k < sv.n_flat; ++k) {
+        const float4 lo = sv.flat[2 * k], hi = sv.flat[2 * k + 1];
+        if (slab_fast(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, sr)) pass |= 1ull << k;
+    }
     TravState s;
     float bt = ANY ? fminf(len, 1e18f) : 1e18f, bb1 = 0.0f, bb2 = 0.0f;
     int bprim = -1, blf = -1;
     uint32_t ntt = 0;
     bool done = false;
-    for (int k = 0; k < sv.n_flat; ++k) {
-        const float4 lo = sv.flat[2 * k], hi = sv.flat[2 * k + 1];
-        if (done || !slab_fast(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, sr)) continue;
-        s.leaf = __float_as_int(lo.w);
+    while (pass != 0 && !done) {
+        const int k = __builtin_ctzll(pass);
+        pass &= pass - 1;
+        s.leaf = __float_as_int(sv.flat[2 * k].w);
         done = trav_leaf<ANY, false>(sv, o, d, double(len), s, bt, bb1, bb2, bprim, blf, ntt);
     }
     if (ANY) return HitRec{done ? bt : 1e18f, bb1, bb2, done ? bprim : -1};
