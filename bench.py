@@ -10,7 +10,9 @@ nr_render_ptnee call (all wavefront batches of the frame, FP64 accumulation incl
 The inverse-rendering iteration (render + loss + path-replay adjoint + Adam on the 256x256
 64-material box room) is timed after the frames and reported as "adjoint"; the same frame with
 sky importance sampling as "env_sampling" (time, variance, equal-error gain); the time to RMSE
-1e-2 of plain ptnee and of the radiance cache at 1920x1080, 1 spp per frame, as "nrc".
+1e-2 of plain ptnee and of the radiance cache at 1920x1080, 1 spp per frame, as "nrc"; the
+reference's own box room at 320x240 x 16 spp (GPU wavefront, GPU path kernel, CPU oracle) as
+"reference_scene".
 
 Printed JSON line: value = Msamples/s over the timed steps, the traversal roofline (HIP events
 around every closest-hit launch; algorithmic bytes from the kernel's own node / triangle
@@ -270,6 +272,43 @@ def large_scene(nr, scenes, spp=256):
             "adjoint_msamples_per_s": aw * ah * aspp / adj_s / 1e6, "nrc_infer_4k_ms": infer_ms}
 
 
+def reference_scene(nr, scenes, with_cpu, reps=5):
+    """The reference's own CPU-sized case: the 36-triangle box room at 320x240 x 16 spp with the
+    reference's defaults (Russian roulette at every vertex, prr 0.8, no depth limit), timed as a
+    wavefront render and as a path-kernel render (one lane per path, the same image), and, on the
+    host, the oracle's restatement of the reference estimator on the same frame (all threads)."""
+    sc = scenes.box_room()
+    W, H, spp = sc["width"], sc["height"], 16
+    dev = nr.Scene(sc)
+    kw = dict(maxdepth=0, rrdepth=0, fetch=False)
+    out = {"workload": f"box_room (36 triangles) {W}x{H}x{spp}spp, reference defaults (RR at every vertex, "
+                       f"prr 0.8, no depth limit)", "samples": W * H * spp}
+    for name, pk in (("wavefront", False), ("path_kernel", True)):
+        dev.render(sc["camera"], W, H, spp, seed=1, path_kernel=pk, **kw)
+        nr.synchronize()
+        t0 = time.perf_counter()
+        for k in range(reps):
+            dev.render(sc["camera"], W, H, spp, seed=2 + k, path_kernel=pk, **kw)
+        nr.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        out[f"{name}_ms"] = ms
+        out[f"{name}_msamples_per_s"] = W * H * spp / ms / 1e3
+    dev.close()
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        from nuray_amd.scenes import camera_array
+        threads = len(os.sched_getaffinity(0))
+        o = po.Scene(sc)
+        t0 = time.perf_counter()
+        o.render(camera_array(sc["camera"]), W, H, spp, seed=2, nthreads=threads)
+        dt = time.perf_counter() - t0
+        out["cpu_ms"] = dt * 1e3
+        out["cpu_msamples_per_s"] = W * H * spp / dt / 1e6
+        out["cpu_threads"] = threads
+    return out
+
+
 def env_sampling(nr, dev, sc, spp):
     """The same 720p frame with and without sky sampling (nr_params.env_is): time per frame, mean
     per-pixel variance of one sample, and the time-to-equal-error gain var_off t_off / (var_on t_on)."""
@@ -405,6 +444,12 @@ def main():
         print(f"# sky sampling: {env['on_ms']:.1f} ms vs {env['off_ms']:.1f} ms per frame, variance "
               f"{env['variance_ratio']:.2f}x lower, {env['equal_error_speedup']:.2f}x faster to equal error",
               file=sys.stderr, flush=True)
+    ref_scene = None if args.frames_only else reference_scene(nr, scenes, not args.no_cpu)
+    if ref_scene:
+        print(f"# reference box room 320x240x16spp: wavefront {ref_scene['wavefront_ms']:.2f} ms, path kernel "
+              f"{ref_scene['path_kernel_ms']:.2f} ms"
+              + (f", CPU oracle {ref_scene['cpu_ms']:.0f} ms" if "cpu_ms" in ref_scene else ""),
+              file=sys.stderr, flush=True)
     nrc = None if args.no_nrc else nrc_time_to_rmse(nr, scenes)
     mlp = None if args.frames_only else nrc_kernels(nr)
     if mlp:
@@ -424,7 +469,7 @@ def main():
                                f"({len(sc['p'])} triangles) + 1024x512 sky", "global_batch": W * H * spp,
                    "seq_len": None, "parallelism": "single-gpu", "bvh_build_ms": info["build_ms"]},
         "roofline": roofline, "cpu_baseline": cpu, "hbm_copy_gbs": copy_gbs,
-        "adjoint": adj, "nrc": nrc, "nrc_mlp": mlp, "env_sampling": env,
+        "adjoint": adj, "nrc": nrc, "nrc_mlp": mlp, "env_sampling": env, "reference_scene": ref_scene,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]

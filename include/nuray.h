@@ -68,8 +68,11 @@ typedef struct nr_params {
                               reference does not have; ptnee and the radiance cache, not the adjoint) */
     int32_t path_kernel;   /* 1: trace each sample's whole path in one lane (one launch per batch) instead of
                               the wavefront (one launch per bounce); the per-sample radiance and the image
-                              are bit-identical.  Pays off for small frames of long paths, e.g. inverse
-                              rendering of a closed room; not with env_is or the radiance cache */
+                              are bit-identical.  Pays off for small scenes and for small frames of long
+                              paths, e.g. inverse rendering of a closed room; not with env_is or the
+                              radiance cache.  2: automatic -- one lane per path for scenes of at most 64
+                              BVH leaves without env_is, the wavefront otherwise (the radiance cache
+                              always uses the wavefront).  0 (default): the wavefront */
 } nr_params;
 
 typedef void (*nr_progress_fn)(float fraction, void* user);

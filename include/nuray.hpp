@@ -517,7 +517,7 @@ public:
         p.maxdepth = cfg.getValueInt("maxdepth", 0);
         p.rrdepth = cfg.getValueInt("rrdepth", 0);
         p.env_is = cfg.getValueInt("envmap_is", 0);
-        p.path_kernel = cfg.getValueInt("path_kernel", 0);
+        p.path_kernel = cfg.getValueInt("path_kernel", 2);  // automatic unless the Config says otherwise
         return p;
     }
 

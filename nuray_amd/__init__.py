@@ -222,8 +222,10 @@ class Scene:
     def render(self, cam, W, H, spp, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0, spp_offset=0, batch_spp=0,
                accumulate=False, fetch=True, progress=None, cancel=None, integrator=0, env_is=False,
                path_kernel=False):
+        """path_kernel: False (wavefront), True (one lane per path) or "auto" (nr_params.path_kernel = 2)."""
+        pk = 2 if path_kernel == "auto" else int(bool(path_kernel))
         p = nr_params(W, H, spp, spp_offset, seed, prr, mis, maxdepth, rrdepth, batch_spp, int(accumulate), integrator,
-                      int(env_is), int(path_kernel))
+                      int(env_is), pk)
         c = make_camera(cam)
         cb = PROGRESS(lambda f, u: progress(f)) if progress else None
         rc = check(lib().nr_render_ptnee(self.h, C.byref(c), C.byref(p), None, None, None, None,

@@ -286,7 +286,7 @@ static nr::RenderParams to_params(const nr_params* pp) {
     p.env_is = pp->env_is;
     if (p.env_is != 0 && p.env_is != 1) throw std::runtime_error("env_is must be 0 or 1");
     p.path_kernel = pp->path_kernel;
-    if (p.path_kernel != 0 && p.path_kernel != 1) throw std::runtime_error("path_kernel must be 0 or 1");
+    if (p.path_kernel < 0 || p.path_kernel > 2) throw std::runtime_error("path_kernel must be 0, 1 or 2");
     return p;
 }
 

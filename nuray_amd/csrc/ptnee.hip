@@ -646,7 +646,10 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
     // BSDF-escape weights; without an environment sampler (no map, or a black one) it is a no-op
     const bool env = p.env_is && sc.view.env_w > 0;
     if (p.env_is && p.integrator != 0) throw std::runtime_error("environment sampling needs the ptnee integrator");
-    if (p.path_kernel && p.env_is) throw std::runtime_error("the path kernel does not sample the sky (env_is)");
+    if (p.path_kernel == 1 && p.env_is) throw std::runtime_error("the path kernel does not sample the sky (env_is)");
+    // automatic choice: small scenes (a flat leaf list) render faster one lane per path, and the
+    // image is the same either way
+    const bool paths = p.path_kernel == 1 || (p.path_kernel == 2 && !p.env_is && sc.view.n_flat > 0);
     if (env) {
         const size_t c = size_t(kb) * npix;
         t.radiance.ensure(sizeof(float4) * 2 * c);
@@ -684,7 +687,7 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
         const int k = std::min(kb, p.spp - b);
         const int P = k * npix;
         float4* const rad_env = env ? t.radiance.as<float4>() + P : nullptr;
-        if (p.path_kernel) {  // one lane per path: the same per-sample radiance without per-bounce launches
+        if (paths) {  // one lane per path: the same per-sample radiance without per-bounce launches
             if (kt) mark(ev_c);
             render_paths(sc, cam, p, uint32_t(p.spp_offset + b), P, t.radiance.as<float4>(), cnt + C_ERR);
             if (kt) mark(ev_c);
@@ -775,7 +778,7 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
     if (!sc.built) throw std::runtime_error("scene not built (call nr_bvh_build_sah)");
     if (p.W <= 0 || p.H <= 0 || p.spp <= 0) throw std::runtime_error("bad image size or spp");
     if (np.train_paths < 0 || np.train_maxlen < 1) throw std::runtime_error("bad radiance-cache parameters");
-    if (p.path_kernel) throw std::runtime_error("the radiance cache renders with the wavefront (path_kernel = 0)");
+    if (p.path_kernel == 1) throw std::runtime_error("the radiance cache renders with the wavefront (path_kernel = 0)");
     if (p.env_is && p.integrator != 0) throw std::runtime_error("environment sampling needs the ptnee integrator");
     const bool env = p.env_is && sc.view.env_w > 0;
     const int npix = p.W * p.H;

@@ -25,7 +25,8 @@ struct RenderParams {
     int accumulate = 0; // keep previous sums instead of clearing them
     int integrator = 0; // 0: RendererPTLS (ptnee), 1: RendererPT (pt, no light sampling)
     int env_is = 0;     // 1: sample the sky at Lambert / GGX-reflect vertices (same expected image)
-    int path_kernel = 0; // 1: one lane per path instead of the wavefront (same radiance, bit for bit)
+    int path_kernel = 0; // 1: one lane per path instead of the wavefront (same radiance, bit for bit);
+                         // 2: one lane per path for small scenes (flat leaf list) without env_is
 };
 
 struct RenderStats {

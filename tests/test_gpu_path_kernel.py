@@ -58,3 +58,25 @@ def test_path_kernel_refusals(nr, scenes):
     with pytest.raises(nr.NuRayError, match="path kernel"):
         dev.render(sc["camera"], 32, 18, 2, env_is=True, path_kernel=True)
     dev.close()
+
+
+def test_path_kernel_auto(nr, scenes):
+    """path_kernel = 2 picks one lane per path for a scene with a flat leaf list and no sky sampling
+    (no wavefront shade launches), the wavefront for larger scenes; the image is the same."""
+    small = scenes.box_room()
+    dev = nr.Scene(small)
+    dev.set_profiling(1)
+    a = dev.render(small["camera"], 64, 48, 4, seed=3, path_kernel="auto")
+    assert dev.stats()["shade_launches"] == 0
+    b = dev.render(small["camera"], 64, 48, 4, seed=3)
+    assert dev.stats()["shade_launches"] > 0
+    assert np.array_equal(a["sum"], b["sum"]) and np.array_equal(a["sum2"], b["sum2"])
+    dev.close()
+    big = scenes.terrain_spheres(n_cells=48, n_spheres=60, width=64, height=36)
+    dev = nr.Scene(big)
+    dev.set_profiling(1)
+    dev.render(big["camera"], 64, 36, 2, seed=3, path_kernel="auto")
+    assert dev.stats()["shade_launches"] > 0
+    dev.render(big["camera"], 64, 36, 2, seed=3, path_kernel="auto", env_is=True)
+    assert dev.stats()["shade_launches"] > 0
+    dev.close()
