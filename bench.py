@@ -360,6 +360,8 @@ def main():
                     help="only time the radiance-cache MLP kernels (inference + training step) and exit")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the plain 1 GiB device copy (HBM counter calibration) and exit")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only time the inverse-rendering iteration (and the reference box room) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "nuray_amd runs on one GPU: a frame shards by pixels x samples, but "
@@ -379,6 +381,10 @@ def main():
         return
     if args.large:
         print(json.dumps(large_scene(nr, scenes)), flush=True)
+        return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_iteration(nr, scenes),
+                          "reference_scene": reference_scene(nr, scenes, False)}), flush=True)
         return
     sc = scenes.terrain_spheres()
     W, H, spp = sc["width"], sc["height"], args.spp
