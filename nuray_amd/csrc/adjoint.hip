@@ -560,25 +560,28 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
         hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, sc.stream, t.adj_gtex.as<double>(), int(ntex));
         gtex = t.adj_gtex.as<double>();
     }
-    int* cnt = t.counters.ptr ? t.counters.as<int>() : nullptr;
-    if (!cnt) {
-        t.counters.ensure(sizeof(int) * 16);
-        cnt = t.counters.as<int>();
-    }
+    t.adj_err.ensure(sizeof(int));
+    int* const d_err = t.adj_err.as<int>();
     const int zero = 0;
-    check_hip(hipMemcpyAsync(cnt + 15, &zero, sizeof(int), hipMemcpyHostToDevice, sc.stream), "H2D");
+    check_hip(hipMemcpyAsync(d_err, &zero, sizeof(int), hipMemcpyHostToDevice, sc.stream), "H2D");
     const long long P = (long long)p.W * p.H * p.spp;
     if (P > 0x7FFFFFFFll) throw std::runtime_error("adjoint: more than 2^31 samples");
     AdjArgs a{sc.view, cam, p.W, p.H, p.W * p.H, int(P), p.seed, uint32_t(p.spp_offset), p.prr, p.mis, p.maxdepth,
               p.rrdepth, stored ? t.radiance.as<float4>() : nullptr, 0, int(P), t.adj_dldi.as<float>(),
-              1.0f / float(p.spp), n_params, t.adj_grad.as<double>(), cnt + 15, tex_id, gtex, p.integrator == 1 ? 0 : 1,
+              1.0f / float(p.spp), n_params, t.adj_grad.as<double>(), d_err, tex_id, gtex, p.integrator == 1 ? 0 : 1,
               0};
-    // without the render's own per-sample radiance, each chunk is first rendered forward (one lane
-    // per path) into a chunk-sized buffer: at most 16 M x 16 B, whatever the sample count
+    // Chunks of a whole number of samples per pixel, about 16 M samples (at least one per pixel).
+    // Without the render's own per-sample radiance each chunk is first rendered forward, by the
+    // faster of the two forms for the scene: one lane per path into a chunk buffer when the scene
+    // has a flat leaf list, else the wavefront into the render target's radiance buffer (the
+    // accumulated image is left alone).  Nothing per path outlives its chunk.
+    const int npix = p.W * p.H;
+    const long long chunk_spp = std::max<long long>(1, PATH_CHUNK / npix);
+    const bool fwd_paths = sc.view.n_flat > 0;
     PathsShape fwd{};
-    if (!stored) {
+    if (!stored && fwd_paths) {
         fwd = paths_shape(sc);
-        t.adj_rad.ensure(sizeof(float4) * size_t(std::min<long long>(P, PATH_CHUNK)));
+        t.adj_rad.ensure(sizeof(float4) * size_t(std::min<long long>(P, chunk_spp * npix)));
     }
     const size_t shmem = size_t(sc.view.stack_cap + 1) * ABLK * sizeof(int) +
                          (n_params <= LDS_PARAMS ? size_t(n_params) * sizeof(float) : 0);
@@ -591,14 +594,17 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adjoint<0u>, ABLK, shmem),
               "occupancy");
     const int grid = std::max(1, per_cu) * device_cu_count();
-    // launches of at most 16 M samples, so that no single dispatch runs for long
-    for (long long s0 = 0; s0 < P; s0 += PATH_CHUNK) {
-        a.slot0 = int(s0);
-        a.slot1 = int(std::min(P, s0 + PATH_CHUNK));
-        if (!stored) {
+    for (long long b = 0; b < p.spp; b += chunk_spp) {
+        a.slot0 = int(b * npix);
+        a.slot1 = int(std::min(P, (b + chunk_spp) * npix));
+        if (!stored && fwd_paths) {
             a.radiance = t.adj_rad.as<float4>();
             a.rad_off = a.slot0;
             launch_paths(sc, fwd, a, t.adj_rad.as<float4>());
+        } else if (!stored) {
+            wavefront_radiance(sc, cam, p, uint32_t(p.spp_offset + b), a.slot1 - a.slot0);
+            a.radiance = t.radiance.as<float4>();
+            a.rad_off = a.slot0;
         }
         if (feat == F_ALL) hipLaunchKernelGGL(k_adjoint<F_ALL>, dim3(grid), dim3(ABLK), shmem, sc.stream, a);
         else if (feat == F_TEX) hipLaunchKernelGGL(k_adjoint<F_TEX>, dim3(grid), dim3(ABLK), shmem, sc.stream, a);
@@ -606,7 +612,7 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
         check_hip(hipGetLastError(), "k_adjoint launch");
     }
     int err = 0;
-    check_hip(hipMemcpyAsync(&err, cnt + 15, sizeof(int), hipMemcpyDeviceToHost, sc.stream), "D2H");
+    check_hip(hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, sc.stream), "D2H");
     check_hip(hipMemcpyAsync(grad_host, t.adj_grad.ptr, sizeof(double) * n_params, hipMemcpyDeviceToHost, sc.stream),
               "D2H");
     if (gtex)

@@ -237,7 +237,7 @@ __device__ __forceinline__ bool slab_filter(float mnx, float mny, float mnz, flo
     const float t_out = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), __builtin_amdgcn_fmed3f(az, bz, 1e18f));
     const float diff = t_out - t_in;
     const float band = fmaf(9.5367431640625e-07f, fabsf(t_in) + fabsf(t_out), s.a2);  // 2^-20
-    clear = (fabsf(t_out) > s.a2) & (fabsf(diff) > band);
+    clear = (int(fabsf(t_out) > s.a2) & int(fabsf(diff) > band)) != 0;  // both tests, no branch
     return t_out >= 0.0f && diff > 0.0f;
 }
 __device__ __forceinline__ bool slab_fast(float mnx, float mny, float mnz, float mxx, float mxy, float mxz,
@@ -249,7 +249,7 @@ __device__ __forceinline__ bool slab_fast(float mnx, float mny, float mnz, float
     const float t_out = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), __builtin_amdgcn_fmed3f(az, bz, 1e18f));
     const float diff = t_out - t_in;
     const float band = fmaf(9.5367431640625e-07f, fabsf(t_in) + fabsf(t_out), s.a2);  // 2^-20
-    const bool clear = (fabsf(t_out) > s.a2) & (fabsf(diff) > band);
+    const bool clear = (int(fabsf(t_out) > s.a2) & int(fabsf(diff) > band)) != 0;
     if (__builtin_expect(clear, 1)) return t_out >= 0.0f && diff > 0.0f;
     return slab(mnx, mny, mnz, mxx, mxy, mxz, s.o, f3(s.d.x + 1e-9f, s.d.y + 1e-9f, s.d.z + 1e-9f));
 }

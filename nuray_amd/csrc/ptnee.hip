@@ -58,6 +58,10 @@ void RenderTarget::ensure(int w, int h, size_t c) {
         W = w; H = h;
         spp_total = 0;
     }
+    ensure_queues(c);
+}
+
+void RenderTarget::ensure_queues(size_t c) {
     if (c > cap) {
         for (int k = 0; k < 2; ++k) {
             rays[k].ensure(sizeof(Ray) * c);
@@ -629,6 +633,95 @@ __global__ void k_tonemap(const double* __restrict__ sum, size_t n3, double inv_
 
 }  // namespace
 
+namespace {
+
+void mark_event(std::vector<hipEvent_t>* v, hipStream_t s) {
+    if (!v) return;
+    hipEvent_t e;
+    check_hip(hipEventCreate(&e), "hipEventCreate");
+    check_hip(hipEventRecord(e, s), "hipEventRecord");
+    v->push_back(e);
+}
+
+// One wavefront batch of P paths -- samples sample0, sample0 + 1, ... of every pixel, slot = sample
+// offset * npix + pixel: raygen, then a closest-hit, a shade and a shadow launch per bounce until no
+// path is left (or maxdepth).  Each path's radiance ends in t.radiance[slot] (the sky rays' share in
+// rad_env with sky sampling).  The ev_* vectors (null: untimed) collect start / stop events around
+// the launches.  Returns 1 if cancelled between bounces.
+int wavefront_batch(Scene& sc, RenderTarget& t, const CamView& cam, const RenderParams& p, uint32_t sample0, int P,
+                    bool env, float4* rad_env, TraceStats* ts, volatile int* cancel, std::vector<hipEvent_t>* ev_c,
+                    std::vector<hipEvent_t>* ev_h, std::vector<hipEvent_t>* ev_s, int& iterations) {
+    const hipStream_t s = sc.stream;
+    const int npix = p.W * p.H;
+    int* cnt = t.counters.as<int>();
+    const int grid_shade = device_cu_count() * 8;
+    hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, int(C_RAY0), P, -1);
+    hipLaunchKernelGGL(k_raygen, dim3((P + BLK - 1) / BLK), dim3(BLK), 0, s, cam, p.W, p.H, npix, P, p.seed,
+                       sample0, t.rays[0].as<Ray>(), t.beta[0].as<float4>(),
+                       t.radiance.as<float4>(), rad_env);
+    for (int it = 0;; ++it) {
+        const int cur = it & 1, nxt = cur ^ 1;
+        hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, -1, 0, int(C_RAY0 + nxt));
+        mark_event(ev_c, s);
+        launch_closest(sc.view, t.rays[cur].as<Ray>(), cnt + C_RAY0 + cur, t.hits.as<HitRec>(), cnt + C_WORK0,
+                       cnt + C_ERR, ts, s, t.retry.as<int>());
+        mark_event(ev_c, s);
+        ShadeArgs a{sc.view, npix, p.seed, sample0, p.prr, p.mis, p.maxdepth, p.rrdepth,
+                    p.integrator == 1 ? 0 : 1,
+                    t.rays[cur].as<Ray>(), t.beta[cur].as<float4>(), t.hits.as<HitRec>(), cnt + C_RAY0 + cur,
+                    t.radiance.as<float4>(), t.rays[nxt].as<Ray>(), t.beta[nxt].as<float4>(), cnt + C_RAY0 + nxt,
+                    t.shadow.as<Ray>(), t.contrib.as<float4>(), cnt + C_SHADOW,
+                    env ? t.skyw.as<float>() : nullptr, env ? P : 0};
+        mark_event(ev_h, s);
+        if (env) {
+            if (sc.features == 0) hipLaunchKernelGGL(k_shade_basic_env, dim3(grid_shade), dim3(BLK), 0, s, a);
+            else hipLaunchKernelGGL(k_shade_full_env, dim3(grid_shade), dim3(BLK), 0, s, a);
+        } else if (sc.features == 0) {
+            hipLaunchKernelGGL(k_shade_basic, dim3(grid_shade), dim3(BLK), 0, s, a);
+        } else {
+            hipLaunchKernelGGL(k_shade_full, dim3(grid_shade), dim3(BLK), 0, s, a);
+        }
+        mark_event(ev_h, s);
+        mark_event(ev_s, s);
+        launch_shadow(sc.view, t.shadow.as<Ray>(), cnt + C_SHADOW, nullptr, nullptr, t.contrib.as<float4>(),
+                      t.radiance.as<float4>(), cnt + C_WORK1, cnt + C_ERR, ts ? ts + 1 : nullptr, s,
+                      t.retry.as<int>());
+        mark_event(ev_s, s);
+        ++iterations;
+        if (p.maxdepth > 0) {
+            if (it + 1 >= p.maxdepth) break;
+        } else {
+            check_hip(hipMemcpyAsync(t.pinned, cnt + C_RAY0 + nxt, sizeof(int), hipMemcpyDeviceToHost, s), "cnt");
+            check_hip(hipStreamSynchronize(s), "sync");
+            if (*t.pinned == 0) break;
+            if (cancel && *cancel) return 1;
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+// Per-sample radiance of samples sample0 ... of every pixel (P paths, a whole number of samples per
+// pixel) into t.radiance, slot = sample offset * npix + pixel, by the wavefront; nothing is
+// accumulated.  The adjoint recomputes the radiance of its samples with it on scenes too large for
+// the flat leaf list, where the wavefront is several times faster than one lane per path.
+void wavefront_radiance(Scene& sc, const CamView& cam, const RenderParams& p, uint32_t sample0, int P) {
+    if (!sc.target) sc.target = new RenderTarget();
+    RenderTarget& t = *sc.target;
+    t.ensure_queues(size_t(P));
+    t.adj_valid = false;  // t.radiance no longer holds the last render's samples
+    int* cnt = t.counters.as<int>();
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(64), 0, sc.stream, cnt);
+    int iterations = 0;
+    wavefront_batch(sc, t, cam, p, sample0, P, false, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                    iterations);
+    check_hip(hipGetLastError(), "wavefront launch");
+    check_hip(hipMemcpyAsync(t.pinned, cnt, sizeof(int) * 8, hipMemcpyDeviceToHost, sc.stream), "counters");
+    check_hip(hipStreamSynchronize(sc.stream), "wavefront sync");
+    if (t.pinned[C_ERR]) throw std::runtime_error("traversal stack overflow");
+}
+
 int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressFn cb, void* user,
                  volatile int* cancel, RenderStats* st, bool trace_stats, TraceStats* tstats, KernelTimes* kt) {
     if (!sc.built) throw std::runtime_error("scene not built (call nr_bvh_build_sah)");
@@ -671,70 +764,23 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
                            size_t(npix) * 3);
         t.spp_total = 0;
     }
-    const int grid_shade = device_cu_count() * 8;
     auto t0 = std::chrono::steady_clock::now();
     RenderStats rs;
     int status = 0;
     std::vector<hipEvent_t> ev_c, ev_s, ev_h;  // start/stop pairs per launch
-    auto mark = [&](std::vector<hipEvent_t>& v) {
-        hipEvent_t e;
-        check_hip(hipEventCreate(&e), "hipEventCreate");
-        check_hip(hipEventRecord(e, s), "hipEventRecord");
-        v.push_back(e);
-    };
     for (int b = 0; b < p.spp; b += kb) {
         if (cancel && *cancel) { status = 1; break; }
         const int k = std::min(kb, p.spp - b);
         const int P = k * npix;
         float4* const rad_env = env ? t.radiance.as<float4>() + P : nullptr;
         if (paths) {  // one lane per path: the same per-sample radiance without per-bounce launches
-            if (kt) mark(ev_c);
+            mark_event(kt ? &ev_c : nullptr, s);
             render_paths(sc, cam, p, uint32_t(p.spp_offset + b), P, t.radiance.as<float4>(), cnt + C_ERR);
-            if (kt) mark(ev_c);
+            mark_event(kt ? &ev_c : nullptr, s);
             rs.iterations++;
         } else {
-            hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, int(C_RAY0), P, -1);
-            hipLaunchKernelGGL(k_raygen, dim3((P + BLK - 1) / BLK), dim3(BLK), 0, s, cam, p.W, p.H, npix, P, p.seed,
-                               uint32_t(p.spp_offset + b), t.rays[0].as<Ray>(), t.beta[0].as<float4>(),
-                               t.radiance.as<float4>(), rad_env);
-            for (int it = 0;; ++it) {
-                const int cur = it & 1, nxt = cur ^ 1;
-                hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, -1, 0, int(C_RAY0 + nxt));
-                if (kt) mark(ev_c);
-                launch_closest(sc.view, t.rays[cur].as<Ray>(), cnt + C_RAY0 + cur, t.hits.as<HitRec>(), cnt + C_WORK0,
-                               cnt + C_ERR, ts, s, t.retry.as<int>());
-                if (kt) mark(ev_c);
-                ShadeArgs a{sc.view, npix, p.seed, uint32_t(p.spp_offset + b), p.prr, p.mis, p.maxdepth, p.rrdepth,
-                            p.integrator == 1 ? 0 : 1,
-                            t.rays[cur].as<Ray>(), t.beta[cur].as<float4>(), t.hits.as<HitRec>(), cnt + C_RAY0 + cur,
-                            t.radiance.as<float4>(), t.rays[nxt].as<Ray>(), t.beta[nxt].as<float4>(), cnt + C_RAY0 + nxt,
-                            t.shadow.as<Ray>(), t.contrib.as<float4>(), cnt + C_SHADOW,
-                            env ? t.skyw.as<float>() : nullptr, env ? P : 0};
-                if (kt) mark(ev_h);
-                if (env) {
-                    if (sc.features == 0) hipLaunchKernelGGL(k_shade_basic_env, dim3(grid_shade), dim3(BLK), 0, s, a);
-                    else hipLaunchKernelGGL(k_shade_full_env, dim3(grid_shade), dim3(BLK), 0, s, a);
-                } else if (sc.features == 0) {
-                    hipLaunchKernelGGL(k_shade_basic, dim3(grid_shade), dim3(BLK), 0, s, a);
-                } else {
-                    hipLaunchKernelGGL(k_shade_full, dim3(grid_shade), dim3(BLK), 0, s, a);
-                }
-                if (kt) mark(ev_h);
-                if (kt) mark(ev_s);
-                launch_shadow(sc.view, t.shadow.as<Ray>(), cnt + C_SHADOW, nullptr, nullptr, t.contrib.as<float4>(),
-                              t.radiance.as<float4>(), cnt + C_WORK1, cnt + C_ERR, ts ? ts + 1 : nullptr, s,
-                              t.retry.as<int>());
-                if (kt) mark(ev_s);
-                rs.iterations++;
-                if (p.maxdepth > 0) {
-                    if (it + 1 >= p.maxdepth) break;
-                } else {
-                    check_hip(hipMemcpyAsync(t.pinned, cnt + C_RAY0 + nxt, sizeof(int), hipMemcpyDeviceToHost, s), "cnt");
-                    check_hip(hipStreamSynchronize(s), "sync");
-                    if (*t.pinned == 0) break;
-                    if (cancel && *cancel) { status = 1; break; }
-                }
-            }
+            status = wavefront_batch(sc, t, cam, p, uint32_t(p.spp_offset + b), P, env, rad_env, ts, cancel,
+                                     kt ? &ev_c : nullptr, kt ? &ev_h : nullptr, kt ? &ev_s : nullptr, rs.iterations);
         }
         if (status) break;
         hipLaunchKernelGGL(k_finalize, dim3((npix + BLK - 1) / BLK), dim3(BLK), 0, s, t.radiance.as<float4>(), rad_env,

@@ -60,10 +60,12 @@ struct RenderTarget {
     CamView adj_cam{};
     DevBuf adj_target, adj_dldi, adj_loss, adj_grad, adj_gtex;
     DevBuf adj_rad;                    // float4 per sample of one adjoint chunk (recomputed radiance)
+    DevBuf adj_err;                    // int: the adjoint's traversal-overflow flag
     // radiance-cache rendering: spread state per queued ray, cache queries, training vertices
     DevBuf nstate[2], nrc_queries, nrc_recs, nrc_lb, nrc_tgt, nrc_batch, nrc_batch_tgt;
     ~RenderTarget();
-    void ensure(int W, int H, size_t cap);
+    void ensure(int W, int H, size_t cap);  // image accumulators for W x H and queues for cap paths
+    void ensure_queues(size_t cap);         // queues only (the accumulators keep their contents)
     // the stored per-sample radiance belongs to exactly the samples of (cam, p)
     bool adj_matches(const CamView& cam, const RenderParams& p) const {
         const RenderParams& q = adj_p;
@@ -103,6 +105,10 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
 // Accumulates into the same sums as render_ptnee (p.accumulate).
 int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, const NrcRenderParams& np,
                NrcFrameStats* fs, RenderStats* st);
+// Per-sample radiance of P paths (a whole number of samples per pixel, from sample0 on) into the
+// render target's radiance buffer, slot = sample offset * W * H + pixel, by the wavefront, without
+// touching the accumulated image.  Invalidates the stored radiance of the last render.
+void wavefront_radiance(Scene& sc, const CamView& cam, const RenderParams& p, uint32_t sample0, int P);
 // Copies the accumulated result: linear mean (float W*H*3), FP64 sums, gamma-2.2 RGB8.
 void render_fetch(Scene& sc, float* rgb, double* sum, double* sum2, uint8_t* rgb8);
 
