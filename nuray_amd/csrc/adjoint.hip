@@ -148,7 +148,23 @@ struct AdjArgs {
     double* gtex;            // w * h * 3 texel gradient (linear texel values)
     int nee;                 // 1: ptnee, 0: pt (no light sampling)
     int rad_off;             // radiance[] holds samples from this index on
+    int* claim;              // samples claimed so far in this launch (zero at launch)
 };
+
+// Hands the next samples of [a.slot0, a.slot1) to the lanes of the wave that ask for one (need):
+// one atomic per wave, samples in lane order.  Called by every lane of the wave at the same point.
+// Returns the lane's new sample, or a.slot1 (no more samples) when it asked; else keeps `slot`.
+__device__ __forceinline__ int claim_samples(const AdjArgs& a, bool need, int slot) {
+    const int lane = int(threadIdx.x & 63);
+    const unsigned long long ask = __ballot(need);
+    if (ask == 0ull) return slot;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(a.claim, __popcll(ask));
+    base = __shfl(base, 0, 64);
+    if (!need) return slot;
+    const long long s = (long long)a.slot0 + base + __popcll(ask & ((1ull << lane) - 1ull));
+    return s < a.slot1 ? int(s) : a.slot1;
+}
 
 __device__ __forceinline__ void gadd(float* lds, double* glob, bool use_lds, int idx, float v) {
     if (v == 0.0f || !(v == v)) return;
@@ -366,30 +382,31 @@ __global__ __launch_bounds__(ABLK) void k_adjoint(AdjArgs a) {
         for (int k = threadIdx.x; k < a.n_params; k += blockDim.x) gacc[k] = 0.0f;
     __syncthreads();
     bool overflow = false;
-    const int stride = int(gridDim.x * blockDim.x);
-    int slot = a.slot0 + int(blockIdx.x * blockDim.x + threadIdx.x);
     PathState p;
-    // the lane's next sample with a non-zero weight: max(0, L) per channel at the camera
-    // (rendererptls.cpp:173) has zero slope where L <= 0 or NaN
-    auto next = [&]() -> bool {
-        for (; slot < a.slot1; slot += stride) {
-            const int pixel = slot % a.npix;
-            const float4 r = a.radiance[slot - a.rad_off];
-            const F3 g = f3(r.x > 0.0f ? a.dldi[3 * pixel] * a.inv_spp : 0.0f,
-                            r.y > 0.0f ? a.dldi[3 * pixel + 1] * a.inv_spp : 0.0f,
-                            r.z > 0.0f ? a.dldi[3 * pixel + 2] * a.inv_spp : 0.0f);
-            if (g.x != 0.0f || g.y != 0.0f || g.z != 0.0f) {
-                path_begin(a, slot, f3(r.x, r.y, r.z), g, p);
-                return true;
+    // Each round every lane that has no path asks for the next sample (claim_samples); a sample
+    // whose weight is zero -- max(0, L) per channel at the camera (rendererptls.cpp:173) has zero
+    // slope where L <= 0 or NaN -- ends at once and the lane asks again next round.
+    int slot = a.slot0;
+    bool need = true, active = false;
+    for (;;) {
+        slot = claim_samples(a, need, slot);
+        if (need) {
+            need = false;
+            if (slot < a.slot1) {
+                const int pixel = slot % a.npix;
+                const float4 r = a.radiance[slot - a.rad_off];
+                const F3 g = f3(r.x > 0.0f ? a.dldi[3 * pixel] * a.inv_spp : 0.0f,
+                                r.y > 0.0f ? a.dldi[3 * pixel + 1] * a.inv_spp : 0.0f,
+                                r.z > 0.0f ? a.dldi[3 * pixel + 2] * a.inv_spp : 0.0f);
+                active = g.x != 0.0f || g.y != 0.0f || g.z != 0.0f;
+                need = !active;
+                if (active) path_begin(a, slot, f3(r.x, r.y, r.z), g, p);
             }
         }
-        return false;
-    };
-    bool active = next();
-    while (active) {
-        if (!path_step<F, true>(a, p, stk, gacc, use_lds, overflow)) {
-            slot += stride;
-            active = next();
+        if (!__any(active || need)) break;
+        if (active && !path_step<F, true>(a, p, stk, gacc, use_lds, overflow)) {
+            active = false;
+            need = true;
         }
     }
     if (overflow) atomicOr(a.err, 1);
@@ -409,15 +426,21 @@ __global__ __launch_bounds__(ABLK) __attribute__((amdgpu_waves_per_eu(4))) void 
     extern __shared__ char smem[];
     int* stk = reinterpret_cast<int*>(smem) + threadIdx.x;
     bool overflow = false;
-    const int stride = int(gridDim.x * blockDim.x);
-    int slot = a.slot0 + int(blockIdx.x * blockDim.x + threadIdx.x);
     PathState p;
-    if (slot < a.slot1) path_begin(a, slot, f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), p);
-    while (slot < a.slot1) {
-        if (!path_step<F, false>(a, p, stk, nullptr, false, overflow)) {
+    int slot = a.slot0;
+    bool need = true, active = false;
+    for (;;) {
+        slot = claim_samples(a, need, slot);
+        if (need) {
+            need = false;
+            active = slot < a.slot1;
+            if (active) path_begin(a, slot, f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), p);
+        }
+        if (!__any(active)) break;
+        if (active && !path_step<F, false>(a, p, stk, nullptr, false, overflow)) {
             radiance[slot - a.rad_off] = make_float4(p.L.x, p.L.y, p.L.z, 0.0f);
-            slot += stride;
-            if (slot < a.slot1) path_begin(a, slot, f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), p);
+            active = false;
+            need = true;
         }
     }
     if (overflow) atomicOr(a.err, 1);
@@ -443,6 +466,10 @@ __global__ void k_loss_l2(const double* __restrict__ sum, double inv_spp, const 
         __syncthreads();
     }
     if (threadIdx.x == 0) atomicAdd(loss, red[0] * inv_n);
+}
+
+__global__ void k_zero_int(int* a) {
+    if (threadIdx.x == 0) *a = 0;
 }
 
 __global__ void k_zero(double* a, int n) {
@@ -508,6 +535,7 @@ PathsShape paths_shape(const Scene& sc) {
 }
 // Samples [a.slot0, a.slot1) into out[slot - a.rad_off].
 void launch_paths(const Scene& sc, const PathsShape& L, const AdjArgs& a, float4* out) {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, sc.stream, a.claim);
     if (L.feat == F_ALL) hipLaunchKernelGGL(k_render_paths<F_ALL>, dim3(L.grid), dim3(ABLK), L.shmem, sc.stream, a, out);
     else if (L.feat == F_TEX) hipLaunchKernelGGL(k_render_paths<F_TEX>, dim3(L.grid), dim3(ABLK), L.shmem, sc.stream, a, out);
     else hipLaunchKernelGGL(k_render_paths<0u>, dim3(L.grid), dim3(ABLK), L.shmem, sc.stream, a, out);
@@ -521,8 +549,11 @@ constexpr long long PATH_CHUNK = 16ll << 20;  // launches of at most 16 M sample
 void render_paths(Scene& sc, const CamView& cam, const RenderParams& p, uint32_t sample0, int P, float4* radiance,
                   int* err) {
     if (p.env_is) throw std::runtime_error("the path kernel does not sample the sky (env_is)");
+    if (!sc.target) sc.target = new RenderTarget();
+    sc.target->path_claim.ensure(sizeof(int));
     AdjArgs a{sc.view, cam, p.W, p.H, p.W * p.H, P, p.seed, sample0, p.prr, p.mis, p.maxdepth, p.rrdepth, nullptr,
-              0, P, nullptr, 0.0f, 0, nullptr, err, -1, nullptr, p.integrator == 1 ? 0 : 1, 0};
+              0, P, nullptr, 0.0f, 0, nullptr, err, -1, nullptr, p.integrator == 1 ? 0 : 1, 0,
+              sc.target->path_claim.as<int>()};
     const PathsShape L = paths_shape(sc);
     for (long long s0 = 0; s0 < P; s0 += PATH_CHUNK) {
         a.slot0 = int(s0);
@@ -562,6 +593,7 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
     }
     t.adj_err.ensure(sizeof(int));
     int* const d_err = t.adj_err.as<int>();
+    t.path_claim.ensure(sizeof(int));
     const int zero = 0;
     check_hip(hipMemcpyAsync(d_err, &zero, sizeof(int), hipMemcpyHostToDevice, sc.stream), "H2D");
     const long long P = (long long)p.W * p.H * p.spp;
@@ -569,7 +601,7 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
     AdjArgs a{sc.view, cam, p.W, p.H, p.W * p.H, int(P), p.seed, uint32_t(p.spp_offset), p.prr, p.mis, p.maxdepth,
               p.rrdepth, stored ? t.radiance.as<float4>() : nullptr, 0, int(P), t.adj_dldi.as<float>(),
               1.0f / float(p.spp), n_params, t.adj_grad.as<double>(), d_err, tex_id, gtex, p.integrator == 1 ? 0 : 1,
-              0};
+              0, t.path_claim.as<int>()};
     // Chunks of a whole number of samples per pixel, about 16 M samples (at least one per pixel).
     // Without the render's own per-sample radiance each chunk is first rendered forward, by the
     // faster of the two forms for the scene: one lane per path into a chunk buffer when the scene
@@ -606,6 +638,7 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
             a.radiance = t.radiance.as<float4>();
             a.rad_off = a.slot0;
         }
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, sc.stream, a.claim);
         if (feat == F_ALL) hipLaunchKernelGGL(k_adjoint<F_ALL>, dim3(grid), dim3(ABLK), shmem, sc.stream, a);
         else if (feat == F_TEX) hipLaunchKernelGGL(k_adjoint<F_TEX>, dim3(grid), dim3(ABLK), shmem, sc.stream, a);
         else hipLaunchKernelGGL(k_adjoint<0u>, dim3(grid), dim3(ABLK), shmem, sc.stream, a);
