@@ -18,9 +18,11 @@
 // exact traversal, and peels each vertex's own contribution off the sample's radiance L, so the
 // radiance still to come through the next bounce is known at every vertex without storing any
 // path.  L is the forward render's per-sample radiance when that render ran as one batch with
-// the same samples; otherwise a forward replay recomputes it first (recompute-only mode, any
-// number of samples).  Material gradients accumulate per block in FP32 in LDS, then into FP64;
-// texel gradients (albedo textures) go straight to FP64 atomics at the four bilinear corners.
+// the same samples; otherwise each chunk of samples is rendered forward first (recompute-only
+// mode, any number of samples: one lane per path on small scenes, the wavefront on the others).
+// Lanes take their samples from a shared counter, one atomic per wave and round.  Material
+// gradients accumulate per block in FP32 in LDS, then into FP64; texel gradients (albedo
+// textures) go straight to FP64 atomics at the four bilinear corners.
 #include <cstring>
 #include "adjoint.h"
 #include "nr_material.h"
@@ -371,7 +373,7 @@ __device__ __forceinline__ bool path_step(const AdjArgs& a, PathState& p, int* s
 }
 
 // Persistent grid over samples [slot0, slot1) with their radiance in a.radiance[slot - a.rad_off]
-// (the last single-batch render, or a forward pass of k_render_paths just before).
+// (the last single-batch render, or a forward pass over the chunk just before).
 template <unsigned F>
 __global__ __launch_bounds__(ABLK) void k_adjoint(AdjArgs a) {
     extern __shared__ char smem[];
