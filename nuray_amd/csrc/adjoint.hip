@@ -150,7 +150,9 @@ struct AdjArgs {
     double* gtex;            // w * h * 3 texel gradient (linear texel values)
     int nee;                 // 1: ptnee, 0: pt (no light sampling)
     int rad_off;             // radiance[] holds samples from this index on
-    int* claim;              // samples claimed so far in this launch (zero at launch)
+    unsigned* claim;         // samples claimed so far in this launch (zero at launch); a launch covers
+                             // fewer than 2^31 samples and each lane asks at most once after the last
+                             // one is handed out, so the count stays below 2^32
 };
 
 // Hands the next samples of [a.slot0, a.slot1) to the lanes of the wave that ask for one (need):
@@ -160,11 +162,11 @@ __device__ __forceinline__ int claim_samples(const AdjArgs& a, bool need, int sl
     const int lane = int(threadIdx.x & 63);
     const unsigned long long ask = __ballot(need);
     if (ask == 0ull) return slot;
-    int base = 0;
-    if (lane == 0) base = atomicAdd(a.claim, __popcll(ask));
+    unsigned base = 0;
+    if (lane == 0) base = atomicAdd(a.claim, unsigned(__popcll(ask)));
     base = __shfl(base, 0, 64);
     if (!need) return slot;
-    const long long s = (long long)a.slot0 + base + __popcll(ask & ((1ull << lane) - 1ull));
+    const long long s = (long long)a.slot0 + (long long)base + __popcll(ask & ((1ull << lane) - 1ull));
     return s < a.slot1 ? int(s) : a.slot1;
 }
 
@@ -470,7 +472,7 @@ __global__ void k_loss_l2(const double* __restrict__ sum, double inv_spp, const 
     if (threadIdx.x == 0) atomicAdd(loss, red[0] * inv_n);
 }
 
-__global__ void k_zero_int(int* a) {
+__global__ void k_zero_int(unsigned* a) {
     if (threadIdx.x == 0) *a = 0;
 }
 
@@ -555,7 +557,7 @@ void render_paths(Scene& sc, const CamView& cam, const RenderParams& p, uint32_t
     sc.target->path_claim.ensure(sizeof(int));
     AdjArgs a{sc.view, cam, p.W, p.H, p.W * p.H, P, p.seed, sample0, p.prr, p.mis, p.maxdepth, p.rrdepth, nullptr,
               0, P, nullptr, 0.0f, 0, nullptr, err, -1, nullptr, p.integrator == 1 ? 0 : 1, 0,
-              sc.target->path_claim.as<int>()};
+              sc.target->path_claim.as<unsigned>()};
     const PathsShape L = paths_shape(sc);
     for (long long s0 = 0; s0 < P; s0 += PATH_CHUNK) {
         a.slot0 = int(s0);
@@ -603,7 +605,7 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
     AdjArgs a{sc.view, cam, p.W, p.H, p.W * p.H, int(P), p.seed, uint32_t(p.spp_offset), p.prr, p.mis, p.maxdepth,
               p.rrdepth, stored ? t.radiance.as<float4>() : nullptr, 0, int(P), t.adj_dldi.as<float>(),
               1.0f / float(p.spp), n_params, t.adj_grad.as<double>(), d_err, tex_id, gtex, p.integrator == 1 ? 0 : 1,
-              0, t.path_claim.as<int>()};
+              0, t.path_claim.as<unsigned>()};
     // Chunks of a whole number of samples per pixel, about 16 M samples (at least one per pixel).
     // Without the render's own per-sample radiance each chunk is first rendered forward, by the
     // faster of the two forms for the scene: one lane per path into a chunk buffer when the scene

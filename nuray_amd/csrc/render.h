@@ -61,7 +61,7 @@ struct RenderTarget {
     DevBuf adj_target, adj_dldi, adj_loss, adj_grad, adj_gtex;
     DevBuf adj_rad;                    // float4 per sample of one adjoint chunk (recomputed radiance)
     DevBuf adj_err;                    // int: the adjoint's traversal-overflow flag
-    DevBuf path_claim;                 // int: samples claimed by a per-lane path kernel launch
+    DevBuf path_claim;                 // unsigned: samples claimed by a per-lane path kernel launch
     // radiance-cache rendering: spread state per queued ray, cache queries, training vertices
     DevBuf nstate[2], nrc_queries, nrc_recs, nrc_lb, nrc_tgt, nrc_batch, nrc_batch_tgt;
     ~RenderTarget();
