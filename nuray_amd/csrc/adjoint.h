@@ -16,7 +16,8 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
                    int tex_id = -1, double* gtex = nullptr);
 // Forward-only render of the P samples from sample0 on, one lane per path (RenderParams::path_kernel):
 // radiance[slot] for slot = sample * W * H + pixel, exactly the wavefront's per-sample radiance.
+// With vrec, also the first vrec_cap vertex records of every sample (vrec[slot * vrec_cap + depth]).
 void render_paths(Scene& sc, const CamView& cam, const RenderParams& p, uint32_t sample0, int P, float4* radiance,
-                  int* err);
+                  int* err, HitRec* vrec = nullptr, int vrec_cap = 0);
 
 }  // namespace nr

@@ -153,7 +153,17 @@ struct AdjArgs {
     unsigned* claim;         // samples claimed so far in this launch (zero at launch); a launch covers
                              // fewer than 2^31 samples and each lane asks at most once after the last
                              // one is handed out, so the count stays below 2^32
+    HitRec* vrec;            // per-vertex records of the first vrec_cap vertices of each sample, at
+    int vrec_cap;            // vrec[(slot - rad_off) * vrec_cap + depth], or null
 };
+
+// A vertex record: the closest hit of the vertex's ray as traced (prim -1: a miss); for a hit, bit 29
+// of prim is set when the vertex traced its NEE shadow ray and bit 30 when that ray found the light
+// visible (primitive indices stay below 2^27).  The forward path kernel writes them; the replay of
+// the same samples reads them instead of tracing both rays again, and traces the shadow ray itself
+// where the record has none.
+constexpr int32_t VREC_TRACED = 1 << 29;
+constexpr int32_t VREC_VISIBLE = 1 << 30;
 
 // Hands the next samples of [a.slot0, a.slot1) to the lanes of the wave that ask for one (need):
 // one atomic per wave, samples in lane order.  Called by every lane of the wave at the same point.
@@ -211,12 +221,14 @@ struct PathState {
     float fac, dfac;  // MIS factor for the next emitter hit and d/d alpha of it
     int prev_mat, depth, pixel;
     uint32_t sample;
+    int rslot;    // slot - rad_off: the sample's index in the radiance and vertex-record buffers
 };
 
 // Camera ray of sample `slot` (identical to k_raygen) and the path's starting state.
 __device__ __forceinline__ void path_begin(const AdjArgs& a, int slot, F3 Lt, F3 g, PathState& p) {
     p.pixel = slot % a.npix;
     p.sample = a.sample0 + uint32_t(slot / a.npix);
+    p.rslot = slot - a.rad_off;
     const int x = p.pixel % a.W, y = p.pixel / a.W;
     const float4 u0 = rng4(a.seed, uint32_t(p.pixel), p.sample, 0u, 0u);
     const float cx = (float(x) + u0.x) / float(a.W) * 2.0f - 1.0f;
@@ -249,7 +261,19 @@ __device__ __forceinline__ bool path_step(const AdjArgs& a, PathState& p, int* s
             scatter_texels(a, sv, m.tex, tu, tv, v);
         }
     };
-    const HitRec h = trace_ray<false, ABLK>(sv, p.o, p.d, 0.0f, stk, overflow);
+    HitRec* const rec = a.vrec && p.depth < a.vrec_cap ? a.vrec + (size_t(p.rslot) * a.vrec_cap + p.depth) : nullptr;
+    HitRec h;
+    int32_t rec_flags = 0;
+    if (GRAD && rec) {
+        h = *rec;
+        if (h.prim >= 0) {
+            rec_flags = h.prim & (VREC_TRACED | VREC_VISIBLE);
+            h.prim &= ~(VREC_TRACED | VREC_VISIBLE);
+        }
+    } else {
+        h = trace_ray<false, ABLK>(sv, p.o, p.d, 0.0f, stk, overflow);
+        if (!GRAD && rec) *rec = h;
+    }
     if (h.prim < 0) {  // environment: no parameter dependence
         if (!GRAD) {
             const F3 e = mulv(p.beta, env_radiance(sv, p.d));
@@ -307,9 +331,15 @@ __device__ __forceinline__ bool path_step(const AdjArgs& a, PathState& p, int* s
         const float cn = (0.0f < ncos ? ncos : 0.0f);
         const F3 c = mul(mulv(mulv(beta, Ll), pl.f), cn * mis);
         if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
-            const F3 so = add(hp, mul(wl, 1e-3f));
-            const HitRec sh = trace_ray<true, ABLK>(sv, so, wl, norm(lv), stk, overflow);
-            if (sh.prim < 0) {  // visible: d(beta Ll f cos mis)
+            bool visible;
+            if (GRAD && (rec_flags & VREC_TRACED)) {
+                visible = (rec_flags & VREC_VISIBLE) != 0;
+            } else {
+                const F3 so = add(hp, mul(wl, 1e-3f));
+                visible = trace_ray<true, ABLK>(sv, so, wl, norm(lv), stk, overflow).prim < 0;
+                if (!GRAD && rec) rec->prim = h.prim | VREC_TRACED | (visible ? VREC_VISIBLE : 0);
+            }
+            if (visible) {  // visible: d(beta Ll f cos mis)
                 if (!GRAD) {
                     p.L = add(p.L, c);
                 } else {
@@ -551,13 +581,13 @@ constexpr long long PATH_CHUNK = 16ll << 20;  // launches of at most 16 M sample
 }  // namespace
 
 void render_paths(Scene& sc, const CamView& cam, const RenderParams& p, uint32_t sample0, int P, float4* radiance,
-                  int* err) {
+                  int* err, HitRec* vrec, int vrec_cap) {
     if (p.env_is) throw std::runtime_error("the path kernel does not sample the sky (env_is)");
     if (!sc.target) sc.target = new RenderTarget();
     sc.target->path_claim.ensure(sizeof(int));
     AdjArgs a{sc.view, cam, p.W, p.H, p.W * p.H, P, p.seed, sample0, p.prr, p.mis, p.maxdepth, p.rrdepth, nullptr,
               0, P, nullptr, 0.0f, 0, nullptr, err, -1, nullptr, p.integrator == 1 ? 0 : 1, 0,
-              sc.target->path_claim.as<unsigned>()};
+              sc.target->path_claim.as<unsigned>(), vrec, vrec ? vrec_cap : 0};
     const PathsShape L = paths_shape(sc);
     for (long long s0 = 0; s0 < P; s0 += PATH_CHUNK) {
         a.slot0 = int(s0);
@@ -605,7 +635,12 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
     AdjArgs a{sc.view, cam, p.W, p.H, p.W * p.H, int(P), p.seed, uint32_t(p.spp_offset), p.prr, p.mis, p.maxdepth,
               p.rrdepth, stored ? t.radiance.as<float4>() : nullptr, 0, int(P), t.adj_dldi.as<float>(),
               1.0f / float(p.spp), n_params, t.adj_grad.as<double>(), d_err, tex_id, gtex, p.integrator == 1 ? 0 : 1,
-              0, t.path_claim.as<unsigned>()};
+              0, t.path_claim.as<unsigned>(), nullptr, 0};
+    // the replay reads the vertex records of a one-lane-per-path render of these very samples
+    if (stored && t.adj_vrec_valid) {
+        a.vrec = t.adj_vrec.as<HitRec>();
+        a.vrec_cap = VREC_CAP;
+    }
     // Chunks of a whole number of samples per pixel, about 16 M samples (at least one per pixel).
     // Without the render's own per-sample radiance each chunk is first rendered forward, by the
     // faster of the two forms for the scene: one lane per path into a chunk buffer when the scene

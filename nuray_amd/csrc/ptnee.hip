@@ -711,6 +711,7 @@ void wavefront_radiance(Scene& sc, const CamView& cam, const RenderParams& p, ui
     RenderTarget& t = *sc.target;
     t.ensure_queues(size_t(P));
     t.adj_valid = false;  // t.radiance no longer holds the last render's samples
+    t.adj_vrec_valid = false;
     int* cnt = t.counters.as<int>();
     hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(64), 0, sc.stream, cnt);
     int iterations = 0;
@@ -754,6 +755,10 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
     t.adj_valid = kb >= p.spp && !p.accumulate && !p.env_is;
     t.adj_p = p;
     t.adj_cam = cam;
+    // a one-batch render one lane per path also keeps vertex records for the adjoint's replay
+    const bool keep_vrec = paths && t.adj_valid && size_t(kb) * npix * VREC_CAP * sizeof(HitRec) <= VREC_BUDGET;
+    if (keep_vrec) t.adj_vrec.ensure(size_t(kb) * npix * VREC_CAP * sizeof(HitRec));
+    t.adj_vrec_valid = keep_vrec;
     hipStream_t s = sc.stream;
     int* cnt = t.counters.as<int>();
     TraceStats* ts = trace_stats ? t.stats.as<TraceStats>() : nullptr;
@@ -775,7 +780,8 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
         float4* const rad_env = env ? t.radiance.as<float4>() + P : nullptr;
         if (paths) {  // one lane per path: the same per-sample radiance without per-bounce launches
             mark_event(kt ? &ev_c : nullptr, s);
-            render_paths(sc, cam, p, uint32_t(p.spp_offset + b), P, t.radiance.as<float4>(), cnt + C_ERR);
+            render_paths(sc, cam, p, uint32_t(p.spp_offset + b), P, t.radiance.as<float4>(), cnt + C_ERR,
+                         keep_vrec ? t.adj_vrec.as<HitRec>() : nullptr, VREC_CAP);
             mark_event(kt ? &ev_c : nullptr, s);
             rs.iterations++;
         } else {

@@ -41,6 +41,11 @@ struct KernelTimes {
     int closest_launches = 0, shadow_launches = 0, shade_launches = 0;
 };
 
+// Vertex records kept by a one-lane-per-path render for the adjoint: the first VREC_CAP vertices of
+// every sample (16 B each), when the whole render fits VREC_BUDGET bytes of them.
+constexpr int VREC_CAP = 16;
+constexpr size_t VREC_BUDGET = size_t(2) << 30;
+
 struct RenderTarget {
     int W = 0, H = 0;
     size_t cap = 0;
@@ -62,6 +67,9 @@ struct RenderTarget {
     DevBuf adj_rad;                    // float4 per sample of one adjoint chunk (recomputed radiance)
     DevBuf adj_err;                    // int: the adjoint's traversal-overflow flag
     DevBuf path_claim;                 // unsigned: samples claimed by a per-lane path kernel launch
+    // vertex records of the last render when it ran one lane per path as one batch (see adjoint.hip)
+    DevBuf adj_vrec;
+    bool adj_vrec_valid = false;
     // radiance-cache rendering: spread state per queued ray, cache queries, training vertices
     DevBuf nstate[2], nrc_queries, nrc_recs, nrc_lb, nrc_tgt, nrc_batch, nrc_batch_tgt;
     ~RenderTarget();

@@ -36,6 +36,10 @@ def test_path_kernel_equals_wavefront(nr, scenes, case):
 
 
 def test_adjoint_after_path_kernel_render(nr, scenes):
+    """After a path-kernel render the replay reads that render's vertex records (the hit of each of
+    the first 16 vertices and its NEE visibility) instead of tracing again, and traces the vertices
+    past the 16th (the problem has no depth limit, so some paths are longer); after a wavefront
+    render it traces every vertex.  Both give the same gradient."""
     sc, theta_star, theta0 = scenes.adjoint_problem(width=48, height=48)
     W, H, M = 48, 48, len(theta0)
     dldi = np.random.Generator(np.random.PCG64(4)).normal(size=(H, W, 3)).astype(np.float32) * 1e-2
