@@ -57,57 +57,6 @@ __device__ __forceinline__ void trav_expand(const SceneView& sv, const SlabRay& 
     }
 }
 
-// Wave-cooperative form of trav_expand for the persistent kernels: every lane keeps expanding
-// internal nodes while other lanes of the wave are still looking for a leaf.  A lane that already
-// holds a leaf goes on expanding internal nodes (its leaf is postponed); when it pops a second leaf
-// it puts that back and parks until the wave's leaf round.  The loop ends when no lane is still
-// looking, or when at most LEAF_ROUND_SEARCHERS lanes are and some lane holds a leaf (waiting for
-// the last few searchers would leave the parked lanes idle; they search on after the leaf round).
-// Every call makes progress: while no lane holds a leaf the loop pops an entry per searching lane.
-// The set of nodes and leaves visited is the same as in the sequential DFS (only their interleaving
-// changes), and the result does not depend on the order.
-template <bool STATS, int STRIDE, int LEAF_ROUND_SEARCHERS = 16>
-__device__ __forceinline__ void trav_expand_wave(const SceneView& sv, const SlabRay& sr, int* stk, int cap,
-                                                 TravState& s, uint32_t& n_pf) {
-    bool parked = false;
-    for (;;) {
-        const int searching = __popcll(__ballot(!s.has_leaf && s.sp > 0));
-        if (searching == 0 || (searching <= LEAF_ROUND_SEARCHERS && __any(s.has_leaf))) return;
-        if (s.sp > 0 && !parked) {
-            const int32_t e = stk[(s.sp - 1) * STRIDE];
-            if (ref_is_leaf(e)) {
-                // hold the first leaf; a second one stays on the stack and parks the lane
-                parked = s.has_leaf;
-                s.leaf = s.has_leaf ? s.leaf : e;
-                s.sp -= s.has_leaf ? 0 : 1;
-                s.has_leaf = true;
-            } else {
-                --s.sp;
-                const NodePair* np = sv.nodes + e;
-                const float4 a = np->a, b = np->b, c = np->c;
-                const int4 ch = np->d;
-                if (STATS) ++n_pf;
-                bool cl, cr;
-                bool hl = slab_filter(a.x, a.y, a.z, a.w, b.x, b.y, sr, cl);
-                bool hr = slab_filter(b.z, b.w, c.x, c.y, c.z, c.w, sr, cr);
-                if (__builtin_expect(!(cl && cr), 0)) {  // inside an error band: the exact test
-                    const F3 dd = f3(sr.d.x + 1e-9f, sr.d.y + 1e-9f, sr.d.z + 1e-9f);
-                    if (!cl) hl = slab(a.x, a.y, a.z, a.w, b.x, b.y, sr.o, dd);
-                    if (!cr) hr = slab(b.z, b.w, c.x, c.y, c.z, c.w, sr.o, dd);
-                }
-                // branch-free push: sp <= cap - 1 here, so both slots are in bounds even when the
-                // push would overflow (which then ends the ray with the overflow flag)
-                const int npush = int(hl) + int(hr);
-                stk[s.sp * STRIDE] = hr ? ch.y : ch.x;
-                stk[(s.sp + 1) * STRIDE] = ch.x;
-                const bool ovf = s.sp + npush > cap;
-                s.overflow = s.overflow || ovf;
-                s.sp = ovf ? 0 : s.sp + npush;
-            }
-        }
-    }
-}
-
 // The 4-wide walk of the persistent trace kernels (Node4, see nr_device.h): the same visiting rule
 // on the same leaves as the binary walk, with half the node steps.  Pushes are branch-free: the four
 // child references are written at their compacted positions in slot order (a missed child's entry is
@@ -122,6 +71,15 @@ __device__ __forceinline__ void trav_begin_wide(const SceneView& sv, const SlabR
     }
 }
 
+// Wave-cooperative expansion of the 4-wide tree for the persistent kernels: every lane keeps expanding
+// internal nodes while other lanes of the wave are still looking for a leaf.  A lane that already
+// holds a leaf goes on expanding internal nodes (its leaf is postponed); when it pops a second leaf
+// it puts that back and parks until the wave's leaf round.  The loop ends when no lane is still
+// looking, or when at most LEAF_ROUND_SEARCHERS lanes are and some lane holds a leaf (waiting for
+// the last few searchers would leave the parked lanes idle; they search on after the leaf round).
+// Every call makes progress: while no lane holds a leaf the loop pops an entry per searching lane.
+// The set of nodes and leaves visited is the same as in the sequential DFS (only their interleaving
+// changes), and the result does not depend on the order.
 template <bool STATS, int STRIDE, int LEAF_ROUND_SEARCHERS = 16>
 __device__ __forceinline__ void trav_expand_wave4(const SceneView& sv, const SlabRay& sr, int* stk, int cap,
                                                   TravState& s, uint32_t& n_nf) {
