@@ -157,13 +157,9 @@ struct AdjArgs {
     int vrec_cap;            // vrec[(slot - rad_off) * vrec_cap + depth], or null
 };
 
-// A vertex record: the closest hit of the vertex's ray as traced (prim -1: a miss); for a hit, bit 29
-// of prim is set when the vertex traced its NEE shadow ray and bit 30 when that ray found the light
-// visible (primitive indices stay below 2^27).  The forward path kernel writes them; the replay of
-// the same samples reads them instead of tracing both rays again, and traces the shadow ray itself
-// where the record has none.
-constexpr int32_t VREC_TRACED = 1 << 29;
-constexpr int32_t VREC_VISIBLE = 1 << 30;
+// Vertex records (HitRec with the VREC_* flags): the forward path kernel and the wavefront write
+// them; the replay of the same samples reads them instead of tracing both rays again, and traces
+// the shadow ray itself where the record has none.
 
 // Hands the next samples of [a.slot0, a.slot1) to the lanes of the wave that ask for one (need):
 // one atomic per wave, samples in lane order.  Called by every lane of the wave at the same point.
@@ -650,9 +646,14 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
     const long long chunk_spp = std::max<long long>(1, PATH_CHUNK / npix);
     const bool fwd_paths = sc.view.n_flat > 0;
     PathsShape fwd{};
+    const size_t chunk_max = size_t(std::min<long long>(P, chunk_spp * npix));
+    // the forward pass of a chunk also writes vertex records when they fit, and the replay reads them
+    const bool fwd_rec = chunk_max * VREC_CAP * sizeof(HitRec) <= VREC_BUDGET;
     if (!stored && fwd_paths) {
         fwd = paths_shape(sc);
-        t.adj_rad.ensure(sizeof(float4) * size_t(std::min<long long>(P, chunk_spp * npix)));
+        t.adj_rad.ensure(sizeof(float4) * chunk_max);
+        if (fwd_rec) t.adj_vrec.ensure(chunk_max * VREC_CAP * sizeof(HitRec));
+        t.adj_vrec_valid = false;  // the chunk's records replace the last render's
     }
     const size_t shmem = size_t(sc.view.stack_cap + 1) * ABLK * sizeof(int) +
                          (n_params <= LDS_PARAMS ? size_t(n_params) * sizeof(float) : 0);
@@ -671,11 +672,15 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
         if (!stored && fwd_paths) {
             a.radiance = t.adj_rad.as<float4>();
             a.rad_off = a.slot0;
+            a.vrec = fwd_rec ? t.adj_vrec.as<HitRec>() : nullptr;
+            a.vrec_cap = VREC_CAP;
             launch_paths(sc, fwd, a, t.adj_rad.as<float4>());
         } else if (!stored) {
-            wavefront_radiance(sc, cam, p, uint32_t(p.spp_offset + b), a.slot1 - a.slot0);
+            const bool rec = wavefront_radiance(sc, cam, p, uint32_t(p.spp_offset + b), a.slot1 - a.slot0, fwd_rec);
             a.radiance = t.radiance.as<float4>();
             a.rad_off = a.slot0;
+            a.vrec = rec ? t.adj_vrec.as<HitRec>() : nullptr;
+            a.vrec_cap = VREC_CAP;
         }
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, sc.stream, a.claim);
         if (feat == F_ALL) hipLaunchKernelGGL(k_adjoint<F_ALL>, dim3(grid), dim3(ABLK), shmem, sc.stream, a);

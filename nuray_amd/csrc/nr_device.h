@@ -117,6 +117,11 @@ struct alignas(16) HitRec {
     float t, b1, b2;
     int32_t prim;  // TriGeom index, -1 for a miss
 };
+// Vertex records kept by a render for the adjoint's replay: the closest hit of the vertex's ray as
+// traced (prim -1: a miss); for a hit, bit 29 of prim is set when the vertex traced its NEE shadow
+// ray and bit 30 when that ray found the light visible (primitive indices stay below 2^27).
+constexpr int32_t VREC_TRACED = 1 << 29;
+constexpr int32_t VREC_VISIBLE = 1 << 30;
 
 // ------------------------------------------------------------------ exact hit arithmetic
 struct F3 {

@@ -211,6 +211,9 @@ struct ShadeArgs {
     // radiance region that sky shadow rays add to (one writer per slot per shadow launch)
     float* skyw;
     int env_base;
+    // vertex records for the adjoint (plain kernels only): vrec[slot * vrec_cap + depth], or null
+    HitRec* vrec;
+    int vrec_cap;
 };
 
 constexpr int NWAVE = BLK / 64;
@@ -306,6 +309,8 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a, const NrcShade& n
         const int j = base + int(threadIdx.x);
         const bool active = j < n;
         bool emit_sh = false, emit_next = false, emit_q = false, emit_rec = false, emit_env = false;
+        int rec_idx = -1;  // this vertex's record (plain kernels with records)
+        HitRec rec_h{};
         Ray sh_ray, next_ray, env_ray;
         float4 sh_c = make_float4(0, 0, 0, 0), next_b = make_float4(0, 0, 0, 0), env_c = make_float4(0, 0, 0, 0);
         float4 ns = make_float4(0, 0, 0, 0), rec_l = make_float4(0, 0, 0, 0);
@@ -316,6 +321,10 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a, const NrcShade& n
             const float4 bf = a.bin[j];
             const int slot = __float_as_int(r.d.w);
             const int depth = __float_as_int(r.o.w);
+            if (!NRC && !ENV && a.vrec && depth < a.vrec_cap) {
+                rec_idx = slot * a.vrec_cap + depth;
+                rec_h = h;
+            }
             const F3 beta = f3(bf.x, bf.y, bf.z);
             const float fac = bf.w;
             const F3 o = f3(r.o.x, r.o.y, r.o.z), d = f3(r.d.x, r.d.y, r.d.z);
@@ -550,6 +559,10 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a, const NrcShade& n
                 a.contrib[pos[2]] = env_c;
             }
         } else {
+            if (rec_idx >= 0)
+                a.vrec[rec_idx] = HitRec{rec_h.t, rec_h.b1, rec_h.b2,
+                                         rec_h.prim >= 0 && emit_sh ? (rec_h.prim | VREC_TRACED) : rec_h.prim};
+            sh_c.w = __int_as_float(rec_idx);  // the shadow launch marks the record when visible
             int ps, pn;
             block_append2(s_app, parity, emit_sh, emit_next, a.n_shadow, a.n_out, ps, pn);
             if (emit_sh) {
@@ -650,7 +663,9 @@ void mark_event(std::vector<hipEvent_t>* v, hipStream_t s) {
 // the launches.  Returns 1 if cancelled between bounces.
 int wavefront_batch(Scene& sc, RenderTarget& t, const CamView& cam, const RenderParams& p, uint32_t sample0, int P,
                     bool env, float4* rad_env, TraceStats* ts, volatile int* cancel, std::vector<hipEvent_t>* ev_c,
-                    std::vector<hipEvent_t>* ev_h, std::vector<hipEvent_t>* ev_s, int& iterations) {
+                    std::vector<hipEvent_t>* ev_h, std::vector<hipEvent_t>* ev_s, int& iterations,
+                    HitRec* vrec = nullptr) {
+    if (env) vrec = nullptr;  // records come from the plain shade kernels only
     const hipStream_t s = sc.stream;
     const int npix = p.W * p.H;
     int* cnt = t.counters.as<int>();
@@ -671,7 +686,7 @@ int wavefront_batch(Scene& sc, RenderTarget& t, const CamView& cam, const Render
                     t.rays[cur].as<Ray>(), t.beta[cur].as<float4>(), t.hits.as<HitRec>(), cnt + C_RAY0 + cur,
                     t.radiance.as<float4>(), t.rays[nxt].as<Ray>(), t.beta[nxt].as<float4>(), cnt + C_RAY0 + nxt,
                     t.shadow.as<Ray>(), t.contrib.as<float4>(), cnt + C_SHADOW,
-                    env ? t.skyw.as<float>() : nullptr, env ? P : 0};
+                    env ? t.skyw.as<float>() : nullptr, env ? P : 0, vrec, VREC_CAP};
         mark_event(ev_h, s);
         if (env) {
             if (sc.features == 0) hipLaunchKernelGGL(k_shade_basic_env, dim3(grid_shade), dim3(BLK), 0, s, a);
@@ -685,7 +700,7 @@ int wavefront_batch(Scene& sc, RenderTarget& t, const CamView& cam, const Render
         mark_event(ev_s, s);
         launch_shadow(sc.view, t.shadow.as<Ray>(), cnt + C_SHADOW, nullptr, nullptr, t.contrib.as<float4>(),
                       t.radiance.as<float4>(), cnt + C_WORK1, cnt + C_ERR, ts ? ts + 1 : nullptr, s,
-                      t.retry.as<int>());
+                      t.retry.as<int>(), vrec);
         mark_event(ev_s, s);
         ++iterations;
         if (p.maxdepth > 0) {
@@ -706,21 +721,25 @@ int wavefront_batch(Scene& sc, RenderTarget& t, const CamView& cam, const Render
 // pixel) into t.radiance, slot = sample offset * npix + pixel, by the wavefront; nothing is
 // accumulated.  The adjoint recomputes the radiance of its samples with it on scenes too large for
 // the flat leaf list, where the wavefront is several times faster than one lane per path.
-void wavefront_radiance(Scene& sc, const CamView& cam, const RenderParams& p, uint32_t sample0, int P) {
+bool wavefront_radiance(Scene& sc, const CamView& cam, const RenderParams& p, uint32_t sample0, int P,
+                        bool records) {
     if (!sc.target) sc.target = new RenderTarget();
     RenderTarget& t = *sc.target;
     t.ensure_queues(size_t(P));
     t.adj_valid = false;  // t.radiance no longer holds the last render's samples
     t.adj_vrec_valid = false;
+    records = records && !p.env_is && size_t(P) * VREC_CAP * sizeof(HitRec) <= VREC_BUDGET;
+    if (records) t.adj_vrec.ensure(size_t(P) * VREC_CAP * sizeof(HitRec));
     int* cnt = t.counters.as<int>();
     hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(64), 0, sc.stream, cnt);
     int iterations = 0;
     wavefront_batch(sc, t, cam, p, sample0, P, false, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                    iterations);
+                    iterations, records ? t.adj_vrec.as<HitRec>() : nullptr);
     check_hip(hipGetLastError(), "wavefront launch");
     check_hip(hipMemcpyAsync(t.pinned, cnt, sizeof(int) * 8, hipMemcpyDeviceToHost, sc.stream), "counters");
     check_hip(hipStreamSynchronize(sc.stream), "wavefront sync");
     if (t.pinned[C_ERR]) throw std::runtime_error("traversal stack overflow");
+    return records;
 }
 
 int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressFn cb, void* user,
@@ -755,8 +774,8 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
     t.adj_valid = kb >= p.spp && !p.accumulate && !p.env_is;
     t.adj_p = p;
     t.adj_cam = cam;
-    // a one-batch render one lane per path also keeps vertex records for the adjoint's replay
-    const bool keep_vrec = paths && t.adj_valid && size_t(kb) * npix * VREC_CAP * sizeof(HitRec) <= VREC_BUDGET;
+    // a one-batch render also keeps vertex records for the adjoint's replay
+    const bool keep_vrec = t.adj_valid && size_t(kb) * npix * VREC_CAP * sizeof(HitRec) <= VREC_BUDGET;
     if (keep_vrec) t.adj_vrec.ensure(size_t(kb) * npix * VREC_CAP * sizeof(HitRec));
     t.adj_vrec_valid = keep_vrec;
     hipStream_t s = sc.stream;
@@ -786,7 +805,8 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
             rs.iterations++;
         } else {
             status = wavefront_batch(sc, t, cam, p, uint32_t(p.spp_offset + b), P, env, rad_env, ts, cancel,
-                                     kt ? &ev_c : nullptr, kt ? &ev_h : nullptr, kt ? &ev_s : nullptr, rs.iterations);
+                                     kt ? &ev_c : nullptr, kt ? &ev_h : nullptr, kt ? &ev_s : nullptr, rs.iterations,
+                                     keep_vrec ? t.adj_vrec.as<HitRec>() : nullptr);
         }
         if (status) break;
         hipLaunchKernelGGL(k_finalize, dim3((npix + BLK - 1) / BLK), dim3(BLK), 0, s, t.radiance.as<float4>(), rad_env,

@@ -44,7 +44,7 @@ struct KernelTimes {
 // Vertex records kept by a one-lane-per-path render for the adjoint: the first VREC_CAP vertices of
 // every sample (16 B each), when the whole render fits VREC_BUDGET bytes of them.
 constexpr int VREC_CAP = 16;
-constexpr size_t VREC_BUDGET = size_t(2) << 30;
+constexpr size_t VREC_BUDGET = size_t(4) << 30;
 
 struct RenderTarget {
     int W = 0, H = 0;
@@ -67,7 +67,7 @@ struct RenderTarget {
     DevBuf adj_rad;                    // float4 per sample of one adjoint chunk (recomputed radiance)
     DevBuf adj_err;                    // int: the adjoint's traversal-overflow flag
     DevBuf path_claim;                 // unsigned: samples claimed by a per-lane path kernel launch
-    // vertex records of the last render when it ran one lane per path as one batch (see adjoint.hip)
+    // vertex records of the last render when it ran as one batch (see adjoint.hip)
     DevBuf adj_vrec;
     bool adj_vrec_valid = false;
     // radiance-cache rendering: spread state per queued ray, cache queries, training vertices
@@ -116,8 +116,11 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
                NrcFrameStats* fs, RenderStats* st);
 // Per-sample radiance of P paths (a whole number of samples per pixel, from sample0 on) into the
 // render target's radiance buffer, slot = sample offset * W * H + pixel, by the wavefront, without
-// touching the accumulated image.  Invalidates the stored radiance of the last render.
-void wavefront_radiance(Scene& sc, const CamView& cam, const RenderParams& p, uint32_t sample0, int P);
+// touching the accumulated image.  Invalidates the stored radiance of the last render.  With
+// `records`, also the vertex records of the first VREC_CAP vertices of every path into adj_vrec
+// when they fit VREC_BUDGET; returns whether it wrote them.
+bool wavefront_radiance(Scene& sc, const CamView& cam, const RenderParams& p, uint32_t sample0, int P,
+                        bool records = false);
 // Copies the accumulated result: linear mean (float W*H*3), FP64 sums, gamma-2.2 RGB8.
 void render_fetch(Scene& sc, float* rgb, double* sum, double* sum2, uint8_t* rgb8);
 

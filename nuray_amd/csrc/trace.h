@@ -24,10 +24,11 @@ void launch_closest(const SceneView& sv, const Ray* rays, const int* n, HitRec* 
 // Shadow test for rays[0 .. *n) with segment length rays[i].o.w: occluded iff some accepted
 // triangle has (double)t + 2e-3 <= (double)len (rendererptls.cpp:70-71).  Writes occ[i] (0/1)
 // and the first occluder found (prim / t) when `hits` is non-null; when `contrib` is non-null
-// a visible ray adds contrib[i].xyz to radiance[slot], slot = bits of rays[i].d.w.
+// a visible ray adds contrib[i].xyz to radiance[slot], slot = bits of rays[i].d.w, and, with vrec,
+// sets VREC_VISIBLE in the vertex record vrec[bits of contrib[i].w] (when that index is >= 0).
 void launch_shadow(const SceneView& sv, const Ray* rays, const int* n, uint8_t* occ, HitRec* hits,
                    const float4* contrib, float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s,
-                   int* retry = nullptr);
+                   int* retry = nullptr, HitRec* vrec = nullptr);
 
 int device_cu_count();
 // Largest dynamic LDS allocation one workgroup may request on the current device, in bytes.

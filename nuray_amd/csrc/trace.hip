@@ -56,7 +56,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                                                   const float4* __restrict__ contrib, float4* __restrict__ radiance,
                                                   int* __restrict__ work, int* __restrict__ err,
                                                   int* __restrict__ retry, int* __restrict__ n_retry, int cap,
-                                                  TraceStats* __restrict__ stats) {
+                                                  TraceStats* __restrict__ stats, HitRec* __restrict__ vrec) {
     extern __shared__ int stk_all[];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -148,6 +148,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                         L.y += cv.y;
                         L.z += cv.z;
                         radiance[slot] = L;
+                        // the vertex record of the ray's path vertex (index in w, -1: none)
+                        const int ri = __float_as_int(cv.w);
+                        if (vrec && ri >= 0) vrec[ri].prim |= VREC_VISIBLE;
                     }
                 }
                 i = -1;
@@ -172,7 +175,7 @@ int grid_for(size_t shmem) {
 
 template <bool ANY>
 void launch(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, uint8_t* occ, const float4* contrib,
-            float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s, int* retry) {
+            float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s, int* retry, HitRec* vrec) {
     const int full = sv.wide_stack_cap;
     const size_t shmem_full = size_t(full + 3) * BLOCK * sizeof(int);  // + the spare push slots
     if (shmem_full > device_max_lds()) throw std::runtime_error("BVH too deep for the LDS traversal stack");
@@ -181,10 +184,10 @@ void launch(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, ui
     const size_t shmem = size_t(cap + 3) * BLOCK * sizeof(int);
     if (stats) {
         hipLaunchKernelGGL((k_trace<ANY, true, false>), dim3(grid_for<ANY, true>(shmem)), dim3(BLOCK), shmem, s, sv, rays, n,
-                           nullptr, hits, occ, contrib, radiance, work, err, rl, work + 1, cap, stats);
+                           nullptr, hits, occ, contrib, radiance, work, err, rl, work + 1, cap, stats, vrec);
     } else {
         hipLaunchKernelGGL((k_trace<ANY, false, false>), dim3(grid_for<ANY, false>(shmem)), dim3(BLOCK), shmem, s, sv, rays,
-                           n, nullptr, hits, occ, contrib, radiance, work, err, rl, work + 1, cap, stats);
+                           n, nullptr, hits, occ, contrib, radiance, work, err, rl, work + 1, cap, stats, vrec);
     }
     check_hip(hipGetLastError(), "k_trace launch");
     if (!rl) return;
@@ -192,10 +195,10 @@ void launch(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, ui
     const dim3 g(device_cu_count());
     if (stats) {
         hipLaunchKernelGGL((k_trace<ANY, true, true>), g, dim3(BLOCK), shmem_full, s, sv, rays, work + 1, rl, hits, occ,
-                           contrib, radiance, work + 2, err, nullptr, nullptr, full, stats);
+                           contrib, radiance, work + 2, err, nullptr, nullptr, full, stats, vrec);
     } else {
         hipLaunchKernelGGL((k_trace<ANY, false, true>), g, dim3(BLOCK), shmem_full, s, sv, rays, work + 1, rl, hits, occ,
-                           contrib, radiance, work + 2, err, nullptr, nullptr, full, stats);
+                           contrib, radiance, work + 2, err, nullptr, nullptr, full, stats, vrec);
     }
     check_hip(hipGetLastError(), "k_trace retry launch");
 }
@@ -224,13 +227,13 @@ int device_cu_count() {
 
 void launch_closest(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, int* work, int* err,
                     TraceStats* stats, hipStream_t s, int* retry) {
-    launch<false>(sv, rays, n, hits, nullptr, nullptr, nullptr, work, err, stats, s, retry);
+    launch<false>(sv, rays, n, hits, nullptr, nullptr, nullptr, work, err, stats, s, retry, nullptr);
 }
 
 void launch_shadow(const SceneView& sv, const Ray* rays, const int* n, uint8_t* occ, HitRec* hits,
                    const float4* contrib, float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s,
-                   int* retry) {
-    launch<true>(sv, rays, n, hits, occ, contrib, radiance, work, err, stats, s, retry);
+                   int* retry, HitRec* vrec) {
+    launch<true>(sv, rays, n, hits, occ, contrib, radiance, work, err, stats, s, retry, vrec);
 }
 
 }  // namespace nr
