@@ -35,20 +35,24 @@ def test_path_kernel_equals_wavefront(nr, scenes, case):
     assert np.array_equal(a["rgb8"], b["rgb8"])
 
 
-def test_adjoint_after_path_kernel_render(nr, scenes):
-    """After a path-kernel render the replay reads that render's vertex records (the hit of each of
-    the first 16 vertices and its NEE visibility) instead of tracing again, and traces the vertices
-    past the 16th (the problem has no depth limit, so some paths are longer); after a wavefront
-    render it traces every vertex.  Both give the same gradient."""
+@pytest.mark.parametrize("maxdepth,rrdepth,integrator", [(0, 0, 0), (4, 2, 0), (6, 3, 1)])
+def test_adjoint_after_path_kernel_render(nr, scenes, maxdepth, rrdepth, integrator):
+    """The replay reads the vertex records of the render it follows (the hit of each of the first
+    16 vertices and its NEE shadow-ray outcome) -- written by the wavefront's shade and shadow
+    launches, or by the path kernel itself -- and traces past the 16th vertex (without a depth
+    limit some paths are longer).  Both renders give the same gradient, with and without a depth
+    limit and for the pt integrator (no NEE)."""
     sc, theta_star, theta0 = scenes.adjoint_problem(width=48, height=48)
     W, H, M = 48, 48, len(theta0)
+    kw = dict(maxdepth=maxdepth, rrdepth=rrdepth, integrator=integrator)
     dldi = np.random.Generator(np.random.PCG64(4)).normal(size=(H, W, 3)).astype(np.float32) * 1e-2
     dev = nr.Scene(sc)
     dev.set_params(theta0)
-    dev.render(sc["camera"], W, H, 8, seed=5, fetch=False)
-    g1 = dev.adjoint(sc["camera"], W, H, 8, M, seed=5, dl_dimg=dldi)
-    dev.render(sc["camera"], W, H, 8, seed=5, fetch=False, path_kernel=True)
-    g2 = dev.adjoint(sc["camera"], W, H, 8, M, seed=5, dl_dimg=dldi)
+    dev.render(sc["camera"], W, H, 8, seed=5, fetch=False, **kw)
+    g1 = dev.adjoint(sc["camera"], W, H, 8, M, seed=5, dl_dimg=dldi, **kw)
+    dev.render(sc["camera"], W, H, 8, seed=5, fetch=False, path_kernel=True, **kw)
+    g2 = dev.adjoint(sc["camera"], W, H, 8, M, seed=5, dl_dimg=dldi, **kw)
+    assert np.linalg.norm(g1) > 0
     # the replay reads the same stored radiance; the gradient rows are summed with float atomics whose
     # order changes from run to run, so the two agree to rounding, not bit for bit
     err = np.linalg.norm(g1 - g2) / np.linalg.norm(g1)
