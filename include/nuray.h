@@ -70,9 +70,10 @@ typedef struct nr_params {
                               the wavefront (one launch per bounce); the per-sample radiance and the image
                               are bit-identical.  Pays off for small scenes and for small frames of long
                               paths, e.g. inverse rendering of a closed room; not with env_is or the
-                              radiance cache.  2: automatic -- one lane per path for scenes of at most 64
-                              BVH leaves without env_is, the wavefront otherwise (the radiance cache
-                              always uses the wavefront).  0 (default): the wavefront */
+                              radiance cache.  2: automatic -- without env_is, one lane per path for
+                              scenes of at most 64 BVH leaves and for renders whose samples x triangles
+                              stay below 5e11, the wavefront otherwise (the radiance cache always uses
+                              the wavefront).  0 (default): the wavefront */
 } nr_params;
 
 typedef void (*nr_progress_fn)(float fraction, void* user);

@@ -760,9 +760,14 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
     const bool env = p.env_is && sc.view.env_w > 0;
     if (p.env_is && p.integrator != 0) throw std::runtime_error("environment sampling needs the ptnee integrator");
     if (p.path_kernel == 1 && p.env_is) throw std::runtime_error("the path kernel does not sample the sky (env_is)");
-    // automatic choice: small scenes (a flat leaf list) render faster one lane per path, and the
-    // image is the same either way
-    const bool paths = p.path_kernel == 1 || (p.path_kernel == 2 && !p.env_is && sc.view.n_flat > 0);
+    // automatic choice (the image is the same either way): one lane per path for small scenes (a
+    // flat leaf list) and for renders whose samples x triangles stay below AUTO_PATHS_WORK, where
+    // the wavefront's per-bounce launches cost more than the per-lane walks' divergence.  Measured
+    // crossovers: 16 k triangles near 59 M samples, 114 k between 2 M and 15 M, 684 k near 2 M.
+    constexpr double AUTO_PATHS_WORK = 5e11;
+    const bool paths = p.path_kernel == 1 ||
+                       (p.path_kernel == 2 && !p.env_is &&
+                        (sc.view.n_flat > 0 || double(npix) * p.spp * sc.n_tris() <= AUTO_PATHS_WORK));
     if (env) {
         const size_t c = size_t(kb) * npix;
         t.radiance.ensure(sizeof(float4) * 2 * c);

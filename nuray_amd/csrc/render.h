@@ -26,7 +26,8 @@ struct RenderParams {
     int integrator = 0; // 0: RendererPTLS (ptnee), 1: RendererPT (pt, no light sampling)
     int env_is = 0;     // 1: sample the sky at Lambert / GGX-reflect vertices (same expected image)
     int path_kernel = 0; // 1: one lane per path instead of the wavefront (same radiance, bit for bit);
-                         // 2: one lane per path for small scenes (flat leaf list) without env_is
+                         // 2: one lane per path for small scenes (flat leaf list) or small renders
+                         // (samples x triangles <= 5e11) without env_is
 };
 
 struct RenderStats {

@@ -69,8 +69,9 @@ def test_path_kernel_refusals(nr, scenes):
 
 
 def test_path_kernel_auto(nr, scenes):
-    """path_kernel = 2 picks one lane per path for a scene with a flat leaf list and no sky sampling
-    (no wavefront shade launches), the wavefront for larger scenes; the image is the same."""
+    """path_kernel = 2 picks one lane per path (no wavefront shade launches) for a scene with a flat
+    leaf list and for a small render of a larger scene, the wavefront for a render whose samples x
+    triangles exceed 5e11 and with sky sampling; the image is the same either way."""
     small = scenes.box_room()
     dev = nr.Scene(small)
     dev.set_profiling(1)
@@ -80,11 +81,17 @@ def test_path_kernel_auto(nr, scenes):
     assert dev.stats()["shade_launches"] > 0
     assert np.array_equal(a["sum"], b["sum"]) and np.array_equal(a["sum2"], b["sum2"])
     dev.close()
-    big = scenes.terrain_spheres(n_cells=48, n_spheres=60, width=64, height=36)
+    mid = scenes.terrain_spheres(n_cells=48, n_spheres=60, width=64, height=36)
+    dev = nr.Scene(mid)
+    dev.set_profiling(1)
+    dev.render(mid["camera"], 64, 36, 2, seed=3, path_kernel="auto")
+    assert dev.stats()["shade_launches"] == 0
+    dev.render(mid["camera"], 64, 36, 2, seed=3, path_kernel="auto", env_is=True)
+    assert dev.stats()["shade_launches"] > 0
+    dev.close()
+    big = scenes.terrain_spheres()  # 684,296 triangles: 1280 x 720 x 1 samples is past 5e11
     dev = nr.Scene(big)
     dev.set_profiling(1)
-    dev.render(big["camera"], 64, 36, 2, seed=3, path_kernel="auto")
-    assert dev.stats()["shade_launches"] > 0
-    dev.render(big["camera"], 64, 36, 2, seed=3, path_kernel="auto", env_is=True)
+    dev.render(big["camera"], 1280, 720, 1, seed=3, maxdepth=4, rrdepth=2, path_kernel="auto", fetch=False)
     assert dev.stats()["shade_launches"] > 0
     dev.close()
