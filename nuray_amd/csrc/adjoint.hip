@@ -406,7 +406,7 @@ template <unsigned F>
 __global__ __launch_bounds__(ABLK) void k_adjoint(AdjArgs a) {
     extern __shared__ char smem[];
     int* stk = reinterpret_cast<int*>(smem) + threadIdx.x;
-    float* gacc = reinterpret_cast<float*>(smem + size_t(a.sv.stack_cap + 1) * ABLK * sizeof(int));
+    float* gacc = reinterpret_cast<float*>(smem + size_t(a.sv.lane_slots) * ABLK * sizeof(int));
     const bool use_lds = a.n_params <= LDS_PARAMS;
     if (use_lds)
         for (int k = threadIdx.x; k < a.n_params; k += blockDim.x) gacc[k] = 0.0f;
@@ -552,7 +552,7 @@ struct PathsShape {
 };
 PathsShape paths_shape(const Scene& sc) {
     PathsShape L;
-    L.shmem = size_t(sc.view.stack_cap + 1) * ABLK * sizeof(int);
+    L.shmem = size_t(sc.view.lane_slots) * ABLK * sizeof(int);
     if (L.shmem > device_max_lds()) throw std::runtime_error("BVH too deep for the LDS traversal stack");
     L.feat = sc.features & (F_GGX | F_EXT) ? F_ALL : (sc.features & F_TEX ? F_TEX : 0u);
     int per_cu = 0;
@@ -655,7 +655,7 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
         if (fwd_rec) t.adj_vrec.ensure(chunk_max * VREC_CAP * sizeof(HitRec));
         t.adj_vrec_valid = false;  // the chunk's records replace the last render's
     }
-    const size_t shmem = size_t(sc.view.stack_cap + 1) * ABLK * sizeof(int) +
+    const size_t shmem = size_t(sc.view.lane_slots) * ABLK * sizeof(int) +
                          (n_params <= LDS_PARAMS ? size_t(n_params) * sizeof(float) : 0);
     if (shmem > device_max_lds()) throw std::runtime_error("adjoint LDS budget exceeded (BVH too deep)");
     // kernel instance for the material features present (as the shade kernels)

@@ -96,6 +96,7 @@ struct SceneView {
     int32_t short_stack; // stack entries of the first persistent trace launch (see trace.hip)
     int32_t wide_root;       // root as a Node4 index, or a leaf reference
     int32_t wide_stack_cap;  // full-depth stack entries of the 4-wide walk: 3 per level + 1
+    int32_t lane_slots;      // LDS stack slots per lane of the per-lane walk (trace_ray): entries + spares
     // small scenes (n_flat > 0 leaves, at most FLAT_LEAVES): the leaves' exact boxes in DFS order,
     // two float4 per leaf {min xyz, leaf reference bits}, {max xyz, 0}, for the per-lane walks
     const float4* flat;

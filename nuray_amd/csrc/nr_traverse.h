@@ -132,6 +132,47 @@ __device__ __forceinline__ void trav_expand_wave4(const SceneView& sv, const Sla
     }
 }
 
+// Per-lane form of the 4-wide walk (trace_ray): pops entries, expanding Node4s, until the lane
+// holds a leaf or the stack is empty.  Same compacted branch-free push as trav_expand_wave4 (three
+// spare slots beyond `cap`); an overflowing push ends the walk with the overflow flag.
+template <bool STATS, int STRIDE>
+__device__ __forceinline__ void trav_expand4(const SceneView& sv, const SlabRay& sr, int* stk, int cap, TravState& s,
+                                             uint32_t& n_nf) {
+    while (s.sp > 0) {
+        --s.sp;
+        const int32_t e = stk[s.sp * STRIDE];
+        if (ref_is_leaf(e)) {
+            s.leaf = e;
+            s.has_leaf = true;
+            return;
+        }
+        const Node4* nd = sv.wide + e;
+        const float4 lx = nd->lox, ly = nd->loy, lz = nd->loz;
+        const float4 hx = nd->hix, hy = nd->hiy, hz = nd->hiz;
+        const int4 ch = nd->ref;
+        if (STATS) ++n_nf;
+        const bool h0 = slab_fast(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, sr);
+        const bool h1 = slab_fast(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, sr);
+        const bool h2 = slab_fast(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, sr);
+        const bool h3 = slab_fast(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, sr);
+        int p = s.sp;  // <= cap - 1, so the writes (at most p + 3) stay inside cap + 3 slots
+        stk[p * STRIDE] = ch.x;
+        p += int(h0);
+        stk[p * STRIDE] = ch.y;
+        p += int(h1);
+        stk[p * STRIDE] = ch.z;
+        p += int(h2);
+        stk[p * STRIDE] = ch.w;
+        p += int(h3);
+        if (p > cap) {
+            s.overflow = true;
+            s.sp = 0;
+            return;
+        }
+        s.sp = p;
+    }
+}
+
 // Leaf loop, bvhnode.cpp:50-66, merged into the running winner with the DFS tie rule (closest
 // hit), or the shadow rule (double)t + 2e-3 <= len (any hit: returns true when blocked).
 template <bool ANY, bool STATS>
@@ -185,21 +226,21 @@ __device__ __forceinline__ HitRec trace_flat(const SceneView& sv, F3 o, F3 d, fl
     return HitRec{bt, bb1, bb2, bprim};
 }
 
-// Whole traversal of one ray by one lane (no refill): closest hit, or shadow test with segment
-// length len (returns hit.prim >= 0 iff blocked).  overflow is set if the stack ran out.  The
-// stack holds stack_cap + 1 slots STRIDE ints apart.
+// Whole traversal of one ray by one lane (no refill) over the 4-wide tree: closest hit, or shadow
+// test with segment length len (returns hit.prim >= 0 iff blocked).  overflow is set if the stack
+// ran out.  The stack holds SceneView::lane_slots slots STRIDE ints apart.
 template <bool ANY, int STRIDE>
 __device__ __forceinline__ HitRec trace_ray(const SceneView& sv, F3 o, F3 d, float len, int* stk, bool& overflow) {
     if (sv.n_flat > 0) return trace_flat<ANY>(sv, o, d, len);
     const SlabRay sr = slab_ray(o, d);
     TravState s;
-    trav_begin(sv, sr, stk, s);
+    trav_begin_wide(sv, sr, stk, s);
     float bt = ANY ? fminf(len, 1e18f) : 1e18f, bb1 = 0.0f, bb2 = 0.0f;
     int bprim = -1, blf = -1;
     uint32_t npf = 0, ntt = 0;
     bool done = false;
     for (;;) {
-        trav_expand<false, STRIDE>(sv, sr, stk, sv.stack_cap, s, npf);
+        trav_expand4<false, STRIDE>(sv, sr, stk, sv.wide_stack_cap, s, npf);
         if (!s.has_leaf) break;
         if (trav_leaf<ANY, false>(sv, o, d, double(len), s, bt, bb1, bb2, bprim, blf, ntt)) {
             done = true;

@@ -255,6 +255,7 @@ void Scene::build(int nthreads) {
     view.short_stack = short_stack;
     view.wide_root = N ? bvh.wide_root : make_leaf(0, 0);
     view.wide_stack_cap = 3 * bvh.wide_depth + 2;
+    view.lane_slots = view.wide_stack_cap + 3;
     upload_materials(false);  // checked above
     upload_textures(false);
     built = true;
