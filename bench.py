@@ -32,7 +32,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes of one closest-hit ray: 128 B per 4-wide node fetch (four child boxes and
-# references: the bytes of two binary node pairs), 48 B per triangle test, 32 B ray in, 16 B hit out
+# references), 48 B per triangle test, 32 B ray in, 16 B hit out
 B_NODE, B_TRI, B_RAY = 128, 48, 48
 
 

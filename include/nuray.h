@@ -72,7 +72,7 @@ typedef struct nr_params {
                               paths, e.g. inverse rendering of a closed room; not with env_is or the
                               radiance cache.  2: automatic -- without env_is, one lane per path for
                               scenes of at most 64 BVH leaves and for renders whose samples x triangles
-                              stay below 5e11, the wavefront otherwise (the radiance cache always uses
+                              stay below 2e12, the wavefront otherwise (the radiance cache always uses
                               the wavefront).  0 (default): the wavefront */
 } nr_params;
 

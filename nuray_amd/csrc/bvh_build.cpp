@@ -197,7 +197,7 @@ BvhBuild build_sah(const float* p, int n, int nthreads) {
     }
     if (n == 0) {
         for (int j = 0; j < 3; ++j) { out.root_min[j] = 1e18f; out.root_max[j] = -1e18f; }
-        out.root_ref = make_leaf(0, 0);
+        out.wide_root = make_leaf(0, 0);
         return out;
     }
     b.nodes.resize(2 * size_t(n));
@@ -213,15 +213,13 @@ BvhBuild build_sah(const float* p, int n, int nthreads) {
                                  "extent whose FP32 surface areas overflow)");
     const int nn = b.n_nodes.load();
 
-    // Preorder numbering over all nodes, preorder numbering of internal nodes for NodePairs,
-    // leaf / internal bounds (bvhnode.cpp:4-33) in a post-order pass.
-    std::vector<int> pre(nn), pair_id(nn, -1), order_nodes;
+    // Preorder numbering over all nodes, leaf / internal bounds (bvhnode.cpp:4-33) in a post-order pass.
+    std::vector<int> pre(nn), order_nodes;
     order_nodes.reserve(nn);
     std::vector<int> depth_of(nn, 0);
     {
         std::vector<int> st{root};
         depth_of[root] = 1;
-        int npairs = 0;
         while (!st.empty()) {
             int v = st.back();
             st.pop_back();
@@ -229,13 +227,11 @@ BvhBuild build_sah(const float* p, int n, int nthreads) {
             order_nodes.push_back(v);
             out.depth = std::max(out.depth, depth_of[v]);
             if (b.nodes[v].l >= 0) {
-                pair_id[v] = npairs++;
                 depth_of[b.nodes[v].l] = depth_of[b.nodes[v].r] = depth_of[v] + 1;
                 st.push_back(b.nodes[v].r);
                 st.push_back(b.nodes[v].l);
             }
         }
-        out.pairs.resize(npairs);
     }
     std::vector<Box> bx(nn);
     for (int k = nn - 1; k >= 0; --k) {  // reverse preorder: children before parents
@@ -257,22 +253,11 @@ BvhBuild build_sah(const float* p, int n, int nthreads) {
             for (int j = 0; j < 3; ++j) { B.mn[j] = smin(L.mn[j], R.mn[j]); B.mx[j] = smax(L.mx[j], R.mx[j]); }
         }
     }
-    auto ref_of = [&](int v) -> int32_t {
+    auto ref_of = [&](int v) -> int32_t {  // leaves only
         const Tmp& t = b.nodes[v];
-        return t.l < 0 ? make_leaf(t.lo, t.hi - t.lo) : int32_t(pair_id[v]);
+        return make_leaf(t.lo, t.hi - t.lo);
     };
-    for (int v : order_nodes) {
-        const Tmp& t = b.nodes[v];
-        if (t.l < 0) continue;
-        NodePair& np = out.pairs[pair_id[v]];
-        const Box &L = bx[t.l], &R = bx[t.r];
-        np.a = make_float4(L.mn[0], L.mn[1], L.mn[2], L.mx[0]);
-        np.b = make_float4(L.mx[1], L.mx[2], R.mn[0], R.mn[1]);
-        np.c = make_float4(R.mn[2], R.mx[0], R.mx[1], R.mx[2]);
-        np.d = make_int4(ref_of(t.l), ref_of(t.r), 0, 0);
-    }
     for (int j = 0; j < 3; ++j) { out.root_min[j] = bx[root].mn[j]; out.root_max[j] = bx[root].mx[j]; }
-    out.root_ref = ref_of(root);
     out.order = b.idx;
     collapse_wide(b.nodes, bx, root, ref_of, out);
     // small trees: the leaves' own boxes in DFS order for the per-lane walks (a leaf is reached by

@@ -8,10 +8,8 @@ namespace nr {
 
 struct BvhBuild {
     // GPU layout
-    std::vector<NodePair> pairs;   // internal nodes, preorder
     std::vector<int32_t> order;    // loader triangle ids in DFS leaf order
     float root_min[3], root_max[3];
-    int32_t root_ref = 0;          // NodePair index 0 or a leaf reference
     int depth = 0;                 // nodes on the longest root-to-leaf path
     std::vector<Node4> wide;       // the tree collapsed 4 wide (Node4 index 0 = root)
     int32_t wide_root = 0;         // Node4 index 0 or a leaf reference

@@ -1,8 +1,7 @@
 // Device-side records and inline helpers of the MI355X ptnee path.
 //
 // Layout in HBM (all arrays 64-byte aligned by hipMalloc):
-//   NodePair  64 B  one per internal BVH node, preorder; holds both children's boxes and refs
-//   Node4    128 B  the same tree collapsed 4 wide (boxes of up to four descendants, refs)
+//   Node4    128 B  the SAH tree collapsed 4 wide (boxes of up to four descendants, refs)
 //   TriGeom   48 B  one per triangle, in DFS leaf order: p0, e1 = p1-p0, e2 = p2-p0, loader id
 //   TriShade  64 B  same order: n0, n1, n2, uv0, uv1, uv2, material row
 //   MatRec    48 B  one per material: {type, r, g, b, alpha, ior, texture, 0, Ks r, g, b, Ks texture}
@@ -19,8 +18,8 @@ namespace nr {
 
 enum MatType { MAT_LAMBERT = 0, MAT_LIGHT = 1, MAT_GGXREFL = 2, MAT_GGX = 3, MAT_BLINNPHONG = 4, MAT_GLASS = 5 };
 
-// Leaf references inside NodePair::ref: bit 31 set, bits 27..30 = primitive count (1..5),
-// bits 0..26 = first primitive in TriGeom order.  Internal references are NodePair indices.
+// Leaf references inside Node4::ref: bit 31 set, bits 27..30 = primitive count (1..5),
+// bits 0..26 = first primitive in TriGeom order.  Internal references are Node4 indices.
 constexpr uint32_t LEAF_BIT = 0x80000000u;
 constexpr int FLAT_LEAVES = 64;  // scenes with at most this many leaves also get a flat leaf list
 __host__ __device__ inline bool ref_is_leaf(int32_t r) { return (uint32_t(r) & LEAF_BIT) != 0; }
@@ -30,13 +29,7 @@ __host__ __device__ inline int32_t make_leaf(int first, int count) {
     return int32_t(LEAF_BIT | (uint32_t(count) << 27) | uint32_t(first));
 }
 
-struct alignas(64) NodePair {
-    float4 a;  // lmin.x lmin.y lmin.z lmax.x
-    float4 b;  // lmax.y lmax.z rmin.x rmin.y
-    float4 c;  // rmin.z rmax.x rmax.y rmax.z
-    int4 d;    // lref rref 0 0
-};
-// The persistent trace kernels walk a 4-wide form of the same tree: each node holds up to four
+// The traversal kernels walk a 4-wide form of the binary SAH tree: each node holds up to four
 // descendants of a binary node (opened largest surface area first) with their exact boxes, as
 // per-axis float4s (child k in component k).  The leaves are the binary tree's own leaves, so every
 // leaf box is tested with its exact bounds, and every other box contains the leaf boxes below it.
@@ -76,8 +69,7 @@ struct TexDesc {
 
 // Everything a kernel needs to read the scene.
 struct SceneView {
-    const NodePair* nodes;
-    const Node4* wide;  // the same tree, 4 wide (persistent trace kernels)
+    const Node4* wide;  // the SAH tree, 4 wide
     const TriGeom* tris;
     const TriShade* shade;
     const MatRec* mats;
@@ -87,12 +79,10 @@ struct SceneView {
     const uint8_t* texpool;
     const float* lut;  // 256-entry RGBA8 decode table
     float root_min[3], root_max[3];
-    int32_t root_ref;  // root as a NodePair index, or a leaf reference
     int32_t n_lights;
     float light_p;
     int32_t env_tex;   // texture index of the environment map or -1
     float env_gain;
-    int32_t stack_cap; // traversal stack entries per lane (full depth)
     int32_t short_stack; // stack entries of the first persistent trace launch (see trace.hip)
     int32_t wide_root;       // root as a Node4 index, or a leaf reference
     int32_t wide_stack_cap;  // full-depth stack entries of the 4-wide walk: 3 per level + 1

@@ -13,52 +13,8 @@ struct TravState {
     bool overflow = false;
 };
 
-// Start a ray: test the root box and seed the stack (stk points at this lane's slot 0; entries are
-// STRIDE ints apart).  Leaves and internal nodes alike go through the stack.
-__device__ __forceinline__ void trav_begin(const SceneView& sv, const SlabRay& sr, int* stk, TravState& s) {
-    s = TravState{};
-    if (slab_fast(sv.root_min[0], sv.root_min[1], sv.root_min[2], sv.root_max[0], sv.root_max[1], sv.root_max[2],
-                  sr)) {
-        stk[0] = sv.root_ref;
-        s.sp = 1;
-    }
-}
-
-// Pop entries, expanding internal nodes, until the lane holds a leaf or the stack is empty.
-// Pushes are branch-free: both child slots are written every time (the entries past the new
-// top are dead), so the stack needs one spare slot beyond `cap` live entries.  Child order
-// does not matter for the result (the tie rule is keyed on leaf position, not visit order).
-template <bool STATS, int STRIDE>
-__device__ __forceinline__ void trav_expand(const SceneView& sv, const SlabRay& sr, int* stk, int cap, TravState& s,
-                                            uint32_t& n_pf) {
-    while (s.sp > 0) {
-        --s.sp;
-        const int32_t e = stk[s.sp * STRIDE];
-        if (ref_is_leaf(e)) {
-            s.leaf = e;
-            s.has_leaf = true;
-            return;
-        }
-        const NodePair* np = sv.nodes + e;
-        const float4 a = np->a, b = np->b, c = np->c;
-        const int4 ch = np->d;
-        if (STATS) ++n_pf;
-        const bool hl = slab_fast(a.x, a.y, a.z, a.w, b.x, b.y, sr);
-        const bool hr = slab_fast(b.z, b.w, c.x, c.y, c.z, c.w, sr);
-        const int npush = int(hl) + int(hr);
-        if (s.sp + npush > cap) {
-            s.overflow = true;
-            s.sp = 0;
-            return;
-        }
-        stk[s.sp * STRIDE] = hr ? ch.y : ch.x;
-        stk[(s.sp + 1) * STRIDE] = ch.x;
-        s.sp += npush;
-    }
-}
-
-// The 4-wide walk of the persistent trace kernels (Node4, see nr_device.h): the same visiting rule
-// on the same leaves as the binary walk, with half the node steps.  Pushes are branch-free: the four
+// The 4-wide walk (Node4, see nr_device.h): the same leaves, and the same hits, as a DFS of the
+// binary SAH tree, with half the node steps.  Pushes are branch-free: the four
 // child references are written at their compacted positions in slot order (a missed child's entry is
 // overwritten by the next hit one or left dead above the new top), so the stack needs three spare
 // slots beyond `cap` live entries.

@@ -27,7 +27,7 @@ struct RenderParams {
     int env_is = 0;     // 1: sample the sky at Lambert / GGX-reflect vertices (same expected image)
     int path_kernel = 0; // 1: one lane per path instead of the wavefront (same radiance, bit for bit);
                          // 2: one lane per path for small scenes (flat leaf list) or small renders
-                         // (samples x triangles <= 5e11) without env_is
+                         // (samples x triangles <= 2e12) without env_is
 };
 
 struct RenderStats {

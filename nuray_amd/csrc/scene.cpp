@@ -229,7 +229,6 @@ void Scene::build(int nthreads) {
         s[k].s2 = make_float4(nn[8], t[0], t[1], t[2]);
         s[k].s3 = make_float4(t[3], t[4], t[5], mf);
     }
-    d_nodes.upload(bvh.pairs.data(), bvh.pairs.size() * sizeof(NodePair), stream);
     d_wide.upload(bvh.wide.data(), bvh.wide.size() * sizeof(Node4), stream);
     d_flat.upload(bvh.flat.data(), bvh.flat.size() * sizeof(float4), stream);
     d_tris.upload(g.data(), g.size() * sizeof(TriGeom), stream);
@@ -239,7 +238,6 @@ void Scene::build(int nthreads) {
     check_hip(hipStreamSynchronize(stream), "upload sync");
 
     view = SceneView{};
-    view.nodes = d_nodes.as<NodePair>();
     view.wide = d_wide.as<Node4>();
     view.flat = d_flat.as<float4>();
     view.n_flat = int(bvh.flat.size() / 2);
@@ -248,10 +246,8 @@ void Scene::build(int nthreads) {
     view.lights = d_lights.as<LightRec>();
     view.light_cdf = d_cdf.as<float>();
     for (int j = 0; j < 3; ++j) { view.root_min[j] = bvh.root_min[j]; view.root_max[j] = bvh.root_max[j]; }
-    view.root_ref = N ? bvh.root_ref : make_leaf(0, 0);
     view.n_lights = int(lights.size());
     view.light_p = float(1.0 / (double(light_sum) + 1e-8));  // LightSampler::p, lightsampler.cpp:100-103
-    view.stack_cap = bvh.depth + 2;
     view.short_stack = short_stack;
     view.wide_root = N ? bvh.wide_root : make_leaf(0, 0);
     view.wide_stack_cap = 3 * bvh.wide_depth + 2;

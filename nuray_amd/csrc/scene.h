@@ -53,7 +53,7 @@ public:
     float light_sum = 0.0f;
 
     // device mirror
-    DevBuf d_nodes, d_wide, d_flat, d_tris, d_shade, d_mats, d_lights, d_cdf, d_tex, d_texpool, d_lut, d_env;
+    DevBuf d_wide, d_flat, d_tris, d_shade, d_mats, d_lights, d_cdf, d_tex, d_texpool, d_lut, d_env;
     std::vector<float4> env_tab;  // environment sampler cells (SceneView::env_tab)
     SceneView view{};
     hipStream_t stream = nullptr;

@@ -71,7 +71,7 @@ def test_path_kernel_refusals(nr, scenes):
 def test_path_kernel_auto(nr, scenes):
     """path_kernel = 2 picks one lane per path (no wavefront shade launches) for a scene with a flat
     leaf list and for a small render of a larger scene, the wavefront for a render whose samples x
-    triangles exceed 5e11 and with sky sampling; the image is the same either way."""
+    triangles exceed 2e12 and with sky sampling; the image is the same either way."""
     small = scenes.box_room()
     dev = nr.Scene(small)
     dev.set_profiling(1)
@@ -89,9 +89,9 @@ def test_path_kernel_auto(nr, scenes):
     dev.render(mid["camera"], 64, 36, 2, seed=3, path_kernel="auto", env_is=True)
     assert dev.stats()["shade_launches"] > 0
     dev.close()
-    big = scenes.terrain_spheres()  # 684,296 triangles: 1280 x 720 x 1 samples is past 5e11
+    big = scenes.terrain_spheres()  # 684,296 triangles: 1280 x 720 x 4 samples is past 2e12
     dev = nr.Scene(big)
     dev.set_profiling(1)
-    dev.render(big["camera"], 1280, 720, 1, seed=3, maxdepth=4, rrdepth=2, path_kernel="auto", fetch=False)
+    dev.render(big["camera"], 1280, 720, 4, seed=3, maxdepth=4, rrdepth=2, path_kernel="auto", fetch=False)
     assert dev.stats()["shade_launches"] > 0
     dev.close()
