@@ -164,7 +164,9 @@ int nr_loss_grad_l2(nr_scene* s, const float* target, double* loss, float* dL_di
  * for the samples of cam / params (the last render's when they match).  When the last
  * nr_render_ptnee ran those samples as one wavefront batch its per-sample radiance is reused;
  * otherwise each sample is replayed forward first (recompute-only: no per-path storage, any
- * sample count).  Gradients for Lambert and GGX-reflect materials, with or without albedo textures
+ * sample count), one lane per path for scenes of at most 64 BVH leaves and as params->path_kernel
+ * chooses for the others (0: the wavefront; 1: one lane per path; 2: one lane per path while a
+ * chunk's samples x triangles stay below 2e12).  Gradients for Lambert and GGX-reflect materials, with or without albedo textures
  * (a textured material's Kd row gets none); paths through translucent GGX dielectrics are
  * replayed, but the dielectrics' own rows get no gradient. */
 int nr_adjoint_ptnee(nr_scene* s, const nr_camera* cam, const nr_params* params, const float* dL_dimg,

@@ -257,10 +257,12 @@ class Scene:
         return (loss.value, g) if want_grad else loss.value
 
     def adjoint(self, cam, W, H, spp, n_mats, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0, spp_offset=0,
-                dl_dimg=None, tex_id=-1, tex_shape=None, integrator=0, env_is=False):
+                dl_dimg=None, tex_id=-1, tex_shape=None, integrator=0, env_is=False, path_kernel=False):
         """Material gradient (n_mats, 4); with tex_id >= 0 also the texel gradient (h, w, 3) of that
-        texture (tex_shape = (h, w))."""
-        p = nr_params(W, H, spp, spp_offset, seed, prr, mis, maxdepth, rrdepth, 0, 0, integrator, int(env_is))
+        texture (tex_shape = (h, w)).  path_kernel (as in render) picks the form of the forward
+        recompute when the last render's radiance is not these samples'."""
+        pk = 2 if path_kernel == "auto" else int(bool(path_kernel))
+        p = nr_params(W, H, spp, spp_offset, seed, prr, mis, maxdepth, rrdepth, 0, 0, integrator, int(env_is), pk)
         c = make_camera(cam)
         grad = np.zeros(n_mats * 4, np.float64)
         dl = None if dl_dimg is None else _f32(dl_dimg).reshape(-1)

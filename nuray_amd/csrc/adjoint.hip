@@ -639,14 +639,16 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
     }
     // Chunks of a whole number of samples per pixel, about 16 M samples (at least one per pixel).
     // Without the render's own per-sample radiance each chunk is first rendered forward, by the
-    // faster of the two forms for the scene: one lane per path into a chunk buffer when the scene
-    // has a flat leaf list, else the wavefront into the render target's radiance buffer (the
-    // accumulated image is left alone).  Nothing per path outlives its chunk.
+    // form the render modes would pick: one lane per path into a chunk buffer when the scene has a
+    // flat leaf list or p.path_kernel asks for it (1, or 2 below AUTO_PATHS_WORK per chunk), else
+    // the wavefront into the render target's radiance buffer (the accumulated image is left alone).
+    // Nothing per path outlives its chunk.
     const int npix = p.W * p.H;
     const long long chunk_spp = std::max<long long>(1, PATH_CHUNK / npix);
-    const bool fwd_paths = sc.view.n_flat > 0;
-    PathsShape fwd{};
     const size_t chunk_max = size_t(std::min<long long>(P, chunk_spp * npix));
+    const bool fwd_paths = sc.view.n_flat > 0 || p.path_kernel == 1 ||
+                           (p.path_kernel == 2 && double(chunk_max) * sc.n_tris() <= AUTO_PATHS_WORK);
+    PathsShape fwd{};
     // the forward pass of a chunk also writes vertex records when they fit, and the replay reads them
     const bool fwd_rec = chunk_max * VREC_CAP * sizeof(HitRec) <= VREC_BUDGET;
     if (!stored && fwd_paths) {

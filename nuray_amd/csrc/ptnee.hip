@@ -761,11 +761,7 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
     if (p.env_is && p.integrator != 0) throw std::runtime_error("environment sampling needs the ptnee integrator");
     if (p.path_kernel == 1 && p.env_is) throw std::runtime_error("the path kernel does not sample the sky (env_is)");
     // automatic choice (the image is the same either way): one lane per path for small scenes (a
-    // flat leaf list) and for renders whose samples x triangles stay below AUTO_PATHS_WORK, where
-    // the wavefront's per-bounce launches cost more than the per-lane walks' divergence.  Measured
-    // (maxdepth 10): one lane per path still wins at 1.7e12 (114 k triangles x 14.7 M samples,
-    // 684 k x 2.1 M) and loses at 2.5e12 (684 k x 3.7 M).
-    constexpr double AUTO_PATHS_WORK = 2e12;
+    // flat leaf list) and for renders whose samples x triangles stay below AUTO_PATHS_WORK
     const bool paths = p.path_kernel == 1 ||
                        (p.path_kernel == 2 && !p.env_is &&
                         (sc.view.n_flat > 0 || double(npix) * p.spp * sc.n_tris() <= AUTO_PATHS_WORK));

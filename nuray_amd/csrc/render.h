@@ -47,6 +47,12 @@ struct KernelTimes {
 constexpr int VREC_CAP = 16;
 constexpr size_t VREC_BUDGET = size_t(4) << 30;
 
+// Automatic render mode (nr_params::path_kernel = 2): one lane per path while samples x triangles
+// stay below this, where the wavefront's per-bounce launches cost more than the per-lane walks'
+// divergence.  Measured (maxdepth 10): one lane per path still wins at 1.7e12 (114 k triangles x
+// 14.7 M samples, 684 k x 2.1 M) and loses at 2.5e12 (684 k x 3.7 M).
+constexpr double AUTO_PATHS_WORK = 2e12;
+
 struct RenderTarget {
     int W = 0, H = 0;
     size_t cap = 0;

@@ -259,10 +259,12 @@ def test_adjoint_through_dielectrics_and_rgba8_texture(nr, po, scenes):
     assert not g[6:8].any()   # the dielectrics' rows
 
 
-def test_recompute_by_wavefront_equals_stored(nr, scenes):
+@pytest.mark.parametrize("pk", [False, True, "auto"])
+def test_recompute_by_wavefront_equals_stored(nr, scenes, pk):
     """On a scene too large for the flat leaf list the adjoint recomputes its samples' radiance with
-    the wavefront, leaving the accumulated image alone; the gradient equals the one taken from the
-    render's stored per-sample radiance up to the order of the float atomic sums."""
+    the wavefront (path_kernel False) or one lane per path (True, and "auto" at this size), leaving
+    the accumulated image alone; the gradient equals the one taken from the render's stored
+    per-sample radiance up to the order of the float atomic sums."""
     sc = scenes.terrain_spheres(n_cells=48, n_spheres=60, width=96, height=54)
     W, H, spp, M = 96, 54, 8, len(sc["mats"])
     kw = dict(maxdepth=6, rrdepth=2)
@@ -272,11 +274,11 @@ def test_recompute_by_wavefront_equals_stored(nr, scenes):
     g_stored = dev.adjoint(sc["camera"], W, H, spp, M, seed=5, dl_dimg=dldi, **kw)
     dev.render(sc["camera"], W, H, spp, seed=6, fetch=False, **kw)  # its radiance is not the adjoint's
     before = dev.fetch(W, H)["sum"].copy()
-    g_re = dev.adjoint(sc["camera"], W, H, spp, M, seed=5, dl_dimg=dldi, **kw)
+    g_re = dev.adjoint(sc["camera"], W, H, spp, M, seed=5, dl_dimg=dldi, path_kernel=pk, **kw)
     after = dev.fetch(W, H)["sum"]
     dev.close()
     err = np.linalg.norm(g_re - g_stored) / np.linalg.norm(g_stored)
-    print(f"wavefront recompute vs stored radiance: relative L2 {err:.2e}, |g| {np.linalg.norm(g_stored):.3e}")
+    print(f"recompute (path_kernel={pk}) vs stored radiance: relative L2 {err:.2e}, |g| {np.linalg.norm(g_stored):.3e}")
     assert np.linalg.norm(g_stored) > 0
     assert err <= 1e-6
     assert np.array_equal(before, after)
