@@ -311,6 +311,31 @@ def reference_scene(nr, scenes, with_cpu, reps=5):
     return out
 
 
+def render_modes(nr, scenes, dev, sc, reps=3):
+    """Wavefront against one lane per path (nr_params.path_kernel 0 / 1, the same image) on frames
+    either side of the automatic mode's samples x triangles threshold (2e12), maxdepth 10: the
+    114 k-triangle terrain at 256^2 x 32 and 1280x720 x 16, and the benchmark scene (684 k) at
+    256^2 x 32 and 640x360 x 16."""
+    mid = scenes.terrain_spheres(n_cells=192, n_spheres=500)
+    dmid = nr.Scene(mid)
+    rows = []
+    for d, s, frames in ((dmid, mid, ((256, 256, 32), (1280, 720, 16))), (dev, sc, ((256, 256, 32), (640, 360, 16)))):
+        for W, H, spp in frames:
+            cam = dict(s["camera"], aspect=W / H)
+            row = {"triangles": len(s["p"]), "frame": f"{W}x{H}x{spp}", "samples_x_triangles": W * H * spp * len(s["p"])}
+            for name, pk in (("wavefront", False), ("path_kernel", True)):
+                d.render(cam, W, H, spp, seed=1, maxdepth=10, rrdepth=3, fetch=False, path_kernel=pk)
+                nr.synchronize()
+                t0 = time.perf_counter()
+                for k in range(reps):
+                    d.render(cam, W, H, spp, seed=2 + k, maxdepth=10, rrdepth=3, fetch=False, path_kernel=pk)
+                nr.synchronize()
+                row[f"{name}_ms"] = (time.perf_counter() - t0) * 1e3 / reps
+            rows.append(row)
+    dmid.close()
+    return rows
+
+
 def env_sampling(nr, dev, sc, spp):
     """The same 720p frame with and without sky sampling (nr_params.env_is): time per frame, mean
     per-pixel variance of one sample, and the time-to-equal-error gain var_off t_off / (var_on t_on)."""
@@ -453,6 +478,11 @@ def main():
               f"{env['variance_ratio']:.2f}x lower, {env['equal_error_speedup']:.2f}x faster to equal error",
               file=sys.stderr, flush=True)
     ref_scene = None if args.frames_only else reference_scene(nr, scenes, not args.no_cpu)
+    modes = None if args.frames_only else render_modes(nr, scenes, dev, sc)
+    if modes:
+        print("# render modes (wavefront / one lane per path, ms): " + ", ".join(
+            f"{m['triangles']} tris {m['frame']} {m['wavefront_ms']:.2f} / {m['path_kernel_ms']:.2f}" for m in modes),
+            file=sys.stderr, flush=True)
     if ref_scene:
         print(f"# reference box room 320x240x16spp: wavefront {ref_scene['wavefront_ms']:.2f} ms, path kernel "
               f"{ref_scene['path_kernel_ms']:.2f} ms"
@@ -478,6 +508,7 @@ def main():
                    "seq_len": None, "parallelism": "single-gpu", "bvh_build_ms": info["build_ms"]},
         "roofline": roofline, "cpu_baseline": cpu, "hbm_copy_gbs": copy_gbs,
         "adjoint": adj, "nrc": nrc, "nrc_mlp": mlp, "env_sampling": env, "reference_scene": ref_scene,
+        "render_modes": modes,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
