@@ -81,6 +81,11 @@ typedef void (*nr_progress_fn)(float fraction, void* user);
 const char* nr_last_error(void);
 /* Selects the HIP device for this host thread. */
 int nr_init(int32_t device);
+/* End of a session: waits for all work on the current device.  Scenes and caches must be
+ * destroyed before.  The
+ * reference's session ends with its process (cmd.cpp:112-113: save, then return from main); a
+ * host that embeds the library calls this before it lets the device go. */
+int nr_shutdown(void);
 int nr_synchronize(void);
 
 /* Replaces Loader::getTriangles() + Triangle::Triangle (src/scene/loader.h:15-19,

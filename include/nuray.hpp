@@ -571,12 +571,16 @@ public:
 // ------------------------------------------------------------------------------------- autograd
 // torch.autograd.Function-style differentiable render: forward saves what backward needs in the
 // context; backward maps dL/dimage (W*H*3 floats) to dL/dtheta (materials x {r, g, b, alpha}).
-// The render runs as one wavefront batch (images up to 64 M samples).
+// The render runs as one wavefront batch (images up to 64 M samples).  The context keeps theta, so
+// a backward after other forwards on the same renderer (other views, other parameters) puts its own
+// parameters back on the scene first; the adjoint then re-renders its samples when the scene's
+// last render was not theirs.
 struct RenderContext {
     nr_scene* scene = nullptr;
     nr_camera cam{};
     nr_params params{};
     uint32_t n_mats = 0;
+    std::vector<float> theta;
 };
 
 struct RenderFunction {
@@ -591,6 +595,7 @@ struct RenderFunction {
         ctx.params.batch_spp = 0;
         ctx.params.accumulate = 0;
         ctx.n_mats = uint32_t(theta.size() / 4);
+        ctx.theta = theta;
         check(nr_scene_set_params(ctx.scene, theta.data(), ctx.n_mats));
         std::vector<float> img(size_t(ctx.params.width) * ctx.params.height * 3);
         check(nr_render_ptnee(ctx.scene, &ctx.cam, &ctx.params, img.data(), nullptr, nullptr, nullptr, nullptr, nullptr,
@@ -601,6 +606,7 @@ struct RenderFunction {
         if (grad_image.size() != size_t(ctx.params.width) * ctx.params.height * 3)
             throw Error("grad_image must have W*H*3 entries");
         std::vector<double> g(size_t(ctx.n_mats) * 4);
+        check(nr_scene_set_params(ctx.scene, ctx.theta.data(), ctx.n_mats));
         check(nr_adjoint_ptnee(ctx.scene, &ctx.cam, &ctx.params, grad_image.data(), g.data()));
         return g;
     }

@@ -510,14 +510,18 @@ __global__ void k_zero(double* a, int n) {
 
 void set_material_params(Scene& sc, const float* theta, int n_mats) {
     if (n_mats != int(sc.mats.size() / 8)) throw std::runtime_error("parameter rows must match the material count");
+    bool changed = false;
     for (int j = 0; j < n_mats; ++j) {
         float* m = &sc.mats[8 * size_t(j)];
         if (int(m[0]) == MAT_LIGHT) continue;  // emitters keep their Ke
-        m[1] = theta[4 * j + 0];
-        m[2] = theta[4 * j + 1];
-        m[3] = theta[4 * j + 2];
-        m[4] = theta[4 * j + 3];
+        for (int k = 0; k < 4; ++k) {
+            changed |= std::memcmp(&m[1 + k], &theta[4 * j + k], sizeof(float)) != 0;
+            m[1 + k] = theta[4 * j + k];
+        }
     }
+    // setting the parameters a render already used keeps its stored samples for the adjoint
+    if (!changed) return;
+    ++sc.generation;
     if (!sc.built) return;
     const std::vector<MatRec> mr = sc.mat_records();
     sc.d_mats.upload(mr.data(), mr.size() * sizeof(MatRec), sc.stream);
@@ -600,7 +604,7 @@ void adjoint_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, const f
     if (!sc.target) sc.target = new RenderTarget();
     RenderTarget& t = *sc.target;
     // the stored per-sample radiance serves when it belongs to exactly these samples
-    const bool stored = t.adj_matches(cam, p);
+    const bool stored = t.adj_matches(cam, p, sc.generation);
     const size_t n3 = size_t(p.W) * p.H * 3;
     if (dldi_host) {
         t.adj_dldi.ensure(n3 * sizeof(float));

@@ -48,6 +48,13 @@ int nr_init(int32_t device) {
     });
 }
 
+int nr_shutdown(void) {
+    return guard([&] {
+        nr::check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        return NR_OK;
+    });
+}
+
 int nr_synchronize(void) {
     return guard([&] {
         nr::check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");

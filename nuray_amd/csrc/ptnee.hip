@@ -776,6 +776,7 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
     t.adj_valid = kb >= p.spp && !p.accumulate && !p.env_is;
     t.adj_p = p;
     t.adj_cam = cam;
+    t.adj_gen = sc.generation;
     // a one-batch render also keeps vertex records for the adjoint's replay
     const bool keep_vrec = t.adj_valid && size_t(kb) * npix * VREC_CAP * sizeof(HitRec) <= VREC_BUDGET;
     if (keep_vrec) t.adj_vrec.ensure(size_t(kb) * npix * VREC_CAP * sizeof(HitRec));

@@ -69,6 +69,7 @@ struct RenderTarget {
     // adjoint: the per-sample radiance of the last render is replayable when it ran as one batch
     bool adj_valid = false;
     RenderParams adj_p;   // the render those samples came from
+    uint64_t adj_gen = 0; // and the scene generation it rendered
     CamView adj_cam{};
     DevBuf adj_target, adj_dldi, adj_loss, adj_grad, adj_gtex;
     DevBuf adj_rad;                    // float4 per sample of one adjoint chunk (recomputed radiance)
@@ -82,10 +83,11 @@ struct RenderTarget {
     ~RenderTarget();
     void ensure(int W, int H, size_t cap);  // image accumulators for W x H and queues for cap paths
     void ensure_queues(size_t cap);         // queues only (the accumulators keep their contents)
-    // the stored per-sample radiance belongs to exactly the samples of (cam, p)
-    bool adj_matches(const CamView& cam, const RenderParams& p) const {
+    // the stored per-sample radiance belongs to exactly the samples of (cam, p) on the scene as
+    // it is now (no material, parameter, texture or sky change since that render)
+    bool adj_matches(const CamView& cam, const RenderParams& p, uint64_t scene_generation) const {
         const RenderParams& q = adj_p;
-        return adj_valid && W == p.W && H == p.H && q.W == p.W && q.H == p.H && q.spp == p.spp &&
+        return adj_valid && adj_gen == scene_generation && W == p.W && H == p.H && q.W == p.W && q.H == p.H && q.spp == p.spp &&
                q.spp_offset == p.spp_offset && q.seed == p.seed && q.prr == p.prr && q.mis == p.mis &&
                q.maxdepth == p.maxdepth && q.rrdepth == p.rrdepth && q.integrator == p.integrator &&
                std::memcmp(&adj_cam, &cam, sizeof(CamView)) == 0;

@@ -84,11 +84,13 @@ std::vector<MatRec> Scene::mat_records() const {
 void Scene::set_material_ext(int id, const float* ext) {
     if (id < 0 || size_t(id) >= mats.size() / 8) throw std::runtime_error("material id out of range");
     for (int k = 0; k < 4; ++k) mats_ext[4 * size_t(id) + k] = ext[k];
+    ++generation;
     if (built) upload_materials();
 }
 
 int Scene::add_texture(int id, int w, int h, int fmt, const void* data) {
     if (id < 0 || w <= 0 || h <= 0) throw std::runtime_error("bad texture");
+    ++generation;
     if (int(textures.size()) <= id) textures.resize(id + 1);
     Texture& t = textures[id];
     // same size and format on a built scene: rewrite the texels in place, no rebuild
@@ -171,6 +173,7 @@ void Scene::upload_textures(bool check) {
 }
 
 void Scene::build(int nthreads) {
+    ++generation;
     const int N = n_tris();
     check_materials();
     bvh = build_sah(p.data(), N, nthreads);

@@ -44,6 +44,9 @@ public:
     float env_gain = 1.0f;
     unsigned features = 0;        // material features present: 1 = GGX dielectric, 2 = albedo textures
     int short_stack = 16;         // LDS stack entries of the first trace launch
+    // Bumped by every change that can alter a render (materials, parameters, textures, sky,
+    // build): per-sample results kept from a render serve the adjoint only at the same value.
+    uint64_t generation = 0;
 
     // derived host data
     BvhBuild bvh;

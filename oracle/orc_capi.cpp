@@ -280,6 +280,16 @@ void orc_material_sample_density(void* h, int m, int n, const double* wo, const 
                                    V3d(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]));
 }
 
+// Radiance the estimator returns for a ray that leaves the scene along d (Tracer::env,
+// rendererptls.cpp:18-28): n unit directions in, n x 3 RGB out (FP32 mode).
+void orc_env_radiance(void* h, int n, const float* d, float* out) {
+    Tracer<float> tr{S(h)->sc, S(h)->bvh, RenderOpts{}};
+    for (int i = 0; i < n; ++i) {
+        V3f c = tr.env(V3f(d[3 * i], d[3 * i + 1], d[3 * i + 2]));
+        out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+    }
+}
+
 int orc_light_pick(void* h, float u) { return S(h)->sc.pick_light(u); }
 float orc_light_p(void* h) { return S(h)->sc.light_p(); }
 

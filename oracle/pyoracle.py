@@ -68,6 +68,7 @@ def lib():
         L.orc_nrc_forward.argtypes = [f32p, f32p, C.c_int, C.c_int, f64p]
         L.orc_nrc_grad.argtypes = [f32p, f32p, f32p, C.c_int, C.c_int, f64p]
         L.orc_nrc_grad.restype = C.c_double
+        L.orc_env_radiance.argtypes = [vp, C.c_int, f32p, f32p]
         L.orc_light_pick.argtypes = [vp, C.c_float]
         L.orc_light_p.argtypes = [vp]
         L.orc_light_p.restype = C.c_float
@@ -205,6 +206,13 @@ class Scene:
         if r != 0:
             raise RuntimeError("oracle adjoint failed")
         return g.reshape(n_mats, 4) if gt is None else (g.reshape(n_mats, 4), gt)
+
+    def env_radiance(self, d):
+        """Radiance of rays that leave the scene along the unit directions d (n x 3)."""
+        d = _f32(d).reshape(-1, 3)
+        out = np.zeros((len(d), 3), np.float32)
+        lib().orc_env_radiance(self.h, len(d), d, out)
+        return out
 
     def light_pick(self, u):
         return lib().orc_light_pick(self.h, float(u))

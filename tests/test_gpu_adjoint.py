@@ -282,3 +282,44 @@ def test_recompute_by_wavefront_equals_stored(nr, scenes, pk):
     assert np.linalg.norm(g_stored) > 0
     assert err <= 1e-6
     assert np.array_equal(before, after)
+
+
+def test_scene_change_after_render_is_not_replayed_stale(nr, scenes):
+    """The adjoint reuses the last render's per-sample radiance and vertex records only while the
+    scene is unchanged: render(theta0), set_params(theta1), adjoint must equal render(theta1),
+    adjoint -- and likewise after an in-place texture update.  (Gradients sum per block in FP32
+    before the FP64 accumulation, so two replays agree to ~1e-7, not bit for bit; a stale replay
+    is off by percents.)"""
+    sc, theta_star, theta0 = _problem(scenes, 32, 32)
+    W, H, spp = 32, 32, 8
+    M = len(theta0)
+    dldi = np.random.Generator(np.random.PCG64(31)).normal(size=(H, W, 3)).astype(np.float32) * 1e-3
+    dev = nr.Scene(sc)
+    dev.set_params(theta0)
+    dev.render(sc["camera"], W, H, spp, seed=5, fetch=False)
+    dev.set_params(theta_star)
+    g_stale = dev.adjoint(sc["camera"], W, H, spp, M, seed=5, dl_dimg=dldi)
+    dev.render(sc["camera"], W, H, spp, seed=5, fetch=False)
+    g_fresh = dev.adjoint(sc["camera"], W, H, spp, M, seed=5, dl_dimg=dldi)
+    dev.set_params(theta0)
+    dev.render(sc["camera"], W, H, spp, seed=5, fetch=False)
+    g_old = dev.adjoint(sc["camera"], W, H, spp, M, seed=5, dl_dimg=dldi)
+    dev.close()
+    err = np.abs(g_stale - g_fresh).max() / np.abs(g_fresh).max()
+    gap = np.abs(g_old - g_fresh).max() / np.abs(g_fresh).max()
+    print(f"params changed after the render: max relative difference {err:.2e} (gradient at theta0: {gap:.2e})")
+    assert err < 1e-5 and gap > 1e-2
+    ts = scenes.textured_room(width=24, height=20)
+    W, H, spp = 24, 20, 4
+    M = len(ts["mats"])
+    dldi = np.random.Generator(np.random.PCG64(32)).normal(size=(H, W, 3)).astype(np.float32) * 1e-3
+    tex = ts["textures"][0]
+    dev = nr.Scene(ts)
+    dev.render(ts["camera"], W, H, spp, seed=6, fetch=False)
+    dev.set_texture_rgb32f(0, tex[::-1].copy() * 0.5)
+    g_stale, t_stale = dev.adjoint(ts["camera"], W, H, spp, M, seed=6, dl_dimg=dldi, tex_id=0, tex_shape=tex.shape[:2])
+    dev.render(ts["camera"], W, H, spp, seed=6, fetch=False)
+    g_fresh, t_fresh = dev.adjoint(ts["camera"], W, H, spp, M, seed=6, dl_dimg=dldi, tex_id=0, tex_shape=tex.shape[:2])
+    dev.close()
+    assert np.abs(g_stale - g_fresh).max() <= 1e-5 * np.abs(g_fresh).max()
+    assert np.abs(t_stale - t_fresh).max() <= 1e-5 * np.abs(t_fresh).max()

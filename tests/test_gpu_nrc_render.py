@@ -93,3 +93,73 @@ def test_trained_cache_gains_over_plain_pt(nr, scenes):
     assert s2 < 1.5 * pt_s2
     assert nrc_time < pt_time
     assert untrained_mse - s2 / 64 > 10 * eps2   # without training the cache's bias dominates
+
+
+def test_cache_at_1080p_on_the_684k_triangle_scene(nr, scenes):
+    """The cache's own configuration: 1920x1080, 1 spp per frame, the 684,296-triangle heightfield +
+    spheres scene (maxdepth 10, RR from vertex 3), 65,536 training paths per frame.
+    (1) With the cache off a frame equals nr_render_ptnee bit for bit.
+    (2) After 256 warm-up frames the cache's squared bias is below 0.1 (1e-2)^2 (within three
+        standard errors of its estimate).  Estimator: two independent 2048-spp plain references
+        R1, R2 and two independent 256-frame cache means A, B; E[(A - R1)(B - R2)] is exactly the
+        squared bias (all four noises are independent), so no variance has to be subtracted.  The
+        same estimator on plain frames (no bias) is the control: it must come out at zero.
+    (3) A cache frame (its training step included) costs no more than a plain frame.
+    Prints the time to RMSE 1e-2 of both (plain: frame time x variance / eps^2; cache: warm-up +
+    frame time x variance / (eps^2 - bias^2))."""
+    import time
+    W, H = 1920, 1080
+    sc = scenes.terrain_spheres(width=W, height=H)
+    assert len(sc["p"]) == 684296
+    cam = sc["camera"]
+    kw = dict(maxdepth=10, rrdepth=3)
+    dev = nr.Scene(sc)
+    # (1) cache off, training on: the frame is the plain frame
+    ref1 = dev.render(cam, W, H, 1, seed=21, spp_offset=7, **kw)
+    cache = nr.Nrc(seed=4)
+    st, got = dev.render_nrc(cache, cam, W, H, spp=1, seed=21, spp_offset=7, use_cache=False, fetch=True, **kw)
+    assert np.array_equal(got["sum"], ref1["sum"]) and np.array_equal(got["sum2"], ref1["sum2"])
+    assert st["trained"] > 0
+    r1 = dev.render(cam, W, H, 2048, seed=999, **kw)["sum"] / 2048
+    r2 = dev.render(cam, W, H, 2048, seed=999, spp_offset=2048, **kw)["sum"] / 2048
+
+    def mean_of(K, fn):
+        """K accumulated frames: (mean image, mean per-pixel frame variance, seconds per frame)."""
+        nr.synchronize()
+        t0 = time.perf_counter()
+        for k in range(K):
+            fn(k)
+        nr.synchronize()
+        t = (time.perf_counter() - t0) / K
+        a = dev.fetch(W, H)
+        m = a["sum"] / K
+        return m, float((np.maximum(a["sum2"] / K - m * m, 0) * K / (K - 1)).mean()), t
+
+    def bias2(ma, mb):
+        prod = ((ma - r1) * (mb - r2)).reshape(-1)
+        return float(prod.mean()), float(prod.std() / np.sqrt(prod.size))
+
+    K = 256
+    pa, pt_s2, pt_t = mean_of(K, lambda k: dev.render(cam, W, H, 1, seed=5, spp_offset=k, accumulate=k > 0,
+                                                       fetch=False, **kw))
+    pb, _, _ = mean_of(K, lambda k: dev.render(cam, W, H, 1, seed=5, spp_offset=K + k, accumulate=k > 0,
+                                               fetch=False, **kw))
+    b2_pt, se_pt = bias2(pa, pb)
+    cache = nr.Nrc(seed=1)
+    warm_t = mean_of(256, lambda k: dev.render_nrc(cache, cam, W, H, spp=1, seed=3, spp_offset=k, **kw))[2] * 256
+    ca, s2, nrc_t = mean_of(K, lambda k: dev.render_nrc(cache, cam, W, H, spp=1, seed=11, spp_offset=256 + k,
+                                                        accumulate=k > 0, **kw))
+    cb, _, _ = mean_of(K, lambda k: dev.render_nrc(cache, cam, W, H, spp=1, seed=11, spp_offset=256 + K + k,
+                                                   accumulate=k > 0, **kw))
+    b2, se = bias2(ca, cb)
+    dev.close()
+    eps2 = 1e-4
+    pt_time = pt_t * pt_s2 / eps2
+    nrc_time = warm_t + nrc_t * s2 / (eps2 - b2) if b2 < eps2 else float("inf")
+    print(f"1080p: plain {pt_t * 1e3:.2f} ms/frame, variance {pt_s2:.3f} -> {pt_time:.1f} s to RMSE 1e-2; cache "
+          f"{nrc_t * 1e3:.2f} ms/frame, variance {s2:.3f}, bias^2 {b2:.2e} +- {se:.1e}, warm-up {warm_t:.2f} s -> "
+          f"{nrc_time:.1f} s (time ratio plain/cache {pt_time / nrc_time:.3f}); control: plain bias^2 "
+          f"{b2_pt:.1e} +- {se_pt:.1e}")
+    assert abs(b2_pt) < 4 * se_pt
+    assert b2 < 0.1 * eps2 + 3 * se
+    assert nrc_t <= 1.05 * pt_t
