@@ -79,10 +79,11 @@ typedef struct nr_params {
 typedef void (*nr_progress_fn)(float fraction, void* user);
 
 const char* nr_last_error(void);
-/* Selects the HIP device for this host thread. */
+/* Selects the HIP device for this host thread (and looks up the device address of the scratch
+ * array nr_intersect_noalloc uses on it). */
 int nr_init(int32_t device);
-/* End of a session: waits for all work on the current device.  Scenes and caches must be
- * destroyed before.  The
+/* End of a session: waits for all work on the current device and forgets the per-device state the
+ * library keeps (the scratch addresses).  Scenes and caches must be destroyed before.  The
  * reference's session ends with its process (cmd.cpp:112-113: save, then return from main); a
  * host that embeds the library calls this before it lets the device go. */
 int nr_shutdown(void);
@@ -132,6 +133,18 @@ int nr_bvh_build_host(const float* p, uint32_t n_tris, int32_t nthreads, int32_t
  * triangle or -1 when the segment is clear. */
 int nr_intersect(nr_scene* s, const float* o, const float* d, uint32_t n, int32_t any_hit, const float* tmax,
                  int32_t* tri, float* t, float* b1, float* b2);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): nr_intersect with every
+ * array in device memory and no allocation at all -- the ray records, hit records, retry list and
+ * counters the traversal needs live in one static per-GPU scratch array (13 MiB, held until the
+ * process ends).  At most NR_INTERSECT_NOALLOC_MAX rays per call (larger n: NR_ERROR).  Results are
+ * those of nr_intersect, bit for bit.  The call runs on the scene's stream and returns after the
+ * results are written.  Calls that overlap on one GPU (several threads or scenes at once) race on
+ * the shared scratch array. */
+#define NR_INTERSECT_NOALLOC_MAX 262144
+int nr_intersect_noalloc(nr_scene* s, const float* o_device, const float* d_device, uint32_t n, int32_t any_hit,
+                         const float* tmax_device, int32_t* tri_device, float* t_device, float* b1_device,
+                         float* b2_device);
 
 /* RendererPTLS::render (rendererptls.cpp:96-236) into the scene's device accumulators.
  * Outputs may be NULL (then fetch later with nr_render_fetch): rgb_linear = mean radiance

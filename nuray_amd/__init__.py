@@ -28,6 +28,7 @@ EXPORTS = [
     "nr_nrc_create", "nr_nrc_destroy", "nr_nrc_set_weights", "nr_nrc_get_weights", "nr_nrc_infer",
     "nr_nrc_train_step", "nr_nrc_gradient", "nr_render_nrc", "nr_nrc_bench", "nr_scene_set_texture_rgb32f",
     "nr_adjoint_ptnee_tex", "nr_scene_set_material_ext", "nr_scene_set_short_stack", "nr_shutdown",
+    "nr_intersect_noalloc",
 ]
 
 NRC_NPARAM = 5 * 4096 + 16 * 64
@@ -80,6 +81,7 @@ def lib():
         L.nr_bvh_build_host.argtypes = [f32p, C.c_uint32, C.c_int32, C.POINTER(C.c_int32), i32p, i32p, f32p,
                                         i32p, i32p, i32p]
         L.nr_intersect.argtypes = [vp, f32p, f32p, C.c_uint32, C.c_int32, vp, i32p, f32p, f32p, f32p]
+        L.nr_intersect_noalloc.argtypes = [vp, vp, vp, C.c_uint32, C.c_int32, vp, vp, vp, vp, vp]
         L.nr_shutdown.argtypes = []
         L.nr_render_ptnee.argtypes = [vp, C.POINTER(nr_camera), C.POINTER(nr_params), vp, vp, vp, vp, vp, vp, vp]
         L.nr_render_fetch.argtypes = [vp, vp, vp, vp, vp]

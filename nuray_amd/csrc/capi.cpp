@@ -4,6 +4,7 @@
 #include <stdexcept>
 #include <string>
 #include "adjoint.h"
+#include "noalloc.h"
 #include "nrc.h"
 #include "render.h"
 #include "scene.h"
@@ -44,6 +45,7 @@ int nr_init(int32_t device) {
     return guard([&] {
         nr::check_hip(hipSetDevice(device), "hipSetDevice");
         nr::check_hip(hipFree(nullptr), "context init");
+        nr::noalloc_setup(device);
         return NR_OK;
     });
 }
@@ -51,6 +53,7 @@ int nr_init(int32_t device) {
 int nr_shutdown(void) {
     return guard([&] {
         nr::check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        nr::noalloc_forget();
         return NR_OK;
     });
 }
@@ -211,6 +214,17 @@ int nr_intersect(nr_scene* s, const float* o, const float* d, uint32_t n, int32_
             b1[i] = h[i].b1;
             b2[i] = h[i].b2;
         }
+        return NR_OK;
+    });
+}
+
+int nr_intersect_noalloc(nr_scene* s, const float* o_device, const float* d_device, uint32_t n, int32_t any_hit,
+                         const float* tmax_device, int32_t* tri_device, float* t_device, float* b1_device,
+                         float* b2_device) {
+    static_assert(NR_INTERSECT_NOALLOC_MAX == nr::NOALLOC_MAX_RAYS, "public and internal ray limits");
+    return guard([&] {
+        nr::intersect_noalloc(*s->sc, o_device, d_device, n, any_hit, tmax_device, tri_device, t_device, b1_device,
+                              b2_device);
         return NR_OK;
     });
 }
