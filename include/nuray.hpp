@@ -4,7 +4,8 @@
 //   Config    src/utils/config.h:7-22      (k=v items, typed getters with defaults)
 //   Camera    src/scene/camera.h:8-34      (Camera(fov, asp, pos, yaw, pitch, roll))
 //   Loader    src/scene/loader.h:8-19      (scene description, OBJ + MTL, loader.cpp:26-361)
-//   Texture   src/scene/texture.cpp:5-31   (8-bit image -> pow(c / 255, 2.2); BMP / PPM here)
+//   Texture   src/scene/texture.cpp:5-31   (8-bit image -> pow(c / 255, 2.2); PNG / BMP / PPM here,
+//                                          link with -lz)
 //   Renderer  src/renderer/renderer.h:15-35 (prepare / render / setEnvmap) on the GPU ptnee path
 //   Image     QImage RGB888 output, saved as BMP (cmd.cpp:112 saves result.bmp)
 // plus RenderFunction, a torch.autograd.Function-style pair forward(ctx, theta) -> image and
@@ -14,7 +15,11 @@
 #ifndef NURAY_HPP
 #define NURAY_HPP
 
+#include <zlib.h>
+
+#include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -28,6 +33,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "nuray.h"
@@ -158,7 +164,7 @@ struct Image {
 struct Texture {
     int w = 0, h = 0;
     std::vector<uint8_t> rgba;
-    // BMP (24 / 32-bit uncompressed) or binary PPM (P6, maxval 255).
+    // PNG, BMP (24 / 32-bit uncompressed) or binary PPM (P6, maxval 255).  JPEG is refused.
     static Texture load(const std::string& path) {
         std::ifstream is(path, std::ios::binary);
         if (!is) throw Error("cannot open texture " + path);
@@ -200,7 +206,116 @@ struct Texture {
             }
             return t;
         }
-        throw Error("unsupported texture format (BMP or binary PPM): " + path);
+        if (d.size() > 8 && std::memcmp(d.data(), "\x89PNG\r\n\x1a\n", 8) == 0) return load_png(d, path);
+        if (d.size() > 2 && d[0] == 0xFF && d[1] == 0xD8)
+            throw Error("JPEG textures are not supported (convert to PNG, BMP or PPM): " + path);
+        throw Error("unsupported texture format (PNG, BMP or binary PPM): " + path);
+    }
+    // PNG (ISO/IEC 15948): IHDR, PLTE, tRNS and the IDAT stream (inflated by zlib); every scanline's
+    // filter (None, Sub, Up, Average, Paeth) undone; greyscale, RGB, palette, grey + alpha and RGBA
+    // at 1-16 bits per sample.  16-bit samples keep their high byte.  Interlaced images are refused.
+    static Texture load_png(const std::vector<uint8_t>& d, const std::string& path) {
+        auto be32 = [&d](size_t o) {
+            return uint32_t(d[o]) << 24 | uint32_t(d[o + 1]) << 16 | uint32_t(d[o + 2]) << 8 | uint32_t(d[o + 3]);
+        };
+        uint32_t W = 0, H = 0;
+        int depth = 0, ctype = -1, interlace = 0;
+        std::vector<uint8_t> z, plte, trns;
+        for (size_t o = 8; o + 12 <= d.size();) {
+            const uint32_t len = be32(o);
+            if (o + 12 + size_t(len) > d.size()) throw Error("truncated PNG " + path);
+            const std::string type(reinterpret_cast<const char*>(&d[o + 4]), 4);
+            const uint8_t* c = &d[o + 8];
+            if (type == "IHDR" && len >= 13) {
+                W = be32(o + 8);
+                H = be32(o + 12);
+                depth = c[8];
+                ctype = c[9];
+                interlace = c[12];
+            } else if (type == "PLTE") {
+                plte.assign(c, c + len);
+            } else if (type == "tRNS") {
+                trns.assign(c, c + len);
+            } else if (type == "IDAT") {
+                z.insert(z.end(), c, c + len);
+            } else if (type == "IEND") {
+                break;
+            }
+            o += 12 + size_t(len);
+        }
+        static const int channels[7] = {1, 0, 3, 1, 2, 0, 4};
+        if (W == 0 || H == 0 || W > (1u << 15) || H > (1u << 15) || ctype < 0 || ctype > 6 || !channels[ctype] ||
+            (depth != 1 && depth != 2 && depth != 4 && depth != 8 && depth != 16) || (ctype == 3 && plte.empty()))
+            throw Error("unsupported PNG header: " + path);
+        if (interlace) throw Error("interlaced PNG textures are not supported: " + path);
+        const int nch = channels[ctype];
+        const size_t bits = size_t(nch) * depth, stride = (size_t(W) * bits + 7) / 8, bpp = std::max<size_t>(1, bits / 8);
+        std::vector<uint8_t> raw(size_t(H) * (stride + 1));
+        z_stream zs{};
+        if (inflateInit(&zs) != Z_OK) throw Error("zlib init failed for " + path);
+        zs.next_in = z.data();
+        zs.avail_in = uInt(z.size());
+        zs.next_out = raw.data();
+        zs.avail_out = uInt(raw.size());
+        const int zr = inflate(&zs, Z_FINISH);
+        const size_t got = raw.size() - zs.avail_out;
+        inflateEnd(&zs);
+        if ((zr != Z_STREAM_END && zr != Z_OK) || got != raw.size()) throw Error("corrupt PNG data in " + path);
+        // undo the filters in place, row by row (the previous row is already unfiltered)
+        for (uint32_t y = 0; y < H; ++y) {
+            uint8_t* row = &raw[size_t(y) * (stride + 1)];
+            const int f = row[0];
+            uint8_t* cur = row + 1;
+            const uint8_t* up = y ? &raw[size_t(y - 1) * (stride + 1) + 1] : nullptr;
+            for (size_t i = 0; i < stride; ++i) {
+                const int a = i >= bpp ? cur[i - bpp] : 0, b = up ? up[i] : 0, cc = (up && i >= bpp) ? up[i - bpp] : 0;
+                int pred = 0;
+                switch (f) {
+                    case 0: pred = 0; break;
+                    case 1: pred = a; break;
+                    case 2: pred = b; break;
+                    case 3: pred = (a + b) >> 1; break;
+                    case 4: {
+                        const int p = a + b - cc, pa = std::abs(p - a), pb = std::abs(p - b), pc = std::abs(p - cc);
+                        pred = (pa <= pb && pa <= pc) ? a : (pb <= pc ? b : cc);
+                        break;
+                    }
+                    default: throw Error("bad PNG filter type in " + path);
+                }
+                cur[i] = uint8_t(cur[i] + pred);
+            }
+        }
+        // one sample (0..255) of channel k of pixel x in an unfiltered row
+        auto sample = [&](const uint8_t* row, uint32_t x, int k) -> int {
+            if (depth == 8) return row[size_t(x) * nch + k];
+            if (depth == 16) return row[(size_t(x) * nch + k) * 2];  // high byte
+            const size_t bit = (size_t(x) * nch + k) * depth;
+            const int v = (row[bit / 8] >> (8 - depth - int(bit % 8))) & ((1 << depth) - 1);
+            return ctype == 3 ? v : v * 255 / ((1 << depth) - 1);
+        };
+        Texture t;
+        t.w = int(W);
+        t.h = int(H);
+        t.rgba.resize(size_t(W) * H * 4);
+        for (uint32_t y = 0; y < H; ++y) {
+            const uint8_t* row = &raw[size_t(y) * (stride + 1) + 1];
+            for (uint32_t x = 0; x < W; ++x) {
+                uint8_t* o = &t.rgba[(size_t(y) * W + x) * 4];
+                if (ctype == 3) {
+                    const int i = sample(row, x, 0);
+                    if (size_t(3 * i + 2) >= plte.size()) throw Error("PNG palette index out of range in " + path);
+                    o[0] = plte[3 * i]; o[1] = plte[3 * i + 1]; o[2] = plte[3 * i + 2];
+                    o[3] = size_t(i) < trns.size() ? trns[i] : 255;
+                } else if (ctype == 0 || ctype == 4) {
+                    o[0] = o[1] = o[2] = uint8_t(sample(row, x, 0));
+                    o[3] = ctype == 4 ? uint8_t(sample(row, x, 1)) : 255;
+                } else {
+                    for (int k = 0; k < 3; ++k) o[k] = uint8_t(sample(row, x, k));
+                    o[3] = ctype == 6 ? uint8_t(sample(row, x, 3)) : 255;
+                }
+            }
+        }
+        return t;
     }
     // Texture::load's decode (texture.cpp:18-20): linear = pow(c / 255, 2.2).
     std::vector<float> linear_rgb() const {
@@ -522,7 +637,8 @@ public:
     }
 
     // Renderer::render (renderer.h:22-29): blocks; progress(fraction) after each wavefront batch;
-    // a zero `con_flag` (set from any thread) cancels, as the reference's con_flag does.
+    // a zero `con_flag` (set from any thread) cancels, as the reference's con_flag does: a watcher
+    // thread raises the render's cancel word, which the render reads between bounces and batches.
     void render(const Camera& cam, Image& img, const Config& cfg, std::function<void(float)> progress = {},
                 std::atomic<int>* con_flag = nullptr) {
         if (!s_) throw Error("render before prepare");
@@ -539,8 +655,22 @@ public:
                 if (*k->f) (*k->f)(x);
                 if (k->flag && k->flag->load() == 0) *k->cancel = 1;
             };
+            std::atomic<bool> done{false};
+            std::thread watcher;
+            if (con_flag)
+                watcher = std::thread([&] {
+                    while (!done.load()) {
+                        if (con_flag->load() == 0) {
+                            cancel = 1;
+                            return;
+                        }
+                        std::this_thread::sleep_for(std::chrono::milliseconds(2));
+                    }
+                });
             const int rc = nr_render_ptnee(s_, &c, &p, img.linear.data(), nullptr, nullptr, img.rgb.data(), cb, &ctx,
                                            &cancel);
+            done.store(true);
+            if (watcher.joinable()) watcher.join();
             check(rc);
             if (rc == NR_CANCELLED) nr_render_fetch(s_, img.linear.data(), nullptr, nullptr, img.rgb.data());
             return;

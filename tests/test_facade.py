@@ -51,7 +51,7 @@ def _parse_obj(path, scale, pos):
 def test_facade_host_checks_and_loader(tmp_path):
     exe = str(tmp_path / "test_facade")
     src = os.path.join(HERE, "cpp", "test_facade.cpp")
-    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-o", exe, src], check=True)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-o", exe, src, "-lz"], check=True)
     dump = str(tmp_path / "dump.bin")
     mirror = os.path.join(os.path.dirname(OBJ), "CornellBox-Mirror.obj")
     r = subprocess.run([exe, OBJ, dump, str(tmp_path / "t.bmp"), mirror], capture_output=True, text=True)

@@ -78,7 +78,7 @@ def test_render_function_backward_vs_central_difference(tmp_path):
     exe = str(tmp_path / "test_autograd")
     src = os.path.join(ROOT, "tests", "cpp", "test_autograd.cpp")
     lib_dir = os.path.join(ROOT, "nuray_amd")
-    subprocess.run(["g++", "-std=c++17", "-O1", "-o", exe, src, f"-L{lib_dir}", "-lnuray", f"-Wl,-rpath,{lib_dir}"],
+    subprocess.run(["g++", "-std=c++17", "-O1", "-o", exe, src, f"-L{lib_dir}", "-lnuray", "-lz", f"-Wl,-rpath,{lib_dir}"],
                    check=True)
     r = subprocess.run([exe, OBJ], capture_output=True, text=True, timeout=120)
     print(r.stdout[-1500:], r.stderr[-500:])

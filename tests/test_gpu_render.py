@@ -133,6 +133,39 @@ def test_progress_and_cancel(nr, scenes):
     assert rc == 1
 
 
+def test_cancel_mid_render_from_another_thread(nr, scenes):
+    """con_flag = 0 from another thread (rendererptls.cpp:150,215,226-230): a second host thread
+    writes the cancel word 0.15 s into a render of 2000 one-sample batches; the call returns
+    NR_CANCELLED soon after, and the sums it leaves are exactly those of the whole batches it
+    completed (equal, bit for bit, to a fresh render of that many samples per pixel)."""
+    import threading
+    import time
+    sc = scenes.box_room()
+    W, H = 320, 240
+    dev = nr.Scene(sc)
+    dev.render(sc["camera"], W, H, 4, batch_spp=1, fetch=False)   # warm-up
+    flag = np.zeros(1, np.int32)
+    t_set = []
+
+    def raise_flag():
+        time.sleep(0.15)
+        t_set.append(time.perf_counter())
+        flag[0] = 1
+    th = threading.Thread(target=raise_flag)
+    th.start()
+    rc = dev.render(sc["camera"], W, H, 2000, seed=9, batch_spp=1, cancel=flag, fetch=False)
+    t_ret = time.perf_counter()
+    th.join()
+    done = int(round(dev.stats()["samples"] / (W * H)))
+    got = dev.fetch(W, H)["sum"].copy()
+    print(f"cancelled after {done} of 2000 samples per pixel, returned {1e3 * (t_ret - t_set[0]):.1f} ms after "
+          f"the flag was raised")
+    assert rc == 1 and 0 < done < 2000
+    assert t_ret - t_set[0] < 0.5
+    ref = dev.render(sc["camera"], W, H, done, seed=9, batch_spp=1)["sum"]
+    assert np.array_equal(got, ref)
+
+
 def test_rgb8_display_transform(nr, scenes):
     """rgb8 = trunc(clamp(pow(mean, 1/2.2) * 255, 0, 255)) (rendererptls.cpp:175-181)."""
     sc = scenes.box_room()
