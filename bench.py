@@ -31,14 +31,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: a wave64 instruction occupies a 16-lane SIMD for 4 cycles, so 256 CUs x 4 SIMDs x
+# 16 lanes x 2.4 GHz = 39.3 T lane-operations per second
+VALU_PEAK_TLANEOPS = 256 * 4 * 16 * 2.4e9 / 1e12
 # algorithmic bytes of one closest-hit ray: 128 B per 4-wide node fetch (four child boxes and
 # references), 48 B per triangle test, 32 B ray in, 16 B hit out
 B_NODE, B_TRI, B_RAY = 128, 48, 48
 
 
-def cpu_baseline(sc, spp_frame, target_s=12.0):
+# crops of the headline frame checked against the FP64 oracle (x0, y0, x1, y1)
+MRE_CROPS = [(100, 60, 148, 108), (600, 300, 648, 348), (1100, 500, 1148, 548), (640, 80, 688, 128)]
+
+
+def cpu_baseline(sc, spp_frame, gpu_mean=None, target_s=12.0):
     """Times the oracle (CPU restatement of RendererPTLS, OpenMP over 8x8 tiles, all host
-    threads) on the same frame: one calibration pass at 1 spp, then k spp sized to ~target_s."""
+    threads) on the same frame: one calibration pass at 1 spp, then k spp sized to ~target_s.
+    With the GPU frame's per-pixel mean (same seed), also its mean relative error against the FP64
+    oracle on four 48 x 48 crops at the frame's spp: mean |gpu - fp64| / (|fp64| + 1e-2), the same
+    random numbers (the oracle consumes the full frame's Philox streams on a crop)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     from nuray_amd.scenes import camera_array
@@ -63,7 +73,19 @@ def cpu_baseline(sc, spp_frame, target_s=12.0):
         o.render(cam, W, H, 2, spp0=1 + k, rect=(0, y, W, y + 1), **dict(kw, nthreads=1))
         one += W * 2
     dt1 = time.perf_counter() - t1
+    mre = None
+    if gpu_mean is not None:
+        sums = (np.zeros(W * H * 3), np.zeros(W * H * 3))
+        for rect in MRE_CROPS:
+            o.render(cam, W, H, spp_frame, rect=rect, sums=sums, fp64=True, **kw)
+        ref = sums[0].reshape(H, W, 3) / spp_frame
+        g = np.concatenate([gpu_mean[y0:y1, x0:x1].reshape(-1) for x0, y0, x1, y1 in MRE_CROPS])
+        r = np.concatenate([ref[y0:y1, x0:x1].reshape(-1) for x0, y0, x1, y1 in MRE_CROPS])
+        mre = {"value": float((np.abs(g - r) / (np.abs(r) + 1e-2)).mean()), "crops": MRE_CROPS,
+               "spp": spp_frame, "definition": "mean over crop pixel channels of |gpu - fp64| / (|fp64| + 1e-2), "
+                                              "same seed (same Philox numbers)"}
     return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "mre_fp64": mre,
             "sample": f"the same 1280x720 frame at {k} spp (samples 1..{k}; {samples} camera samples in {dt:.1f} s)",
             "cpu_model": _cpu_model(), "one_thread_value": one / dt1 / 1e6,
             "one_thread_sample": f"every 8th row of the frame at 2 spp ({one} camera samples in {dt1:.1f} s)"}
@@ -360,15 +382,61 @@ def env_sampling(nr, dev, sc, spp):
             "variance_ratio": v0 / v1, "equal_error_speedup": v0 * t0_ / (v1 * t1)}
 
 
-def measured_traffic():
-    """Calibrated HBM bytes per closest-hit launch from the committed rocprofv3 counter summary."""
-    path = os.path.join(ROOT, "profiles", "trace_closest_720p_traffic.txt")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch"):
-            return float(line.split()[1])
-    return None
+def profile_figures(name):
+    """key -> value lines of a committed rocprofv3 summary under profiles/ (counter runs cannot be
+    taken inside the timed run: a profiler slows the kernels it counts)."""
+    path = os.path.join(ROOT, "profiles", name)
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            parts = line.split()
+            if len(parts) >= 2 and not line.startswith("#"):
+                try:
+                    out[parts[0]] = float(parts[1])
+                except ValueError:
+                    out[parts[0]] = " ".join(parts[1:])
+    return out
+
+
+def traversal_roofline(cnt, closest_ms, launches, steps):
+    """Roofline of the dominant kernel, the closest-hit traversal k_trace<closest>.
+
+    Its counters show it bound by VALU issue, not by HBM: the exactness contract fixes the work
+    (every box the reference's slab test accepts is visited), and the node and triangle bytes are
+    mostly served by the L2s and the Infinity Cache.  So the roofline is VALU issue: useful
+    lane-operations per second (VALU instructions x active lanes) against 39.3 T lane-ops/s.  The
+    lane-operations per ray come from the committed counter run
+    (profiles/trace_closest_720p_lane_ops.txt: SQ_INSTS_VALU x SQ_THREAD_CYCLES_VALU /
+    SQ_ACTIVE_INST_VALU over the same frame's rays); the rays and the launch time from this run.
+    HBM: the measured traffic per launch (FETCH_SIZE / WRITE_SIZE counters calibrated on a copy
+    of known size, profiles/trace_closest_720p_traffic.txt) over this run's launch time.  The
+    no-reuse byte count (128 B per node fetch, 48 B per triangle test and per ray) is reported as
+    bytes_no_reuse_per_launch: it counts cache hits as traffic, so it is not a fraction of HBM peak."""
+    avg_ms = closest_ms / max(launches, 1)
+    rays_launch = cnt["rays"] / max(cnt["closest_launches"], 1)
+    lane = profile_figures("trace_closest_720p_lane_ops.txt")
+    lane_ops_ray = lane.get("lane_ops_per_ray")
+    achieved = lane_ops_ray * rays_launch / (avg_ms * 1e-3) / 1e12 if lane_ops_ray and avg_ms > 0 else None
+    hbm = profile_figures("trace_closest_720p_traffic.txt")
+    traffic = hbm.get("traffic_bytes_per_launch")
+    hbm_gbs = traffic / (avg_ms * 1e-3) / 1e9 if traffic and avg_ms > 0 else None
+    no_reuse = (B_NODE * cnt["node_fetch"] + B_TRI * cnt["tri_tests"] + B_RAY * cnt["rays"]) / max(
+        cnt["closest_launches"], 1)
+    return {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TLANEOPS, "unit": "T lane-ops/s",
+            "frac": achieved / VALU_PEAK_TLANEOPS if achieved else None, "traffic": traffic,
+            "kernel": "k_trace<closest>", "avg_launch_ms": avg_ms, "launches_per_frame": launches / steps,
+            "rays_per_launch": rays_launch, "lane_ops_per_ray": lane_ops_ray,
+            "valu_busy_in_counter_run": lane.get("valu_busy"),
+            "active_lanes_per_valu_inst": lane.get("active_lanes_per_inst"),
+            "lane_ops_source": lane.get("source"),
+            "hbm": {"achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": hbm_gbs / HBM_PEAK_GBS if hbm_gbs else None, "traffic": traffic,
+                    "traffic_source": hbm.get("source")},
+            "bytes_no_reuse_per_launch": no_reuse,
+            "no_reuse_bytes_per_s_over_hbm_peak": no_reuse / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if avg_ms > 0 else None,
+            "rays_per_frame": cnt["rays"], "node_fetch_per_ray": cnt["node_fetch"] / max(cnt["rays"], 1),
+            "tri_tests_per_ray": cnt["tri_tests"] / max(cnt["rays"], 1),
+            "rays_retried_with_full_stack": cnt["retried"]}
 
 
 def main():
@@ -443,28 +511,14 @@ def main():
         launches += st["closest_launches"]
     nr.synchronize()
     elapsed = time.perf_counter() - t0
+    frame_mean = dev.fetch(W, H)["sum"] / spp   # the timed frames' image (same seed every frame)
     samples = W * H * spp * args.steps
     value = samples / elapsed / 1e6
     print(f"# timed {args.steps} frames in {elapsed:.3f} s; per frame: closest-hit {closest_ms / args.steps:.1f} ms, "
           f"shade {shade_ms / args.steps:.1f} ms, shadow {shadow_ms / args.steps:.1f} ms; "
           f"{st['iterations']:.0f} iterations", file=sys.stderr, flush=True)
 
-    bytes_frame = B_NODE * cnt["node_fetch"] + B_TRI * cnt["tri_tests"] + B_RAY * cnt["rays"]
-    achieved = bytes_frame * args.steps / (closest_ms / 1e3) / 1e9 if closest_ms > 0 else None
-    traffic = measured_traffic()
-    avg_ms = closest_ms / max(launches, 1)
-    # achieved counts algorithmic bytes; the L2s and the Infinity Cache serve most of them, so it can
-    # exceed the HBM peak.  hbm_gbs is the measured HBM traffic (rocprofv3 counters) over the same
-    # launch time: the kernel's own share of HBM bandwidth.
-    hbm_gbs = traffic / (avg_ms * 1e-3) / 1e9 if traffic and avg_ms > 0 else None
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
-                "hbm_gbs": hbm_gbs, "hbm_frac": hbm_gbs / HBM_PEAK_GBS if hbm_gbs else None,
-                "kernel": "k_trace<closest>", "bytes_per_launch": bytes_frame / max(cnt["closest_launches"], 1),
-                "avg_launch_ms": avg_ms,
-                "rays_per_frame": cnt["rays"], "node_fetch_per_ray": cnt["node_fetch"] / max(cnt["rays"], 1),
-                "tri_tests_per_ray": cnt["tri_tests"] / max(cnt["rays"], 1),
-                "rays_retried_with_full_stack": cnt["retried"]}
+    roofline = traversal_roofline(cnt, closest_ms, launches, args.steps)
     if args.frames_only:
         args.no_cpu = args.no_nrc = True
     adj = None if args.frames_only else adjoint_iteration(nr, scenes)
@@ -498,7 +552,7 @@ def main():
         for lab, r in (("", nrc), (" with sky sampling", nrc["with_sky_sampling"])):
             print(f"# time to RMSE 1e-2 at 1080p{lab}: PT {r['pt_s']:.1f} s, radiance cache "
                   f"{r['nrc_s'] if r['nrc_s'] is None else round(r['nrc_s'], 1)} s", file=sys.stderr, flush=True)
-    cpu = None if args.no_cpu else cpu_baseline(sc, spp)
+    cpu = None if args.no_cpu else cpu_baseline(sc, spp, frame_mean)
     out = {
         "metric": "Msamples/s at 720p", "value": value, "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
@@ -512,6 +566,7 @@ def main():
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
+        out["mre_fp64"] = cpu["mre_fp64"]
     print(json.dumps(out), flush=True)
 
 

@@ -19,10 +19,11 @@ public:
     int device = 0;
     hipStream_t stream = nullptr;
     DevBuf w32, w16, m, v;     // FP32 master weights, BF16 copy (training), Adam moments
+    DevBuf w16t;               // BF16 weights transposed per layer (row = input): the backward's operand
     DevBuf wema, w16inf;       // FP32 weight average, BF16 inference weights
     DevBuf act, gz;            // training: layer inputs and pre-activation gradients, [6][batch][64] bf16
-    DevBuf part;               // training: per-wave weight-gradient partials
-    DevBuf scal;               // float[4]: loss sum
+    DevBuf part;               // training: per-block weight-gradient partials
+    DevBuf lossp;              // training: summed loss of each 64-sample chunk
     DevBuf q_tmp, t_tmp, o_tmp;  // staging for host-buffer entry points
 
     Nrc(uint32_t seed, const NrcConfig& c);

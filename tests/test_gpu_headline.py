@@ -63,3 +63,18 @@ def test_headline_frame_ztest(headline, scenes):
     print(f"headline frame crops, z-test: {frac3:.4f} of channels beyond 3 sigma, mean z {z.mean():.4f}")
     assert frac3 < 0.0054
     assert abs(float(z.mean())) < 0.06
+
+
+def test_headline_frame_fp64_mre(headline, scenes):
+    """The benchmark's frame (64 spp, seed 7) against the FP64 oracle of the same estimator on the
+    same random numbers, on the crops: mean |gpu - fp64| / (|fp64| + 1e-2) (the figure the
+    benchmark line reports as mre_fp64)."""
+    sc, dev, ref = headline
+    kw = dict(maxdepth=10, rrdepth=3)
+    spp = 64
+    g = dev.render(sc["camera"], sc["width"], sc["height"], spp, seed=7, **kw)
+    s, _ = _oracle_crops(ref, scenes, sc, spp, 7, dict(kw, fp64=True))
+    m_g, m_o = _crop(g["sum"]) / spp, _crop(s) / spp
+    mre = float((np.abs(m_g - m_o) / (np.abs(m_o) + 1e-2)).mean())
+    print(f"headline frame crops: mean relative error against the FP64 oracle {mre:.2e}")
+    assert mre < 5e-3
