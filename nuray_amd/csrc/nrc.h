@@ -12,6 +12,8 @@ struct NrcConfig {
     int max_batch = 1 << 16;   // training samples per step
 };
 
+constexpr int TRAIN_MAX_BLOCKS = 256;  // training blocks per step (one weight-gradient partial each)
+
 class Nrc {
 public:
     NrcConfig cfg;
@@ -21,8 +23,7 @@ public:
     DevBuf w32, w16, m, v;     // FP32 master weights, BF16 copy (training), Adam moments
     DevBuf w16t;               // BF16 weights transposed per layer (row = input): the backward's operand
     DevBuf wema, w16inf;       // FP32 weight average, BF16 inference weights
-    DevBuf act, gz;            // training: layer inputs and pre-activation gradients, [6][batch][64] bf16
-    DevBuf part;               // training: per-block weight-gradient partials
+    DevBuf part;               // training: per-block weight-gradient partials, [6][blocks][4096] fp32
     DevBuf lossp;              // training: summed loss of each 64-sample chunk
     DevBuf q_tmp, t_tmp, o_tmp;  // staging for host-buffer entry points
 
@@ -50,7 +51,7 @@ public:
     double bench(int n, int reps, double* train_ms);
 
 private:
-    int splits_for(int n) const;
+    int train_blocks(int n) const;
 };
 
 // Training targets of recorded path vertices: target = (L_path - L_before) / beta, per channel,

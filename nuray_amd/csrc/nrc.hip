@@ -1,5 +1,5 @@
-// Neural radiance cache kernels for gfx950: inference, training forward + backward, split-K weight
-// gradient and Adam, all matrix work on BF16 MFMA with FP32 accumulation.
+// Neural radiance cache kernels for gfx950: inference, a fused training pass (forward, backward and
+// weight gradient) and Adam, all matrix work on BF16 MFMA with FP32 accumulation.
 //
 // The reference's cache (NeuralRadianceCache / NNNode*, renderernrc.h:17-43, nnnode.cpp:1-208) is a
 // per-sample scalar MLP trained by SGD; this one is the 44 -> 5 x 64 (ReLU) -> 3 network trained
@@ -10,19 +10,16 @@
 // Kernels
 //   k_nrc_infer      persistent; weights in LDS once per block (12 waves: 3 per SIMD, the most
 //                    the 138 VGPRs allow), one 64-sample tile per wave
-//   k_nrc_fwd_bwd    one 64-sample tile per block of 4 waves, each wave 16 features of every layer:
-//                    forward, loss gradient, backward (ReLU masks kept in registers), weights read
-//                    from L2 (W^T for the forward, the transposed copy W for the backward), two
-//                    LDS tiles (18 KB a block, so many blocks share a CU and hide each other's
-//                    latency); writes each layer's input X_l and pre-activation gradient G_l (bf16,
-//                    [sample][64]) for the weight gradient, and the tile's loss to its own slot (no
-//                    atomics: a device-scope atomic per tile on one address serialises the grid
-//                    across the XCDs)
-//   k_nrc_wgrad      dW_l^T = sum_s G_l[s]^T X_l[s]: K = samples, split over (split, layer)
-//                    blocks of 4 waves that prefetch their next 32-sample chunk; the 4 waves'
-//                    partials are summed in LDS, one partial per block (no atomics)
-//   k_nrc_adam       sums the blocks' partials (independent loads, 8 in flight) and applies Adam
-//                    to the FP32 master weights, refreshes the BF16 copies
+//   k_nrc_train      persistent blocks of 8 waves over 64-sample tiles: forward, loss gradient,
+//                    backward (ReLU masks kept in registers) and the weight gradient, with every
+//                    layer's input and gradient tile kept in LDS in both sample-major and
+//                    feature-major layout (nothing per sample goes to global memory); each
+//                    block's weight-gradient partial is written once; the weights come from L2
+//                    (W^T for the forward, the transposed copy W for the backward); per-tile loss
+//                    slots instead of one device-wide atomic
+//   k_nrc_adam       sums the blocks' partials (16 slices per parameter, 8 loads in flight per
+//                    thread) and applies Adam
+//                    to the FP32 master weights, refreshes the BF16 copies (W^T and W)
 #include <cmath>
 #include <cstring>
 #include <type_traits>
@@ -83,29 +80,32 @@ __global__ __launch_bounds__(INF_WAVES * 64) void k_nrc_infer(const __bf16* __re
     }
 }
 
-// The 16 features [16 w, 16 w + 16) of the 64 rows of tile T to global rows [s0, s0 + 64) of a
-// [batch][64] bf16 array (lane = row; two 16-byte stores).
-__device__ __forceinline__ void slice_to_global(const __bf16* T, __bf16* g, int s0, int n, int w, int lane) {
-    const int s = s0 + lane;
-    if (s >= n) return;
-    const uint4* src = reinterpret_cast<const uint4*>(T + lane * NRC_LD + 16 * w);
-    uint4* dst = reinterpret_cast<uint4*>(g + size_t(s) * 64 + 16 * w);
-    dst[0] = src[0];
-    dst[1] = src[1];
-}
+// One training pass over the batch: forward, loss gradient, backward and the weight gradient,
+// fused.  Persistent blocks of 8 waves (two per SIMD); block b takes the 64-sample tiles b, b + B, ...
+//
+// Every activation and gradient tile lives in LDS in two layouts (padded rows of NRC_LD bf16):
+// sample-major ([sample][feature], the K = feature operand of the forward and backward products)
+// and feature-major ([feature][sample], the K = sample operands of the weight gradient).  A
+// 16x16x32 MFMA issued with its operands swapped computes D^T = B^T A^T, which lands in the lane
+// layout of the transposed tile, so the feature-major copy costs one more MFMA per product and a
+// plain 8-byte store.  LDS: X_0..X_5 in both layouts (kept for the backward, whose ReLU masks are
+// read back from them) and the gradients G_l in both layouts (two tiles each): 144 KB, one block
+// per CU.
+//
+// Wave (w, h) = (wave & 3, wave >> 2) computes output features [16 w, 16 w + 16) of samples
+// [32 h, 32 h + 32) of every hidden layer and, in the backward, input features [16 w, 16 w + 16) of
+// the same samples.  It accumulates rows [16 w, 16 w + 16), columns [32 h, 32 h + 32) of every
+// hidden layer's weight gradient dW_l = G_l^T X_l over its tiles in registers (waves h = 0 also
+// the output layer's rows 0..15, columns [16 w, 16 w + 16), and its loss over samples
+// [16 w, 16 w + 16)), and writes them once, at the end, as the block's partial
+// part[(l B + b) 4096 + row 64 + col].  The weights come from global memory (L2-resident, one layer
+// ahead): W^T rows (w16) for the forward, W rows (w16t, kept by Adam) for the backward.  The next
+// tile's records are loaded during the backward.  loss[4 tile + w] = the summed relative L2 loss
+// of samples [16 w, 16 w + 16) of the tile (no atomics).
+constexpr int FT_WAVES = 8;
 
-// Forward, loss gradient and backward of one 64-sample tile per block of 4 waves.  Wave w computes
-// output features [16 w, 16 w + 16) of every layer (the 16-row output layer: wave 0) and, in the
-// backward, input features [16 w, 16 w + 16): exactly the features whose ReLU masks it kept from
-// the forward (a lane's backward results sit at the fragment positions of its forward results).
-// Activations ping-pong between two LDS tiles, one block barrier per layer; the weights come from
-// global memory (L2-resident): W^T rows (w16) as the forward's A operand, W rows (w16t, kept beside
-// them by Adam) as the backward's.  Layer inputs X_l and pre-activation gradients G_l go to global
-// memory for the weight gradient; loss[tile] = the tile's summed relative L2 loss.
-constexpr int FB_WAVES = 4;
-
-// The A operand fragments of one weight layer for lane (r, qd): row `row` of a [64][64] bf16 matrix,
-// k = 8 qd .. 8 qd + 7 and 32 + 8 qd ...
+// The A operand fragments of one weight layer for lane (r, qd): row `row` of a [64][64] bf16
+// matrix, k = 8 qd .. 8 qd + 7 and 32 + 8 qd ...
 struct AFrag {
     bf16x8 a0, a1;
 };
@@ -114,96 +114,137 @@ __device__ __forceinline__ AFrag load_afrag(const __bf16* m, int row, int qd) {
     return AFrag{*reinterpret_cast<const bf16x8*>(p), *reinterpret_cast<const bf16x8*>(p + 32)};
 }
 
-__global__ __launch_bounds__(FB_WAVES * 64) void k_nrc_fwd_bwd(const __bf16* __restrict__ w16,
-                                                               const __bf16* __restrict__ w16t,
-                                                               const NrcQuery* __restrict__ q,
-                                                               const float4* __restrict__ tgt, int n, float inv_n,
-                                                               __bf16* __restrict__ act, __bf16* __restrict__ gz,
-                                                               float* __restrict__ loss) {
-    __shared__ __attribute__((aligned(16))) __bf16 sT[2][NRC_TILE];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, qd = lane >> 4;
-    const int s0 = blockIdx.x * 64;
-    const size_t ls = size_t(n) * 64;
-    // Loads are issued one layer ahead: on gfx950 the vector-memory counter covers stores as well,
-    // so a weight load issued after a layer's activation stores would make its wait cover them too.
-    AFrag nxt = load_afrag(w16, 16 * w + r, qd);
-    float4 tv[4];
-    if (w == 0 && qd == 0) {
+__device__ __forceinline__ bf16x8 lds8(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+__device__ __forceinline__ bf16x4 lds4(const __bf16* p) { return *reinterpret_cast<const bf16x4*>(p); }
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+// Round-to-nearest-even bf16 of two floats as one packed conversion, viewed as int16 pairs.
+// (Written pairwise: the compiler lowers a four-wide conversion one element at a time.)
+__device__ __forceinline__ s16x2 bf16_pair(float a, float b) {
+    return __builtin_bit_cast(s16x2, __builtin_convertvector(f32x2{a, b}, bf16x2));
+}
+__device__ __forceinline__ bf16x4 join_pairs(s16x2 lo, s16x2 hi) {
+    return __builtin_bit_cast(bf16x4, u32x2{__builtin_bit_cast(unsigned, lo), __builtin_bit_cast(unsigned, hi)});
+}
+
+// ReLU of the bf16-rounded values: a bf16 is negative (or -0) exactly when its bits are negative
+// as int16, so the packed integer max with 0 clears those (the same values as rounding max(c, 0)).
+__device__ __forceinline__ bf16x4 relu_bf16(f32x4 c) {
+    return join_pairs(__builtin_elementwise_max(bf16_pair(c[0], c[1]), s16x2{0, 0}),
+                      __builtin_elementwise_max(bf16_pair(c[2], c[3]), s16x2{0, 0}));
+}
+
+// The bf16-rounded c where the (non-negative) activation x is > 0, else 0: min(x, 1) negated is
+// an all-ones or all-zeros 16-bit mask.  (Kept four-wide: applied to one 32-bit half of x at a
+// time, this compiler selects the packed min on the wrong half.)
+__device__ __forceinline__ bf16x4 masked_bf16(f32x4 c, bf16x4 x) {
+    const s16x4 m = -__builtin_elementwise_min(__builtin_bit_cast(s16x4, x), s16x4{1, 1, 1, 1});
+    const bf16x4 g = join_pairs(bf16_pair(c[0], c[1]), bf16_pair(c[2], c[3]));
+    return __builtin_bit_cast(bf16x4, __builtin_bit_cast(s16x4, g) & m);
+}
+
+// Input features 8 g .. 8 g + 7 of one record (g wave-uniform; each case folds to its features).
+template <int F0>
+__device__ __forceinline__ bf16x8 encode8(const NrcQuery& rec, bool valid) {
+    bf16x8 v;
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            const int s = s0 + 16 * nt + r;
-            tv[nt] = s < n ? tgt[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        }
+    for (int j = 0; j < 8; ++j) v[j] = (__bf16)(valid ? nrc_feature(rec, F0 + j) : 0.0f);
+    return v;
+}
+__device__ __forceinline__ bf16x8 encode_group(int g, const NrcQuery& rec, bool valid) {
+    switch (g) {
+        case 0: return encode8<0>(rec, valid);
+        case 1: return encode8<8>(rec, valid);
+        case 2: return encode8<16>(rec, valid);
+        case 3: return encode8<24>(rec, valid);
+        case 4: return encode8<32>(rec, valid);
+        case 5: return encode8<40>(rec, valid);
+        case 6: return encode8<48>(rec, valid);
+        default: return encode8<56>(rec, valid);
     }
-    // encode: wave w writes features [16 w, 16 w + 16) of the lane's sample
-    {
-        const int s = s0 + lane;
-        const bool valid = s < n;
-        const NrcQuery rec = valid ? q[s] : NrcQuery{};
-        bf16x8 v[2];
+}
+
+__global__ __launch_bounds__(FT_WAVES * 64) void k_nrc_train(const __bf16* __restrict__ w16,
+                                                             const __bf16* __restrict__ w16t,
+                                                             const NrcQuery* __restrict__ q,
+                                                             const float4* __restrict__ tgt, int n, float inv_n,
+                                                             float* __restrict__ part, float* __restrict__ loss) {
+    __shared__ __attribute__((aligned(16))) __bf16 sX[6][NRC_TILE];
+    __shared__ __attribute__((aligned(16))) __bf16 sXT[6][NRC_TILE];
+    __shared__ __attribute__((aligned(16))) __bf16 sG[2][NRC_TILE];
+    __shared__ __attribute__((aligned(16))) __bf16 sGT[2][NRC_TILE];
+    const int lane = threadIdx.x & 63, r = lane & 15, qd = lane >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int w = wave & 3, h = wave >> 2;
+    const int tiles = (n + 63) / 64;
+    f32x4 dw[5][2], dw5 = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
+    for (int l = 0; l < 5; ++l)
 #pragma unroll
-            for (int j = 0; j < 8; ++j) v[h][j] = (__bf16)(valid ? nrc_feature(rec, 16 * w + 8 * h + j) : 0.0f);
-        *reinterpret_cast<bf16x8*>(sT[0] + lane * NRC_LD + 16 * w) = v[0];
-        *reinterpret_cast<bf16x8*>(sT[0] + lane * NRC_LD + 16 * w + 8) = v[1];
-        if (valid) {
-            *reinterpret_cast<bf16x8*>(act + size_t(s) * 64 + 16 * w) = v[0];
-            *reinterpret_cast<bf16x8*>(act + size_t(s) * 64 + 16 * w + 8) = v[1];
+        for (int j = 0; j < 2; ++j) dw[l][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    // this lane's record, for the first tile (later tiles: fetched during the previous backward)
+    NrcQuery rec{};
+    auto fetch = [&](int t) {
+        const int s = t * 64 + lane;
+        rec = s < n ? q[s] : NrcQuery{};
+    };
+    fetch(blockIdx.x);
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const int s0 = tile * 64;
+        AFrag nxt = load_afrag(w16, 16 * w + r, qd);
+        float4 tv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // waves h = 0, qd = 0: target of sample 16 w + r
+        if (h == 0 && qd == 0 && s0 + 16 * w + r < n) tv = tgt[s0 + 16 * w + r];
+        // encode: this wave writes features 8 wave .. 8 wave + 7 of sample `lane`, both layouts
+        {
+            const bf16x8 v = encode_group(wave, rec, s0 + lane < n);
+            *reinterpret_cast<bf16x8*>(sX[0] + lane * NRC_LD + 8 * wave) = v;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) sXT[0][(8 * wave + j) * NRC_LD + lane] = v[j];
         }
-    }
-    __syncthreads();
-    // forward; bit (4 nt + i) of mask[l]: output feature 16 w + 4 qd + i of sample 16 nt + r is > 0
-    uint32_t mask[5];
-    int cur = 0;
-#pragma unroll
-    for (int l = 0; l < 5; ++l) {
-        const AFrag A = nxt;
-        // next: layer l + 1 (the output layer's 16 rows after layer 4: wave 0 only, others idle)
-        nxt = load_afrag(w16 + (l + 1) * 4096, l < 4 ? 16 * w + r : r, qd);
-        const __bf16* X = sT[cur];
-        __bf16* Y = sT[cur ^ 1];
-        uint32_t m = 0;
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(X + (16 * nt + r) * NRC_LD + 8 * qd);
-            const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(X + (16 * nt + r) * NRC_LD + 32 + 8 * qd);
-            f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-            c = mfma16(A.a0, b0, c);
-            c = mfma16(A.a1, b1, c);
-            bf16x4 y;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                y[i] = (__bf16)fmaxf(c[i], 0.0f);
-                if ((float)y[i] > 0.0f) m |= 1u << (4 * nt + i);
-            }
-            *reinterpret_cast<bf16x4*>(Y + (16 * nt + r) * NRC_LD + 16 * w + 4 * qd) = y;
-        }
-        mask[l] = m;
-        wave_sync();
-        slice_to_global(Y, act + (l + 1) * ls, s0, n, w, lane);
         __syncthreads();
-        cur ^= 1;
-    }
-    // output layer (rows 0..15 of layer 5, wave 0) and its gradient G_5 (columns 0..2 of the
-    // other tile; columns 3..31 zero: the backward's K of layer 5 spans 32 outputs)
-    const AFrag A5 = nxt;
-    nxt = load_afrag(w16t + 5 * 4096, 16 * w + r, qd);  // the backward's first layer
-    __bf16* G = sT[cur ^ 1];
-    if (w == 0) {
-        const __bf16* X = sT[cur];
-        float loss_acc = 0.0f;
+        // forward: X_{l+1} = ReLU(W_l X_l), both layouts
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(X + (16 * nt + r) * NRC_LD + 8 * qd);
-            const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(X + (16 * nt + r) * NRC_LD + 32 + 8 * qd);
+        for (int l = 0; l < 5; ++l) {
+            const AFrag A = nxt;
+            nxt = load_afrag(w16 + (l + 1) * 4096, l < 4 ? 16 * w + r : r, qd);
+            const __bf16* X = sX[l];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int nt = 2 * h + j;
+                const bf16x8 b0 = lds8(X + (16 * nt + r) * NRC_LD + 8 * qd);
+                const bf16x8 b1 = lds8(X + (16 * nt + r) * NRC_LD + 32 + 8 * qd);
+                f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f}, ct = {0.0f, 0.0f, 0.0f, 0.0f};
+                c = mfma16(A.a0, b0, c);
+                ct = mfma16(b0, A.a0, ct);
+                c = mfma16(A.a1, b1, c);
+                ct = mfma16(b1, A.a1, ct);
+                *reinterpret_cast<bf16x4*>(sX[l + 1] + (16 * nt + r) * NRC_LD + 16 * w + 4 * qd) = relu_bf16(c);
+                *reinterpret_cast<bf16x4*>(sXT[l + 1] + (16 * w + r) * NRC_LD + 16 * nt + 4 * qd) = relu_bf16(ct);
+            }
+            __syncthreads();
+        }
+        // output layer (rows 0..15 of layer 5) on waves h = 0, samples 16 w .. 16 w + 15, and its
+        // gradient G_5: sample-major columns 0..2 of sG[0] (columns 3..31 zero: the backward's K of
+        // layer 5 spans 32 outputs), feature-major rows 0..2 of sGT[0] (rows 3..15 zero)
+        const AFrag A5 = nxt;
+        nxt = load_afrag(w16t + 5 * 4096, 16 * w + r, qd);  // the backward's first layer
+        if (h == 0) {
+            const int nt = w;
+            const bf16x8 b0 = lds8(sX[5] + (16 * nt + r) * NRC_LD + 8 * qd);
+            const bf16x8 b1 = lds8(sX[5] + (16 * nt + r) * NRC_LD + 32 + 8 * qd);
             f32x4 y = {0.0f, 0.0f, 0.0f, 0.0f};
             y = mfma16(A5.a0, b0, y);
             y = mfma16(A5.a1, b1, y);
             const int s = s0 + 16 * nt + r;
+            float loss_acc = 0.0f;
             bf16x4 gv = {(__bf16)0.0f, (__bf16)0.0f, (__bf16)0.0f, (__bf16)0.0f};
             if (qd == 0 && s < n) {
-                const float t3[3] = {tv[nt].x, tv[nt].y, tv[nt].z};
+                const float t3[3] = {tv.x, tv.y, tv.z};
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const float den = y[c] * y[c] + 0.01f, e = y[c] - t3[c];
@@ -212,160 +253,125 @@ __global__ __launch_bounds__(FB_WAVES * 64) void k_nrc_fwd_bwd(const __bf16* __r
                 }
             }
             const bf16x4 z = {(__bf16)0.0f, (__bf16)0.0f, (__bf16)0.0f, (__bf16)0.0f};
-            *reinterpret_cast<bf16x4*>(G + (16 * nt + r) * NRC_LD + 4 * qd) = gv;
-            *reinterpret_cast<bf16x4*>(G + (16 * nt + r) * NRC_LD + 16 + 4 * qd) = z;
-        }
-        for (int off = 32; off > 0; off >>= 1) loss_acc += __shfl_xor(loss_acc, off);
-        if (lane == 0) loss[blockIdx.x] = loss_acc;
-        wave_sync();
-        slice_to_global(G, gz + 5 * ls, s0, n, 0, lane);
-    }
-    __syncthreads();
-    // backward: G_{l-1} = (W_l^T G_l) * [Z_{l-1} > 0] for this wave's 16 input features
-    int gc = cur ^ 1;
+            *reinterpret_cast<bf16x4*>(sG[0] + (16 * nt + r) * NRC_LD + 4 * qd) = gv;
+            *reinterpret_cast<bf16x4*>(sG[0] + (16 * nt + r) * NRC_LD + 16 + 4 * qd) = z;
+            if (qd == 0) {
 #pragma unroll
-    for (int l = 5; l >= 1; --l) {
-        const AFrag A = nxt;
-        if (l > 1) nxt = load_afrag(w16t + (l - 1) * 4096, 16 * w + r, qd);
-        const __bf16* Gl = sT[gc];
-        __bf16* Gp = sT[gc ^ 1];
-        const uint32_t m = mask[l - 1];
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(Gl + (16 * nt + r) * NRC_LD + 8 * qd);
-            f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-            c = mfma16(A.a0, b0, c);
-            if (l < 5) {
-                const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(Gl + (16 * nt + r) * NRC_LD + 32 + 8 * qd);
-                c = mfma16(A.a1, b1, c);
+                for (int c = 0; c < 3; ++c) sGT[0][c * NRC_LD + 16 * nt + r] = gv[c];
             }
-            bf16x4 gv;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) gv[i] = (__bf16)(((m >> (4 * nt + i)) & 1u) ? c[i] : 0.0f);
-            *reinterpret_cast<bf16x4*>(Gp + (16 * nt + r) * NRC_LD + 16 * w + 4 * qd) = gv;
+            if (lane < 26) {  // rows 3..15, columns 16 nt .. 16 nt + 15: two 8-sample stores a row
+                const bf16x8 z8 = {};
+                *reinterpret_cast<bf16x8*>(sGT[0] + (3 + (lane >> 1)) * NRC_LD + 16 * nt + 8 * (lane & 1)) = z8;
+            }
+            for (int off = 32; off > 0; off >>= 1) loss_acc += __shfl_xor(loss_acc, off);
+            if (lane == 0) loss[4 * tile + w] = loss_acc;
         }
-        wave_sync();
-        slice_to_global(Gp, gz + (l - 1) * ls, s0, n, w, lane);
         __syncthreads();
-        gc ^= 1;
-    }
-}
-
-// Weight gradient of layer l over the samples of this wave's chunks of 32 (chunks first, first +
-// stride, ...): acc[mt][nt] += G^T X over the chunk.  The next chunk's 8 KB are loaded into
-// registers while the MFMAs run on the current one (staged through LDS for the transposed reads).
-template <int MT>
-__device__ __forceinline__ void wgrad_body(const __bf16* __restrict__ X, const __bf16* __restrict__ G, int n,
-                                           int first, int stride, __bf16* sG, __bf16* sX, f32x4 (&acc)[MT][4],
-                                           int lane) {
-    const int r = lane & 15, qd = lane >> 4;
+        // backward: dW_l += G_l^T X_l (this wave's 16 rows x 32 columns of layer l), then
+        // G_{l-1} = (W_l^T G_l) * [X_l > 0] for this wave's 16 input features and 32 samples, both
+        // layouts (layer 0 needs G_0 feature-major only)
+        int gc = 0;
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
+        for (int l = 5; l >= 1; --l) {
+            const AFrag A = nxt;
+            if (l > 1) nxt = load_afrag(w16t + (l - 1) * 4096, 16 * w + r, qd);
+            if (l == 2 && tile + int(gridDim.x) < tiles) fetch(tile + gridDim.x);
+            const __bf16* Gl = sG[gc];
+            const __bf16* GTl = sGT[gc];
+            if (l < 5) {
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    // 32 rows of G and X: 256 16-byte pieces each, 4 per lane
-    uint4 g[4], x[4];
-    auto fetch = [&](int c) {
+                for (int ks = 0; ks < 2; ++ks) {
+                    const bf16x8 a = lds8(GTl + (16 * w + r) * NRC_LD + 32 * ks + 8 * qd);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int piece = lane + 64 * k, row = piece >> 3, col = piece & 7;
-            const int s = c * 32 + row;
-            g[k] = make_uint4(0, 0, 0, 0);
-            x[k] = make_uint4(0, 0, 0, 0);
-            if (s < n) {
-                g[k] = reinterpret_cast<const uint4*>(G + size_t(s) * 64)[col];
-                x[k] = reinterpret_cast<const uint4*>(X + size_t(s) * 64)[col];
+                    for (int j = 0; j < 2; ++j)
+                        dw[l][j] = mfma16(a, lds8(sXT[l] + (16 * (2 * h + j) + r) * NRC_LD + 32 * ks + 8 * qd), dw[l][j]);
+                }
+            } else if (h == 0) {
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks)
+                    dw5 = mfma16(lds8(GTl + r * NRC_LD + 32 * ks + 8 * qd),
+                                 lds8(sXT[5] + (16 * w + r) * NRC_LD + 32 * ks + 8 * qd), dw5);
             }
-        }
-    };
-    if (first * 32 < n) fetch(first);
-    for (int c0 = first; c0 * 32 < n; c0 += stride) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int piece = lane + 64 * k, row = piece >> 3, col = piece & 7;
-            *reinterpret_cast<uint4*>(sG + row * NRC_LD + col * 8) = g[k];
-            *reinterpret_cast<uint4*>(sX + row * NRC_LD + col * 8) = x[k];
-        }
-        wave_sync();
-        if ((c0 + stride) * 32 < n) fetch(c0 + stride);
-        bf16x8 b[4];
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt)
-#pragma unroll
-            for (int j = 0; j < 8; ++j) b[nt][j] = sX[(8 * qd + j) * NRC_LD + 16 * nt + r];
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-            bf16x8 a;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) a[j] = sG[(8 * qd + j) * NRC_LD + 16 * mt + r];
-#pragma unroll
-            for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = mfma16(a, b[nt], acc[mt][nt]);
-        }
-        wave_sync();
-    }
-}
-
-// grid (splits, 6), 4 waves per block: block (s, l) sums layer l's gradient over the chunks
-// s * 4 + w + k * 4 S of its waves w and writes it to part[(l * S + s) * 4096 ..] (rows of W^T).
-__global__ __launch_bounds__(256) void k_nrc_wgrad(const __bf16* __restrict__ act, const __bf16* __restrict__ gz,
-                                                   int n, float* __restrict__ part) {
-    __shared__ __attribute__((aligned(16))) __bf16 sG[4][32 * NRC_LD];
-    __shared__ __attribute__((aligned(16))) __bf16 sX[4][32 * NRC_LD];
-    __shared__ float sP[4096];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l = blockIdx.y;
-    const int S = gridDim.x, s = blockIdx.x;
-    const int r = lane & 15, qd = lane >> 4;
-    const size_t ls = size_t(n) * 64;
-    float* dst = part + (size_t(l) * S + s) * 4096;
-    // the four waves' partials are added into sP one wave after the other
-    auto reduce = [&](auto& acc, auto mt_count) {
-        constexpr int MT = decltype(mt_count)::value;
-        for (int w = 0; w < 4; ++w) {
-            if (wave == w) {
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-                    for (int nt = 0; nt < 4; ++nt)
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            float& o = sP[(16 * mt + 4 * qd + i) * 64 + 16 * nt + r];
-                            o = w == 0 ? acc[mt][nt][i] : o + acc[mt][nt][i];
-                        }
+            for (int j = 0; j < 2; ++j) {
+                const int nt = 2 * h + j;
+                const bf16x8 b0 = lds8(Gl + (16 * nt + r) * NRC_LD + 8 * qd);
+                f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f}, ct = {0.0f, 0.0f, 0.0f, 0.0f};
+                if (l > 1) c = mfma16(A.a0, b0, c);
+                ct = mfma16(b0, A.a0, ct);
+                if (l < 5) {
+                    const bf16x8 b1 = lds8(Gl + (16 * nt + r) * NRC_LD + 32 + 8 * qd);
+                    if (l > 1) c = mfma16(A.a1, b1, c);
+                    ct = mfma16(b1, A.a1, ct);
+                }
+                if (l > 1)
+                    *reinterpret_cast<bf16x4*>(sG[gc ^ 1] + (16 * nt + r) * NRC_LD + 16 * w + 4 * qd) =
+                        masked_bf16(c, lds4(sX[l] + (16 * nt + r) * NRC_LD + 16 * w + 4 * qd));
+                *reinterpret_cast<bf16x4*>(sGT[gc ^ 1] + (16 * w + r) * NRC_LD + 16 * nt + 4 * qd) =
+                    masked_bf16(ct, lds4(sXT[l] + (16 * w + r) * NRC_LD + 16 * nt + 4 * qd));
             }
             __syncthreads();
+            gc ^= 1;
         }
-        for (int i = threadIdx.x; i < MT * 16 * 64; i += 256) dst[i] = sP[i];
-    };
-    if (l < 5) {
-        f32x4 acc[4][4];
-        wgrad_body<4>(act + l * ls, gz + l * ls, n, s * 4 + wave, S * 4, sG[wave], sX[wave], acc, lane);
-        reduce(acc, std::integral_constant<int, 4>{});
-    } else {
-        f32x4 acc[1][4];
-        wgrad_body<1>(act + l * ls, gz + l * ls, n, s * 4 + wave, S * 4, sG[wave], sX[wave], acc, lane);
-        reduce(acc, std::integral_constant<int, 1>{});
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const bf16x8 a = lds8(sGT[gc] + (16 * w + r) * NRC_LD + 32 * ks + 8 * qd);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                dw[0][j] = mfma16(a, lds8(sXT[0] + (16 * (2 * h + j) + r) * NRC_LD + 32 * ks + 8 * qd), dw[0][j]);
+        }
+        __syncthreads();  // the next tile overwrites X_0 and the gradient tiles
+    }
+    // this block's partial: hidden layers rows 16 w + 4 qd + i, columns 16 (2 h + j) + r; the
+    // output layer (waves h = 0) rows 4 qd + i, columns 16 w + r
+    const int B = gridDim.x;
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+        float* dst = part + (size_t(l) * B + blockIdx.x) * 4096;
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) dst[(16 * w + 4 * qd + i) * 64 + 16 * (2 * h + j) + r] = dw[l][j][i];
+    }
+    if (h == 0) {
+        float* dst = part + (size_t(5) * B + blockIdx.x) * 4096;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dst[(4 * qd + i) * 64 + 16 * w + r] = dw5[i];
     }
 }
 
-__global__ void k_nrc_adam(float* __restrict__ w32, __bf16* __restrict__ w16, __bf16* __restrict__ w16t,
-                           float* __restrict__ m,
-                           float* __restrict__ v, float* __restrict__ wema, __bf16* __restrict__ w16inf,
-                           const float* __restrict__ part, int P, float lr, float b1, float b2, float eps, float c1,
-                           float c2, float ema, float ce, float* __restrict__ grad_out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= NRC_NPARAM) return;
+// Adam over the summed partials: a block takes 64 consecutive parameters and sums their P
+// partials in 16 interleaved slices (coalesced 256-byte rows, 8 independent loads in flight per
+// thread, 16 waves a block: enough reads in flight to stream the partials), then one thread per
+// parameter adds the slices in a fixed order and applies the update.
+constexpr int ADAM_PARAMS = 64, ADAM_SLICES = 16;
+
+__global__ __launch_bounds__(ADAM_PARAMS* ADAM_SLICES) void k_nrc_adam(
+    float* __restrict__ w32, __bf16* __restrict__ w16, __bf16* __restrict__ w16t, float* __restrict__ m,
+    float* __restrict__ v, float* __restrict__ wema, __bf16* __restrict__ w16inf, const float* __restrict__ part, int P,
+    float lr, float b1, float b2, float eps, float c1, float c2, float ema, float ce, float* __restrict__ grad_out) {
+    __shared__ float red[ADAM_SLICES - 1][ADAM_PARAMS];
+    const int t = threadIdx.x % ADAM_PARAMS, sl = threadIdx.x / ADAM_PARAMS;
+    const int i = blockIdx.x * ADAM_PARAMS + t;
     const int l = i < 5 * 4096 ? i / 4096 : 5, k = i - l * 4096;
-    const float* src = part + size_t(l) * P * 4096 + k;
     float g = 0.0f;
-    int p = 0;
-    for (; p + 8 <= P; p += 8) {
-        float t[8];
+    if (i < NRC_NPARAM) {
+        const float* src = part + size_t(l) * P * 4096 + k;
+        int p = sl;
+        for (; p + 7 * ADAM_SLICES < P; p += 8 * ADAM_SLICES) {
+            float tt[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) t[j] = src[size_t(p + j) * 4096];
+            for (int j = 0; j < 8; ++j) tt[j] = src[size_t(p + j * ADAM_SLICES) * 4096];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) g += t[j];
+            for (int j = 0; j < 8; ++j) g += tt[j];
+        }
+        for (; p < P; p += ADAM_SLICES) g += src[size_t(p) * 4096];
     }
-    for (; p < P; ++p) g += src[size_t(p) * 4096];
+    if (sl > 0) red[sl - 1][t] = g;
+    __syncthreads();
+    if (sl > 0 || i >= NRC_NPARAM) return;
+#pragma unroll
+    for (int j = 0; j < ADAM_SLICES - 1; ++j) g += red[j][t];
     if (grad_out) {
         grad_out[i] = g;
         return;
@@ -513,10 +519,8 @@ Nrc::Nrc(uint32_t seed, const NrcConfig& c) : cfg(c) {
     v.ensure(sizeof(float) * NRC_NPARAM);
     wema.ensure(sizeof(float) * NRC_NPARAM);
     w16inf.ensure(sizeof(__bf16) * NRC_NPARAM);
-    act.ensure(sizeof(__bf16) * 6 * 64 * size_t(cfg.max_batch));
-    gz.ensure(sizeof(__bf16) * 6 * 64 * size_t(cfg.max_batch));
-    part.ensure(sizeof(float) * 6 * size_t(64) * 4096);  // at most 64 splits (blocks) per layer
-    lossp.ensure(sizeof(float) * size_t((cfg.max_batch + 63) / 64));
+    part.ensure(sizeof(float) * 6 * size_t(TRAIN_MAX_BLOCKS) * 4096);  // one partial per training block
+    lossp.ensure(sizeof(float) * 4 * size_t((cfg.max_batch + 63) / 64));  // 4 slots a 64-sample tile
     init_weights(seed);
 }
 
@@ -577,10 +581,11 @@ void Nrc::infer_accumulate(const NrcQuery* q, const int* count, int max_n, float
     check_hip(hipGetLastError(), "k_nrc_infer<accumulate>");
 }
 
-int Nrc::splits_for(int n) const {
-    // blocks per layer: at least 2 chunks of 32 samples per wave, at most 64 (6 x 64 = 384 blocks)
-    const int chunks = (n + 31) / 32;
-    return std::max(1, std::min(64, chunks / 8));
+int Nrc::train_blocks(int n) const {
+    // two 64-sample tiles per block, at most one block per CU (144 KB of LDS): each block keeps a
+    // full 84 KB weight-gradient partial in registers and writes it once
+    const int tiles = (n + 63) / 64;
+    return std::max(1, std::min(TRAIN_MAX_BLOCKS, (tiles + 1) / 2));
 }
 
 double Nrc::train(const NrcQuery* q, const float4* target, int n, bool want_loss, float* grad_out, hipStream_t s) {
@@ -588,12 +593,10 @@ double Nrc::train(const NrcQuery* q, const float4* target, int n, bool want_loss
     if (n <= 0) return 0.0;
     if (n > cfg.max_batch) throw std::runtime_error("nrc: training batch larger than max_batch");
     const int chunks = (n + 63) / 64;
-    k_nrc_fwd_bwd<<<chunks, FB_WAVES * 64, 0, s>>>(w16.as<__bf16>(), w16t.as<__bf16>(), q, target, n, 1.0f / float(n),
-                                                  act.as<__bf16>(), gz.as<__bf16>(), lossp.as<float>());
-    check_hip(hipGetLastError(), "k_nrc_fwd_bwd");
-    const int splits = splits_for(n);
-    k_nrc_wgrad<<<dim3(splits, 6), 256, 0, s>>>(act.as<__bf16>(), gz.as<__bf16>(), n, part.as<float>());
-    check_hip(hipGetLastError(), "k_nrc_wgrad");
+    const int blocks = train_blocks(n);
+    k_nrc_train<<<blocks, FT_WAVES * 64, 0, s>>>(w16.as<__bf16>(), w16t.as<__bf16>(), q, target, n, 1.0f / float(n),
+                                                 part.as<float>(), lossp.as<float>());
+    check_hip(hipGetLastError(), "k_nrc_train");
     DevBuf gtmp;
     float* gdev = nullptr;
     if (grad_out) {
@@ -605,10 +608,10 @@ double Nrc::train(const NrcQuery* q, const float4* target, int n, bool want_loss
     const float c1 = 1.0f - std::pow(cfg.beta1, float(std::max(step, 1)));
     const float c2 = 1.0f - std::pow(cfg.beta2, float(std::max(step, 1)));
     const float ce = 1.0f - std::pow(cfg.ema, float(std::max(step, 1)));
-    k_nrc_adam<<<(NRC_NPARAM + 255) / 256, 256, 0, s>>>(w32.as<float>(), w16.as<__bf16>(), w16t.as<__bf16>(),
+    k_nrc_adam<<<(NRC_NPARAM + ADAM_PARAMS - 1) / ADAM_PARAMS, ADAM_PARAMS * ADAM_SLICES, 0, s>>>(w32.as<float>(), w16.as<__bf16>(), w16t.as<__bf16>(),
                                                             m.as<float>(),
                                                             v.as<float>(), wema.as<float>(), w16inf.as<__bf16>(),
-                                                            part.as<float>(), splits, cfg.lr, cfg.beta1,
+                                                            part.as<float>(), blocks, cfg.lr, cfg.beta1,
                                                             cfg.beta2, cfg.eps, c1, c2, cfg.ema, ce, gdev);
     check_hip(hipGetLastError(), "k_nrc_adam");
     double loss = 0.0;
@@ -616,8 +619,8 @@ double Nrc::train(const NrcQuery* q, const float4* target, int n, bool want_loss
         check_hip(hipMemcpyAsync(grad_out, gdev, sizeof(float) * NRC_NPARAM, hipMemcpyDeviceToHost, s), "grad");
     }
     if (want_loss || grad_out) {
-        std::vector<float> lp(chunks);
-        check_hip(hipMemcpyAsync(lp.data(), lossp.ptr, sizeof(float) * chunks, hipMemcpyDeviceToHost, s), "loss");
+        std::vector<float> lp(4 * size_t(chunks));
+        check_hip(hipMemcpyAsync(lp.data(), lossp.ptr, sizeof(float) * lp.size(), hipMemcpyDeviceToHost, s), "loss");
         check_hip(hipStreamSynchronize(s), "nrc train");
         for (float x : lp) loss += double(x);
         loss /= n;
