@@ -449,6 +449,9 @@ def main():
     ap.add_argument("--no-nrc", action="store_true", help="skip the time-to-RMSE measurement (about 40 s)")
     ap.add_argument("--frames-only", action="store_true",
                     help="only the ptnee frames (no adjoint, cache or CPU legs), e.g. under a profiler")
+    ap.add_argument("--trace-order", type=int, default=0,
+                    help="order of the wavefront trace launches: 0 queue, 1 octant per 1024 rays, 2 64 direction "
+                         "bins per 4096 rays (same results)")
     ap.add_argument("--large", action="store_true",
                     help="only measure the large textured scene (3840x2160 x 256 spp, texel adjoint) and exit")
     ap.add_argument("--mlp-only", action="store_true",
@@ -486,6 +489,7 @@ def main():
     t0 = time.perf_counter()
     dev = nr.Scene(sc)
     prep_s = time.perf_counter() - t0
+    dev.set_trace_order(args.trace_order)
     info = dev.info()
     rp = dict(spp=spp, seed=7, maxdepth=10, rrdepth=3, prr=0.8, fetch=False)
     print(f"# scene {sc['name']}: {len(sc['p'])} triangles, BVH {info['n_nodes']} nodes depth {info['depth']}, "
@@ -559,7 +563,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"ptnee {W}x{H}x{spp}spp maxdepth10 rrdepth3 prr0.8, {sc['name']} "
                                f"({len(sc['p'])} triangles) + 1024x512 sky", "global_batch": W * H * spp,
-                   "seq_len": None, "parallelism": "single-gpu", "bvh_build_ms": info["build_ms"]},
+                   "seq_len": None, "parallelism": "single-gpu", "bvh_build_ms": info["build_ms"],
+                   "trace_order": args.trace_order},
         "roofline": roofline, "cpu_baseline": cpu, "hbm_copy_gbs": copy_gbs,
         "adjoint": adj, "nrc": nrc, "nrc_mlp": mlp, "env_sampling": env, "reference_scene": ref_scene,
         "render_modes": modes,

@@ -27,7 +27,7 @@ EXPORTS = [
     "nr_set_profiling", "nr_bench_copy", "nr_scene_set_params", "nr_loss_grad_l2", "nr_adjoint_ptnee",
     "nr_nrc_create", "nr_nrc_destroy", "nr_nrc_set_weights", "nr_nrc_get_weights", "nr_nrc_infer",
     "nr_nrc_train_step", "nr_nrc_gradient", "nr_render_nrc", "nr_nrc_bench", "nr_scene_set_texture_rgb32f",
-    "nr_adjoint_ptnee_tex", "nr_scene_set_material_ext", "nr_scene_set_short_stack", "nr_shutdown",
+    "nr_adjoint_ptnee_tex", "nr_scene_set_material_ext", "nr_scene_set_short_stack", "nr_scene_set_trace_order", "nr_shutdown",
     "nr_intersect_noalloc",
 ]
 
@@ -88,6 +88,7 @@ def lib():
         L.nr_render_stats.argtypes = [vp, f64p]
         L.nr_set_profiling.argtypes = [vp, C.c_int32]
         L.nr_scene_set_short_stack.argtypes = [vp, C.c_int32]
+        L.nr_scene_set_trace_order.argtypes = [vp, C.c_int32]
         L.nr_bench_copy.argtypes = [C.c_uint64, C.c_int32, C.POINTER(C.c_double)]
         L.nr_scene_set_params.argtypes = [vp, f32p, C.c_uint32]
         L.nr_loss_grad_l2.argtypes = [vp, f32p, C.POINTER(C.c_double), vp]
@@ -314,6 +315,11 @@ class Scene:
     def set_short_stack(self, entries):
         """LDS stack entries of the first trace launch (rays needing more are traced again)."""
         check(lib().nr_scene_set_short_stack(self.h, entries))
+
+    def set_trace_order(self, mode):
+        """Order the wavefront's trace launches take rays in: 0 queue order, 1 direction octant per
+        1024 rays, 2 sixty-four direction bins per 4096 rays (results are the same)."""
+        check(lib().nr_scene_set_trace_order(self.h, mode))
 
     def stats(self):
         out = np.zeros(16, np.float64)

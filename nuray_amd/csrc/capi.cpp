@@ -283,6 +283,14 @@ int nr_scene_set_short_stack(nr_scene* s, int32_t entries) {
     });
 }
 
+int nr_scene_set_trace_order(nr_scene* s, int32_t mode) {
+    return guard([&] {
+        if (mode < 0 || mode > 2) throw std::runtime_error("trace order must be 0, 1 or 2");
+        s->sc->trace_order = mode;
+        return NR_OK;
+    });
+}
+
 int nr_scene_set_params(nr_scene* s, const float* theta, uint32_t n_mats) {
     return guard([&] {
         nr::set_material_params(*s->sc, theta, int(n_mats));

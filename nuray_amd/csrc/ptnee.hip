@@ -670,6 +670,7 @@ int wavefront_batch(Scene& sc, RenderTarget& t, const CamView& cam, const Render
     const int npix = p.W * p.H;
     int* cnt = t.counters.as<int>();
     const int grid_shade = device_cu_count() * 8;
+    if (sc.trace_order) t.order.ensure(sizeof(int) * 3 * t.cap);  // ray queue (cap), shadow queue (2 cap)
     hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, int(C_RAY0), P, -1);
     hipLaunchKernelGGL(k_raygen, dim3((P + BLK - 1) / BLK), dim3(BLK), 0, s, cam, p.W, p.H, npix, P, p.seed,
                        sample0, t.rays[0].as<Ray>(), t.beta[0].as<float4>(),
@@ -677,9 +678,12 @@ int wavefront_batch(Scene& sc, RenderTarget& t, const CamView& cam, const Render
     for (int it = 0;; ++it) {
         const int cur = it & 1, nxt = cur ^ 1;
         hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, -1, 0, int(C_RAY0 + nxt));
+        // optional direction grouping of secondary and shadow rays (primary rays: pixel order)
+        int* const ord = sc.trace_order ? t.order.as<int>() : nullptr;
         mark_event(ev_c, s);
+        if (ord && it > 0) sort_by_direction(t.rays[cur].as<Ray>(), cnt + C_RAY0 + cur, P, sc.trace_order, ord, s);
         launch_closest(sc.view, t.rays[cur].as<Ray>(), cnt + C_RAY0 + cur, t.hits.as<HitRec>(), cnt + C_WORK0,
-                       cnt + C_ERR, ts, s, t.retry.as<int>());
+                       cnt + C_ERR, ts, s, t.retry.as<int>(), (ord && it > 0) ? ord : nullptr);
         mark_event(ev_c, s);
         ShadeArgs a{sc.view, npix, p.seed, sample0, p.prr, p.mis, p.maxdepth, p.rrdepth,
                     p.integrator == 1 ? 0 : 1,
@@ -698,9 +702,10 @@ int wavefront_batch(Scene& sc, RenderTarget& t, const CamView& cam, const Render
         }
         mark_event(ev_h, s);
         mark_event(ev_s, s);
+        if (ord) sort_by_direction(t.shadow.as<Ray>(), cnt + C_SHADOW, env ? 2 * P : P, sc.trace_order, ord + t.cap, s);
         launch_shadow(sc.view, t.shadow.as<Ray>(), cnt + C_SHADOW, nullptr, nullptr, t.contrib.as<float4>(),
                       t.radiance.as<float4>(), cnt + C_WORK1, cnt + C_ERR, ts ? ts + 1 : nullptr, s,
-                      t.retry.as<int>(), vrec);
+                      t.retry.as<int>(), vrec, ord ? ord + t.cap : nullptr);
         mark_event(ev_s, s);
         ++iterations;
         if (p.maxdepth > 0) {

@@ -19,8 +19,10 @@ struct TraceStats {
 // full-depth stack at once.  A ray that overflows the full-depth stack sets *err.
 //
 // Closest hit for rays[0 .. *n): writes hits[i].
+// With `order`, rays are taken in the order rays[order[0]], rays[order[1]], ... (a permutation of
+// 0 .. *n - 1); results still go to the rays' own indices.
 void launch_closest(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, int* work, int* err,
-                    TraceStats* stats, hipStream_t s, int* retry = nullptr);
+                    TraceStats* stats, hipStream_t s, int* retry = nullptr, const int* order = nullptr);
 // Shadow test for rays[0 .. *n) with segment length rays[i].o.w: occluded iff some accepted
 // triangle has (double)t + 2e-3 <= (double)len (rendererptls.cpp:70-71).  Writes occ[i] (0/1)
 // and the first occluder found (prim / t) when `hits` is non-null; when `contrib` is non-null
@@ -28,7 +30,13 @@ void launch_closest(const SceneView& sv, const Ray* rays, const int* n, HitRec* 
 // sets VREC_VISIBLE in the vertex record vrec[bits of contrib[i].w] (when that index is >= 0).
 void launch_shadow(const SceneView& sv, const Ray* rays, const int* n, uint8_t* occ, HitRec* hits,
                    const float4* contrib, float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s,
-                   int* retry = nullptr, HitRec* vrec = nullptr);
+                   int* retry = nullptr, HitRec* vrec = nullptr, const int* order = nullptr);
+
+// order[0 .. *n): queue positions grouped by ray direction inside chunks of consecutive
+// positions, so that the rays a wave traces together point the same way.  mode 1: the 8 octants
+// of rays[i].d within chunks of 1024; mode 2: 64 bins (each component quantised to 4) within
+// chunks of 4096.  `max_n` bounds *n (sizes the grid).
+void sort_by_direction(const Ray* rays, const int* n, int max_n, int mode, int* order, hipStream_t s);
 
 int device_cu_count();
 // Largest dynamic LDS allocation one workgroup may request on the current device, in bytes.

@@ -123,7 +123,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
             // ---- finished: write the result, go idle
             if ((ts.sp == 0 && !ts.has_leaf) || done || ts.overflow) {
                 if (ts.overflow && retry) {  // traced again from the start with the full stack
-                    retry[atomicAdd(n_retry, 1)] = i;
+                    retry[atomicAdd(n_retry, 1)] = idx ? idx[i] : i;
                     if (STATS) atomicAdd(&stats->retried, 1ull);
                     i = -1;
                     continue;
@@ -159,6 +159,52 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     }
 }
 
+constexpr int SORT_BLOCK = 256, SORT_BINS = 64;
+
+template <bool OCT>
+__device__ __forceinline__ int dir_bin(float4 d) {
+    if (OCT) return (d.x < 0.0f ? 4 : 0) | (d.y < 0.0f ? 2 : 0) | (d.z < 0.0f ? 1 : 0);
+    auto q = [](float x) { return min(3, max(0, int((x + 1.0f) * 2.0f))); };
+    return q(d.x) * 16 + q(d.y) * 4 + q(d.z);
+}
+
+// One chunk of CHUNK queue positions per block: bin counts in LDS, an exclusive scan, then
+// every position is written to its bin's next free place (the order inside a bin is whatever the
+// LDS atomics give: the traversal's results do not depend on the order rays are traced in).
+template <int CHUNK, bool OCT>
+__global__ __launch_bounds__(SORT_BLOCK) void k_sort_dir(const Ray* __restrict__ rays, const int* __restrict__ n_ptr,
+                                                        int* __restrict__ order) {
+    __shared__ int bins[SORT_BINS];
+    const int n = *n_ptr;
+    const int c0 = blockIdx.x * CHUNK;
+    if (c0 >= n) return;
+    const int m = min(CHUNK, n - c0);
+    if (threadIdx.x < SORT_BINS) bins[threadIdx.x] = 0;
+    __syncthreads();
+    constexpr int PER = CHUNK / SORT_BLOCK;
+    int key[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int j = int(threadIdx.x) + k * SORT_BLOCK;
+        key[k] = j < m ? dir_bin<OCT>(rays[c0 + j].d) : -1;
+        if (key[k] >= 0) atomicAdd(&bins[key[k]], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {  // exclusive scan of the 64 bin counts by wave 0
+        const int v = bins[threadIdx.x];
+        int incl = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int u = __shfl_up(incl, off, 64);
+            if (int(threadIdx.x) >= off) incl += u;
+        }
+        bins[threadIdx.x] = incl - v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+        if (key[k] >= 0) order[c0 + atomicAdd(&bins[key[k]], 1)] = c0 + int(threadIdx.x) + k * SORT_BLOCK;
+}
+
 template <bool ANY, bool STATS>
 int grid_for(size_t shmem) {
     static int cached_shmem = -1, cached_blocks = 0;
@@ -175,7 +221,8 @@ int grid_for(size_t shmem) {
 
 template <bool ANY>
 void launch(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, uint8_t* occ, const float4* contrib,
-            float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s, int* retry, HitRec* vrec) {
+            float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s, int* retry, HitRec* vrec,
+            const int* order) {
     const int full = sv.wide_stack_cap;
     const size_t shmem_full = size_t(full + 3) * BLOCK * sizeof(int);  // + the spare push slots
     if (shmem_full > device_max_lds()) throw std::runtime_error("BVH too deep for the LDS traversal stack");
@@ -184,10 +231,10 @@ void launch(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, ui
     const size_t shmem = size_t(cap + 3) * BLOCK * sizeof(int);
     if (stats) {
         hipLaunchKernelGGL((k_trace<ANY, true, false>), dim3(grid_for<ANY, true>(shmem)), dim3(BLOCK), shmem, s, sv, rays, n,
-                           nullptr, hits, occ, contrib, radiance, work, err, rl, work + 1, cap, stats, vrec);
+                           order, hits, occ, contrib, radiance, work, err, rl, work + 1, cap, stats, vrec);
     } else {
         hipLaunchKernelGGL((k_trace<ANY, false, false>), dim3(grid_for<ANY, false>(shmem)), dim3(BLOCK), shmem, s, sv, rays,
-                           n, nullptr, hits, occ, contrib, radiance, work, err, rl, work + 1, cap, stats, vrec);
+                           n, order, hits, occ, contrib, radiance, work, err, rl, work + 1, cap, stats, vrec);
     }
     check_hip(hipGetLastError(), "k_trace launch");
     if (!rl) return;
@@ -226,14 +273,23 @@ int device_cu_count() {
 }
 
 void launch_closest(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, int* work, int* err,
-                    TraceStats* stats, hipStream_t s, int* retry) {
-    launch<false>(sv, rays, n, hits, nullptr, nullptr, nullptr, work, err, stats, s, retry, nullptr);
+                    TraceStats* stats, hipStream_t s, int* retry, const int* order) {
+    launch<false>(sv, rays, n, hits, nullptr, nullptr, nullptr, work, err, stats, s, retry, nullptr, order);
 }
 
 void launch_shadow(const SceneView& sv, const Ray* rays, const int* n, uint8_t* occ, HitRec* hits,
                    const float4* contrib, float4* radiance, int* work, int* err, TraceStats* stats, hipStream_t s,
-                   int* retry, HitRec* vrec) {
-    launch<true>(sv, rays, n, hits, occ, contrib, radiance, work, err, stats, s, retry, vrec);
+                   int* retry, HitRec* vrec, const int* order) {
+    launch<true>(sv, rays, n, hits, occ, contrib, radiance, work, err, stats, s, retry, vrec, order);
+}
+
+void sort_by_direction(const Ray* rays, const int* n, int max_n, int mode, int* order, hipStream_t s) {
+    if (max_n <= 0) return;
+    if (mode == 1)
+        hipLaunchKernelGGL((k_sort_dir<1024, true>), dim3((max_n + 1023) / 1024), dim3(SORT_BLOCK), 0, s, rays, n, order);
+    else
+        hipLaunchKernelGGL((k_sort_dir<4096, false>), dim3((max_n + 4095) / 4096), dim3(SORT_BLOCK), 0, s, rays, n, order);
+    check_hip(hipGetLastError(), "k_sort_dir");
 }
 
 }  // namespace nr

@@ -100,6 +100,26 @@ def test_short_stack_retry_is_bit_exact(entries, nr, po, scenes):
     assert np.array_equal(a["sum"], b["sum"]) and np.array_equal(a["sum2"], b["sum2"])
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_trace_order_is_bit_exact(mode, nr, scenes):
+    """Secondary and shadow rays traced in direction-sorted order give the queue-order render bit
+    for bit (the winner among equal t is fixed by leaf order, not by visit order), also when most
+    rays overflow a 1-entry first-launch stack and go through the retry list."""
+    sc = SCENES["terrain64"](scenes)
+    dev = nr.Scene(sc)
+    out = []
+    for order, stack in ((0, 16), (mode, 16), (mode, 1)):
+        dev.set_trace_order(order)
+        dev.set_short_stack(stack)
+        out.append(dev.render(sc["camera"], 96, 54, 4, seed=5, maxdepth=6, rrdepth=2))
+    with pytest.raises(RuntimeError):
+        dev.set_trace_order(3)
+    dev.close()
+    print(f"trace order {mode}: {len(out)} renders")
+    for r in out[1:]:
+        assert np.array_equal(r["sum"], out[0]["sum"]) and np.array_equal(r["sum2"], out[0]["sum2"])
+
+
 def test_deep_chain_tree(nr, po, scenes):
     """100 coincident triangles stacked in the room make a 100-level chain in the reference's SAH
     tree (all splits cost the same, the first one wins).  Rays that reach the stack overflow the
