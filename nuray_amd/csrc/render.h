@@ -61,7 +61,7 @@ struct RenderTarget {
     DevBuf hits, shadow, contrib;      // closest hits; shadow rays and their NEE contributions
     DevBuf radiance;                   // float4 per path slot (x2 with sky sampling: sky rays' region)
     DevBuf skyw;                       // sky sampling: BSDF-escape MIS weight per path slot
-    DevBuf retry;                      // rays a trace launch hands to its full-stack retry launch
+    DevBuf retry, retry_sh;            // rays the closest-hit / shadow launch hands to its full-stack retry launch
     DevBuf order;                      // traversal order of the ray queue, then of the shadow queue
     DevBuf counters;                   // int[16]
     DevBuf stats;                      // TraceStats[2]

@@ -62,10 +62,16 @@ Scene::Scene(const float* P, const float* N, const float* UV, const int32_t* M, 
     }
     check_hip(hipGetDevice(&device), "hipGetDevice");
     check_hip(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    check_hip(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking), "hipStreamCreate");
+    check_hip(hipEventCreateWithFlags(&shadow_ready, hipEventDisableTiming), "hipEventCreate");
+    check_hip(hipEventCreateWithFlags(&shadow_done, hipEventDisableTiming), "hipEventCreate");
 }
 
 Scene::~Scene() {
     delete target;
+    if (shadow_ready) (void)hipEventDestroy(shadow_ready);
+    if (shadow_done) (void)hipEventDestroy(shadow_done);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
 }
 

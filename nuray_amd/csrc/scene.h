@@ -61,6 +61,10 @@ public:
     std::vector<float4> env_tab;  // environment sampler cells (SceneView::env_tab)
     SceneView view{};
     hipStream_t stream = nullptr;
+    // a wavefront bounce's shadow launch runs here, beside the next bounce's closest-hit launch
+    // on `stream`; shadow_ready / shadow_done order it after the shade and before the next shade
+    hipStream_t stream2 = nullptr;
+    hipEvent_t shadow_ready = nullptr, shadow_done = nullptr;
     int device = 0;
     RenderTarget* target = nullptr;  // lazily created render buffers
 
