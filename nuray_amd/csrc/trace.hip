@@ -63,6 +63,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     int* const stk = stk_all + tid;
     const unsigned long long below = (1ull << lane) - 1ull;
     const int n = *n_ptr;
+    if (n <= 0) return;  // nothing queued (the retry launch, most of the time): no claims at all
     const int n_waves = int(gridDim.x) * (BLOCK / 64);
     const int claim = max(1, min(CLAIM, (n + n_waves - 1) / n_waves));
 
