@@ -116,3 +116,71 @@ def test_large_render_fp64_mre(large, scenes):
     print(f"9.19 M-triangle scene crop: mean relative error against the FP64 oracle {mre:.2e} "
           f"(means {gc.mean():.5f} / {mc.mean():.5f})")
     assert mre < 3e-2
+
+
+def test_large_full_size_frame(large, scenes):
+    """The scene's own frame size, 3840 x 2160, at 4 spp (maxdepth 12, RR from vertex 3) as one
+    render (33 M paths: one wavefront batch), against the oracle on two 64 x 36 crops rendered with
+    the full frame's pixel indices: the same seed (few pixels off, as on the small frame), and the
+    mean relative error against the FP64 oracle no larger than the FP32 oracle's own (at 4 spp on
+    unit-sized triangles FP32 and FP64 hit points part on a few per cent of paths, so that error is
+    set by the arithmetic, not by the kernels)."""
+    sc, dev, ref = large
+    W, H, spp = 3840, 2160, 4
+    cam = dict(sc["camera"], aspect=W / H)
+    kw = dict(maxdepth=12, rrdepth=3)
+    t0 = time.time()
+    g = dev.render(cam, W, H, spp, seed=13, **kw)["sum"].reshape(H, W, 3) / spp
+    t = time.time() - t0
+    print(f"3840x2160 x {spp} spp: {t * 1e3:.0f} ms including the FP64 readback "
+          f"({W * H * spp / t / 1e6:.0f} Msamples/s)")
+    for rect in ((1888, 1062, 1952, 1098), (200, 1900, 264, 1936)):
+        x0, y0, x1, y1 = rect
+        gc = g[y0:y1, x0:x1]
+        s32 = ref.render(scenes.camera_array(cam), W, H, spp, seed=13, rect=rect, **kw)[0].reshape(H, W, 3)
+        s64 = ref.render(scenes.camera_array(cam), W, H, spp, seed=13, rect=rect, fp64=True, **kw)[0].reshape(H, W, 3)
+        m32, m64 = s32[y0:y1, x0:x1] / spp, s64[y0:y1, x0:x1] / spp
+        frac = float((np.abs(gc - m32) / (np.abs(m32) + 1e-3) > 1e-3).any(-1).mean())
+        mre = float((np.abs(gc - m64) / (np.abs(m64) + 1e-2)).mean())
+        mre32 = float((np.abs(m32 - m64) / (np.abs(m64) + 1e-2)).mean())
+        print(f"crop {rect}: same seed {frac:.2%} of pixels beyond 1e-3; MRE against FP64 {mre:.2e} (FP32 oracle "
+              f"{mre32:.2e}); means {gc.mean():.5f} / {m64.mean():.5f}")
+        assert m64.mean() > 1e-3
+        assert frac < 0.05
+        assert mre <= 1.25 * mre32 + 2e-3
+        assert abs(gc.mean() - m64.mean()) < 2e-2 * m64.mean()
+
+
+def test_large_full_size_texel_adjoint(large):
+    """The 2048^2 texel adjoint at 1920 x 1080 x 4 spp (8.3 M samples, the benchmark's size): the
+    gradient's directional derivative along a random +-1 pattern over all 12.6 M texels against a
+    central difference of the GPU render with common random numbers (texel values enter the image
+    linearly along each path and move no path)."""
+    sc, dev, _ = large
+    W, H, spp = 1920, 1080, 4
+    cam = dict(sc["camera"], aspect=W / H)
+    kw = dict(maxdepth=12, rrdepth=3)
+    M = len(sc["mats"])
+    rng = np.random.Generator(np.random.PCG64(31))
+    dldi = rng.normal(size=(H, W, 3)).astype(np.float32)
+    t0 = time.time()
+    _, gt = dev.adjoint(cam, W, H, spp, M, seed=7, dl_dimg=dldi, tex_id=16, tex_shape=(2048, 2048), **kw)
+    t_adj = time.time() - t0
+    tex = sc["textures"][16].copy()
+    v = rng.choice(np.array([-1.0, 1.0], np.float32), size=tex.shape)
+    h = 1e-3
+    vals = []
+    for sgn in (1.0, -1.0):
+        t = (tex + np.float32(sgn * h) * v).astype(np.float32)
+        dev.set_texture_rgb32f(16, t)
+        vals.append((dev.render(cam, W, H, spp, seed=7, **kw)["sum"].reshape(H, W, 3) / spp * dldi).sum())
+    dev.set_texture_rgb32f(16, tex)
+    # each texel's exact float32 step between the two renders
+    step = (tex + np.float32(h) * v).astype(np.float32).astype(np.float64) - \
+        (tex - np.float32(h) * v).astype(np.float32).astype(np.float64)
+    fd = vals[0] - vals[1]
+    gv = float((gt.astype(np.float64) * step).sum())
+    print(f"1080p x {spp} spp texel adjoint: {np.count_nonzero(gt)} nonzero texel gradients, {t_adj * 1e3:.0f} ms "
+          f"with readback; gradient . step {gv:.6e}, central difference {fd:.6e}")
+    assert np.count_nonzero(gt) > 10000
+    assert abs(fd - gv) <= 5e-3 * abs(fd)
