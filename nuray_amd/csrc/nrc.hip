@@ -47,30 +47,43 @@ __global__ __launch_bounds__(INF_WAVES * 64) void k_nrc_infer(const __bf16* __re
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int n = count ? min(*count, n_max) : n_max;
     __bf16* X = sX[wave];
-    for (int chunk = blockIdx.x * INF_WAVES + wave; chunk * 64 < n; chunk += gridDim.x * INF_WAVES) {
+    const int stride = gridDim.x * INF_WAVES;
+    // the lane's query of the wave's next tile is loaded while the current tile runs its layers
+    auto fetch = [&](int chunk) { return chunk * 64 + lane < n ? q[chunk * 64 + lane] : NrcQuery{}; };
+    NrcQuery nxt = fetch(blockIdx.x * INF_WAVES + wave);
+    for (int chunk = blockIdx.x * INF_WAVES + wave; chunk * 64 < n; chunk += stride) {
         const int s = chunk * 64 + lane;
-        NrcQuery rec;
-        const bool valid = s < n;
-        if (valid) rec = q[s];
-        else rec = NrcQuery{};
-        nrc_encode_row(rec, valid, X, lane);
+        const NrcQuery rec = nxt;
+        nrc_encode_row(rec, s < n, X, lane);
         wave_sync();
+        if ((chunk + stride) * 64 < n) nxt = fetch(chunk + stride);
         float4 y[4];
         nrc_forward(sW, X, y, lane);
         wave_sync();
+        // lanes 0..15 finish samples 16 nt + lane, whose records sit in lanes 16 nt + lane
+        float4 qc[4], qd[4];
+        if (ACC) {
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const int src = 16 * nt + (lane & 15);
+                qc[nt].w = __shfl(rec.c.w, src, 64);
+                qd[nt].x = __shfl(rec.d.x, src, 64);
+                qd[nt].y = __shfl(rec.d.y, src, 64);
+                qd[nt].z = __shfl(rec.d.z, src, 64);
+            }
+        }
         if (lane < 16) {
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
                 const int si = chunk * 64 + 16 * nt + lane;
                 if (si >= n) continue;
                 if (ACC) {
-                    const float4 qc = q[si].c, qd = q[si].d;
-                    const int slot = __float_as_int(qc.w);
+                    const int slot = __float_as_int(qc[nt].w);
                     if (slot < 0) continue;
                     float4 r = out[slot];
-                    r.x += qd.x * fmaxf(y[nt].x, 0.0f);
-                    r.y += qd.y * fmaxf(y[nt].y, 0.0f);
-                    r.z += qd.z * fmaxf(y[nt].z, 0.0f);
+                    r.x += qd[nt].x * fmaxf(y[nt].x, 0.0f);
+                    r.y += qd[nt].y * fmaxf(y[nt].y, 0.0f);
+                    r.z += qd[nt].z * fmaxf(y[nt].z, 0.0f);
                     out[slot] = r;
                 } else {
                     out[si] = y[nt];
@@ -116,37 +129,6 @@ __device__ __forceinline__ AFrag load_afrag(const __bf16* m, int row, int qd) {
 
 __device__ __forceinline__ bf16x8 lds8(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
 __device__ __forceinline__ bf16x4 lds4(const __bf16* p) { return *reinterpret_cast<const bf16x4*>(p); }
-
-typedef short s16x4 __attribute__((ext_vector_type(4)));
-typedef short s16x2 __attribute__((ext_vector_type(2)));
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-
-// Round-to-nearest-even bf16 of two floats as one packed conversion, viewed as int16 pairs.
-// (Written pairwise: the compiler lowers a four-wide conversion one element at a time.)
-__device__ __forceinline__ s16x2 bf16_pair(float a, float b) {
-    return __builtin_bit_cast(s16x2, __builtin_convertvector(f32x2{a, b}, bf16x2));
-}
-__device__ __forceinline__ bf16x4 join_pairs(s16x2 lo, s16x2 hi) {
-    return __builtin_bit_cast(bf16x4, u32x2{__builtin_bit_cast(unsigned, lo), __builtin_bit_cast(unsigned, hi)});
-}
-
-// ReLU of the bf16-rounded values: a bf16 is negative (or -0) exactly when its bits are negative
-// as int16, so the packed integer max with 0 clears those (the same values as rounding max(c, 0)).
-__device__ __forceinline__ bf16x4 relu_bf16(f32x4 c) {
-    return join_pairs(__builtin_elementwise_max(bf16_pair(c[0], c[1]), s16x2{0, 0}),
-                      __builtin_elementwise_max(bf16_pair(c[2], c[3]), s16x2{0, 0}));
-}
-
-// The bf16-rounded c where the (non-negative) activation x is > 0, else 0: min(x, 1) negated is
-// an all-ones or all-zeros 16-bit mask.  (Kept four-wide: applied to one 32-bit half of x at a
-// time, this compiler selects the packed min on the wrong half.)
-__device__ __forceinline__ bf16x4 masked_bf16(f32x4 c, bf16x4 x) {
-    const s16x4 m = -__builtin_elementwise_min(__builtin_bit_cast(s16x4, x), s16x4{1, 1, 1, 1});
-    const bf16x4 g = join_pairs(bf16_pair(c[0], c[1]), bf16_pair(c[2], c[3]));
-    return __builtin_bit_cast(bf16x4, __builtin_bit_cast(s16x4, g) & m);
-}
 
 // Input features 8 g .. 8 g + 7 of one record (g wave-uniform; each case folds to its features).
 template <int F0>

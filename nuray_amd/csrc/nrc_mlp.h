@@ -97,6 +97,37 @@ NRC_DEVICE void wave_sync() {
 
 NRC_DEVICE f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+// Round-to-nearest-even bf16 of two floats as one packed conversion, viewed as int16 pairs.
+// (Written pairwise: the compiler lowers a four-wide conversion one element at a time.)
+NRC_DEVICE s16x2 bf16_pair(float a, float b) {
+    return __builtin_bit_cast(s16x2, __builtin_convertvector(f32x2{a, b}, bf16x2));
+}
+NRC_DEVICE bf16x4 join_pairs(s16x2 lo, s16x2 hi) {
+    return __builtin_bit_cast(bf16x4, u32x2{__builtin_bit_cast(unsigned, lo), __builtin_bit_cast(unsigned, hi)});
+}
+
+// ReLU of the bf16-rounded values: a bf16 is negative (or -0) exactly when its bits are negative
+// as int16, so the packed integer max with 0 clears those (the same values as rounding max(c, 0)).
+NRC_DEVICE bf16x4 relu_bf16(f32x4 c) {
+    return join_pairs(__builtin_elementwise_max(bf16_pair(c[0], c[1]), s16x2{0, 0}),
+                      __builtin_elementwise_max(bf16_pair(c[2], c[3]), s16x2{0, 0}));
+}
+
+// The bf16-rounded c where the (non-negative) activation x is > 0, else 0: min(x, 1) negated is
+// an all-ones or all-zeros 16-bit mask.  (Kept four-wide: applied to one 32-bit half of x at a
+// time, this compiler selects the packed min on the wrong half.)
+NRC_DEVICE bf16x4 masked_bf16(f32x4 c, bf16x4 x) {
+    const s16x4 m = -__builtin_elementwise_min(__builtin_bit_cast(s16x4, x), s16x4{1, 1, 1, 1});
+    const bf16x4 g = join_pairs(bf16_pair(c[0], c[1]), bf16_pair(c[2], c[3]));
+    return __builtin_bit_cast(bf16x4, __builtin_bit_cast(s16x4, g) & m);
+}
+
 // Block-cooperative copy of the bf16 weights (global, dense rows of 64) into padded LDS rows.
 NRC_DEVICE void nrc_load_weights(const __bf16* __restrict__ w, __bf16* lds) {
     for (int i = threadIdx.x; i < NRC_WROWS * 8; i += blockDim.x) {
@@ -117,60 +148,66 @@ NRC_DEVICE void nrc_encode_row(const NrcQuery& q, bool valid, __bf16* X, int lan
     }
 }
 
-// acc[mt][nt] = rows 16 mt .. of W^T  x  samples 16 nt .. of X^T, for MT output tiles and 4 sample
-// tiles (64 samples), K = 64 inputs.
-template <int MT>
-NRC_DEVICE void nrc_layer(const __bf16* Wl, const __bf16* X, f32x4 (&acc)[MT][4], int lane) {
-    const int r = lane & 15, q = lane >> 4;
-    bf16x8 bx[4][2];
+// A fragment of hidden or output layer l >= 1 for lane (r, qd) and K chunk ks: row `row` of W^T,
+// input features 32 ks + 4 qd + {0..3} and 32 ks + 16 + 4 qd + {0..3} -- the order in which the
+// previous layer's output tiles 2 ks and 2 ks + 1 hold them in the lane's registers.
+NRC_DEVICE bf16x8 nrc_wfrag_perm(const __bf16* Wl, int row, int ks, int qd) {
+    const __bf16* p = Wl + row * NRC_LD + 32 * ks + 4 * qd;
+    const bf16x4 lo = *reinterpret_cast<const bf16x4*>(p);
+    const bf16x4 hi = *reinterpret_cast<const bf16x4*>(p + 16);
+    return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
+// Full forward pass of 64 samples whose encoded inputs are in the sample-major tile X (read once).
+// The activations stay in registers: output tile mt of a 16x16x32 product leaves features
+// 16 mt + 4 qd .. + 3 of sample 16 nt + r in lane (r, qd), so the lane's 8 K values of the next
+// product's B operand for chunk ks are tiles 2 ks and 2 ks + 1 side by side -- the K order of
+// nrc_wfrag_perm, which reads the weights in that order.  Returns the RGB output of sample
+// 16 nt + (lane & 15) in out[nt] for lanes with lane < 16.
+NRC_DEVICE void nrc_forward(const __bf16* W, const __bf16* X, float4 (&out)[4], int lane) {
+    const int r = lane & 15, qd = lane >> 4;
+    bf16x8 b[4][2];
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks) bx[nt][ks] = *reinterpret_cast<const bf16x8*>(X + (16 * nt + r) * NRC_LD + 32 * ks + 8 * q);
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-        const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(Wl + (16 * mt + r) * NRC_LD + 8 * q);
-        const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(Wl + (16 * mt + r) * NRC_LD + 32 + 8 * q);
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-            c = mfma16(a0, bx[nt][0], c);
-            acc[mt][nt] = mfma16(a1, bx[nt][1], c);
-        }
-    }
-}
-
-// Writes relu(acc) (bf16) into the sample-major tile Y: sample 16 nt + (lane & 15),
-// features 16 mt + 4 (lane >> 4) .. +3.
-template <int MT>
-NRC_DEVICE void nrc_store_relu(const f32x4 (&acc)[MT][4], __bf16* Y, int lane) {
-    const int r = lane & 15, q = lane >> 4;
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            bf16x4 v;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) v[i] = (__bf16)fmaxf(acc[mt][nt][i], 0.0f);
-            *reinterpret_cast<bf16x4*>(Y + (16 * nt + r) * NRC_LD + 16 * mt + 4 * q) = v;
-        }
-}
-
-// Full forward pass of 64 samples whose encoded inputs are in X; X is overwritten layer by layer.
-// Returns the RGB output of sample 16 nt + (lane & 15) in out[nt] for lanes with lane < 16.
-NRC_DEVICE void nrc_forward(const __bf16* W, __bf16* X, float4 (&out)[4], int lane) {
+        for (int ks = 0; ks < 2; ++ks) b[nt][ks] = *reinterpret_cast<const bf16x8*>(X + (16 * nt + r) * NRC_LD + 32 * ks + 8 * qd);
 #pragma unroll 1
     for (int l = 0; l < 5; ++l) {
+        const __bf16* Wl = W + l * 64 * NRC_LD;
         f32x4 acc[4][4];
-        nrc_layer<4>(W + l * 64 * NRC_LD, X, acc, lane);
-        wave_sync();
-        nrc_store_relu<4>(acc, X, lane);
-        wave_sync();
-    }
-    f32x4 acc[1][4];
-    nrc_layer<1>(W + 5 * 64 * NRC_LD, X, acc, lane);
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) out[nt] = make_float4(acc[0][nt][0], acc[0][nt][1], acc[0][nt][2], 0.0f);
+        for (int mt = 0; mt < 4; ++mt) {
+            bf16x8 a0, a1;
+            if (l == 0) {  // the encoded inputs are in natural order
+                a0 = *reinterpret_cast<const bf16x8*>(Wl + (16 * mt + r) * NRC_LD + 8 * qd);
+                a1 = *reinterpret_cast<const bf16x8*>(Wl + (16 * mt + r) * NRC_LD + 32 + 8 * qd);
+            } else {
+                a0 = nrc_wfrag_perm(Wl, 16 * mt + r, 0, qd);
+                a1 = nrc_wfrag_perm(Wl, 16 * mt + r, 1, qd);
+            }
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+                c = mfma16(a0, b[nt][0], c);
+                acc[mt][nt] = mfma16(a1, b[nt][1], c);
+            }
+        }
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks)
+                b[nt][ks] = __builtin_shufflevector(relu_bf16(acc[2 * ks][nt]), relu_bf16(acc[2 * ks + 1][nt]), 0, 1, 2,
+                                                    3, 4, 5, 6, 7);
+    }
+    const __bf16* W5 = W + 5 * 64 * NRC_LD;
+    const bf16x8 a0 = nrc_wfrag_perm(W5, r, 0, qd), a1 = nrc_wfrag_perm(W5, r, 1, qd);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+        c = mfma16(a0, b[nt][0], c);
+        c = mfma16(a1, b[nt][1], c);
+        out[nt] = make_float4(c[0], c[1], c[2], 0.0f);
+    }
 }
 
 }  // namespace nr
