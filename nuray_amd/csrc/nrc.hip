@@ -31,9 +31,9 @@ namespace nr {
 
 namespace {
 
-// waves per inference block: 48 KB of weights + 12 x 9 KB tiles = 155 KB of LDS, one block per CU,
-// 3 waves per SIMD (the 138 VGPRs of the kernel allow 3)
-constexpr int INF_WAVES = 12;
+// waves per inference block: 48 KB of weights + 16 x 5 KB half-width input tiles = 128 KB of LDS,
+// one block per CU, 4 waves per SIMD (both forms of the kernel stay within 128 VGPRs)
+constexpr int INF_WAVES = 16;
 
 template <bool ACC>
 __global__ __launch_bounds__(INF_WAVES * 64) void k_nrc_infer(const __bf16* __restrict__ w16,
@@ -41,7 +41,7 @@ __global__ __launch_bounds__(INF_WAVES * 64) void k_nrc_infer(const __bf16* __re
                                                               const int* __restrict__ count, int n_max,
                                                               float4* __restrict__ out) {
     __shared__ __attribute__((aligned(16))) __bf16 sW[NRC_LDS_W];
-    __shared__ __attribute__((aligned(16))) __bf16 sX[INF_WAVES][NRC_TILE];
+    __shared__ __attribute__((aligned(16))) __bf16 sX[INF_WAVES][NRC_TILE_HALF];
     nrc_load_weights(w16, sW);
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -54,40 +54,38 @@ __global__ __launch_bounds__(INF_WAVES * 64) void k_nrc_infer(const __bf16* __re
     for (int chunk = blockIdx.x * INF_WAVES + wave; chunk * 64 < n; chunk += stride) {
         const int s = chunk * 64 + lane;
         const NrcQuery rec = nxt;
-        nrc_encode_row(rec, s < n, X, lane);
-        wave_sync();
+        // the layer-0 B operands, through the half-width tile: features 0..31, then 32..63
+        bf16x8 b[4][2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            nrc_encode_half(rec, s < n, X, lane, h);
+            wave_sync();
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+                b[nt][h] = *reinterpret_cast<const bf16x8*>(X + (16 * nt + (lane & 15)) * NRC_LDH + 8 * (lane >> 4));
+            wave_sync();
+        }
         if ((chunk + stride) * 64 < n) nxt = fetch(chunk + stride);
         float4 y[4];
-        nrc_forward(sW, X, y, lane);
-        wave_sync();
+        nrc_forward(sW, b, y, lane);
         // lanes 0..15 finish samples 16 nt + lane, whose records sit in lanes 16 nt + lane
-        float4 qc[4], qd[4];
-        if (ACC) {
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt) {
+        for (int nt = 0; nt < 4; ++nt) {
+            const int si = chunk * 64 + 16 * nt + lane;
+            if (ACC) {
                 const int src = 16 * nt + (lane & 15);
-                qc[nt].w = __shfl(rec.c.w, src, 64);
-                qd[nt].x = __shfl(rec.d.x, src, 64);
-                qd[nt].y = __shfl(rec.d.y, src, 64);
-                qd[nt].z = __shfl(rec.d.z, src, 64);
-            }
-        }
-        if (lane < 16) {
-#pragma unroll
-            for (int nt = 0; nt < 4; ++nt) {
-                const int si = chunk * 64 + 16 * nt + lane;
-                if (si >= n) continue;
-                if (ACC) {
-                    const int slot = __float_as_int(qc[nt].w);
-                    if (slot < 0) continue;
+                const int slot = __shfl(__float_as_int(rec.c.w), src, 64);
+                const float dx = __shfl(rec.d.x, src, 64), dy = __shfl(rec.d.y, src, 64),
+                            dz = __shfl(rec.d.z, src, 64);
+                if (lane < 16 && si < n && slot >= 0) {
                     float4 r = out[slot];
-                    r.x += qd[nt].x * fmaxf(y[nt].x, 0.0f);
-                    r.y += qd[nt].y * fmaxf(y[nt].y, 0.0f);
-                    r.z += qd[nt].z * fmaxf(y[nt].z, 0.0f);
+                    r.x += dx * fmaxf(y[nt].x, 0.0f);
+                    r.y += dy * fmaxf(y[nt].y, 0.0f);
+                    r.z += dz * fmaxf(y[nt].z, 0.0f);
                     out[slot] = r;
-                } else {
-                    out[si] = y[nt];
                 }
+            } else if (lane < 16 && si < n) {
+                out[si] = y[nt];
             }
         }
     }

@@ -136,14 +136,20 @@ NRC_DEVICE void nrc_load_weights(const __bf16* __restrict__ w, __bf16* lds) {
     }
 }
 
-// Lane `lane` encodes one query into row `lane` of the tile X (or zeros when !valid).
-NRC_DEVICE void nrc_encode_row(const NrcQuery& q, bool valid, __bf16* X, int lane) {
-    __bf16* row = X + lane * NRC_LD;
+// Half-width input tile of the inference kernel: 64 samples x 32 features, 80-byte rows (their
+// starting banks differ by 20 dwords, so 16-byte accesses of 16 lanes hit all 64 banks once).
+constexpr int NRC_LDH = 40;
+constexpr int NRC_TILE_HALF = 64 * NRC_LDH;
+
+// Lane `lane` encodes features 32 h .. 32 h + 31 of one query into row `lane` of the half tile X
+// (zeros when !valid).
+NRC_DEVICE void nrc_encode_half(const NrcQuery& q, bool valid, __bf16* X, int lane, int h) {
+    __bf16* row = X + lane * NRC_LDH;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
+    for (int c = 0; c < 4; ++c) {
         bf16x8 v;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = (__bf16)(valid ? nrc_feature(q, c * 8 + j) : 0.0f);
+        for (int j = 0; j < 8; ++j) v[j] = (__bf16)(valid ? nrc_feature(q, 32 * h + c * 8 + j) : 0.0f);
         *reinterpret_cast<bf16x8*>(row + c * 8) = v;
     }
 }
@@ -158,19 +164,14 @@ NRC_DEVICE bf16x8 nrc_wfrag_perm(const __bf16* Wl, int row, int ks, int qd) {
     return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
 }
 
-// Full forward pass of 64 samples whose encoded inputs are in the sample-major tile X (read once).
-// The activations stay in registers: output tile mt of a 16x16x32 product leaves features
+// Full forward pass of 64 samples whose encoded inputs are the layer-0 B operands b (lane (r, qd):
+// features 32 ks + 8 qd .. + 7 of sample 16 nt + r in b[nt][ks]).  The activations stay in registers: output tile mt of a 16x16x32 product leaves features
 // 16 mt + 4 qd .. + 3 of sample 16 nt + r in lane (r, qd), so the lane's 8 K values of the next
 // product's B operand for chunk ks are tiles 2 ks and 2 ks + 1 side by side -- the K order of
 // nrc_wfrag_perm, which reads the weights in that order.  Returns the RGB output of sample
 // 16 nt + (lane & 15) in out[nt] for lanes with lane < 16.
-NRC_DEVICE void nrc_forward(const __bf16* W, const __bf16* X, float4 (&out)[4], int lane) {
+NRC_DEVICE void nrc_forward(const __bf16* W, bf16x8 (&b)[4][2], float4 (&out)[4], int lane) {
     const int r = lane & 15, qd = lane >> 4;
-    bf16x8 b[4][2];
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt)
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks) b[nt][ks] = *reinterpret_cast<const bf16x8*>(X + (16 * nt + r) * NRC_LD + 32 * ks + 8 * qd);
 #pragma unroll 1
     for (int l = 0; l < 5; ++l) {
         const __bf16* Wl = W + l * 64 * NRC_LD;
