@@ -253,6 +253,15 @@ typedef struct nr_nrc_params {
 } nr_nrc_params;
 int nr_render_nrc(nr_scene* s, nr_nrc* c, const nr_camera* cam, const nr_params* params, const nr_nrc_params* np,
                   double* stats);
+/* No reference counterpart (the reference keeps its training records inside RendererNRC,
+ * renderernrc.cpp:128-133, 175-182): the training records of the scene's last radiance-cache frame
+ * and their targets, for checking the cache's data path.  recs: cap rows of 16 floats in the query
+ * layout above, except float 11 = the path slot (int bits) and floats 12..14 = the path throughput
+ * beta on arrival at the vertex; targets: cap rows of 4 floats, xyz = (L_path - L_before) / beta
+ * (L_before: the radiance the path had collected before the vertex; 0 where beta <= 1e-8).
+ * Records of one path are in vertex order.  *n = the number of records with targets (at most the
+ * cache's max_batch), also when it exceeds cap. */
+int nr_nrc_frame_records(nr_scene* s, float* recs, float* targets, int32_t cap, int32_t* n);
 
 /* No reference counterpart: a plain device-to-device copy of `bytes` bytes (16-byte vector loads
  * and stores), run `reps` times; *gbs = bytes read + written per second.  Calibrates the

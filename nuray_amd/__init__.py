@@ -26,7 +26,7 @@ EXPORTS = [
     "nr_bvh_build_host", "nr_intersect", "nr_render_ptnee", "nr_render_fetch", "nr_render_stats",
     "nr_set_profiling", "nr_bench_copy", "nr_scene_set_params", "nr_loss_grad_l2", "nr_adjoint_ptnee",
     "nr_nrc_create", "nr_nrc_destroy", "nr_nrc_set_weights", "nr_nrc_get_weights", "nr_nrc_infer",
-    "nr_nrc_train_step", "nr_nrc_gradient", "nr_render_nrc", "nr_nrc_bench", "nr_scene_set_texture_rgb32f",
+    "nr_nrc_train_step", "nr_nrc_gradient", "nr_render_nrc", "nr_nrc_frame_records", "nr_nrc_bench", "nr_scene_set_texture_rgb32f",
     "nr_adjoint_ptnee_tex", "nr_scene_set_material_ext", "nr_scene_set_short_stack", "nr_scene_set_trace_order", "nr_shutdown",
     "nr_intersect_noalloc",
 ]
@@ -106,6 +106,7 @@ def lib():
         L.nr_nrc_gradient.argtypes = [vp, f32p, f32p, C.c_uint32, f32p, C.POINTER(C.c_double)]
         L.nr_nrc_bench.argtypes = [vp, C.c_uint32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.nr_render_nrc.argtypes = [vp, vp, C.POINTER(nr_camera), C.POINTER(nr_params), C.POINTER(nr_nrc_params), vp]
+        L.nr_nrc_frame_records.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
         _lib = L
     return _lib
 
@@ -308,6 +309,16 @@ class Scene:
         if fetch:
             return stats, self.fetch(W, H)
         return stats
+
+    def nrc_frame_records(self, cap=1 << 16):
+        """The last cache frame's training records (n x 16 floats: float 11 = slot bits, 12..14 = beta)
+        and targets (n x 4; xyz used) (nr_nrc_frame_records)."""
+        recs = np.zeros((cap, 16), np.float32)
+        tgt = np.zeros((cap, 4), np.float32)
+        n = C.c_int32(0)
+        check(lib().nr_nrc_frame_records(self.h, recs.ctypes.data, tgt.ctypes.data, cap, C.byref(n)))
+        k = min(n.value, cap)
+        return recs[:k], tgt[:k]
 
     def set_profiling(self, mode):
         check(lib().nr_set_profiling(self.h, mode))

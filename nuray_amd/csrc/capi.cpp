@@ -452,6 +452,14 @@ int nr_nrc_bench(nr_nrc* c, uint32_t n, int32_t reps, double* infer_ms, double* 
     });
 }
 
+int nr_nrc_frame_records(nr_scene* s, float* recs, float* targets, int32_t cap, int32_t* n) {
+    return guard([&] {
+        if (!s || !n || cap < 0 || (cap > 0 && (!recs || !targets))) throw std::runtime_error("bad argument");
+        *n = nr::nrc_frame_records(*s->sc, recs, targets, cap);
+        return NR_OK;
+    });
+}
+
 int nr_render_nrc(nr_scene* s, nr_nrc* c, const nr_camera* cam, const nr_params* params, const nr_nrc_params* np,
                   double* stats) {
     return guard([&] {

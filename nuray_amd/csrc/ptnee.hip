@@ -984,10 +984,12 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
     NrcFrameStats f;
     f.queries = t.pinned[C_QUERY];
     f.records = std::min<long long>(t.pinned[C_REC], (long long)rec_cap);
+    t.nrc_last_records = int(std::min<long long>(f.records, nrc.cfg.max_batch));
+    if (t.nrc_last_records > 0)  // targets (L_path - L_before) / beta of the frame's records
+        nrc_targets(t.nrc_recs.as<NrcQuery>(), t.nrc_lb.as<float4>(), t.radiance.as<float4>(), t.nrc_last_records,
+                    t.nrc_tgt.as<float4>(), s);
     if (np.train && f.records > 0) {
-        const int n = int(std::min<long long>(f.records, nrc.cfg.max_batch));
-        nrc_targets(t.nrc_recs.as<NrcQuery>(), t.nrc_lb.as<float4>(), t.radiance.as<float4>(), n, t.nrc_tgt.as<float4>(),
-                    s);
+        const int n = t.nrc_last_records;
         // vertices are recorded bounce by bounce: interleave them so every batch mixes all depths
         const int steps = std::max(1, std::min(np.train_steps, n));
         t.nrc_batch.ensure(sizeof(NrcQuery) * size_t(n));
@@ -1010,6 +1012,20 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
     if (fs) *fs = f;
     if (st) *st = rs;
     return 0;
+}
+
+int nrc_frame_records(Scene& sc, float* recs, float* targets, int cap) {
+    if (!sc.target) return 0;
+    RenderTarget& t = *sc.target;
+    const int n = std::min(cap, t.nrc_last_records);
+    if (n > 0) {
+        check_hip(hipMemcpyAsync(recs, t.nrc_recs.ptr, sizeof(NrcQuery) * size_t(n), hipMemcpyDeviceToHost, sc.stream),
+                  "records D2H");
+        check_hip(hipMemcpyAsync(targets, t.nrc_tgt.ptr, sizeof(float4) * size_t(n), hipMemcpyDeviceToHost, sc.stream),
+                  "targets D2H");
+        check_hip(hipStreamSynchronize(sc.stream), "records sync");
+    }
+    return t.nrc_last_records;
 }
 
 void render_fetch(Scene& sc, float* rgb, double* sum, double* sum2, uint8_t* rgb8) {

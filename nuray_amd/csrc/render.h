@@ -81,6 +81,7 @@ struct RenderTarget {
     bool adj_vrec_valid = false;
     // radiance-cache rendering: spread state per queued ray, cache queries, training vertices
     DevBuf nstate[2], nrc_queries, nrc_recs, nrc_lb, nrc_tgt, nrc_batch, nrc_batch_tgt;
+    int nrc_last_records = 0;  // records of the last cache frame that have targets (nrc_recs, nrc_tgt)
     ~RenderTarget();
     void ensure(int W, int H, size_t cap);  // image accumulators for W x H and queues for cap paths
     void ensure_queues(size_t cap);         // queues only (the accumulators keep their contents)
@@ -124,6 +125,9 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
 // Accumulates into the same sums as render_ptnee (p.accumulate).
 int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, const NrcRenderParams& np,
                NrcFrameStats* fs, RenderStats* st);
+// The last radiance-cache frame's training records (NrcQuery, 16 floats each) and their targets
+// (float4), the first min(cap, n) of them; returns n.
+int nrc_frame_records(Scene& sc, float* recs, float* targets, int cap);
 // Per-sample radiance of P paths (a whole number of samples per pixel, from sample0 on) into the
 // render target's radiance buffer, slot = sample offset * W * H + pixel, by the wavefront, without
 // touching the accumulated image.  Invalidates the stored radiance of the last render.  With

@@ -69,6 +69,10 @@ def lib():
         L.orc_nrc_grad.argtypes = [f32p, f32p, f32p, C.c_int, C.c_int, f64p]
         L.orc_nrc_grad.restype = C.c_double
         L.orc_env_radiance.argtypes = [vp, C.c_int, f32p, f32p]
+        L.orc_nrc_frame.argtypes = [vp, f32p, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_float, C.c_int,
+                                    C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, f32p, C.c_int,
+                                    f64p, f64p, f32p, f32p, f32p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int)]
         L.orc_light_pick.argtypes = [vp, C.c_float]
         L.orc_light_p.argtypes = [vp]
         L.orc_light_p.restype = C.c_float
@@ -150,6 +154,28 @@ class Scene:
         if r != 0:
             raise RuntimeError("oracle render failed")
         return sums
+
+    def nrc_frame(self, cam, W, H, spp, w, sample0=0, seed=0, prr=0.8, mis=1, maxdepth=0, rrdepth=0, train_paths=65536,
+                  train_maxlen=5, c_spread=0.01, use_cache=True, min_depth=1, nthreads=0, rec_cap=None):
+        """One radiance-cache frame with the network w answering every query (orc_nrc_frame.cpp).
+        Returns a dict: per-pixel sums sum / sum2, the training records recs (n x 16 floats, slot
+        and depth order), the radiance their paths had collected before them lb (n x 3), their
+        targets tgt (n x 3), the number of queries, and qdepth: the query vertex of every camera
+        path (-1: none)."""
+        rec_cap = rec_cap if rec_cap is not None else train_paths * max(1, train_maxlen - 1)
+        s, s2 = np.zeros(W * H * 3), np.zeros(W * H * 3)
+        recs = np.zeros((max(rec_cap, 1), 16), np.float32)
+        lb = np.zeros((max(rec_cap, 1), 3), np.float32)
+        tgt = np.zeros((max(rec_cap, 1), 3), np.float32)
+        nr, nq = C.c_int(0), C.c_int(0)
+        qd = np.zeros(W * H * spp, np.int32)
+        r = lib().orc_nrc_frame(self.h, _f32(cam), W, H, spp, sample0, seed, prr, mis, maxdepth, rrdepth, train_paths,
+                                train_maxlen, c_spread, int(use_cache), min_depth, _f32(w), nthreads, s, s2, recs, lb,
+                                tgt, rec_cap, C.byref(nr), C.byref(nq), qd.ctypes.data_as(C.POINTER(C.c_int)))
+        if r != 0:
+            raise RuntimeError("oracle cache frame failed")
+        n = min(nr.value, rec_cap)
+        return dict(sum=s, sum2=s2, recs=recs[:n], lb=lb[:n], tgt=tgt[:n], queries=nq.value, qdepth=qd)
 
     def texture_lookup(self, tid, u, v):
         u = _f32(u); v = _f32(v)
