@@ -8,18 +8,20 @@
 // whose output gradient has the reference's form (o - a) / (o^2 + 0.01) (nnnode.cpp:195-196).
 //
 // Kernels
-//   k_nrc_infer      persistent; weights in LDS once per block (12 waves: 3 per SIMD, the most
-//                    the 138 VGPRs allow), one 64-sample tile per wave
+//   k_nrc_infer      persistent; weights in LDS once per block (16 waves: 4 per SIMD), one
+//                    64-sample tile per wave, the encoded inputs through a half-width LDS tile and
+//                    the hidden activations in registers (nrc_forward); the next tile's queries are
+//                    loaded while the current one runs
 //   k_nrc_train      persistent blocks of 8 waves over 64-sample tiles: forward, loss gradient,
-//                    backward (ReLU masks kept in registers) and the weight gradient, with every
-//                    layer's input and gradient tile kept in LDS in both sample-major and
-//                    feature-major layout (nothing per sample goes to global memory); each
-//                    block's weight-gradient partial is written once; the weights come from L2
-//                    (W^T for the forward, the transposed copy W for the backward); per-tile loss
-//                    slots instead of one device-wide atomic
+//                    backward (ReLU masks read back from the kept activations) and the weight
+//                    gradient, with every layer's input and gradient tile kept in LDS in both
+//                    sample-major and feature-major layout (nothing per sample goes to global
+//                    memory); each block's weight-gradient partial is written once; the weights
+//                    come from L2 (W^T for the forward, the transposed copy W for the backward);
+//                    per-tile loss slots instead of one device-wide atomic
 //   k_nrc_adam       sums the blocks' partials (16 slices per parameter, 8 loads in flight per
-//                    thread) and applies Adam
-//                    to the FP32 master weights, refreshes the BF16 copies (W^T and W)
+//                    thread) and applies Adam to the FP32 master weights, refreshes the BF16
+//                    copies (W^T and W)
 #include <cmath>
 #include <cstring>
 #include <type_traits>
