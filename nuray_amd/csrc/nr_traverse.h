@@ -137,8 +137,12 @@ __device__ __forceinline__ bool trav_leaf(const SceneView& sv, F3 o, F3 d, doubl
     s.has_leaf = false;
     const int first = ref_first(s.leaf), cnt = ref_count(s.leaf);
     if (STATS) n_tt += cnt;
+    if (cnt == 0) return false;
+    // the next triangle's loads are in flight while the current one is tested
+    TriGeom nxt = sv.tris[first];
     for (int k = 0; k < cnt; ++k) {
-        const TriGeom g = sv.tris[first + k];
+        const TriGeom g = nxt;
+        if (k + 1 < cnt) nxt = sv.tris[first + k + 1];
         float t, b1, b2;
         if (!tri_fast(g, o, d, bt, t, b1, b2)) continue;
         if (ANY) {
@@ -149,6 +153,97 @@ __device__ __forceinline__ bool trav_leaf(const SceneView& sv, F3 o, F3 d, doubl
         } else if (t < bt || first > blf) {  // t <= bt here: equal t -> later leaf wins
             bt = t; bb1 = b1; bb2 = b2; bprim = first + k; blf = first;
         }
+    }
+    return false;
+}
+
+// Wave-cooperative leaf round of the persistent kernels (trav_leaf's role there).  The triangles of
+// every lane's pending leaf are laid out one after another in lane order and tested 64 at a time,
+// one per lane, so lanes without a leaf of their own test triangles of the others' leaves (the
+// per-lane loop runs as many iterations as the wave's longest leaf with a third of the lanes busy).
+// Closest hit: each owner takes its leaf's smallest t, the first triangle among equal t (the loop's
+// strict '<'), and merges it into its running winner with the DFS tie rule.  Any hit: the owner's
+// first triangle with (double)t + 2e-3 <= len blocks the ray (returns true).  Bit-identical to
+// trav_leaf: tri_fast accepts a triangle against a winner w iff the triangle's own t is <= w (the
+// rcp filter's margin never rejects such a t), so testing every triangle against the winner held
+// at the start of the round and keeping the leaf's minimum is the loop's outcome; the any-hit
+// winner never changes before the ray is blocked.
+// All 64 lanes call it (wave-uniform control flow); `mark` is 64 ints of LDS owned by the wave.
+template <bool ANY, bool STATS>
+__device__ __forceinline__ bool trav_leaf_wave(const SceneView& sv, F3 o, F3 d, float len, bool own, TravState& s,
+                                               float& bt, float& bb1, float& bb2, int& bprim, int& blf,
+                                               uint32_t& n_tt, int* mark, int lane) {
+    if (__ballot(own) == 0ull) return false;
+    const int cnt = own ? ref_count(s.leaf) : 0;
+    const int first = own ? ref_first(s.leaf) : 0;
+    if (own) s.has_leaf = false;
+    if (STATS) n_tt += cnt;
+    // exclusive prefix of the 4-bit counts over the lanes, and their total (wave-uniform)
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int excl = 0, total = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const unsigned long long m = __ballot((cnt >> b) & 1);
+        excl += __popcll(m & below) << b;
+        total += __popcll(m) << b;
+    }
+    float lt = __builtin_inff(), lb1 = 0.0f, lb2 = 0.0f;  // the owner's leaf winner so far
+    int lk = -1;
+    for (int base = 0; base < total; base += 64) {
+        // owner of position base + lane: the last owner whose run starts at or before it (marked in
+        // LDS at the run's start), else the owner whose run began in an earlier pass
+        const int st = excl - base;
+        __hip_atomic_store(&mark[lane], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        if (cnt > 0 && st >= 0 && st < 64) __hip_atomic_store(&mark[st], lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        const int v = __hip_atomic_load(&mark[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        const unsigned long long upto = __ballot(v >= 0) & (~0ull >> (63 - lane));
+        const unsigned long long carried = __ballot(cnt > 0 && st < 0);
+        const int vs = __shfl(v, upto ? 63 - __builtin_clzll(upto) : 0);
+        const int ow = upto ? vs : (carried ? 63 - __builtin_clzll(carried) : 0);
+        // the owner's ray, winner and triangle offset; one triangle per position
+        const int p = base + lane;
+        const F3 po = f3(__shfl(o.x, ow), __shfl(o.y, ow), __shfl(o.z, ow));
+        const F3 pd = f3(__shfl(d.x, ow), __shfl(d.y, ow), __shfl(d.z, ow));
+        const float pbt = __shfl(bt, ow);
+        const float plen = ANY ? __shfl(len, ow) : 0.0f;  // (shuffles only here, with every lane active)
+        const int tri = __shfl(first - excl, ow) + p;
+        float t = __builtin_inff(), b1 = 0.0f, b2 = 0.0f;
+        bool blocked = false;
+        if (p < total) {
+            float tt, u1, u2;
+            if (tri_fast(sv.tris[tri], po, pd, pbt, tt, u1, u2)) {
+                t = tt, b1 = u1, b2 = u2;
+                if (ANY) blocked = double(tt) + 2e-3 <= double(plen);
+            }
+        }
+        // the owner's positions in this pass: q0 .. q0 + nq - 1, in triangle order
+        const int q0 = max(st, 0), nq = max(min(st + cnt, 64) - q0, 0);
+        if (ANY) {
+            const unsigned long long seg = nq == 0 ? 0ull : nq == 64 ? ~0ull : ((1ull << nq) - 1ull) << q0;
+            const unsigned long long hm = __ballot(blocked) & seg;
+            const int hq = hm ? __builtin_ctzll(hm) : 0;
+            const float ht = __shfl(t, hq), h1 = __shfl(b1, hq), h2 = __shfl(b2, hq);
+            if (hm && lk < 0) lt = ht, lb1 = h1, lb2 = h2, lk = base + hq - excl;
+        } else {
+            float pt = __builtin_inff();
+            int pq = q0;
+            for (int j = 0; __ballot(j < nq) != 0ull; ++j) {
+                const float tj = __shfl(t, q0 + j);
+                if (j < nq && tj < pt) pt = tj, pq = q0 + j;
+            }
+            const float w1 = __shfl(b1, pq), w2 = __shfl(b2, pq);
+            if (pt < lt) lt = pt, lb1 = w1, lb2 = w2, lk = base + pq - excl;  // earlier passes hold lower k
+        }
+    }
+    if (ANY) {
+        if (lk < 0) return false;
+        bt = lt; bb1 = lb1; bb2 = lb2; bprim = first + lk;
+        return true;
+    }
+    if (lk >= 0 && (lt < bt || first > blf)) {  // lt <= bt: equal t -> later leaf wins
+        bt = lt; bb1 = lb1; bb2 = lb2; bprim = first + lk; blf = first;
     }
     return false;
 }

@@ -14,7 +14,8 @@
 // leaves and boxes, half the node steps).  Every lane traverses one ray with a private stack kept
 // in LDS at stack[slot * BLOCK + tid] (one 4-byte bank per lane, conflict-free).  Internal nodes
 // are expanded wave-cooperatively (trav_expand_wave4) until (nearly) every lane holds a leaf, then
-// leaves are tested together; a lane whose ray is done goes idle, and once REFILL_MIN lanes are
+// the wave tests the held leaves' triangles together, one triangle per lane, idle lanes included
+// (trav_leaf_wave); a lane whose ray is done goes idle, and once REFILL_MIN lanes are
 // idle one atomic claims new rays for all of them (ballot + prefix count), so long rays do not hold
 // a whole wave hostage.  The first launch gives each lane SceneView::short_stack live entries (plus
 // three spare slots for the branch-free push).  The rare ray
@@ -58,6 +59,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                                                   int* __restrict__ retry, int* __restrict__ n_retry, int cap,
                                                   TraceStats* __restrict__ stats, HitRec* __restrict__ vrec) {
     extern __shared__ int stk_all[];
+    __shared__ int leaf_mark[BLOCK];  // the leaf round's run starts, 64 per wave
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     int* const stk = stk_all + tid;
@@ -118,9 +120,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
             if (drained && rb == re) break;
             continue;
         }
-        if (i >= 0) {
+        if (i >= 0)
             trav_expand_wave4<STATS, BLOCK, ANY ? SHADOW_SEARCHERS : CLOSEST_SEARCHERS>(sv, sr, stk, cap, ts, n_pf);
-            if (ts.has_leaf) done = trav_leaf<ANY, STATS>(sv, o, d, double(wr), ts, bt, bb1, bb2, bprim, blf, n_tt);
+        // the leaf round: every lane of the wave takes part, idle ones included
+        if (trav_leaf_wave<ANY, STATS>(sv, o, d, wr, i >= 0 && ts.has_leaf, ts, bt, bb1, bb2, bprim, blf, n_tt,
+                                       leaf_mark + (tid & ~63), lane))
+            done = true;
+        if (i >= 0) {
             // ---- finished: write the result, go idle
             if ((ts.sp == 0 && !ts.has_leaf) || done || ts.overflow) {
                 if (ts.overflow && retry) {  // traced again from the start with the full stack
@@ -226,7 +232,8 @@ void launch(const SceneView& sv, const Ray* rays, const int* n, HitRec* hits, ui
             const int* order) {
     const int full = sv.wide_stack_cap;
     const size_t shmem_full = size_t(full + 3) * BLOCK * sizeof(int);  // + the spare push slots
-    if (shmem_full > device_max_lds()) throw std::runtime_error("BVH too deep for the LDS traversal stack");
+    if (shmem_full + BLOCK * sizeof(int) > device_max_lds())  // + the leaf round's marks
+        throw std::runtime_error("BVH too deep for the LDS traversal stack");
     const int cap = retry ? std::min(full, std::max(1, sv.short_stack)) : full;
     int* const rl = cap < full ? retry : nullptr;
     const size_t shmem = size_t(cap + 3) * BLOCK * sizeof(int);
