@@ -38,10 +38,12 @@ constexpr int BLOCK = 256;
 // rays at once instead of a few waves working through long private ranges (the launch then
 // takes about one ray's latency, not several).
 constexpr int REFILL_MIN = 24;
-// Leaf-round thresholds of the wave-cooperative expansion (sweeps of 4 to 32 on the 720p terrain
-// frame put the optimum near these values; within a few per cent of each other).
-constexpr int CLOSEST_SEARCHERS = 24;
-constexpr int SHADOW_SEARCHERS = 8;
+// Leaf-round thresholds of the wave-cooperative expansion.  With the lane-parallel leaf round a
+// round costs about ceil(triangles / 64) triangle tests, so it pays to start one earlier than with
+// the per-lane loop (which favoured 24 / 8): on the 720p terrain frame 28-36 (closest) and 12-20
+// (shadow) are within 0.5 % of each other and 0.5-0.8 % ahead of 24 / 8.
+constexpr int CLOSEST_SEARCHERS = 32;
+constexpr int SHADOW_SEARCHERS = 16;
 constexpr int CLAIM = 128;
 // Live stack entries of the first launch default to SceneView::short_stack = 16:
 // (16 + 3) x 256 x 4 B = 19 KB of LDS per block.  The kernel is compiled for 6 waves per SIMD
