@@ -37,6 +37,11 @@ VALU_PEAK_TLANEOPS = 256 * 4 * 16 * 2.4e9 / 1e12
 # algorithmic bytes of one closest-hit ray: 128 B per 4-wide node fetch (four child boxes and
 # references), 48 B per triangle test, 32 B ray in, 16 B hit out
 B_NODE, B_TRI, B_RAY = 128, 48, 48
+# algorithmic FP32 operations of the reference's tests: a slab test (bvhnode.cpp:35-48) is 6 sub +
+# 6 div + 12 min / max / compare = 24, so a 4-wide node fetch (four child boxes) is 96; a triangle
+# test (triangle.cpp:27-33) is 9 sub + 12 for the two crosses + 4 dots + 3 div = 44 (each one
+# lane-operation, whatever the instructions the kernel spends on it)
+OPS_NODE, OPS_TRI = 4 * 24, 44
 
 
 # crops of the headline frame checked against the FP64 oracle (x0, y0, x1, y1)
@@ -403,32 +408,37 @@ def traversal_roofline(cnt, closest_ms, launches, steps):
 
     Its counters show it bound by VALU issue, not by HBM: the exactness contract fixes the work
     (every box the reference's slab test accepts is visited), and the node and triangle bytes are
-    mostly served by the L2s and the Infinity Cache.  So the roofline is VALU issue: useful
-    lane-operations per second (VALU instructions x active lanes) against 39.3 T lane-ops/s.  The
-    lane-operations per ray come from the committed counter run
-    (profiles/trace_closest_720p_lane_ops.txt: SQ_INSTS_VALU x SQ_THREAD_CYCLES_VALU /
-    SQ_ACTIVE_INST_VALU over the same frame's rays); the rays and the launch time from this run.
+    mostly served by the L2s and the Infinity Cache.  So the roofline is VALU issue against 39.3 T
+    lane-operations/s, and `achieved` is algorithmic: the reference's FP32 operations per box test
+    and per triangle test (OPS_NODE, OPS_TRI) times the box and triangle tests of this run's own
+    counter pass, over this run's launch time.  Beside it, `lane_slots`: the lane-operations the
+    kernel actually issues (VALU instructions x active lanes, bookkeeping included) from the
+    committed counter run profiles/trace_closest_720p_lane_ops.txt, over the same rays and time.
     HBM: the measured traffic per launch (FETCH_SIZE / WRITE_SIZE counters calibrated on a copy
     of known size, profiles/trace_closest_720p_traffic.txt) over this run's launch time.  The
     no-reuse byte count (128 B per node fetch, 48 B per triangle test and per ray) is reported as
     bytes_no_reuse_per_launch: it counts cache hits as traffic, so it is not a fraction of HBM peak."""
     avg_ms = closest_ms / max(launches, 1)
-    rays_launch = cnt["rays"] / max(cnt["closest_launches"], 1)
+    n_launch = max(cnt["closest_launches"], 1)
+    rays_launch = cnt["rays"] / n_launch
+    alg_ops = (OPS_NODE * cnt["node_fetch"] + OPS_TRI * cnt["tri_tests"]) / n_launch
+    achieved = alg_ops / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else None
     lane = profile_figures("trace_closest_720p_lane_ops.txt")
     lane_ops_ray = lane.get("lane_ops_per_ray")
-    achieved = lane_ops_ray * rays_launch / (avg_ms * 1e-3) / 1e12 if lane_ops_ray and avg_ms > 0 else None
+    issued = lane_ops_ray * rays_launch / (avg_ms * 1e-3) / 1e12 if lane_ops_ray and avg_ms > 0 else None
     hbm = profile_figures("trace_closest_720p_traffic.txt")
     traffic = hbm.get("traffic_bytes_per_launch")
     hbm_gbs = traffic / (avg_ms * 1e-3) / 1e9 if traffic and avg_ms > 0 else None
-    no_reuse = (B_NODE * cnt["node_fetch"] + B_TRI * cnt["tri_tests"] + B_RAY * cnt["rays"]) / max(
-        cnt["closest_launches"], 1)
+    no_reuse = (B_NODE * cnt["node_fetch"] + B_TRI * cnt["tri_tests"] + B_RAY * cnt["rays"]) / n_launch
     return {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TLANEOPS, "unit": "T lane-ops/s",
             "frac": achieved / VALU_PEAK_TLANEOPS if achieved else None, "traffic": traffic,
             "kernel": "k_trace<closest>", "avg_launch_ms": avg_ms, "launches_per_frame": launches / steps,
-            "rays_per_launch": rays_launch, "lane_ops_per_ray": lane_ops_ray,
-            "valu_busy_in_counter_run": lane.get("valu_busy"),
-            "active_lanes_per_valu_inst": lane.get("active_lanes_per_inst"),
-            "lane_ops_source": lane.get("source"),
+            "rays_per_launch": rays_launch, "alg_ops_per_launch": alg_ops,
+            "alg_ops_per_node_fetch": OPS_NODE, "alg_ops_per_tri_test": OPS_TRI,
+            "lane_slots": {"achieved": issued, "frac": issued / VALU_PEAK_TLANEOPS if issued else None,
+                           "lane_ops_per_ray": lane_ops_ray, "valu_busy_in_counter_run": lane.get("valu_busy"),
+                           "active_lanes_per_valu_inst": lane.get("active_lanes_per_inst"),
+                           "source": lane.get("source")},
             "hbm": {"achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": hbm_gbs / HBM_PEAK_GBS if hbm_gbs else None, "traffic": traffic,
                     "traffic_source": hbm.get("source")},
