@@ -9,8 +9,8 @@
 // shade() handles one path vertex: miss -> environment, emitter -> Le * fac (MIS weight of the
 // previous vertex's NEE direction, rendererptls.cpp:33-36), back face -> stop, NEE with the
 // reference's MIS weight and shadow-ray test, Russian roulette, BSDF sampling.  New shadow
-// and extension rays are appended with one atomic per wave: ballot the lanes that emit,
-// prefix-count them, and let lane 0 reserve the wave's range.
+// and extension rays are appended with one atomic per block and queue: ballot the lanes that
+// emit, prefix-count them, and let one thread reserve the block's range.
 //
 // Random numbers come from Philox4x32-10 keyed by (seed, pixel, sample, vertex, slot), so any
 // split of pixels or samples over batches or devices gives the same per-sample results.
@@ -25,6 +25,22 @@
 #include "render.h"
 
 namespace nr {
+
+namespace {
+// Counters of the wavefront, each on a 128-byte line of its own: device-scope atomics on one line
+// are serialised (about 11 ns each, 88 M/s on MI355X), while atomics on different lines proceed in
+// parallel, so the shade step's two queue appends, and the ray claims of two trace launches that
+// run side by side, do not queue behind each other.  C_WORK0 / C_WORK1 hold three ints: ray claims,
+// retry count, retry claims of a trace launch.  The shadow queue's count and its launch's claims
+// alternate by bounce parity (C_SHADOW / C_SHADOW_B, C_WORK1 / C_WORK1_B): bounce k's shadow launch
+// may still run while bounce k + 1 resets its own.
+constexpr int CSTRIDE = 32;  // ints per line
+enum { C_RAY0 = 0, C_RAY1 = 1, C_SHADOW = 2, C_SHADOW_B = 3, C_ERR = 4, C_QUERY = 5, C_REC = 6, C_WORK0 = 7,
+       C_WORK1 = 8, C_WORK1_B = 9, N_COUNTER_LINES = 10 };
+__host__ __device__ constexpr int at(int line) { return line * CSTRIDE; }
+__host__ __device__ constexpr int shadow_slot(int it) { return at((it & 1) ? C_SHADOW_B : C_SHADOW); }
+__host__ __device__ constexpr int work1_slot(int it) { return at((it & 1) ? C_WORK1_B : C_WORK1); }
+}  // namespace
 
 CamView make_camera(const float pos[3], const float euler[3], float fov, float aspect) {
     CamView c;
@@ -75,9 +91,11 @@ void RenderTarget::ensure_queues(size_t c) {
         retry_sh.ensure(sizeof(int) * c);
         cap = c;
     }
-    counters.ensure(sizeof(int) * 24);
+    counters.ensure(sizeof(int) * N_COUNTER_LINES * CSTRIDE);
     stats.ensure(sizeof(TraceStats) * 2);
-    if (!pinned) check_hip(hipHostMalloc((void**)&pinned, sizeof(int) * 16, hipHostMallocDefault), "hipHostMalloc");
+    if (!pinned)
+        check_hip(hipHostMalloc((void**)&pinned, sizeof(int) * N_COUNTER_LINES * CSTRIDE, hipHostMallocDefault),
+                  "hipHostMalloc");
 }
 
 namespace {
@@ -85,20 +103,12 @@ namespace {
 constexpr int BLK = 256;
 constexpr int LDS_LIGHTS = 2048;  // light CDFs up to this size are searched in LDS
 
-// Counter slots
-// (C_WORK0 and C_WORK1 are three ints each: ray claims, retry count, retry claims of a trace launch).
-// The shadow queue's count and its launch's claims alternate by bounce parity (C_SHADOW / C_SHADOW_B,
-// C_WORK1 / C_WORK1_B): bounce k's shadow launch may still run while bounce k + 1 resets its own.
-enum { C_RAY0 = 0, C_RAY1 = 1, C_SHADOW = 2, C_SHADOW_B = 3, C_ERR = 5, C_QUERY = 6, C_REC = 7, C_WORK0 = 8,
-       C_WORK1 = 11, C_WORK1_B = 16 };
-__host__ __device__ constexpr int shadow_slot(int it) { return (it & 1) ? C_SHADOW_B : C_SHADOW; }
 // Below this many paths per wavefront batch the trace launches are bound by their tails (a few
 // rays per lane: the launch lasts as long as its slowest rays), and a bounce's shadow launch fills
 // the next closest-hit launch's idle CUs (1080p x 1 spp frames: 10-14 % faster).  Larger batches
 // fill the GPU on their own: overlapping them gained 3 % at 720p x 64 spp while stretching every
 // launch, so they run one launch after the other.
 constexpr int OVERLAP_PATHS = 1 << 24;
-__host__ __device__ constexpr int work1_slot(int it) { return (it & 1) ? C_WORK1_B : C_WORK1; }
 
 __global__ void k_zero_doubles(double* a, double* b, size_t n) {
     for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
@@ -108,18 +118,18 @@ __global__ void k_zero_doubles(double* a, double* b, size_t n) {
 }
 
 __global__ void k_clear_counters(int* c) {
-    if (threadIdx.x < 24) c[threadIdx.x] = 0;
+    if (threadIdx.x < N_COUNTER_LINES * 3) c[at(threadIdx.x / 3) + threadIdx.x % 3] = 0;
 }
 
 // Before bounce `it`: the ray counter (raygen) or the next queue's counter, and the counters of
 // this bounce's closest-hit and shadow launches and of its shadow queue.
 __global__ void k_set_counters(int* c, int ray_slot, int ray_value, int clear_slot, int it) {
     if (threadIdx.x == 0) {
-        if (ray_slot >= 0) c[ray_slot] = ray_value;
-        if (clear_slot >= 0) c[clear_slot] = 0;
+        if (ray_slot >= 0) c[at(ray_slot)] = ray_value;
+        if (clear_slot >= 0) c[at(clear_slot)] = 0;
         c[shadow_slot(it)] = 0;
         for (int k = 0; k < 3; ++k) {
-            c[C_WORK0 + k] = 0;
+            c[at(C_WORK0) + k] = 0;
             c[work1_slot(it) + k] = 0;
         }
     }
@@ -702,14 +712,14 @@ int wavefront_batch(Scene& sc, RenderTarget& t, const CamView& cam, const Render
         // optional direction grouping of secondary and shadow rays (primary rays: pixel order)
         int* const ord = sc.trace_order ? t.order.as<int>() : nullptr;
         mark_event(ev_c, s);
-        if (ord && it > 0) sort_by_direction(t.rays[cur].as<Ray>(), cnt + C_RAY0 + cur, P, sc.trace_order, ord, s);
-        launch_closest(sc.view, t.rays[cur].as<Ray>(), cnt + C_RAY0 + cur, t.hits.as<HitRec>(), cnt + C_WORK0,
-                       cnt + C_ERR, ts, s, t.retry.as<int>(), (ord && it > 0) ? ord : nullptr);
+        if (ord && it > 0) sort_by_direction(t.rays[cur].as<Ray>(), cnt + at(C_RAY0 + cur), P, sc.trace_order, ord, s);
+        launch_closest(sc.view, t.rays[cur].as<Ray>(), cnt + at(C_RAY0 + cur), t.hits.as<HitRec>(), cnt + at(C_WORK0),
+                       cnt + at(C_ERR), ts, s, t.retry.as<int>(), (ord && it > 0) ? ord : nullptr);
         mark_event(ev_c, s);
         ShadeArgs a{sc.view, npix, p.seed, sample0, p.prr, p.mis, p.maxdepth, p.rrdepth,
                     p.integrator == 1 ? 0 : 1,
-                    t.rays[cur].as<Ray>(), t.beta[cur].as<float4>(), t.hits.as<HitRec>(), cnt + C_RAY0 + cur,
-                    t.radiance.as<float4>(), t.rays[nxt].as<Ray>(), t.beta[nxt].as<float4>(), cnt + C_RAY0 + nxt,
+                    t.rays[cur].as<Ray>(), t.beta[cur].as<float4>(), t.hits.as<HitRec>(), cnt + at(C_RAY0 + cur),
+                    t.radiance.as<float4>(), t.rays[nxt].as<Ray>(), t.beta[nxt].as<float4>(), cnt + at(C_RAY0 + nxt),
                     t.shadow.as<Ray>(), t.contrib.as<float4>(), sh_cnt,
                     env ? t.skyw.as<float>() : nullptr, env ? P : 0, vrec, VREC_CAP};
         if (it > 0) join();  // the previous bounce's shadow launch has read the shadow queue
@@ -728,7 +738,7 @@ int wavefront_batch(Scene& sc, RenderTarget& t, const CamView& cam, const Render
         mark_event(ev_s, s2);
         if (ord) sort_by_direction(t.shadow.as<Ray>(), sh_cnt, env ? 2 * P : P, sc.trace_order, ord + t.cap, s2);
         launch_shadow(sc.view, t.shadow.as<Ray>(), sh_cnt, nullptr, nullptr, t.contrib.as<float4>(),
-                      t.radiance.as<float4>(), cnt + work1_slot(it), cnt + C_ERR, ts ? ts + 1 : nullptr, s2,
+                      t.radiance.as<float4>(), cnt + work1_slot(it), cnt + at(C_ERR), ts ? ts + 1 : nullptr, s2,
                       t.retry_sh.as<int>(), vrec, ord ? ord + t.cap : nullptr);
         mark_event(ev_s, s2);
         check_hip(hipEventRecord(sc.shadow_done, s2), "shadow done");
@@ -736,7 +746,7 @@ int wavefront_batch(Scene& sc, RenderTarget& t, const CamView& cam, const Render
         if (p.maxdepth > 0) {
             if (it + 1 >= p.maxdepth) break;
         } else {
-            check_hip(hipMemcpyAsync(t.pinned, cnt + C_RAY0 + nxt, sizeof(int), hipMemcpyDeviceToHost, s), "cnt");
+            check_hip(hipMemcpyAsync(t.pinned, cnt + at(C_RAY0 + nxt), sizeof(int), hipMemcpyDeviceToHost, s), "cnt");
             check_hip(hipStreamSynchronize(s), "sync");
             if (*t.pinned == 0) break;
             if (cancel && *cancel) {
@@ -770,9 +780,9 @@ bool wavefront_radiance(Scene& sc, const CamView& cam, const RenderParams& p, ui
     wavefront_batch(sc, t, cam, p, sample0, P, false, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                     iterations, records ? t.adj_vrec.as<HitRec>() : nullptr);
     check_hip(hipGetLastError(), "wavefront launch");
-    check_hip(hipMemcpyAsync(t.pinned, cnt, sizeof(int) * 8, hipMemcpyDeviceToHost, sc.stream), "counters");
+    check_hip(hipMemcpyAsync(t.pinned, cnt, sizeof(int) * at(C_REC + 1), hipMemcpyDeviceToHost, sc.stream), "counters");
     check_hip(hipStreamSynchronize(sc.stream), "wavefront sync");
-    if (t.pinned[C_ERR]) throw std::runtime_error("traversal stack overflow");
+    if (t.pinned[at(C_ERR)]) throw std::runtime_error("traversal stack overflow");
     return records;
 }
 
@@ -836,7 +846,7 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
         float4* const rad_env = env ? t.radiance.as<float4>() + P : nullptr;
         if (paths) {  // one lane per path: the same per-sample radiance without per-bounce launches
             mark_event(kt ? &ev_c : nullptr, s);
-            render_paths(sc, cam, p, uint32_t(p.spp_offset + b), P, t.radiance.as<float4>(), cnt + C_ERR,
+            render_paths(sc, cam, p, uint32_t(p.spp_offset + b), P, t.radiance.as<float4>(), cnt + at(C_ERR),
                          keep_vrec ? t.adj_vrec.as<HitRec>() : nullptr, VREC_CAP);
             mark_event(kt ? &ev_c : nullptr, s);
             rs.iterations++;
@@ -856,10 +866,10 @@ int render_ptnee(Scene& sc, const CamView& cam, const RenderParams& p, ProgressF
             cb(float(b + k) / float(p.spp), user);
         }
     }
-    check_hip(hipMemcpyAsync(t.pinned, cnt, sizeof(int) * 8, hipMemcpyDeviceToHost, s), "counters");
+    check_hip(hipMemcpyAsync(t.pinned, cnt, sizeof(int) * at(C_REC + 1), hipMemcpyDeviceToHost, s), "counters");
     check_hip(hipStreamSynchronize(s), "render sync");
     if (status) t.adj_valid = false;
-    if (t.pinned[C_ERR]) throw std::runtime_error("traversal stack overflow");
+    if (t.pinned[at(C_ERR)]) throw std::runtime_error("traversal stack overflow");
     rs.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (kt) {
         *kt = KernelTimes{};
@@ -932,16 +942,16 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
         const int cur = it & 1, nxt = cur ^ 1;
         int* const sh_cnt = cnt + shadow_slot(it);
         hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, -1, 0, int(C_RAY0 + nxt), it);
-        launch_closest(sc.view, t.rays[cur].as<Ray>(), cnt + C_RAY0 + cur, t.hits.as<HitRec>(), cnt + C_WORK0,
-                       cnt + C_ERR, nullptr, s, t.retry.as<int>());
+        launch_closest(sc.view, t.rays[cur].as<Ray>(), cnt + at(C_RAY0 + cur), t.hits.as<HitRec>(), cnt + at(C_WORK0),
+                       cnt + at(C_ERR), nullptr, s, t.retry.as<int>());
         ShadeArgs a{sc.view, npix, p.seed, uint32_t(p.spp_offset), p.prr, p.mis, p.maxdepth, p.rrdepth,
                     p.integrator == 1 ? 0 : 1,
-                    t.rays[cur].as<Ray>(), t.beta[cur].as<float4>(), t.hits.as<HitRec>(), cnt + C_RAY0 + cur,
-                    t.radiance.as<float4>(), t.rays[nxt].as<Ray>(), t.beta[nxt].as<float4>(), cnt + C_RAY0 + nxt,
+                    t.rays[cur].as<Ray>(), t.beta[cur].as<float4>(), t.hits.as<HitRec>(), cnt + at(C_RAY0 + cur),
+                    t.radiance.as<float4>(), t.rays[nxt].as<Ray>(), t.beta[nxt].as<float4>(), cnt + at(C_RAY0 + nxt),
                     t.shadow.as<Ray>(), t.contrib.as<float4>(), sh_cnt, env ? t.skyw.as<float>() : nullptr,
                     env ? int(cap) : 0};
         NrcShade ns{t.nstate[cur].as<float4>(), t.nstate[nxt].as<float4>(), t.nrc_queries.as<NrcQuery>(),
-                    t.nrc_recs.as<NrcQuery>(), t.nrc_lb.as<float4>(), cnt + C_QUERY, cnt + C_REC, int(cap),
+                    t.nrc_recs.as<NrcQuery>(), t.nrc_lb.as<float4>(), cnt + at(C_QUERY), cnt + at(C_REC), int(cap),
                     int(rec_cap), int(P), int(T), np.c_spread, np.train_maxlen, np.use_cache,
                     std::max(1, np.min_depth)};
         if (it > 0) join();  // the previous bounce's shadow launch has read the shadow queue
@@ -956,13 +966,13 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
         check_hip(hipEventRecord(sc.shadow_ready, s), "shade done");
         check_hip(hipStreamWaitEvent(s2, sc.shadow_ready, 0), "shadow after shade");
         launch_shadow(sc.view, t.shadow.as<Ray>(), sh_cnt, nullptr, nullptr, t.contrib.as<float4>(),
-                      t.radiance.as<float4>(), cnt + work1_slot(it), cnt + C_ERR, nullptr, s2, t.retry_sh.as<int>());
+                      t.radiance.as<float4>(), cnt + work1_slot(it), cnt + at(C_ERR), nullptr, s2, t.retry_sh.as<int>());
         check_hip(hipEventRecord(sc.shadow_done, s2), "shadow done");
         rs.iterations++;
         if (p.maxdepth > 0) {
             if (it + 1 >= p.maxdepth) break;
         } else {
-            check_hip(hipMemcpyAsync(t.pinned, cnt + C_RAY0 + nxt, sizeof(int), hipMemcpyDeviceToHost, s), "cnt");
+            check_hip(hipMemcpyAsync(t.pinned, cnt + at(C_RAY0 + nxt), sizeof(int), hipMemcpyDeviceToHost, s), "cnt");
             check_hip(hipStreamSynchronize(s), "sync");
             if (*t.pinned == 0) break;
         }
@@ -972,18 +982,18 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
         hipLaunchKernelGGL(k_fold_env, dim3((cap + BLK - 1) / BLK), dim3(BLK), 0, s, t.radiance.as<float4>(), rad_env,
                            int(cap));
     // the cache answers every stopped path (render paths and training-path tails)
-    nrc.infer_accumulate(t.nrc_queries.as<NrcQuery>(), cnt + C_QUERY, int(cap), t.radiance.as<float4>(), s);
+    nrc.infer_accumulate(t.nrc_queries.as<NrcQuery>(), cnt + at(C_QUERY), int(cap), t.radiance.as<float4>(), s);
     hipLaunchKernelGGL(k_finalize, dim3((npix + BLK - 1) / BLK), dim3(BLK), 0, s, t.radiance.as<float4>(), nullptr, npix,
                        p.spp, t.sum.as<double>(), t.sum2.as<double>());
     check_hip(hipGetLastError(), "nrc frame launch");
     t.spp_total += p.spp;
     rs.samples = (long long)P;
-    check_hip(hipMemcpyAsync(t.pinned, cnt, sizeof(int) * 8, hipMemcpyDeviceToHost, s), "counters");
+    check_hip(hipMemcpyAsync(t.pinned, cnt, sizeof(int) * at(C_REC + 1), hipMemcpyDeviceToHost, s), "counters");
     check_hip(hipStreamSynchronize(s), "nrc frame sync");
-    if (t.pinned[C_ERR]) throw std::runtime_error("traversal stack overflow");
+    if (t.pinned[at(C_ERR)]) throw std::runtime_error("traversal stack overflow");
     NrcFrameStats f;
-    f.queries = t.pinned[C_QUERY];
-    f.records = std::min<long long>(t.pinned[C_REC], (long long)rec_cap);
+    f.queries = t.pinned[at(C_QUERY)];
+    f.records = std::min<long long>(t.pinned[at(C_REC)], (long long)rec_cap);
     t.nrc_last_records = int(std::min<long long>(f.records, nrc.cfg.max_batch));
     if (t.nrc_last_records > 0)  // targets (L_path - L_before) / beta of the frame's records
         nrc_targets(t.nrc_recs.as<NrcQuery>(), t.nrc_lb.as<float4>(), t.radiance.as<float4>(), t.nrc_last_records,

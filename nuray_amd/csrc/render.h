@@ -63,7 +63,7 @@ struct RenderTarget {
     DevBuf skyw;                       // sky sampling: BSDF-escape MIS weight per path slot
     DevBuf retry, retry_sh;            // rays the closest-hit / shadow launch hands to its full-stack retry launch
     DevBuf order;                      // traversal order of the ray queue, then of the shadow queue
-    DevBuf counters;                   // int[16]
+    DevBuf counters;                   // the wavefront's counters, one 128-byte line each
     DevBuf stats;                      // TraceStats[2]
     int* pinned = nullptr;             // host-visible counter readback
     long long spp_total = 0;
