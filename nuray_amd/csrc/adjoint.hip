@@ -151,8 +151,8 @@ struct AdjArgs {
     int nee;                 // 1: ptnee, 0: pt (no light sampling)
     int rad_off;             // radiance[] holds samples from this index on
     unsigned* claim;         // samples claimed so far in this launch (zero at launch); a launch covers
-                             // fewer than 2^31 samples and each lane asks at most once after the last
-                             // one is handed out, so the count stays below 2^32
+                             // fewer than 2^31 samples and each wave claims once more at most after
+                             // the last one is handed out, so the count stays below 2^32
     HitRec* vrec;            // per-vertex records of the first vrec_cap vertices of each sample, at
     int vrec_cap;            // vrec[(slot - rad_off) * vrec_cap + depth], or null
 };
@@ -161,19 +161,56 @@ struct AdjArgs {
 // them; the replay of the same samples reads them instead of tracing both rays again, and traces
 // the shadow ray itself where the record has none.
 
-// Hands the next samples of [a.slot0, a.slot1) to the lanes of the wave that ask for one (need):
-// one atomic per wave, samples in lane order.  Called by every lane of the wave at the same point.
-// Returns the lane's new sample, or a.slot1 (no more samples) when it asked; else keeps `slot`.
-__device__ __forceinline__ int claim_samples(const AdjArgs& a, bool need, int slot) {
+// Per-wave sample source: a private range [rb, re) of claim offsets, refilled from the launch's
+// counter.  Device-scope atomics on one address serialise (about 11 ns each), so one atomic per
+// round of every wave bounds a launch of short steps (the replay reads its vertices instead of
+// tracing them: 1.27 -> 0.68 ms per 256^2 x 32 spp replay with 128-sample refills).  Guided
+// refills (a quarter of the samples left per wave, at most 128) suit long paths (the forward
+// render): the ranges shrink towards the end of the launch, so no wave is left holding many
+// samples while others idle.  A refill is never smaller than what the asking lanes need.
+struct SampleSource {
+    unsigned rb = 0, re = 0, chunk = 1, waves = 1;
+    bool drained = false, guided = false;
+};
+__device__ __forceinline__ unsigned refill_size(unsigned left, unsigned waves, bool guided) {
+    return max(1u, min(128u, guided ? left / (4u * waves) : (left + waves - 1) / waves));
+}
+__device__ __forceinline__ SampleSource sample_source(const AdjArgs& a, bool guided) {
+    SampleSource src;
+    src.waves = gridDim.x * (blockDim.x / 64);
+    src.guided = guided;
+    src.chunk = refill_size(unsigned(a.slot1 - a.slot0), src.waves, guided);
+    return src;
+}
+
+// Hands the next samples of [a.slot0, a.slot1) to the lanes of the wave that ask for one (need),
+// in lane order, from the wave's private range (claiming more when it runs short).  Called by every
+// lane of the wave at the same point.  Returns the lane's new sample, or a.slot1 (no more samples)
+// when it asked; else keeps `slot`.
+__device__ __forceinline__ int claim_samples(const AdjArgs& a, SampleSource& src, bool need, int slot) {
     const int lane = int(threadIdx.x & 63);
     const unsigned long long ask = __ballot(need);
     if (ask == 0ull) return slot;
-    unsigned base = 0;
-    if (lane == 0) base = atomicAdd(a.claim, unsigned(__popcll(ask)));
-    base = __shfl(base, 0, 64);
+    const unsigned n_ask = unsigned(__popcll(ask)), rank = unsigned(__popcll(ask & ((1ull << lane) - 1ull)));
+    const unsigned have = src.re - src.rb;
+    unsigned pick;  // this lane's claim offset (wave-uniform bookkeeping below)
+    if (n_ask <= have || src.drained) {
+        pick = rank < have ? src.rb + rank : ~0u;
+        src.rb += min(n_ask, have);
+    } else {
+        const unsigned want = max(src.chunk, n_ask - have);
+        unsigned base = 0;
+        if (lane == 0) base = atomicAdd(a.claim, want);
+        base = __shfl(base, 0, 64);
+        const unsigned total = unsigned(a.slot1 - a.slot0);
+        pick = rank < have ? src.rb + rank : base + (rank - have);
+        src.rb = base + (n_ask - have);
+        src.re = base + want;
+        if (base + want >= total) src.drained = true;
+        else if (src.guided) src.chunk = refill_size(total - (base + want), src.waves, true);
+    }
     if (!need) return slot;
-    const long long s = (long long)a.slot0 + (long long)base + __popcll(ask & ((1ull << lane) - 1ull));
-    return s < a.slot1 ? int(s) : a.slot1;
+    return pick < unsigned(a.slot1 - a.slot0) ? a.slot0 + int(pick) : a.slot1;
 }
 
 __device__ __forceinline__ void gadd(float* lds, double* glob, bool use_lds, int idx, float v) {
@@ -418,8 +455,9 @@ __global__ __launch_bounds__(ABLK) void k_adjoint(AdjArgs a) {
     // slope where L <= 0 or NaN -- ends at once and the lane asks again next round.
     int slot = a.slot0;
     bool need = true, active = false;
+    SampleSource src = sample_source(a, false);
     for (;;) {
-        slot = claim_samples(a, need, slot);
+        slot = claim_samples(a, src, need, slot);
         if (need) {
             need = false;
             if (slot < a.slot1) {
@@ -459,8 +497,9 @@ __global__ __launch_bounds__(ABLK) __attribute__((amdgpu_waves_per_eu(4))) void 
     PathState p;
     int slot = a.slot0;
     bool need = true, active = false;
+    SampleSource src = sample_source(a, true);
     for (;;) {
-        slot = claim_samples(a, need, slot);
+        slot = claim_samples(a, src, need, slot);
         if (need) {
             need = false;
             active = slot < a.slot1;
