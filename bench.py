@@ -468,6 +468,9 @@ def main():
                     help="only time the radiance-cache MLP kernels (inference + training step) and exit")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the plain 1 GiB device copy (HBM counter calibration) and exit")
+    ap.add_argument("--atomics-only", action="store_true",
+                    help="only measure device-scope atomic adds on counters sharing a 128-byte line or not, "
+                         "and exit")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only time the inverse-rendering iteration (and the reference box room) and exit")
     args = ap.parse_args()
@@ -483,6 +486,13 @@ def main():
     copy_gbs = nr.bench_copy(1 << 30, 5)
     if args.copy_only:
         print(json.dumps({"copy_bytes": 1 << 30, "reps": 5, "copy_gbs": copy_gbs}), flush=True)
+        return
+    if args.atomics_only:
+        # (counters, ints apart): one counter; two in one 128-byte line; two, eight and 32 on lines of
+        # their own
+        cases = [(1, 1), (2, 1), (2, 32), (8, 32), (32, 32)]
+        print(json.dumps({"atomic_adds_mops": [{"counters": c, "bytes_apart": 4 * s,
+                                                 "mops": nr.bench_atomics(c, s)} for c, s in cases]}), flush=True)
         return
     if args.mlp_only:
         print(json.dumps(nrc_kernels(nr)), flush=True)

@@ -268,6 +268,13 @@ int nr_nrc_frame_records(nr_scene* s, float* recs, float* targets, int32_t cap, 
  * hardware byte counters (a known number of HBM bytes per dispatch) and the achievable HBM rate. */
 int nr_bench_copy(uint64_t bytes, int32_t reps, double* gbs);
 
+/* No reference counterpart: the rate (*mops, million per second) of device-scope atomic adds in
+ * the wavefront's append pattern -- 5 blocks of 256 threads per CU, each block iteration one add
+ * by one thread, waited for by the block -- with block b adding to counter b % counters, the
+ * counters stride_ints ints apart, iters iterations per block.  Measures how such atomics share a
+ * cache line (the layout of the wavefront's counters follows from it). */
+int nr_bench_atomics(int32_t counters, int32_t stride_ints, int32_t iters, double* mops);
+
 #ifdef __cplusplus
 }
 #endif

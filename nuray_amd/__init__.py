@@ -24,7 +24,7 @@ EXPORTS = [
     "nr_last_error", "nr_init", "nr_synchronize", "nr_scene_create", "nr_scene_destroy",
     "nr_scene_set_texture_rgba8", "nr_scene_set_envmap", "nr_bvh_build_sah", "nr_bvh_info", "nr_bvh_export",
     "nr_bvh_build_host", "nr_intersect", "nr_render_ptnee", "nr_render_fetch", "nr_render_stats",
-    "nr_set_profiling", "nr_bench_copy", "nr_scene_set_params", "nr_loss_grad_l2", "nr_adjoint_ptnee",
+    "nr_set_profiling", "nr_bench_copy", "nr_bench_atomics", "nr_scene_set_params", "nr_loss_grad_l2", "nr_adjoint_ptnee",
     "nr_nrc_create", "nr_nrc_destroy", "nr_nrc_set_weights", "nr_nrc_get_weights", "nr_nrc_infer",
     "nr_nrc_train_step", "nr_nrc_gradient", "nr_render_nrc", "nr_nrc_frame_records", "nr_nrc_bench", "nr_scene_set_texture_rgb32f",
     "nr_adjoint_ptnee_tex", "nr_scene_set_material_ext", "nr_scene_set_short_stack", "nr_scene_set_trace_order", "nr_shutdown",
@@ -90,6 +90,7 @@ def lib():
         L.nr_scene_set_short_stack.argtypes = [vp, C.c_int32]
         L.nr_scene_set_trace_order.argtypes = [vp, C.c_int32]
         L.nr_bench_copy.argtypes = [C.c_uint64, C.c_int32, C.POINTER(C.c_double)]
+        L.nr_bench_atomics.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
         L.nr_scene_set_params.argtypes = [vp, f32p, C.c_uint32]
         L.nr_loss_grad_l2.argtypes = [vp, f32p, C.POINTER(C.c_double), vp]
         L.nr_adjoint_ptnee.argtypes = [vp, C.POINTER(nr_camera), C.POINTER(nr_params), vp, f64p]
@@ -123,6 +124,14 @@ def init(device=0):
 
 def synchronize():
     check(lib().nr_synchronize())
+
+
+def bench_atomics(counters, stride_ints, iters=90):
+    """Million device-scope atomic adds per second in the wavefront's append pattern (block b adds
+    to counter b % counters, the counters stride_ints ints apart)."""
+    m = C.c_double()
+    check(lib().nr_bench_atomics(counters, stride_ints, iters, C.byref(m)))
+    return m.value
 
 
 def bench_copy(nbytes, reps=5):

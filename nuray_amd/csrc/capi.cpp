@@ -11,6 +11,7 @@
 
 namespace nr {
 double bench_copy(size_t bytes, int reps);
+double bench_atomics(int n_cnt, int stride, int iters);
 }
 
 struct nr_scene {
@@ -339,6 +340,13 @@ int nr_adjoint_ptnee_tex(nr_scene* s, const nr_camera* cam, const nr_params* par
 int nr_bench_copy(uint64_t bytes, int32_t reps, double* gbs) {
     return guard([&] {
         *gbs = nr::bench_copy(size_t(bytes), reps);
+        return NR_OK;
+    });
+}
+
+int nr_bench_atomics(int32_t counters, int32_t stride_ints, int32_t iters, double* mops) {
+    return guard([&] {
+        *mops = nr::bench_atomics(counters, stride_ints, iters);
         return NR_OK;
     });
 }
