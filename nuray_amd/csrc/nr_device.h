@@ -193,11 +193,18 @@ __device__ __forceinline__ bool tri_valid(float t, float b1, float b2) {
 // quotients (dd != 0: `finite`) the reference's swap / std::max / std::min chain reduces to
 // t_in = max(-1e18, min pairs), t_out = min(1e18, max pairs); max and min move by at most the
 // largest error of the elements that attain them, so |t_in' - t_in| <= 4.2 u |t_in'| + 1.2 u A,
-// and likewise for t_out.  Hence, with a20 = max(2^-20 A, 1e-30):
-//   |t_out'| > a20                                      => t_out >= 0  <=>  t_out' >= 0, and
-//   |t_out' - t_in'| > 2^-20 (|t_in'| + |t_out'|) + 2 a20 => t_in <= t_out <=> t_out' > t_in'
-// (2^-20 = 16 u covers 4.2 u per side plus the rounding of the difference; the 1e-30 floor keeps
-// every value the bounds rely on in the normal range).  Everything else takes the exact path.
+// and likewise for t_out.  Hence, with a20 = max(SLAB_BAND_ABS A, 1e-30), SLAB_BAND_ABS = 2^-22:
+//   |t_out'| > a20                                             => t_out >= 0 <=> t_out' >= 0, and
+//   |t_out' - t_in'| > SLAB_BAND_REL (|t_in'| + |t_out'|) + 2 a20 => t_in <= t_out <=> t_out' > t_in'
+// (SLAB_BAND_REL = 2^-21 = 8 u covers 4.2 u per side plus the rounding of the difference and of the
+// band itself; 2 a20 = 8 u A covers 2 x 1.2 u A; the 1e-30 floor keeps every value the bounds rely
+// on in the normal range).  Everything else takes the exact path.  A host check restates the
+// filter and the reference's test and compares every proven decision on 2e7 random and
+// adversarial ray / box pairs (tests/cpp/slab_filter_check.cpp: no difference at these constants
+// nor at 2^-22 / 2^-24; 814 at 2^-23 / 2^-25, below the bound).  On the 720p frame a lane of the
+// wave is inside a band in 6.8 % of the closest-hit node steps (14.5 % with 2^-20 / 2^-20) and in
+// 45 % of the shadow ones: shadow rays aimed at a light cross its flat leaf box, where t_in' and
+// t_out' come from the same axis.
 // Both comparisons use a2 = 2 a20 (the first one is then stricter than it needs to be), and
 // a2 = +inf when some dd component is zero, which sends every box of such a ray to the exact
 // path: one per-ray threshold and no branches in the filter.  The reference's -1e18 / 1e18 start
@@ -206,6 +213,8 @@ __device__ __forceinline__ bool tri_valid(float t, float b1, float b2) {
 // both z values lie beyond the same bound; then t_out' < -1e18 (both below) or t_in' >= t_out'
 // (both above), so the filter reports a miss, as the reference does, or hands the box to the
 // exact test.
+constexpr float SLAB_BAND_REL = 4.76837158203125e-07f;   // 2^-21
+constexpr float SLAB_BAND_ABS = 2.384185791015625e-07f;  // 2^-22
 struct SlabRay {
     F3 o, d, r, c;
     float a2;  // 2 a20, or +inf when some component of dd is zero
@@ -217,7 +226,7 @@ __device__ __forceinline__ SlabRay slab_ray(F3 o, F3 d) {
     const F3 dd = f3(d.x + 1e-9f, d.y + 1e-9f, d.z + 1e-9f);
     s.r = f3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
     s.c = f3(o.x * s.r.x, o.y * s.r.y, o.z * s.r.z);
-    const float a20 = fmaxf(fmaxf(fmaxf(fabsf(s.c.x), fabsf(s.c.y)), fabsf(s.c.z)) * 9.5367431640625e-07f, 1e-30f);
+    const float a20 = fmaxf(fmaxf(fmaxf(fabsf(s.c.x), fabsf(s.c.y)), fabsf(s.c.z)) * SLAB_BAND_ABS, 1e-30f);
     const bool finite = dd.x != 0.0f && dd.y != 0.0f && dd.z != 0.0f;
     s.a2 = finite ? 2.0f * a20 : __builtin_inff();
     return s;
@@ -232,7 +241,7 @@ __device__ __forceinline__ bool slab_filter(float mnx, float mny, float mnz, flo
     const float t_in = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), __builtin_amdgcn_fmed3f(az, bz, -1e18f));
     const float t_out = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), __builtin_amdgcn_fmed3f(az, bz, 1e18f));
     const float diff = t_out - t_in;
-    const float band = fmaf(9.5367431640625e-07f, fabsf(t_in) + fabsf(t_out), s.a2);  // 2^-20
+    const float band = fmaf(SLAB_BAND_REL, fabsf(t_in) + fabsf(t_out), s.a2);
     clear = (int(fabsf(t_out) > s.a2) & int(fabsf(diff) > band)) != 0;  // both tests, no branch
     return t_out >= 0.0f && diff > 0.0f;
 }
@@ -244,7 +253,7 @@ __device__ __forceinline__ bool slab_fast(float mnx, float mny, float mnz, float
     const float t_in = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), __builtin_amdgcn_fmed3f(az, bz, -1e18f));
     const float t_out = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), __builtin_amdgcn_fmed3f(az, bz, 1e18f));
     const float diff = t_out - t_in;
-    const float band = fmaf(9.5367431640625e-07f, fabsf(t_in) + fabsf(t_out), s.a2);  // 2^-20
+    const float band = fmaf(SLAB_BAND_REL, fabsf(t_in) + fabsf(t_out), s.a2);
     const bool clear = (int(fabsf(t_out) > s.a2) & int(fabsf(diff) > band)) != 0;
     if (__builtin_expect(clear, 1)) return t_out >= 0.0f && diff > 0.0f;
     return slab(mnx, mny, mnz, mxx, mxy, mxz, s.o, f3(s.d.x + 1e-9f, s.d.y + 1e-9f, s.d.z + 1e-9f));
