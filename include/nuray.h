@@ -162,7 +162,10 @@ int nr_render_fetch(nr_scene* s, float* rgb_linear, double* sum, double* sum2, u
  * (profiling level 2).
  * Kernel times are filled when the render ran with timing on (nr_set_profiling). */
 int nr_render_stats(nr_scene* s, double* out16);
-/* 1: time the traversal launches with HIP events; 2: also count node fetches / triangle tests. */
+/* 1: time the traversal launches with HIP events; 2: also count node fetches / triangle tests.
+ * With either on, every launch of a wavefront render runs on one stream (otherwise a bounce's
+ * shadow launch runs beside the next bounce's closest-hit launch), so that each kernel's time and
+ * counts are its own. */
 int nr_set_profiling(nr_scene* s, int32_t mode);
 /* Traversal tuning (no counterpart in the reference, whose traversal recurses on the CPU stack):
  * live LDS stack entries per lane of the first persistent trace launch (default 16, at least 1).

@@ -103,13 +103,6 @@ namespace {
 constexpr int BLK = 256;
 constexpr int LDS_LIGHTS = 2048;  // light CDFs up to this size are searched in LDS
 
-// Below this many paths per wavefront batch the trace launches are bound by their tails (a few
-// rays per lane: the launch lasts as long as its slowest rays), and a bounce's shadow launch fills
-// the next closest-hit launch's idle CUs (1080p x 1 spp frames: 10-14 % faster).  Larger batches
-// fill the GPU on their own: overlapping them gained 3 % at 720p x 64 spp while stretching every
-// launch, so they run one launch after the other.
-constexpr int OVERLAP_PATHS = 1 << 24;
-
 __global__ void k_zero_doubles(double* a, double* b, size_t n) {
     for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
         a[i] = 0.0;
@@ -683,12 +676,16 @@ void mark_event(std::vector<hipEvent_t>* v, hipStream_t s) {
 // One wavefront batch of P paths -- samples sample0, sample0 + 1, ... of every pixel, slot = sample
 // offset * npix + pixel: raygen, then a closest-hit, a shade and a shadow launch per bounce until no
 // path is left (or maxdepth).  Each path's radiance ends in t.radiance[slot] (the sky rays' share in
-// rad_env with sky sampling).  In batches of fewer than OVERLAP_PATHS paths a bounce's shadow launch
-// runs on sc.stream2, beside the next bounce's closest-hit launch (the two share no data: shadow
-// rays read the shadow queue and add to the radiance, closest hits read the next ray queue); the
-// next shade waits for it, since it refills the shadow queue and adds to the same radiance.
-// Everything has joined sc.stream on return.  The ev_* vectors (null: untimed) collect start /
-// stop events around the launches.  Returns 1 if cancelled between bounces.
+// rad_env with sky sampling).  A bounce's shadow launch runs on sc.stream2, beside the next
+// bounce's closest-hit launch (the two share no data: shadow rays read the shadow queue and add to
+// the radiance, closest hits read the next ray queue); the next shade waits for it, since it
+// refills the shadow queue and adds to the same radiance.  Small batches gain most (their trace
+// launches are bound by their slowest rays, and the shadow launch fills the idle CUs: 1080p x 1
+// spp frames 10-14 % faster), the 720p x 64 spp frame 2.3 %.  Timed or counted runs (the ev_*
+// vectors or ts non-null) keep every launch on sc.stream instead, so that each kernel's time and
+// counts are its own.  Everything has joined sc.stream on return.  The ev_* vectors (null:
+// untimed) collect start / stop events around the launches.  Returns 1 if cancelled between
+// bounces.
 int wavefront_batch(Scene& sc, RenderTarget& t, const CamView& cam, const RenderParams& p, uint32_t sample0, int P,
                     bool env, float4* rad_env, TraceStats* ts, volatile int* cancel, std::vector<hipEvent_t>* ev_c,
                     std::vector<hipEvent_t>* ev_h, std::vector<hipEvent_t>* ev_s, int& iterations,
@@ -698,7 +695,7 @@ int wavefront_batch(Scene& sc, RenderTarget& t, const CamView& cam, const Render
     const int npix = p.W * p.H;
     int* cnt = t.counters.as<int>();
     const int grid_shade = device_cu_count() * 8;
-    const hipStream_t s2 = P < OVERLAP_PATHS ? sc.stream2 : s;
+    const hipStream_t s2 = (ev_c || ts) ? s : sc.stream2;
     if (sc.trace_order) t.order.ensure(sizeof(int) * 3 * t.cap);  // ray queue (cap), shadow queue (2 cap)
     hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, int(C_RAY0), P, -1, 0);
     hipLaunchKernelGGL(k_raygen, dim3((P + BLK - 1) / BLK), dim3(BLK), 0, s, cam, p.W, p.H, npix, P, p.seed,
@@ -932,7 +929,7 @@ int render_nrc(Scene& sc, Nrc& nrc, const CamView& cam, const RenderParams& p, c
     const int grid_shade = device_cu_count() * 8;
     auto t0 = std::chrono::steady_clock::now();
     RenderStats rs;
-    const hipStream_t s2 = cap < size_t(OVERLAP_PATHS) ? sc.stream2 : s;
+    const hipStream_t s2 = sc.stream2;  // the shadow launches beside the next closest-hit ones
     auto join = [&]() { check_hip(hipStreamWaitEvent(s, sc.shadow_done, 0), "join shadow stream"); };
     hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, int(C_RAY0), int(cap), -1, 0);
     hipLaunchKernelGGL(k_raygen_nrc, dim3((cap + BLK - 1) / BLK), dim3(BLK), 0, s, cam, p.W, p.H, npix, int(P), int(T),
