@@ -460,6 +460,11 @@ def main():
     ap.add_argument("--no-nrc", action="store_true", help="skip the time-to-RMSE measurement (about 40 s)")
     ap.add_argument("--frames-only", action="store_true",
                     help="only the ptnee frames (no adjoint, cache or CPU legs), e.g. under a profiler")
+    ap.add_argument("--serial-frames", action="store_true",
+                    help="run the warm-up and timed frames with HIP-event timing on, so that their launches run one "
+                         "after another as in the kernel-time frame (for kernel-trace profiles, whose per-dispatch "
+                         "durations then match the roofline's launch time); the default overlaps each bounce's "
+                         "shadow launch with the next closest-hit launch")
     ap.add_argument("--trace-order", type=int, default=0,
                     help="order of the wavefront trace launches: 0 queue, 1 octant per 1024 rays, 2 64 direction "
                          "bins per 4096 rays (same results)")
@@ -516,6 +521,7 @@ def main():
     print(f"# scene {sc['name']}: {len(sc['p'])} triangles, BVH {info['n_nodes']} nodes depth {info['depth']}, "
           f"build {info['build_ms']:.0f} ms, prepare {prep_s:.2f} s", file=sys.stderr, flush=True)
 
+    dev.set_profiling(1 if args.serial_frames else 0)
     for _ in range(args.warmup):
         dev.render(sc["camera"], W, H, **rp)
     # counters pass (untimed): node fetches / triangle tests / rays of the closest-hit kernel
@@ -529,7 +535,7 @@ def main():
     dev.render(sc["camera"], W, H, **rp)
     kt = dev.stats()
     closest_ms, shade_ms, shadow_ms = kt["closest_ms"], kt["shade_ms"], kt["shadow_ms"]
-    dev.set_profiling(0)
+    dev.set_profiling(1 if args.serial_frames else 0)
     nr.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -585,7 +591,8 @@ def main():
         "config": {"workload": f"ptnee {W}x{H}x{spp}spp maxdepth10 rrdepth3 prr0.8, {sc['name']} "
                                f"({len(sc['p'])} triangles) + 1024x512 sky", "global_batch": W * H * spp,
                    "seq_len": None, "parallelism": "single-gpu", "bvh_build_ms": info["build_ms"],
-                   "trace_order": args.trace_order},
+                   "trace_order": args.trace_order,
+                   "launches": "serial" if args.serial_frames else "shadow beside the next closest hit"},
         "roofline": roofline, "cpu_baseline": cpu, "hbm_copy_gbs": copy_gbs,
         "adjoint": adj, "nrc": nrc, "nrc_mlp": mlp, "env_sampling": env, "reference_scene": ref_scene,
         "render_modes": modes,
