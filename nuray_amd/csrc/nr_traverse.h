@@ -36,18 +36,24 @@ __device__ __forceinline__ void trav_begin_wide(const SceneView& sv, const SlabR
 // Every call makes progress: while no lane holds a leaf the loop pops an entry per searching lane.
 // The set of nodes and leaves visited is the same as in the sequential DFS (only their interleaving
 // changes), and the result does not depend on the order.
+// The top entry lives in a register (`top`): a pop takes it from there and reads the entry below
+// it from LDS at once, so that read is in flight together with the popped node's loads; after the
+// push the new top is the last child pushed, or that entry when no child was.
 template <bool STATS, int STRIDE, int LEAF_ROUND_SEARCHERS = 16>
 __device__ __forceinline__ void trav_expand_wave4(const SceneView& sv, const SlabRay& sr, int* stk, int cap,
                                                   TravState& s, uint32_t& n_nf) {
     bool parked = false;
+    int32_t top = stk[max(s.sp - 1, 0) * STRIDE];
     for (;;) {
         const int searching = __popcll(__ballot(!s.has_leaf && s.sp > 0));
         if (searching == 0 || (searching <= LEAF_ROUND_SEARCHERS && __any(s.has_leaf))) return;
         if (s.sp > 0 && !parked) {
-            const int32_t e = stk[(s.sp - 1) * STRIDE];
+            const int32_t e = top;
+            const int32_t below = stk[max(s.sp - 2, 0) * STRIDE];  // the top after this pop
             if (ref_is_leaf(e)) {
                 parked = s.has_leaf;
                 s.leaf = s.has_leaf ? s.leaf : e;
+                top = s.has_leaf ? top : below;
                 s.sp -= s.has_leaf ? 0 : 1;
                 s.has_leaf = true;
             } else {
@@ -80,6 +86,7 @@ __device__ __forceinline__ void trav_expand_wave4(const SceneView& sv, const Sla
                 p += int(h2);
                 stk[p * STRIDE] = ch.w;
                 p += int(h3);
+                top = h3 ? ch.w : h2 ? ch.z : h1 ? ch.y : h0 ? ch.x : below;
                 const bool ovf = p > cap;
                 s.overflow = s.overflow || ovf;
                 s.sp = ovf ? 0 : p;
