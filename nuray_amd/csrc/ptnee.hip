@@ -142,17 +142,24 @@ __device__ __forceinline__ F3 camera_dir(const CamView& cam, int W, int H, int p
     return normalized(sub(add(gaze, mul(mul(mul(hand, cam.fw), cx), 0.5f)), mul(mul(mul(up, cam.fh), cy), 0.5f)));
 }
 
+// Camera rays of a batch of kb = P / npix samples per pixel.  Queue position q holds sample
+// q % kb of pixel q / kb: the rays a wave traces, and those the whole GPU has in flight at any
+// moment, come from a few neighbouring pixels, so the nodes and triangles they reach stay in the
+// L2s (and the compacting appends of later bounces keep that order).  The path's slot, which
+// indexes its radiance and vertex records, stays sample offset * npix + pixel.
 __global__ void k_raygen(CamView cam, int W, int H, int npix, int P, uint32_t seed, uint32_t sample0, Ray* rays,
                          float4* beta, float4* radiance, float4* radiance_env) {
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= P) return;
-    const int pixel = slot % npix;
-    const F3 d = camera_dir(cam, W, H, pixel, seed, sample0 + uint32_t(slot / npix));
-    rays[slot].o = make_float4(cam.pos[0], cam.pos[1], cam.pos[2], __int_as_float(0));
-    rays[slot].d = make_float4(d.x, d.y, d.z, __int_as_float(slot));
-    beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-    radiance[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (radiance_env) radiance_env[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= P) return;
+    const int kb = P / npix;
+    const int pixel = q / kb, k = q - pixel * kb;
+    const int slot = k * npix + pixel;
+    const F3 d = camera_dir(cam, W, H, pixel, seed, sample0 + uint32_t(k));
+    rays[q].o = make_float4(cam.pos[0], cam.pos[1], cam.pos[2], __int_as_float(0));
+    rays[q].d = make_float4(d.x, d.y, d.z, __int_as_float(slot));
+    beta[q] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    radiance[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // slots [0, P): every one is some q
+    if (radiance_env) radiance_env[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // Sample index of training path i of a frame: its own Philox counters, disjoint from the render
@@ -697,6 +704,7 @@ int wavefront_batch(Scene& sc, RenderTarget& t, const CamView& cam, const Render
     const int grid_shade = device_cu_count() * 8;
     const hipStream_t s2 = (ev_c || ts) ? s : sc.stream2;
     if (sc.trace_order) t.order.ensure(sizeof(int) * 3 * t.cap);  // ray queue (cap), shadow queue (2 cap)
+    if (P % npix != 0) throw std::runtime_error("wavefront batch of a partial sample per pixel");
     hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, int(C_RAY0), P, -1, 0);
     hipLaunchKernelGGL(k_raygen, dim3((P + BLK - 1) / BLK), dim3(BLK), 0, s, cam, p.W, p.H, npix, P, p.seed,
                        sample0, t.rays[0].as<Ray>(), t.beta[0].as<float4>(),
