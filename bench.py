@@ -594,6 +594,11 @@ def main():
                    "trace_order": args.trace_order,
                    "launches": "serial" if args.serial_frames else "shadow beside the next closest hit"},
         "roofline": roofline, "cpu_baseline": cpu, "hbm_copy_gbs": copy_gbs,
+        "kernel_ms_per_frame": {"closest_hit": closest_ms, "shade": shade_ms, "shadow": shadow_ms,
+                                "from": "one untimed frame with serial launches"},
+        "shadow_rays": {"rays_per_frame": cnt["shadow_rays"],
+                        "node_fetch_per_ray": cnt["shadow_node_fetch"] / max(cnt["shadow_rays"], 1),
+                        "tri_tests_per_ray": cnt["shadow_tri_tests"] / max(cnt["shadow_rays"], 1)},
         "adjoint": adj, "nrc": nrc, "nrc_mlp": mlp, "env_sampling": env, "reference_scene": ref_scene,
         "render_modes": modes,
     }
