@@ -15,7 +15,7 @@
 // in LDS at stack[slot * BLOCK + tid] (one 4-byte bank per lane, conflict-free).  Internal nodes
 // are expanded wave-cooperatively (trav_expand_wave4) until (nearly) every lane holds a leaf, then
 // the wave tests the held leaves' triangles together, one triangle per lane, idle lanes included
-// (trav_leaf_wave); a lane whose ray is done goes idle, and once REFILL_MIN lanes are
+// (trav_leaf_wave); a lane whose ray is done goes idle, and once refill_min lanes are
 // idle one atomic claims new rays for all of them (ballot + prefix count), so long rays do not hold
 // a whole wave hostage.  The first launch gives each lane SceneView::short_stack live entries (plus
 // three spare slots for the branch-free push).  The rare ray
@@ -36,8 +36,11 @@ constexpr int BLOCK = 256;
 // rays from the global counter up to CLAIM at a time and hands them out from that private range;
 // for small batches the claim shrinks to the batch's share per wave, so that every wave starts
 // rays at once instead of a few waves working through long private ranges (the launch then
-// takes about one ray's latency, not several).
-constexpr int REFILL_MIN = 24;
+// takes about one ray's latency, not several).  Large launches wait for more idle lanes (fewer,
+// fuller refills): on the 720p x 64 spp terrain frame (camera rays queued pixel by pixel) 40 idle
+// lanes measured best for both launches (36 and 44: -0.5 %, 24 and 48: -1.5 to -3 %), while the
+// 1080p x 1 spp frames, whose launches hold at most 2 M rays, are 1.5 % faster with 24.
+constexpr int REFILL_MIN = 24, REFILL_MIN_LARGE = 40, REFILL_LARGE_RAYS = 1 << 22;
 // Leaf-round thresholds of the wave-cooperative expansion.  With the lane-parallel leaf round a
 // round costs about ceil(triangles / 64) triangle tests, so it pays to start one earlier than with
 // the per-lane loop (which favoured 24 / 8): on the 720p terrain frame 28-36 (closest) and 12-20
@@ -70,6 +73,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     if (n <= 0) return;  // nothing queued (the retry launch, most of the time): no claims at all
     const int n_waves = int(gridDim.x) * (BLOCK / 64);
     const int claim = max(1, min(CLAIM, (n + n_waves - 1) / n_waves));
+    const int refill_min = n >= REFILL_LARGE_RAYS ? REFILL_MIN_LARGE : REFILL_MIN;
 
     // per-lane ray state; i < 0 means the lane is idle
     int i = -1;
@@ -89,7 +93,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
         // ---- refill idle lanes from the wave's claimed range; claim more when it is empty
         const unsigned long long idle_mask = __ballot(i < 0);
         const int n_idle = __popcll(idle_mask);
-        if ((n_idle >= REFILL_MIN || n_idle == 64) && !(drained && rb == re)) {
+        if ((n_idle >= refill_min || n_idle == 64) && !(drained && rb == re)) {
             if (rb == re) {
                 int base = 0;
                 if (lane == 0) base = atomicAdd(work, claim);
