@@ -43,9 +43,10 @@ constexpr int BLOCK = 256;
 constexpr int REFILL_MIN = 24, REFILL_MIN_LARGE = 40, REFILL_LARGE_RAYS = 1 << 22;
 // Leaf-round thresholds of the wave-cooperative expansion.  With the lane-parallel leaf round a
 // round costs about ceil(triangles / 64) triangle tests, so it pays to start one earlier than with
-// the per-lane loop (which favoured 24 / 8): on the 720p terrain frame 28-36 (closest) and 12-20
-// (shadow) are within 0.5 % of each other and 0.5-0.8 % ahead of 24 / 8.
-constexpr int CLOSEST_SEARCHERS = 32;
+// the per-lane loop (which favoured 24 / 8) for shadow rays: on the 720p terrain frame 12-20 are
+// within 0.5 % of each other.  With camera rays queued pixel by pixel and 40-lane refills, 24
+// closest-hit searchers measured 0.4 % ahead of 32 and 1.3 % ahead of 40.
+constexpr int CLOSEST_SEARCHERS = 24;
 constexpr int SHADOW_SEARCHERS = 16;
 constexpr int CLAIM = 128;
 // Live stack entries of the first launch default to SceneView::short_stack = 16:
