@@ -465,9 +465,10 @@ def main():
                          "after another as in the kernel-time frame (for kernel-trace profiles, whose per-dispatch "
                          "durations then match the roofline's launch time); the default overlaps each bounce's "
                          "shadow launch with the next closest-hit launch")
-    ap.add_argument("--trace-order", type=int, default=0,
+    ap.add_argument("--trace-order", type=int, default=3,
                     help="order of the wavefront trace launches: 0 queue, 1 octant per 1024 rays, 2 64 direction "
-                         "bins per 4096 rays (same results)")
+                         "bins per 4096 rays, 3 (the library's default) shadow rays alone by 64 bins per 256 rays, "
+                         "4 the same per 1024 rays (same results)")
     ap.add_argument("--large", action="store_true",
                     help="only measure the large textured scene (3840x2160 x 256 spp, texel adjoint) and exit")
     ap.add_argument("--mlp-only", action="store_true",

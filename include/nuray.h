@@ -173,11 +173,12 @@ int nr_set_profiling(nr_scene* s, int32_t mode);
  * results do not depend on it; smaller values raise occupancy, larger ones retry fewer rays. */
 int nr_scene_set_short_stack(nr_scene* s, int32_t entries);
 /* Traversal tuning (no counterpart in the reference): the order in which a wavefront render's
- * trace launches take their queued rays.  0 (default): queue order, in which neighbouring rays
- * come from neighbouring pixels; 1: secondary and shadow rays grouped by direction octant within
- * chunks of 1024 queue positions; 2: by 64 direction bins within chunks of 4096.  Results do not
- * depend on it (ties are broken by leaf order, not by visit order); on the 720p terrain frame both
- * sorted orders are slower, since every ray still visits the same nodes. */
+ * trace launches take their queued rays.  0: queue order, in which neighbouring rays come from
+ * neighbouring pixels; 1: secondary and shadow rays grouped by direction octant within chunks of
+ * 1024 queue positions; 2: by 64 direction bins within chunks of 4096; 3 (default): shadow rays
+ * alone by the 64 bins within chunks of 256, so that a wave's shadow rays leave a few neighbouring
+ * pixels towards the same light; 4: the same within chunks of 1024.  Results do not depend on it
+ * (ties are broken by leaf order, not by visit order). */
 int nr_scene_set_trace_order(nr_scene* s, int32_t mode);
 
 /* ---- Differentiable ptnee (no counterpart in the reference, which has no adjoint).

@@ -338,7 +338,8 @@ class Scene:
 
     def set_trace_order(self, mode):
         """Order the wavefront's trace launches take rays in: 0 queue order, 1 direction octant per
-        1024 rays, 2 sixty-four direction bins per 4096 rays (results are the same)."""
+        1024 rays, 2 sixty-four direction bins per 4096 rays, 3 (default) shadow rays alone by the
+        64 bins per 256 rays, 4 the same per 1024 rays (results are the same)."""
         check(lib().nr_scene_set_trace_order(self.h, mode))
 
     def stats(self):

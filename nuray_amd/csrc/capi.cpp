@@ -286,7 +286,7 @@ int nr_scene_set_short_stack(nr_scene* s, int32_t entries) {
 
 int nr_scene_set_trace_order(nr_scene* s, int32_t mode) {
     return guard([&] {
-        if (mode < 0 || mode > 2) throw std::runtime_error("trace order must be 0, 1 or 2");
+        if (mode < 0 || mode > 4) throw std::runtime_error("trace order must be 0 to 4");
         s->sc->trace_order = mode;
         return NR_OK;
     });

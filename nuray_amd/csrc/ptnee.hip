@@ -717,9 +717,10 @@ int wavefront_batch(Scene& sc, RenderTarget& t, const CamView& cam, const Render
         // optional direction grouping of secondary and shadow rays (primary rays: pixel order)
         int* const ord = sc.trace_order ? t.order.as<int>() : nullptr;
         mark_event(ev_c, s);
-        if (ord && it > 0) sort_by_direction(t.rays[cur].as<Ray>(), cnt + at(C_RAY0 + cur), P, sc.trace_order, ord, s);
+        const bool sort_ext = ord && it > 0 && sc.trace_order <= 2;  // modes 3 and 4 sort shadow rays only
+        if (sort_ext) sort_by_direction(t.rays[cur].as<Ray>(), cnt + at(C_RAY0 + cur), P, sc.trace_order, ord, s);
         launch_closest(sc.view, t.rays[cur].as<Ray>(), cnt + at(C_RAY0 + cur), t.hits.as<HitRec>(), cnt + at(C_WORK0),
-                       cnt + at(C_ERR), ts, s, t.retry.as<int>(), (ord && it > 0) ? ord : nullptr);
+                       cnt + at(C_ERR), ts, s, t.retry.as<int>(), sort_ext ? ord : nullptr);
         mark_event(ev_c, s);
         ShadeArgs a{sc.view, npix, p.seed, sample0, p.prr, p.mis, p.maxdepth, p.rrdepth,
                     p.integrator == 1 ? 0 : 1,

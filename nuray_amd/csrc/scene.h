@@ -44,7 +44,7 @@ public:
     float env_gain = 1.0f;
     unsigned features = 0;        // material features present: 1 = GGX dielectric, 2 = albedo textures
     int short_stack = 16;         // LDS stack entries of the first trace launch
-    int trace_order = 0;          // ray order of the wavefront's trace launches (sort_by_direction)
+    int trace_order = 3;          // ray order of the wavefront's trace launches (sort_by_direction)
     // Bumped by every change that can alter a render (materials, parameters, textures, sky,
     // build): per-sample results kept from a render serve the adjoint only at the same value.
     uint64_t generation = 0;

@@ -35,7 +35,8 @@ void launch_shadow(const SceneView& sv, const Ray* rays, const int* n, uint8_t* 
 // order[0 .. *n): queue positions grouped by ray direction inside chunks of consecutive
 // positions, so that the rays a wave traces together point the same way.  mode 1: the 8 octants
 // of rays[i].d within chunks of 1024; mode 2: 64 bins (each component quantised to 4) within
-// chunks of 4096.  `max_n` bounds *n (sizes the grid).
+// chunks of 4096; mode 3: the 64 bins within chunks of 256; mode 4: within chunks of 1024.
+// `max_n` bounds *n (sizes the grid).
 void sort_by_direction(const Ray* rays, const int* n, int max_n, int mode, int* order, hipStream_t s);
 
 int device_cu_count();

@@ -302,6 +302,10 @@ void sort_by_direction(const Ray* rays, const int* n, int max_n, int mode, int* 
     if (max_n <= 0) return;
     if (mode == 1)
         hipLaunchKernelGGL((k_sort_dir<1024, true>), dim3((max_n + 1023) / 1024), dim3(SORT_BLOCK), 0, s, rays, n, order);
+    else if (mode == 3)
+        hipLaunchKernelGGL((k_sort_dir<256, false>), dim3((max_n + 255) / 256), dim3(SORT_BLOCK), 0, s, rays, n, order);
+    else if (mode == 4)
+        hipLaunchKernelGGL((k_sort_dir<1024, false>), dim3((max_n + 1023) / 1024), dim3(SORT_BLOCK), 0, s, rays, n, order);
     else
         hipLaunchKernelGGL((k_sort_dir<4096, false>), dim3((max_n + 4095) / 4096), dim3(SORT_BLOCK), 0, s, rays, n, order);
     check_hip(hipGetLastError(), "k_sort_dir");

@@ -100,11 +100,12 @@ def test_short_stack_retry_is_bit_exact(entries, nr, po, scenes):
     assert np.array_equal(a["sum"], b["sum"]) and np.array_equal(a["sum2"], b["sum2"])
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
 def test_trace_order_is_bit_exact(mode, nr, scenes):
-    """Secondary and shadow rays traced in direction-sorted order give the queue-order render bit
-    for bit (the winner among equal t is fixed by leaf order, not by visit order), also when most
-    rays overflow a 1-entry first-launch stack and go through the retry list."""
+    """Secondary and shadow rays (modes 1, 2), or shadow rays alone (3, 4), traced in
+    direction-sorted order give the queue-order render bit for bit (the winner among equal t is
+    fixed by leaf order, not by visit order), also when most rays overflow a 1-entry first-launch
+    stack and go through the retry list."""
     sc = SCENES["terrain64"](scenes)
     dev = nr.Scene(sc)
     out = []
@@ -113,7 +114,7 @@ def test_trace_order_is_bit_exact(mode, nr, scenes):
         dev.set_short_stack(stack)
         out.append(dev.render(sc["camera"], 96, 54, 4, seed=5, maxdepth=6, rrdepth=2))
     with pytest.raises(RuntimeError):
-        dev.set_trace_order(3)
+        dev.set_trace_order(5)
     dev.close()
     print(f"trace order {mode}: {len(out)} renders")
     for r in out[1:]:
