@@ -122,6 +122,30 @@ def test_split_renders_equal_one_render(nr, scenes):
     assert np.array_equal(a["rgb8"], b["rgb8"])
 
 
+@pytest.mark.parametrize("env_is", [False, True])
+def test_overlapped_launches_equal_serial_ones(nr, scenes, env_is):
+    """Each bounce's shadow launch runs on a second stream beside the next bounce's closest-hit
+    launch unless kernel timing is on (nr_set_profiling), which keeps every launch on one stream.
+    The two schedules must give the same image bit for bit: the overlapped launches share no data
+    (the next shade step waits for the shadow launch), so a difference would mean a race.  Several
+    batches (batch_spp) and sky shadow rays (env_is) included."""
+    sc = scenes.terrain_spheres(n_cells=48, n_spheres=60, width=128, height=72)
+    dev = nr.Scene(sc)
+    kw = dict(seed=6, maxdepth=10, rrdepth=3, batch_spp=5, env_is=env_is)
+    dev.set_profiling(0)
+    a = dev.render(sc["camera"], 128, 72, 16, **kw)
+    dev.set_profiling(1)
+    b = dev.render(sc["camera"], 128, 72, 16, **kw)
+    st = dev.stats()
+    dev.set_profiling(0)
+    c = dev.render(sc["camera"], 128, 72, 16, **kw)
+    dev.close()
+    print(f"env_is {env_is}: {st['iterations']:.0f} bounces, closest {st['closest_ms']:.2f} ms, "
+          f"shadow {st['shadow_ms']:.2f} ms (serial run), image mean {a['sum'].mean() / 16:.5f}")
+    for r in (b, c):
+        assert np.array_equal(r["sum"], a["sum"]) and np.array_equal(r["sum2"], a["sum2"])
+
+
 def test_progress_and_cancel(nr, scenes):
     sc = scenes.box_room()
     dev = nr.Scene(sc)
