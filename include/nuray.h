@@ -48,7 +48,7 @@ typedef struct nr_camera {
 } nr_camera;
 
 /* The Config keys RendererPTLS::render / ::trace read (rendererptls.cpp:65,81,98-100,130,191)
- * plus this build's seed / maxdepth / rrdepth.  maxdepth = 0 and rrdepth = 0 are the reference's
+ * plus the library's own seed / maxdepth / rrdepth.  maxdepth = 0 and rrdepth = 0 are the reference's
  * semantics (no depth limit, Russian roulette at every vertex). */
 typedef struct nr_params {
     int32_t width, height; /* imgw, imgh */

@@ -43,7 +43,7 @@ struct Builder {
     std::vector<Tmp> nodes;
     std::atomic<int> n_nodes{0};
     // set when no split had a cost below the initial 1e38 (surface areas that overflow FP32, or
-    // NaN coordinates): the reference would recurse on an empty part forever; this build stops
+    // NaN coordinates): the reference would recurse on an empty part forever; this builder stops
     // and reports it
     std::atomic<bool> degenerate{false};
 
