@@ -146,9 +146,10 @@ __device__ __forceinline__ F3 camera_dir(const CamView& cam, int W, int H, int p
 // q % kb of pixel q / kb: the rays a wave traces, and those the whole GPU has in flight at any
 // moment, come from a few neighbouring pixels, so the nodes and triangles they reach stay in the
 // L2s (and the compacting appends of later bounces keep that order).  The path's slot, which
-// indexes its radiance and vertex records, stays sample offset * npix + pixel.
+// indexes its radiance and vertex records, stays sample offset * npix + pixel.  The throughput of
+// every camera ray is 1 (the first shade step is given no throughput array).
 __global__ void k_raygen(CamView cam, int W, int H, int npix, int P, uint32_t seed, uint32_t sample0, Ray* rays,
-                         float4* beta, float4* radiance, float4* radiance_env) {
+                         float4* radiance, float4* radiance_env) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= P) return;
     const int kb = P / npix;
@@ -157,7 +158,6 @@ __global__ void k_raygen(CamView cam, int W, int H, int npix, int P, uint32_t se
     const F3 d = camera_dir(cam, W, H, pixel, seed, sample0 + uint32_t(k));
     rays[q].o = make_float4(cam.pos[0], cam.pos[1], cam.pos[2], __int_as_float(0));
     rays[q].d = make_float4(d.x, d.y, d.z, __int_as_float(slot));
-    beta[q] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
     radiance[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // slots [0, P): every one is some q
     if (radiance_env) radiance_env[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
@@ -221,7 +221,7 @@ struct ShadeArgs {
     int mis, maxdepth, rrdepth;
     int nee;  // 1: ptnee; 0: pt (rendererpt.cpp:13-61: no light sampling, so emitters keep fac = 1)
     const Ray* rin;
-    const float4* bin;
+    const float4* bin;  // throughput and MIS factor per queued ray; null: (1, 1, 1, 1) for every ray
     const HitRec* hits;
     const int* n_in;
     float4* radiance;
@@ -342,7 +342,7 @@ __device__ __forceinline__ void shade_body(const ShadeArgs& a, const NrcShade& n
         if (active) {
             const Ray r = a.rin[j];
             const HitRec h = a.hits[j];
-            const float4 bf = a.bin[j];
+            const float4 bf = a.bin ? a.bin[j] : make_float4(1.0f, 1.0f, 1.0f, 1.0f);  // null: camera rays
             const int slot = __float_as_int(r.d.w);
             const int depth = __float_as_int(r.o.w);
             if (!NRC && !ENV && a.vrec && depth < a.vrec_cap) {
@@ -707,8 +707,7 @@ int wavefront_batch(Scene& sc, RenderTarget& t, const CamView& cam, const Render
     if (P % npix != 0) throw std::runtime_error("wavefront batch of a partial sample per pixel");
     hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, cnt, int(C_RAY0), P, -1, 0);
     hipLaunchKernelGGL(k_raygen, dim3((P + BLK - 1) / BLK), dim3(BLK), 0, s, cam, p.W, p.H, npix, P, p.seed,
-                       sample0, t.rays[0].as<Ray>(), t.beta[0].as<float4>(),
-                       t.radiance.as<float4>(), rad_env);
+                       sample0, t.rays[0].as<Ray>(), t.radiance.as<float4>(), rad_env);
     auto join = [&]() { check_hip(hipStreamWaitEvent(s, sc.shadow_done, 0), "join shadow stream"); };
     for (int it = 0;; ++it) {
         const int cur = it & 1, nxt = cur ^ 1;
@@ -724,9 +723,9 @@ int wavefront_batch(Scene& sc, RenderTarget& t, const CamView& cam, const Render
         mark_event(ev_c, s);
         ShadeArgs a{sc.view, npix, p.seed, sample0, p.prr, p.mis, p.maxdepth, p.rrdepth,
                     p.integrator == 1 ? 0 : 1,
-                    t.rays[cur].as<Ray>(), t.beta[cur].as<float4>(), t.hits.as<HitRec>(), cnt + at(C_RAY0 + cur),
-                    t.radiance.as<float4>(), t.rays[nxt].as<Ray>(), t.beta[nxt].as<float4>(), cnt + at(C_RAY0 + nxt),
-                    t.shadow.as<Ray>(), t.contrib.as<float4>(), sh_cnt,
+                    t.rays[cur].as<Ray>(), it == 0 ? nullptr : t.beta[cur].as<float4>(), t.hits.as<HitRec>(),
+                    cnt + at(C_RAY0 + cur), t.radiance.as<float4>(), t.rays[nxt].as<Ray>(), t.beta[nxt].as<float4>(),
+                    cnt + at(C_RAY0 + nxt), t.shadow.as<Ray>(), t.contrib.as<float4>(), sh_cnt,
                     env ? t.skyw.as<float>() : nullptr, env ? P : 0, vrec, VREC_CAP};
         if (it > 0) join();  // the previous bounce's shadow launch has read the shadow queue
         mark_event(ev_h, s);
